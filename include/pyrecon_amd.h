@@ -1,0 +1,243 @@
+/*
+ * pyrecon_amd -- C ABI of the MI355X (gfx950) MultiGridReconstruction library.
+ *
+ * The reference (pyrecon 1.0.0) binds two native libraries per precision
+ * through ctypes: mesh_{float,double}.so (src/mesh.c) and
+ * multigrid_{float,double}.so (src/multigrid.c); see pyrecon/mesh.py:126-154,
+ * 529-653 and pyrecon/multigrid.py:115-133.  This header declares the
+ * replacement: one shared library, libpyrecon_amd.so, whose entry points take
+ * DEVICE pointers for every mesh, position, weight and output array and HOST
+ * pointers for the small geometry arrays (nmesh, boxsize, boxcenter, los,
+ * radius), plus a HIP stream (void*, NULL = default stream).  Every call
+ * returns an int status (PRH_OK or a negative PRH_ERR_*); the message of the
+ * last failure on the calling thread is returned by prh_last_error().
+ *
+ * Suffix _f32 / _f64 selects the mesh precision, like the reference's
+ * FLOAT32 build switch (src/utils.h:9-19).  All kernel entry points are
+ * asynchronous on the given stream unless stated otherwise.
+ */
+#ifndef PYRECON_AMD_H
+#define PYRECON_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define PRH_OK 0
+#define PRH_ERR_ARG (-1)       /* invalid argument */
+#define PRH_ERR_RANGE (-2)     /* positions outside the box (reference: MeshError, mesh.py:549-551) */
+#define PRH_ERR_NONFINITE (-3) /* NaN/inf positions or weights (reference: MeshError, mesh.py:552-553) */
+#define PRH_ERR_HIP (-4)       /* HIP runtime error */
+#define PRH_ERR_NOMEM (-5)     /* device allocation failed */
+#define PRH_ERR_FFT (-6)       /* hipFFT error */
+#define PRH_ERR_CAPACITY (-7)  /* request larger than the workspace was built for */
+#define PRH_ERR_STATE (-8)     /* call out of order */
+
+/* ---- element type codes (positions, weights, outputs) --------------------- */
+#define PRH_F32 4
+#define PRH_F64 8
+
+/* ---- shift fields (reference: recon.py:283-302, multigrid.py:137-158) ----- */
+#define PRH_FIELD_DISP 0
+#define PRH_FIELD_RSD 1
+#define PRH_FIELD_DISP_RSD 2
+
+/* ---- runtime plumbing (no reference counterpart: device memory is what
+ *      replaces the reference's numpy buffers) ---------------------------- */
+const char *prh_last_error(void);
+int prh_version(void);
+int prh_device_count(int *count);
+int prh_set_device(int device);
+int prh_get_device_name(char *buffer, size_t size);
+int prh_synchronize(void *stream);
+int prh_stream_create(void **stream);
+int prh_stream_destroy(void *stream);
+int prh_malloc(void **ptr, size_t bytes);
+int prh_free(void *ptr);
+int prh_host_alloc(void **ptr, size_t bytes); /* pinned host memory */
+int prh_host_free(void *ptr);
+int prh_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int prh_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int prh_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int prh_memset(void *dst, int value, size_t bytes, void *stream);
+int prh_mem_info(size_t *free_bytes, size_t *total_bytes);
+int prh_event_create(void **event);
+int prh_event_destroy(void *event);
+int prh_event_record(void *event, void *stream);
+int prh_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronizes on stop */
+
+/* ---- workspace ------------------------------------------------------------
+ * Owns everything a reconstruction of one mesh shape needs besides the
+ * caller's meshes: the multigrid hierarchy (replacing the malloc/calloc per
+ * level per call of src/multigrid.c:280-297, 314-324), the paint scratch
+ * (points are painted in chunks of at most paint_chunk points), the hipFFT
+ * plans and spectrum buffer, and the FP32 hierarchy of the mixed-precision
+ * solve.  Nothing is allocated by the hot calls.  flags: PRH_WS_* below.
+ * A workspace may be used by one stream at a time.                          */
+typedef struct prh_workspace prh_workspace;
+#define PRH_WS_MULTIGRID 1u /* multigrid hierarchy in the workspace precision */
+#define PRH_WS_PAINT 2u     /* CIC paint scratch */
+#define PRH_WS_FFT 4u       /* hipFFT plans + spectrum buffer */
+#define PRH_WS_MIXED 8u     /* FP32 hierarchy for prh_fmg_mixed (FP64 workspaces) */
+#define PRH_WS_ALL 15u
+int prh_workspace_create(prh_workspace **ws, const size_t *nmesh, int dtype, size_t paint_chunk,
+                         unsigned flags);
+int prh_workspace_destroy(prh_workspace *ws);
+int prh_workspace_bytes(const prh_workspace *ws, size_t *bytes);
+/* bytes a workspace with these arguments would allocate (admission control) */
+int prh_workspace_footprint(const size_t *nmesh, int dtype, size_t paint_chunk, unsigned flags,
+                            size_t *bytes);
+
+/* ---- reference-shaped kernels (mesh units, device pointers) ---------------
+ * Same arguments and meaning as the reference C functions; positions are
+ * already in mesh units, (N, 3) C-order, cast to the mesh precision.       */
+
+/* replaces assign_cic, src/mesh.c:26-63.  Deterministic: the result equals
+ * the reference's serial accumulation in point order, bit for bit.        */
+int prh_assign_cic_f32(prh_workspace *ws, float *mesh, const size_t *nmesh, const float *positions,
+                       const float *weights, size_t npositions, void *stream);
+int prh_assign_cic_f64(prh_workspace *ws, double *mesh, const size_t *nmesh,
+                       const double *positions, const double *weights, size_t npositions,
+                       void *stream);
+/* replaces read_cic, src/mesh.c:269-310 */
+int prh_read_cic_f32(const float *mesh, const size_t *nmesh, const float *positions, float *values,
+                     size_t npositions, void *stream);
+int prh_read_cic_f64(const double *mesh, const size_t *nmesh, const double *positions,
+                     double *values, size_t npositions, void *stream);
+/* replaces read_finite_difference_cic, src/mesh.c:187-266 */
+int prh_read_finite_difference_cic_f32(const float *mesh, const size_t *nmesh,
+                                       const float *boxsize, const float *positions,
+                                       float *shifts, size_t npositions, void *stream);
+int prh_read_finite_difference_cic_f64(const double *mesh, const size_t *nmesh,
+                                       const double *boxsize, const double *positions,
+                                       double *shifts, size_t npositions, void *stream);
+/* replaces jacobi, src/multigrid.c:44-104 (ws supplies the second iterate) */
+int prh_jacobi_f32(prh_workspace *ws, float *v, const float *f, const size_t *nmesh,
+                   const float *boxsize, const float *boxcenter, float beta, float damping_factor,
+                   int niterations, const float *los, void *stream);
+int prh_jacobi_f64(prh_workspace *ws, double *v, const double *f, const size_t *nmesh,
+                   const double *boxsize, const double *boxcenter, double beta,
+                   double damping_factor, int niterations, const double *los, void *stream);
+/* replaces residual, src/multigrid.c:107-164 */
+int prh_residual_f32(const float *v, const float *f, float *r, const size_t *nmesh,
+                     const float *boxsize, const float *boxcenter, float beta, const float *los,
+                     void *stream);
+int prh_residual_f64(const double *v, const double *f, double *r, const size_t *nmesh,
+                     const double *boxsize, const double *boxcenter, double beta,
+                     const double *los, void *stream);
+/* replaces prolong, src/multigrid.c:167-209 (nmesh = coarse shape) */
+int prh_prolong_f32(const float *v2h, float *v1h, const size_t *nmesh, void *stream);
+int prh_prolong_f64(const double *v2h, double *v1h, const size_t *nmesh, void *stream);
+/* replaces reduce, src/multigrid.c:212-266 (nmesh = fine shape) */
+int prh_reduce_f32(const float *v1h, float *v2h, const size_t *nmesh, void *stream);
+int prh_reduce_f64(const double *v1h, double *v2h, const size_t *nmesh, void *stream);
+/* replaces vcycle, src/multigrid.c:270-300 */
+int prh_vcycle_f32(prh_workspace *ws, float *v, const float *f, const size_t *nmesh,
+                   const float *boxsize, const float *boxcenter, float beta, float damping_factor,
+                   int niterations, const float *los, void *stream);
+int prh_vcycle_f64(prh_workspace *ws, double *v, const double *f, const size_t *nmesh,
+                   const double *boxsize, const double *boxcenter, double beta,
+                   double damping_factor, int niterations, const double *los, void *stream);
+/* replaces fmg, src/multigrid.c:303-328: v1h is the initial iterate and
+ * receives the solution (the reference's return value is v1h too).       */
+int prh_fmg_f32(prh_workspace *ws, const float *f1h, float *v1h, const size_t *nmesh,
+                const float *boxsize, const float *boxcenter, float beta,
+                float jacobi_damping_factor, int jacobi_niterations, int vcycle_niterations,
+                const float *los, void *stream);
+int prh_fmg_f64(prh_workspace *ws, const double *f1h, double *v1h, const size_t *nmesh,
+                const double *boxsize, const double *boxcenter, double beta,
+                double jacobi_damping_factor, int jacobi_niterations, int vcycle_niterations,
+                const double *los, void *stream);
+
+/* ---- fused pipeline entry points (what the drop-in Python class calls) ----
+ * positions: device (N, 3) array of pos_dtype (PRH_F32/PRH_F64) in
+ * Cartesian coordinates; the mesh-unit transform, the wrap or range check and
+ * the finite check of pyrecon/mesh.py:529-568 are done on the device with
+ * numpy's type promotion.  boxsize/boxcenter are host arrays in the mesh
+ * precision (MeshInfo stores them so, mesh.py:447-455).                    */
+
+/* replaces RealMesh.assign_cic (mesh.py:529-568) + assign_cic (mesh.c:26).
+ * weights may be NULL (unit weights).  Returns PRH_ERR_RANGE / _NONFINITE
+ * (mesh untouched) like the reference's MeshError.  Synchronizes stream.  */
+int prh_paint_f32(prh_workspace *ws, float *mesh, const size_t *nmesh, const float *boxsize,
+                  const float *boxcenter, const void *positions, int pos_dtype,
+                  const void *weights, int weight_dtype, size_t npositions, int wrap,
+                  void *stream);
+int prh_paint_f64(prh_workspace *ws, double *mesh, const size_t *nmesh, const double *boxsize,
+                  const double *boxcenter, const void *positions, int pos_dtype,
+                  const void *weights, int weight_dtype, size_t npositions, int wrap,
+                  void *stream);
+
+/* replaces RealMesh.smooth_gaussian(method='fft') (mesh.py:655-680): in-place
+ * multiplication of the spectrum by exp(-(R k)^2 / 2), R2C/C2R hipFFT.     */
+int prh_smooth_gaussian_f32(prh_workspace *ws, float *mesh, const size_t *nmesh,
+                            const float *boxsize, const double *radius, void *stream);
+int prh_smooth_gaussian_f64(prh_workspace *ws, double *mesh, const size_t *nmesh,
+                            const double *boxsize, const double *radius, void *stream);
+
+/* Deterministic FP64 sums of one or two meshes (recon.py:227 np.sum calls).
+ * sums: host double[2]; synchronizes stream.                               */
+int prh_mesh_sums_f32(prh_workspace *ws, const float *a, const float *b, size_t size,
+                      double *sums, void *stream);
+int prh_mesh_sums_f64(prh_workspace *ws, const double *a, const double *b, size_t size,
+                      double *sums, void *stream);
+
+/* replaces the arithmetic of BaseReconstruction.set_density_contrast
+ * (recon.py:229-244): delta = data - alpha*randoms; delta /= bias_alpha *
+ * randoms where randoms > threshold, 0 elsewhere.  delta may alias data.
+ * With randoms == NULL (recon.py:246-247): delta = (data / mean - 1) / bias,
+ * where alpha carries mean.  Scalars are in the mesh precision, computed by
+ * the caller as numpy does.                                                */
+int prh_density_contrast_f32(const float *data, const float *randoms, float *delta, size_t size,
+                             float alpha, float threshold, float bias_alpha, float bias,
+                             void *stream);
+int prh_density_contrast_f64(const double *data, const double *randoms, double *delta,
+                             size_t size, double alpha, double threshold, double bias_alpha,
+                             double bias, void *stream);
+
+/* replaces read_shifts / read_shifted_positions of MultiGridReconstruction
+ * (multigrid.py:137-158, recon.py:304-325, mesh.py:612-653): finite-
+ * difference CIC read of phi, the RSD term along the local (los == NULL) or
+ * global line of sight, and, if shifted != 0, positions - shifts (wrapped when
+ * wrap != 0).  out is (N, 3) of out_dtype, which the caller derives with
+ * numpy's promotion rules (see pyrecon_amd/multigrid.py).  Returns
+ * PRH_ERR_RANGE if wrap == 0 and a position is outside the box; sets
+ * *nonfinite (host int, may be NULL) when positions hold NaN/inf (the
+ * reference only warns, mesh.py:637-639).  ws holds the device status
+ * flags (any workspace; one stream at a time).  Synchronizes stream.       */
+int prh_read_shifts_f32(prh_workspace *ws, const float *phi, const size_t *nmesh, const float *boxsize,
+                        const float *boxcenter, const void *positions, int pos_dtype,
+                        size_t npositions, double f, const float *los, int field, int wrap,
+                        int shifted, void *out, int out_dtype, int *nonfinite, void *stream);
+int prh_read_shifts_f64(prh_workspace *ws, const double *phi, const size_t *nmesh, const double *boxsize,
+                        const double *boxcenter, const void *positions, int pos_dtype,
+                        size_t npositions, double f, const double *los, int field, int wrap,
+                        int shifted, void *out, int out_dtype, int *nonfinite, void *stream);
+
+/* Mixed-precision solve (no reference counterpart; same contract as
+ * prh_fmg_f64): an FP32 full multigrid gives the first iterate, then
+ * nrefine defect-correction steps r64 = f - A v (FP64), e = V-cycles in FP32
+ * on r64 (vcycles_per_refine of them, zero start), v += e.  Requires a
+ * workspace built with PRH_WS_MIXED.  If residual_norms is not NULL it
+ * receives nrefine+1 FP64 RMS residual norms (host) and the call
+ * synchronizes.                                                            */
+int prh_fmg_mixed(prh_workspace *ws, const double *f1h, double *v1h, const size_t *nmesh,
+                  const double *boxsize, const double *boxcenter, double beta,
+                  double jacobi_damping_factor, int jacobi_niterations, int vcycle_niterations,
+                  const double *los, int nrefine, int vcycles_per_refine, double *residual_norms,
+                  void *stream);
+
+/* RMS of r = f - A v in FP64 (diagnostics and tests); host result, syncs. */
+int prh_residual_norm_f64(prh_workspace *ws, const double *v, const double *f, const size_t *nmesh,
+                          const double *boxsize, const double *boxcenter, double beta,
+                          const double *los, double *rms, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYRECON_AMD_H */

@@ -1,0 +1,205 @@
+"""End-to-end CPU oracle of ``MultiGridReconstruction`` -- TEST INFRASTRUCTURE ONLY.
+
+A restatement of the reference's Python layer with numpy, calling the C oracle
+(:mod:`oracle.cpu`) for the native kernels.  It follows, step for step:
+
+- mesh geometry and the mesh-unit transform: pyrecon/mesh.py:462-479, 529-568,
+  612-653 (``offset``, ``wrap``, range/finite checks, ``((x - c) / L + 0.5) * n``);
+- FFT Gaussian smoothing with numpy's complex-to-complex transform, as the
+  reference does with ``hermitian=False``: pyrecon/mesh.py:655-680, 697-705,
+  812-820, 827-844, 974-988, pyrecon/recon.py:117;
+- the density contrast: pyrecon/recon.py:198-249 (used by
+  MultiGridReconstruction, pyrecon/multigrid.py:165-167);
+- the solve: pyrecon/multigrid.py:99-135 (``beta = f / bias``, recon.py:122-125);
+- the shifts: pyrecon/multigrid.py:137-158 and pyrecon/recon.py:304-325.
+
+The numpy operations (and therefore the dtypes of every intermediate) are the
+reference's, so with the same native kernels this reproduces the reference's
+output bit for bit.  Nothing in the shipped package imports this module.
+"""
+import numpy as np
+
+from .cpu import OracleKernels
+
+
+class OracleError(Exception):
+    """Raised where the reference raises MeshError / ReconstructionError."""
+
+
+def distance(positions):
+    """Cartesian norm along the last axis (pyrecon/utils.py:146-148)."""
+    return np.sqrt((positions**2).sum(axis=-1))
+
+
+def _vec3(value, dtype):
+    out = np.empty(3, dtype=dtype)
+    out[...] = value
+    return out
+
+
+class OracleMultiGridReconstruction(object):
+    """Same constructor arguments and methods as the reference's class, for explicit
+    ``nmesh``, ``boxsize`` and ``boxcenter``."""
+
+    def __init__(self, f=0., bias=1., los=None, nmesh=None, boxsize=None, boxcenter=None, dtype='f8', wrap=False,
+                 kernels=None):
+        self.f = f
+        self.bias = bias
+        self.wrap = wrap
+        self.dtype = np.dtype(dtype)
+        # MeshInfo stores nmesh as c_int and boxsize / boxcenter in the mesh dtype (mesh.py:447-460)
+        self.nmesh = _vec3(nmesh, np.intc)
+        self.boxsize = _vec3(boxsize, self.dtype)
+        self.boxcenter = _vec3(boxcenter, self.dtype)
+        self.kernels = kernels if kernels is not None else OracleKernels()
+        if los in [None, 'local']:
+            self.los = None
+        else:
+            if isinstance(los, str):
+                los = 'xyz'.index(los)
+            if np.ndim(los) == 0:
+                ilos = los
+                los = np.zeros(3, dtype=self.dtype)
+                los[ilos] = 1.
+            los = np.array(los, dtype=self.dtype)
+            self.los = los / distance(los)
+        self.mesh_data = None
+        self.mesh_randoms = None
+        self._size_randoms = 0
+
+    @property
+    def beta(self):
+        return self.f / self.bias
+
+    @property
+    def offset(self):
+        return self.boxcenter - self.boxsize / 2.
+
+    @property
+    def shape(self):
+        return tuple(int(n) for n in self.nmesh)
+
+    def wrap_positions(self, positions):
+        return ((positions - self.offset) % self.boxsize) + self.offset
+
+    def _check_range(self, positions):
+        low, high = self.boxcenter - self.boxsize / 2., self.boxcenter + self.boxsize / 2.
+        if not np.all((positions >= low) & (positions <= high)):
+            raise OracleError('positions not in box range {} - {}'.format(low, high))
+
+    def mesh_units(self, positions):
+        """Transform to mesh units and cast to the mesh dtype (mesh.py:554-555)."""
+        positions = ((positions - self.boxcenter) / self.boxsize + 0.5) * self.nmesh
+        return positions.astype(self.dtype, copy=False)
+
+    def _paint(self, mesh, positions, weights):
+        size = len(positions)
+        if weights is None: weights = np.ones_like(positions, shape=size, dtype=self.dtype)
+        if self.wrap:
+            positions = self.wrap_positions(positions)
+        else:
+            self._check_range(positions)
+        if not (np.isfinite(positions).all() and np.isfinite(weights).all()):
+            raise OracleError('Some positions or weights are NaN/inf')
+        positions = self.mesh_units(positions)
+        weights = weights.astype(self.dtype, copy=False)
+        if mesh is None: mesh = np.zeros(self.shape, dtype=self.dtype)
+        self.kernels.assign_cic(mesh, positions, weights)
+        return mesh
+
+    def assign_data(self, positions, weights=None):
+        self.mesh_data = self._paint(self.mesh_data, positions, weights)
+
+    def assign_randoms(self, positions, weights=None):
+        if self.mesh_randoms is None: self._size_randoms = 0
+        self.mesh_randoms = self._paint(self.mesh_randoms, positions, weights)
+        self._size_randoms += len(positions)
+
+    @property
+    def has_randoms(self):
+        return self.mesh_randoms is not None
+
+    def smooth_gaussian(self, value, radius):
+        """Complex-to-complex FFT smoothing (hermitian=False), numpy engine."""
+        radius_ = np.empty_like(self.boxsize, order='C')
+        radius_[:] = radius
+        ctype = np.dtype('c{:d}'.format(2 * self.dtype.itemsize))
+        valuek = np.fft.fftn(value).astype(ctype, copy=False)
+        coords = []
+        for n, d in zip(self.nmesh, self.boxsize / self.nmesh):
+            coords.append(2 * np.pi * np.fft.fftfreq(n, d=d))
+        grids = []
+        for iaxis, k in enumerate(coords):
+            sl = [None] * 3
+            sl[iaxis] = slice(None)
+            grids.append(k[tuple(sl)])
+        k2 = sum(-0.5 * (r * k)**2 for r, k in zip(radius_, grids))
+        valuek *= np.exp(k2)
+        real = np.fft.ifftn(valuek).astype(ctype, copy=False).real
+        return real.astype(self.dtype, copy=False)
+
+    def set_density_contrast(self, ran_min=0.01, smoothing_radius=15.):
+        data = self.smooth_gaussian(self.mesh_data, smoothing_radius)
+        if self.has_randoms:
+            randoms = self.smooth_gaussian(self.mesh_randoms, smoothing_radius)
+            sum_data, sum_randoms = np.sum(data), np.sum(randoms)
+            alpha = sum_data * 1. / sum_randoms
+            delta = data - alpha * randoms
+            threshold = ran_min * sum_randoms / self._size_randoms
+            mask = randoms > threshold
+            delta[mask] /= (self.bias * alpha * randoms[mask])
+            delta[~mask] = 0.
+            self.alpha, self.threshold, self.mask_count = alpha, threshold, int(mask.sum())
+        else:
+            delta = data / np.mean(data) - 1.
+            delta /= self.bias
+        self.mesh_delta = delta
+        self.mesh_data = self.mesh_randoms = None
+
+    def run(self, jacobi_damping_factor=0.4, jacobi_niterations=5, vcycle_niterations=6):
+        los = None if self.los is None else self.los.astype(self.dtype, copy=False)
+        self.mesh_phi = self.kernels.fmg(self.mesh_delta, self.boxsize, self.boxcenter, self.beta,
+                                         jacobi_damping_factor, jacobi_niterations, vcycle_niterations, los=los)
+        del self.mesh_delta
+
+    def read_finite_difference_cic(self, positions):
+        dtype = positions.dtype
+        if self.wrap:
+            positions = self.wrap_positions(positions)
+        else:
+            self._check_range(positions)
+        positions = self.mesh_units(positions)
+        values = self.kernels.read_finite_difference_cic(self.mesh_phi, self.boxsize, positions)
+        return values.astype(dtype=dtype, copy=False)
+
+    def read_cic(self, mesh, positions):
+        dtype = positions.dtype
+        if self.wrap:
+            positions = self.wrap_positions(positions)
+        else:
+            self._check_range(positions)
+        positions = self.mesh_units(positions)
+        return self.kernels.read_cic(mesh, positions).astype(dtype=dtype, copy=False)
+
+    def read_shifts(self, positions, field='disp+rsd'):
+        field = field.lower()
+        if field not in ['disp', 'rsd', 'disp+rsd']:
+            raise OracleError('Unknown field {}'.format(field))
+        shifts = self.read_finite_difference_cic(positions)
+        if field == 'disp':
+            return shifts
+        if self.los is None:
+            los = positions / distance(positions)[:, None]
+        else:
+            los = self.los
+        rsd = self.f * np.sum(shifts * los, axis=-1)[:, None] * los
+        if field == 'rsd':
+            return rsd
+        shifts += rsd
+        return shifts
+
+    def read_shifted_positions(self, positions, field='disp+rsd'):
+        shifts = self.read_shifts(positions, field=field)
+        positions = positions - shifts
+        if self.wrap: positions = self.wrap_positions(positions)
+        return positions

@@ -1,0 +1,894 @@
+// Cloud-in-cell paint and reads on the device.
+//
+// Paint (replaces assign_cic, src/mesh.c:26-63, whose loop is serial).  The
+// reference adds each point's 8 corner weights to the mesh in point order, so
+// every cell receives its contributions in increasing point index.  Here,
+// per chunk of points:
+//   1. key:     base cell b(i) and fractional offsets of every point, with
+//               the Cartesian -> mesh-unit transform, the wrap or range check
+//               and the finite check of pyrecon/mesh.py:544-556 fused in;
+//               a per-cell counter gives each point an arrival rank;
+//   2. scan:    exclusive prefix of the per-cell counts (cell start offsets);
+//   3. scatter: point indices grouped by base cell;
+//   4. order:   each cell's group sorted by point index (groups are short;
+//               long groups get one workgroup each), which makes the result
+//               independent of the arrival order of step 1;
+//   5. gather:  each cell merges the 8 groups of the base cells that touch
+//               it by point index and accumulates the corner weights into its
+//               current value in that order.
+// The arithmetic per contribution is the reference's ((wx*wy)*wz)*w, so the
+// mesh is bit-identical to the reference's serial accumulation, and repeated
+// runs give identical bits.  Chunks are applied in order, which preserves
+// the point order across chunk boundaries.
+#include <type_traits>
+
+#include "prh_reduce.hpp"
+
+namespace prh {
+
+constexpr int kCicThreads = 256;
+constexpr uint32_t kErrRange = 1u, kErrNonFinite = 2u;
+constexpr uint32_t kShortSegment = 32;
+constexpr int kLongSortCapacity = 8192;  // u32 keys sorted in LDS by one workgroup
+
+template <typename A, typename B>
+using promote_t = typename std::conditional<(sizeof(A) >= sizeof(B)), A, B>::type;
+
+template <typename R>
+struct Payload {
+  R t[3];
+  R w;
+};
+
+// numpy's float remainder (npy_divmod): fmod, moved to the sign of b.
+template <typename T>
+__device__ __forceinline__ T np_mod(T a, T b) {
+  T m = fmod(a, b);
+  if (m != T(0)) {
+    if ((b < T(0)) != (m < T(0))) m += b;
+  } else {
+    m = copysign(T(0), b);
+  }
+  return m;
+}
+
+// Geometry of the Cartesian -> mesh-unit transform, in the mesh precision R.
+template <typename R>
+struct Frame {
+  R bs[3], bc[3], off[3], lo[3], hi[3];
+  int n[3];
+  double nd[3];
+};
+
+template <typename R>
+static Frame<R> make_frame(const size_t* nmesh, const R* boxsize, const R* boxcenter) {
+  Frame<R> F;
+  for (int d = 0; d < 3; d++) {
+    F.n[d] = static_cast<int>(nmesh[d]);
+    F.nd[d] = static_cast<double>(static_cast<int>(nmesh[d]));
+    F.bs[d] = boxsize[d];
+    F.bc[d] = boxcenter[d];
+    // MeshInfo.offset and the range-check bounds, in R (mesh.py:462-465, 548)
+    F.off[d] = boxcenter[d] - boxsize[d] / R(2);
+    F.lo[d] = boxcenter[d] - boxsize[d] / R(2);
+    F.hi[d] = boxcenter[d] + boxsize[d] / R(2);
+  }
+  return F;
+}
+
+// Transform of one Cartesian coordinate triple (positions dtype P) to mesh
+// units in R with numpy's promotion: Q = promote(P, R) for the box arithmetic,
+// float64 for the product with the int32 nmesh (mesh.py:554-555).  Sets
+// error bits for out-of-box (no wrap) and non-finite coordinates.
+template <typename R, typename P>
+__device__ __forceinline__ void to_mesh_units(const P x[3], const Frame<R>& F, int wrap,
+                                              R out[3], uint32_t& err) {
+  using Q = promote_t<P, R>;
+  for (int d = 0; d < 3; d++) {
+    Q p;
+    if (wrap) {
+      p = np_mod(static_cast<Q>(x[d]) - static_cast<Q>(F.off[d]), static_cast<Q>(F.bs[d])) +
+          static_cast<Q>(F.off[d]);
+    } else {
+      p = static_cast<Q>(x[d]);
+      if (!(static_cast<Q>(x[d]) >= static_cast<Q>(F.lo[d]) &&
+            static_cast<Q>(x[d]) <= static_cast<Q>(F.hi[d])))
+        err |= kErrRange;
+    }
+    if (!isfinite(p)) err |= kErrNonFinite;
+    Q a = p - static_cast<Q>(F.bc[d]);
+    a = a / static_cast<Q>(F.bs[d]);
+    a = a + static_cast<Q>(0.5);
+    out[d] = static_cast<R>(static_cast<double>(a) * F.nd[d]);
+  }
+}
+
+// ((int) p) % n with the reference's conversion of the int to size_t.
+__device__ __forceinline__ int base_index(int ip, int n) {
+  if (ip >= 0) return ip % n;
+  const unsigned long long u = static_cast<unsigned long long>(static_cast<long long>(ip));
+  return static_cast<int>(u % static_cast<unsigned long long>(n));
+}
+
+// ---------------------------------------------------------------------------
+// Paint step 1 (MODE 0: positions already in mesh units, no checks;
+// MODE 1: Cartesian positions, range check; MODE 2: Cartesian, wrapped)
+// ---------------------------------------------------------------------------
+template <typename R, typename P, typename W, int MODE>
+__global__ __launch_bounds__(kCicThreads) void k_paint_key(
+    const P* __restrict__ pos, const W* __restrict__ w, size_t m, Frame<R> F,
+    uint32_t* __restrict__ key, uint32_t* __restrict__ rank, Payload<R>* __restrict__ payload,
+    uint32_t* __restrict__ count, uint32_t* __restrict__ misc, int check_only) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  uint32_t err = 0;
+  if (i < m) {
+    const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    R p[3];
+    if (MODE == 0) {
+      for (int d = 0; d < 3; d++) p[d] = static_cast<R>(x[d]);
+    } else {
+      to_mesh_units<R, P>(x, F, MODE == 2, p, err);
+      if (w && !isfinite(w[i])) err |= kErrNonFinite;
+    }
+    if (!check_only) {
+      int b[3];
+      Payload<R> q;
+      for (int d = 0; d < 3; d++) {
+        b[d] = base_index(static_cast<int>(p[d]), F.n[d]);
+        q.t[d] = p[d] - static_cast<R>(b[d]);
+      }
+      q.w = w ? static_cast<R>(w[i]) : R(1);
+      const uint32_t k = (static_cast<uint32_t>(b[0]) * F.n[1] + b[1]) * F.n[2] + b[2];
+      key[i] = k;
+      payload[i] = q;
+      rank[i] = atomicAdd(&count[k], 1u);
+    }
+  }
+  // one atomic per wavefront with a non-zero error
+  const uint64_t bad = __ballot(err != 0);
+  if (bad) {
+    uint32_t e = err;
+    for (int off = 32; off > 0; off >>= 1) e |= __shfl_xor(e, off);
+    if ((threadIdx.x & 63) == 0) atomicOr(&misc[0], e);
+  }
+}
+
+// Paint step 2: exclusive scan of count[0, n) into start[0, n], zeroing count.
+constexpr int kScanItems = 32;
+constexpr int kScanBlock = kCicThreads * kScanItems;  // 8192 cells per block
+
+__global__ __launch_bounds__(kCicThreads) void k_scan_reduce(const uint32_t* __restrict__ count,
+                                                             size_t n,
+                                                             uint32_t* __restrict__ partial) {
+  __shared__ uint32_t lds[kCicThreads];
+  const size_t base = blockIdx.x * static_cast<size_t>(kScanBlock);
+  uint32_t acc = 0;
+  for (int k = 0; k < kScanItems; k++) {
+    const size_t i = base + k * kCicThreads + threadIdx.x;
+    if (i < n) acc += count[i];
+  }
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kCicThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds,
+                                                         uint32_t& total) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kCicThreads; off <<= 1) {
+    uint32_t t = threadIdx.x >= static_cast<unsigned>(off) ? lds[threadIdx.x - off] : 0u;
+    __syncthreads();
+    lds[threadIdx.x] += t;
+    __syncthreads();
+  }
+  total = lds[kCicThreads - 1];
+  const uint32_t incl = lds[threadIdx.x];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kCicThreads) void k_scan_partials(uint32_t* __restrict__ partial,
+                                                               int np) {
+  __shared__ uint32_t lds[kCicThreads];
+  uint32_t carry = 0;
+  for (int base = 0; base < np; base += kCicThreads) {
+    const int i = base + threadIdx.x;
+    const uint32_t v = i < np ? partial[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, lds, total);
+    if (i < np) partial[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) partial[np] = carry;
+}
+
+__global__ __launch_bounds__(kCicThreads) void k_scan_final(uint32_t* __restrict__ count,
+                                                            size_t n,
+                                                            const uint32_t* __restrict__ partial,
+                                                            uint32_t* __restrict__ start) {
+  __shared__ uint32_t lds[kCicThreads];
+  const size_t base = blockIdx.x * static_cast<size_t>(kScanBlock);
+  // each thread owns kScanItems consecutive cells
+  uint32_t c[kScanItems];
+  uint32_t acc = 0;
+  const size_t mine = base + static_cast<size_t>(threadIdx.x) * kScanItems;
+  for (int k = 0; k < kScanItems; k++) {
+    const size_t i = mine + k;
+    c[k] = i < n ? count[i] : 0u;
+    acc += c[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(acc, lds, total) + partial[blockIdx.x];
+  for (int k = 0; k < kScanItems; k++) {
+    const size_t i = mine + k;
+    if (i < n) {
+      start[i] = run;
+      count[i] = 0u;
+    }
+    run += c[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = partial[gridDim.x];
+}
+
+// Paint step 3
+__global__ __launch_bounds__(kCicThreads) void k_paint_scatter(
+    const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank, size_t m,
+    const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (i < m) sorted[start[key[i]] + rank[i]] = static_cast<uint32_t>(i);
+}
+
+// Paint step 4a: short groups sorted by one thread; long ones queued.
+__global__ __launch_bounds__(kCicThreads) void k_paint_order(const uint32_t* __restrict__ start,
+                                                             size_t ncell,
+                                                             uint32_t* __restrict__ sorted,
+                                                             uint32_t* __restrict__ misc,
+                                                             uint32_t* __restrict__ big,
+                                                             uint32_t nbig) {
+  const size_t c = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (c >= ncell) return;
+  const uint32_t s = start[c], e = start[c + 1];
+  const uint32_t len = e - s;
+  if (len < 2) return;
+  if (len > kShortSegment) {
+    const uint32_t slot = atomicAdd(&misc[1], 1u);
+    if (slot < nbig) big[slot] = static_cast<uint32_t>(c);
+    return;
+  }
+  for (uint32_t a = s + 1; a < e; a++) {
+    const uint32_t v = sorted[a];
+    uint32_t b = a;
+    while (b > s && sorted[b - 1] > v) {
+      sorted[b] = sorted[b - 1];
+      b--;
+    }
+    sorted[b] = v;
+  }
+}
+
+// Paint step 4b: one workgroup per long group.  Groups up to
+// kLongSortCapacity are bitonic-sorted in LDS; longer ones are placed by
+// rank (count of smaller indices; indices are distinct) through `scratch`.
+__global__ __launch_bounds__(kCicThreads) void k_paint_order_long(
+    const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted,
+    const uint32_t* __restrict__ misc, const uint32_t* __restrict__ big, uint32_t nbig,
+    uint32_t* __restrict__ scratch) {
+  __shared__ uint32_t lds[kLongSortCapacity];
+  const uint32_t nlong = misc[1] < nbig ? misc[1] : nbig;
+  for (uint32_t q = blockIdx.x; q < nlong; q += gridDim.x) {
+    const uint32_t c = big[q];
+    const uint32_t s = start[c], len = start[c + 1] - s;
+    if (len <= static_cast<uint32_t>(kLongSortCapacity)) {
+      uint32_t p2 = 1;
+      while (p2 < len) p2 <<= 1;
+      for (uint32_t i = threadIdx.x; i < p2; i += kCicThreads)
+        lds[i] = i < len ? sorted[s + i] : 0xffffffffu;
+      __syncthreads();
+      for (uint32_t k = 2; k <= p2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          for (uint32_t i = threadIdx.x; i < p2; i += kCicThreads) {
+            const uint32_t l = i ^ j;
+            if (l > i) {
+              const bool up = (i & k) == 0;
+              const uint32_t a = lds[i], b = lds[l];
+              if ((a > b) == up) {
+                lds[i] = b;
+                lds[l] = a;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (uint32_t i = threadIdx.x; i < len; i += kCicThreads) sorted[s + i] = lds[i];
+      __syncthreads();
+    } else {
+      for (uint32_t i = threadIdx.x; i < len; i += kCicThreads) {
+        const uint32_t v = sorted[s + i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < len; j++) r += sorted[s + j] < v;
+        scratch[s + r] = v;
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < len; i += kCicThreads) sorted[s + i] = scratch[s + i];
+      __syncthreads();
+    }
+  }
+}
+
+// Paint step 5: per target cell, merge the groups of its 8 source base cells
+// by point index and accumulate.  Source k = (a, b, c) is the base cell at
+// (x - a, y - b, z - c): the target is that point's corner (a, b, c).
+template <typename R>
+__global__ __launch_bounds__(kCicThreads) void k_paint_gather(
+    R* __restrict__ mesh, int n0, int n1, int n2, const uint32_t* __restrict__ start,
+    const uint32_t* __restrict__ sorted, const Payload<R>* __restrict__ payload,
+    const uint32_t* __restrict__ misc) {
+  if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
+  const size_t total = static_cast<size_t>(n0) * n1 * n2;
+  const size_t c = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (c >= total) return;
+  const int z = static_cast<int>(c % n2);
+  const size_t t = c / n2;
+  const int y = static_cast<int>(t % n1), x = static_cast<int>(t / n1);
+  const int xm = x == 0 ? n0 - 1 : x - 1, ym = y == 0 ? n1 - 1 : y - 1,
+            zm = z == 0 ? n2 - 1 : z - 1;
+  uint32_t beg[8], end[8], cur[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int sx = (k & 4) ? xm : x, sy = (k & 2) ? ym : y, sz = (k & 1) ? zm : z;
+    const size_t sc = (static_cast<size_t>(sx) * n1 + sy) * n2 + sz;
+    beg[k] = start[sc];
+    end[k] = start[sc + 1];
+    cur[k] = beg[k] < end[k] ? sorted[beg[k]] : 0xffffffffu;
+  }
+  bool any = false;
+  R acc = R(0);
+  for (;;) {
+    uint32_t best = cur[0];
+    int bk = 0;
+#pragma unroll
+    for (int k = 1; k < 8; k++)
+      if (cur[k] < best) {
+        best = cur[k];
+        bk = k;
+      }
+    if (best == 0xffffffffu) break;
+    if (!any) {
+      acc = mesh[c];
+      any = true;
+    }
+    const Payload<R> q = payload[best];
+    const R fx = (bk & 4) ? q.t[0] : R(1) - q.t[0];
+    const R fy = (bk & 2) ? q.t[1] : R(1) - q.t[1];
+    const R fz = (bk & 1) ? q.t[2] : R(1) - q.t[2];
+    acc += fx * fy * fz * q.w;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k == bk) {
+        beg[k]++;
+        cur[k] = beg[k] < end[k] ? sorted[beg[k]] : 0xffffffffu;
+      }
+  }
+  if (any) mesh[c] = acc;
+}
+
+__global__ void k_zero_misc(uint32_t* misc) {
+  if (threadIdx.x < 4) misc[threadIdx.x] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Reads
+// ---------------------------------------------------------------------------
+
+// src/mesh.c:277-306
+template <typename R>
+__device__ __forceinline__ R cic_value(const R* __restrict__ mesh, int n0, int n1, int n2,
+                                       const R p[3]) {
+  R t[3];
+  int b[3];
+  const int nn[3] = {n0, n1, n2};
+  for (int d = 0; d < 3; d++) {
+    const int ip = static_cast<int>(p[d]);
+    t[d] = p[d] - static_cast<R>(ip);
+    b[d] = base_index(ip, nn[d]);
+  }
+  const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
+  const size_t X[2] = {b[0] * s0, (b[0] + 1 == n0 ? 0 : b[0] + 1) * s0};
+  const size_t Y[2] = {b[1] * s1, (b[1] + 1 == n1 ? 0 : b[1] + 1) * s1};
+  const size_t Z[2] = {static_cast<size_t>(b[2]), static_cast<size_t>(b[2] + 1 == n2 ? 0 : b[2] + 1)};
+  R acc = R(0);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int a = (k >> 2) & 1, bb = (k >> 1) & 1, e = k & 1;
+    const R wx = a ? t[0] : R(1) - t[0];
+    const R wy = bb ? t[1] : R(1) - t[1];
+    const R wz = e ? t[2] : R(1) - t[2];
+    const R term = mesh[X[a] + Y[bb] + Z[e]] * wx * wy * wz;
+    acc = (k == 0) ? term : acc + term;
+  }
+  return acc;
+}
+
+// src/mesh.c:205-262: central differences at the 8 corners, corner order
+// 000, 100, 010, 001, 110, 101, 011, 111; divided by cell = 2 L / n.
+template <typename R>
+__device__ __forceinline__ void fd_gradient(const R* __restrict__ mesh, int n0, int n1, int n2,
+                                            const R cell[3], const R p[3], R g[3]) {
+  R t[3];
+  int b[3];
+  const int nn[3] = {n0, n1, n2};
+  for (int d = 0; d < 3; d++) {
+    const int ip = static_cast<int>(p[d]);
+    t[d] = p[d] - static_cast<R>(ip);
+    b[d] = base_index(ip, nn[d]);
+  }
+  const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
+  size_t X[4], Y[4], Z[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    X[k] = static_cast<size_t>((b[0] + n0 - 1 + k) % n0) * s0;
+    Y[k] = static_cast<size_t>((b[1] + n1 - 1 + k) % n1) * s1;
+    Z[k] = static_cast<size_t>((b[2] + n2 - 1 + k) % n2);
+  }
+  constexpr int corner[8] = {0, 4, 2, 1, 6, 5, 3, 7};  // bit 2: x, bit 1: y, bit 0: z
+  R gx = R(0), gy = R(0), gz = R(0);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int a = (corner[q] >> 2) & 1, bb = (corner[q] >> 1) & 1, e = corner[q] & 1;
+    const R wx = a ? t[0] : R(1) - t[0];
+    const R wy = bb ? t[1] : R(1) - t[1];
+    const R wz = e ? t[2] : R(1) - t[2];
+    const R wt = wx * wy * wz;
+    const size_t cx = X[1 + a], cy = Y[1 + bb], cz = Z[1 + e];
+    const R dx = (mesh[X[2 + a] + cy + cz] - mesh[X[a] + cy + cz]) * wt;
+    const R dy = (mesh[cx + Y[2 + bb] + cz] - mesh[cx + Y[bb] + cz]) * wt;
+    const R dz = (mesh[cx + cy + Z[2 + e]] - mesh[cx + cy + Z[e]]) * wt;
+    if (q == 0) {
+      gx = dx;
+      gy = dy;
+      gz = dz;
+    } else {
+      gx += dx;
+      gy += dy;
+      gz += dz;
+    }
+  }
+  g[0] = gx / cell[0];
+  g[1] = gy / cell[1];
+  g[2] = gz / cell[2];
+}
+
+template <typename R>
+__global__ __launch_bounds__(kCicThreads) void k_read_cic(const R* __restrict__ mesh, int n0,
+                                                          int n1, int n2,
+                                                          const R* __restrict__ pos,
+                                                          R* __restrict__ out, size_t m) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (i >= m) return;
+  const R p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  out[i] = cic_value(mesh, n0, n1, n2, p);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kCicThreads) void k_read_fd(const R* __restrict__ mesh, int n0,
+                                                         int n1, int n2, R c0, R c1, R c2,
+                                                         const R* __restrict__ pos,
+                                                         R* __restrict__ out, size_t m) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (i >= m) return;
+  const R p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  const R cell[3] = {c0, c1, c2};
+  R g[3];
+  fd_gradient(mesh, n0, n1, n2, cell, p, g);
+  out[3 * i] = g[0];
+  out[3 * i + 1] = g[1];
+  out[3 * i + 2] = g[2];
+}
+
+// Fused read of the shifts and shifted positions (multigrid.py:137-158,
+// recon.py:304-325), with numpy's dtypes: the gradient is cast to the
+// positions dtype P; the local line of sight x/|x| is in P, computed from
+// the unwrapped input; with a global line of sight (mesh dtype R) the RSD term
+// is in Q = promote(P, R).  O is the output dtype.
+template <typename R, typename P, typename O>
+__global__ __launch_bounds__(kCicThreads) void k_read_shifts(
+    const R* __restrict__ phi, Frame<R> F, R c0, R c1, R c2, const P* __restrict__ pos, size_t m,
+    double fgrowth, R l0, R l1, R l2, int local, int field, int wrap, int shifted,
+    O* __restrict__ out, uint32_t* __restrict__ misc) {
+  using Q = promote_t<P, R>;
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  uint32_t err = 0;
+  if (i < m) {
+    const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    R p[3];
+    to_mesh_units<R, P>(x, F, wrap, p, err);
+    const R cell[3] = {c0, c1, c2};
+    R g[3];
+    fd_gradient(phi, F.n[0], F.n[1], F.n[2], cell, p, g);
+    P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
+    // shifts in P (disp, disp+rsd, local rsd) or in Q (global rsd)
+    O res[3];
+    if (field == PRH_FIELD_DISP) {
+      for (int d = 0; d < 3; d++) res[d] = static_cast<O>(s[d]);
+      if (shifted)
+        for (int d = 0; d < 3; d++) res[d] = static_cast<O>(x[d] - s[d]);
+    } else if (local) {
+      const P dist = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+      const P l[3] = {x[0] / dist, x[1] / dist, x[2] / dist};
+      const P dot = s[0] * l[0] + s[1] * l[1] + s[2] * l[2];
+      const P fd = static_cast<P>(fgrowth) * dot;
+      P sh[3];
+      for (int d = 0; d < 3; d++) {
+        const P rsd = fd * l[d];
+        sh[d] = (field == PRH_FIELD_RSD) ? rsd : static_cast<P>(s[d] + rsd);
+      }
+      for (int d = 0; d < 3; d++)
+        res[d] = static_cast<O>(shifted ? static_cast<P>(x[d] - sh[d]) : sh[d]);
+    } else {
+      const Q l[3] = {static_cast<Q>(l0), static_cast<Q>(l1), static_cast<Q>(l2)};
+      const Q dot = static_cast<Q>(s[0]) * l[0] + static_cast<Q>(s[1]) * l[1] +
+                    static_cast<Q>(s[2]) * l[2];
+      const Q fd = static_cast<Q>(fgrowth) * dot;
+      for (int d = 0; d < 3; d++) {
+        const Q rsd = fd * l[d];
+        if (field == PRH_FIELD_RSD) {
+          res[d] = static_cast<O>(shifted ? static_cast<Q>(static_cast<Q>(x[d]) - rsd) : rsd);
+        } else {
+          const P sh = static_cast<P>(static_cast<Q>(s[d]) + rsd);
+          res[d] = static_cast<O>(shifted ? static_cast<P>(x[d] - sh) : sh);
+        }
+      }
+    }
+    if (shifted && wrap) {
+      using QO = promote_t<O, R>;
+      for (int d = 0; d < 3; d++)
+        res[d] = static_cast<O>(np_mod(static_cast<QO>(res[d]) - static_cast<QO>(F.off[d]),
+                                       static_cast<QO>(F.bs[d])) +
+                                static_cast<QO>(F.off[d]));
+    }
+    out[3 * i] = res[0];
+    out[3 * i + 1] = res[1];
+    out[3 * i + 2] = res[2];
+  }
+  const uint64_t bad = __ballot(err != 0);
+  if (bad) {
+    uint32_t e = err;
+    for (int off = 32; off > 0; off >>= 1) e |= __shfl_xor(e, off);
+    if ((threadIdx.x & 63) == 0) atomicOr(&misc[0], e);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers
+// ---------------------------------------------------------------------------
+
+template <typename R, typename P, typename W, int MODE>
+static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
+                       size_t m, bool check_only, hipStream_t s) {
+  PaintScratch& S = ws->paint;
+  const unsigned grid = ceil_div(m, kCicThreads);
+  k_paint_key<R, P, W, MODE><<<grid, kCicThreads, 0, s>>>(
+      pos, w, m, F, S.key, S.rank, static_cast<Payload<R>*>(S.payload), S.count, S.misc,
+      check_only ? 1 : 0);
+  PRH_LAUNCHED();
+  if (check_only) return PRH_OK;
+  const size_t ncell = S.ncell;
+  const unsigned nscan = ceil_div(ncell, kScanBlock);
+  k_scan_reduce<<<nscan, kCicThreads, 0, s>>>(S.count, ncell, S.partial);
+  PRH_LAUNCHED();
+  k_scan_partials<<<1, kCicThreads, 0, s>>>(S.partial, static_cast<int>(nscan));
+  PRH_LAUNCHED();
+  k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.count, ncell, S.partial, S.start);
+  PRH_LAUNCHED();
+  k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
+  PRH_LAUNCHED();
+  const unsigned gcell = ceil_div(ncell, kCicThreads);
+  k_paint_order<<<gcell, kCicThreads, 0, s>>>(S.start, ncell, S.sorted, S.misc, S.big,
+                                              static_cast<uint32_t>(S.nbig));
+  PRH_LAUNCHED();
+  k_paint_order_long<<<256, kCicThreads, 0, s>>>(S.start, S.sorted, S.misc, S.big,
+                                                 static_cast<uint32_t>(S.nbig), S.rank);
+  PRH_LAUNCHED();
+  k_paint_gather<R><<<gcell, kCicThreads, 0, s>>>(mesh, F.n[0], F.n[1], F.n[2], S.start,
+                                                  S.sorted,
+                                                  static_cast<const Payload<R>*>(S.payload),
+                                                  S.misc);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+static int read_flags(prh_workspace* ws, hipStream_t s, uint32_t* flags) {
+  PRH_HIP(hipMemcpyAsync(ws->flag_host, ws->paint.misc, 2 * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, s));
+  PRH_HIP(hipStreamSynchronize(s));
+  *flags = ws->flag_host[0];
+  return PRH_OK;
+}
+
+static int flags_to_status(uint32_t flags) {
+  if (flags & kErrRange) {
+    set_error("positions not in box range");
+    return PRH_ERR_RANGE;
+  }
+  if (flags & kErrNonFinite) {
+    set_error("Some positions or weights are NaN/inf");
+    return PRH_ERR_NONFINITE;
+  }
+  return PRH_OK;
+}
+
+template <typename R, typename P, typename W, int MODE>
+static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
+                     size_t n, hipStream_t s) {
+  PaintScratch& S = ws->paint;
+  k_zero_misc<<<1, 64, 0, s>>>(S.misc);
+  PRH_LAUNCHED();
+  if (n == 0) return PRH_OK;
+  const bool multi = n > S.chunk;
+  if (multi && MODE != 0) {
+    // validate every point before the first chunk touches the mesh
+    for (size_t o = 0; o < n; o += S.chunk) {
+      const size_t m = n - o < S.chunk ? n - o : S.chunk;
+      PRH_TRY((paint_chunk<R, P, W, MODE>(ws, mesh, F, pos + 3 * o, w ? w + o : nullptr, m, true, s)));
+    }
+    uint32_t flags;
+    PRH_TRY(read_flags(ws, s, &flags));
+    PRH_TRY(flags_to_status(flags));
+  }
+  for (size_t o = 0; o < n; o += S.chunk) {
+    const size_t m = n - o < S.chunk ? n - o : S.chunk;
+    if (o > 0) {
+      k_zero_misc<<<1, 64, 0, s>>>(S.misc);
+      PRH_LAUNCHED();
+    }
+    PRH_TRY((paint_chunk<R, P, W, MODE>(ws, mesh, F, pos + 3 * o, w ? w + o : nullptr, m, false, s)));
+  }
+  if (MODE == 0 || multi) return PRH_OK;
+  uint32_t flags;
+  PRH_TRY(read_flags(ws, s, &flags));
+  return flags_to_status(flags);
+}
+
+static int check_paint_ws(const prh_workspace* ws, const size_t* nmesh, int dtype) {
+  if (!ws || !nmesh) {
+    set_error("workspace and nmesh are required");
+    return PRH_ERR_ARG;
+  }
+  if (!(ws->flags & PRH_WS_PAINT)) {
+    set_error("workspace has no paint scratch");
+    return PRH_ERR_STATE;
+  }
+  if (ws->dtype != dtype) {
+    set_error("workspace precision does not match the mesh");
+    return PRH_ERR_ARG;
+  }
+  for (int d = 0; d < 3; d++)
+    if (ws->n[d] != nmesh[d]) {
+      set_error("workspace mesh shape does not match nmesh");
+      return PRH_ERR_CAPACITY;
+    }
+  return PRH_OK;
+}
+
+template <typename R>
+static int do_assign_cic(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* pos,
+                         const R* w, size_t n, void* stream) {
+  PRH_TRY(check_paint_ws(ws, nmesh, Tag<R>::code));
+  if (n && (!mesh || !pos || !w)) return PRH_ERR_ARG;
+  const R one[3] = {R(1), R(1), R(1)}, zero[3] = {R(0), R(0), R(0)};
+  const Frame<R> F = make_frame<R>(nmesh, one, zero);
+  return paint_all<R, R, R, 0>(ws, mesh, F, pos, w, n, as_stream(stream));
+}
+
+template <typename R, typename P, typename W>
+static int paint_dispatch_w(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos,
+                            const W* w, size_t n, int wrap, hipStream_t s) {
+  if (wrap) return paint_all<R, P, W, 2>(ws, mesh, F, pos, w, n, s);
+  return paint_all<R, P, W, 1>(ws, mesh, F, pos, w, n, s);
+}
+
+template <typename R, typename P>
+static int paint_dispatch_p(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos,
+                            const void* w, int wdt, size_t n, int wrap, hipStream_t s) {
+  if (!w || wdt == PRH_F64)
+    return paint_dispatch_w<R, P, double>(ws, mesh, F, pos, static_cast<const double*>(w), n,
+                                          wrap, s);
+  if (wdt == PRH_F32)
+    return paint_dispatch_w<R, P, float>(ws, mesh, F, pos, static_cast<const float*>(w), n, wrap,
+                                         s);
+  set_error("weight dtype must be PRH_F32 or PRH_F64");
+  return PRH_ERR_ARG;
+}
+
+template <typename R>
+static int do_paint(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* boxsize,
+                    const R* boxcenter, const void* pos, int pdt, const void* w, int wdt, size_t n,
+                    int wrap, void* stream) {
+  PRH_TRY(check_paint_ws(ws, nmesh, Tag<R>::code));
+  if (!boxsize || !boxcenter || (n && (!mesh || !pos))) return PRH_ERR_ARG;
+  const Frame<R> F = make_frame<R>(nmesh, boxsize, boxcenter);
+  hipStream_t s = as_stream(stream);
+  if (pdt == PRH_F64)
+    return paint_dispatch_p<R, double>(ws, mesh, F, static_cast<const double*>(pos), w, wdt, n,
+                                       wrap, s);
+  if (pdt == PRH_F32)
+    return paint_dispatch_p<R, float>(ws, mesh, F, static_cast<const float*>(pos), w, wdt, n,
+                                      wrap, s);
+  set_error("positions dtype must be PRH_F32 or PRH_F64");
+  return PRH_ERR_ARG;
+}
+
+template <typename R>
+static int do_read_cic(const R* mesh, const size_t* nmesh, const R* pos, R* out, size_t n,
+                       void* stream) {
+  if (!nmesh || (n && (!mesh || !pos || !out))) return PRH_ERR_ARG;
+  if (n == 0) return PRH_OK;
+  k_read_cic<R><<<ceil_div(n, kCicThreads), kCicThreads, 0, as_stream(stream)>>>(
+      mesh, static_cast<int>(nmesh[0]), static_cast<int>(nmesh[1]), static_cast<int>(nmesh[2]),
+      pos, out, n);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+// cell = 2.0 * boxsize / nmesh evaluated in double and stored in R (src/mesh.c:197)
+template <typename R>
+static void fd_cells(const size_t* nmesh, const R* boxsize, R cell[3]) {
+  for (int d = 0; d < 3; d++)
+    cell[d] = static_cast<R>(2.0 * static_cast<double>(boxsize[d]) / static_cast<double>(nmesh[d]));
+}
+
+template <typename R>
+static int do_read_fd(const R* mesh, const size_t* nmesh, const R* boxsize, const R* pos, R* out,
+                      size_t n, void* stream) {
+  if (!nmesh || !boxsize || (n && (!mesh || !pos || !out))) return PRH_ERR_ARG;
+  if (n == 0) return PRH_OK;
+  R cell[3];
+  fd_cells(nmesh, boxsize, cell);
+  k_read_fd<R><<<ceil_div(n, kCicThreads), kCicThreads, 0, as_stream(stream)>>>(
+      mesh, static_cast<int>(nmesh[0]), static_cast<int>(nmesh[1]), static_cast<int>(nmesh[2]),
+      cell[0], cell[1], cell[2], pos, out, n);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+template <typename R, typename P, typename O>
+static int launch_read_shifts(const R* phi, const Frame<R>& F, const R cell[3], const P* pos,
+                              size_t n, double f, const R* los, int field, int wrap, int shifted,
+                              void* out, uint32_t* misc, hipStream_t s) {
+  const R l0 = los ? los[0] : R(0), l1 = los ? los[1] : R(0), l2 = los ? los[2] : R(0);
+  k_read_shifts<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(
+      phi, F, cell[0], cell[1], cell[2], pos, n, f, l0, l1, l2, los ? 0 : 1, field, wrap,
+      shifted, static_cast<O*>(out), misc);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+template <typename R>
+static int do_read_shifts(prh_workspace* ws, const R* phi, const size_t* nmesh, const R* boxsize, const R* boxcenter,
+                          const void* pos, int pdt, size_t n, double f, const R* los, int field,
+                          int wrap, int shifted, void* out, int odt, int* nonfinite,
+                          void* stream) {
+  if (!ws || !nmesh || !boxsize || !boxcenter || (n && (!phi || !pos || !out))) return PRH_ERR_ARG;
+  if (field != PRH_FIELD_DISP && field != PRH_FIELD_RSD && field != PRH_FIELD_DISP_RSD) {
+    set_error("unknown field %d", field);
+    return PRH_ERR_ARG;
+  }
+  if ((pdt != PRH_F32 && pdt != PRH_F64) || (odt != PRH_F32 && odt != PRH_F64)) {
+    set_error("positions/output dtype must be PRH_F32 or PRH_F64");
+    return PRH_ERR_ARG;
+  }
+  // expected output dtype under numpy's promotion rules
+  const int q = pdt > Tag<R>::code ? pdt : Tag<R>::code;
+  int expect = pdt;
+  if (field == PRH_FIELD_RSD && los) expect = q;
+  if (shifted && wrap) expect = expect > Tag<R>::code ? expect : Tag<R>::code;
+  if (odt != expect) {
+    set_error("output dtype %d does not follow numpy promotion (expected %d)", odt, expect);
+    return PRH_ERR_ARG;
+  }
+  if (nonfinite) *nonfinite = 0;
+  if (n == 0) return PRH_OK;
+  uint32_t* misc = ws->misc;
+  hipStream_t s = as_stream(stream);
+  k_zero_misc<<<1, 64, 0, s>>>(misc);
+  PRH_LAUNCHED();
+  const Frame<R> F = make_frame<R>(nmesh, boxsize, boxcenter);
+  R cell[3];
+  fd_cells(nmesh, boxsize, cell);
+  int st;
+  if (pdt == PRH_F64) {
+    st = launch_read_shifts<R, double, double>(phi, F, cell, static_cast<const double*>(pos), n, f,
+                                               los, field, wrap, shifted, out, misc, s);
+  } else if (odt == PRH_F32) {
+    st = launch_read_shifts<R, float, float>(phi, F, cell, static_cast<const float*>(pos), n, f,
+                                             los, field, wrap, shifted, out, misc, s);
+  } else {
+    st = launch_read_shifts<R, float, double>(phi, F, cell, static_cast<const float*>(pos), n, f,
+                                              los, field, wrap, shifted, out, misc, s);
+  }
+  PRH_TRY(st);
+  uint32_t* host = ws->flag_host;
+  PRH_HIP(hipMemcpyAsync(host, misc, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  PRH_HIP(hipStreamSynchronize(s));
+  if (nonfinite && (host[0] & kErrNonFinite)) *nonfinite = 1;
+  if (!wrap && (host[0] & kErrRange)) {
+    set_error("positions not in box range");
+    return PRH_ERR_RANGE;
+  }
+  return PRH_OK;
+}
+
+}  // namespace prh
+
+using namespace prh;
+
+extern "C" {
+
+int prh_assign_cic_f32(prh_workspace* ws, float* mesh, const size_t* nmesh, const float* positions,
+                       const float* weights, size_t npositions, void* stream) {
+  return do_assign_cic<float>(ws, mesh, nmesh, positions, weights, npositions, stream);
+}
+int prh_assign_cic_f64(prh_workspace* ws, double* mesh, const size_t* nmesh,
+                       const double* positions, const double* weights, size_t npositions,
+                       void* stream) {
+  return do_assign_cic<double>(ws, mesh, nmesh, positions, weights, npositions, stream);
+}
+
+int prh_read_cic_f32(const float* mesh, const size_t* nmesh, const float* positions, float* values,
+                     size_t npositions, void* stream) {
+  return do_read_cic<float>(mesh, nmesh, positions, values, npositions, stream);
+}
+int prh_read_cic_f64(const double* mesh, const size_t* nmesh, const double* positions,
+                     double* values, size_t npositions, void* stream) {
+  return do_read_cic<double>(mesh, nmesh, positions, values, npositions, stream);
+}
+
+int prh_read_finite_difference_cic_f32(const float* mesh, const size_t* nmesh,
+                                       const float* boxsize, const float* positions,
+                                       float* shifts, size_t npositions, void* stream) {
+  return do_read_fd<float>(mesh, nmesh, boxsize, positions, shifts, npositions, stream);
+}
+int prh_read_finite_difference_cic_f64(const double* mesh, const size_t* nmesh,
+                                       const double* boxsize, const double* positions,
+                                       double* shifts, size_t npositions, void* stream) {
+  return do_read_fd<double>(mesh, nmesh, boxsize, positions, shifts, npositions, stream);
+}
+
+int prh_paint_f32(prh_workspace* ws, float* mesh, const size_t* nmesh, const float* boxsize,
+                  const float* boxcenter, const void* positions, int pos_dtype,
+                  const void* weights, int weight_dtype, size_t npositions, int wrap,
+                  void* stream) {
+  return do_paint<float>(ws, mesh, nmesh, boxsize, boxcenter, positions, pos_dtype, weights,
+                         weight_dtype, npositions, wrap, stream);
+}
+int prh_paint_f64(prh_workspace* ws, double* mesh, const size_t* nmesh, const double* boxsize,
+                  const double* boxcenter, const void* positions, int pos_dtype,
+                  const void* weights, int weight_dtype, size_t npositions, int wrap,
+                  void* stream) {
+  return do_paint<double>(ws, mesh, nmesh, boxsize, boxcenter, positions, pos_dtype, weights,
+                          weight_dtype, npositions, wrap, stream);
+}
+
+int prh_read_shifts_f32(prh_workspace* ws, const float* phi, const size_t* nmesh, const float* boxsize,
+                        const float* boxcenter, const void* positions, int pos_dtype,
+                        size_t npositions, double f, const float* los, int field, int wrap,
+                        int shifted, void* out, int out_dtype, int* nonfinite, void* stream) {
+  return do_read_shifts<float>(ws, phi, nmesh, boxsize, boxcenter, positions, pos_dtype, npositions,
+                               f, los, field, wrap, shifted, out, out_dtype, nonfinite, stream);
+}
+int prh_read_shifts_f64(prh_workspace* ws, const double* phi, const size_t* nmesh, const double* boxsize,
+                        const double* boxcenter, const void* positions, int pos_dtype,
+                        size_t npositions, double f, const double* los, int field, int wrap,
+                        int shifted, void* out, int out_dtype, int* nonfinite, void* stream) {
+  return do_read_shifts<double>(ws, phi, nmesh, boxsize, boxcenter, positions, pos_dtype, npositions,
+                                f, los, field, wrap, shifted, out, out_dtype, nonfinite, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,399 @@
+// Runtime plumbing of the C ABI: errors, device memory, streams, events,
+// and the workspace (allocation of every buffer the hot calls use).
+#include <cstdarg>
+#include <cstdlib>
+
+#include "prh_workspace.hpp"
+
+namespace prh {
+
+static thread_local char g_error[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_error, sizeof(g_error), fmt, ap);
+  va_end(ap);
+}
+
+const char* get_error() { return g_error; }
+
+int ws_alloc(prh_workspace* ws, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return PRH_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    set_error("hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return PRH_ERR_NOMEM;
+  }
+  ws->bytes += bytes;
+  return PRH_OK;
+}
+
+// Level shapes of the hierarchy: halve while every dimension is even and > 4
+// (src/multigrid.c:274-275, 308-309).
+static int level_shapes(const size_t* n, size_t shapes[kMaxLevels][3]) {
+  size_t m[3] = {n[0], n[1], n[2]};
+  int l = 0;
+  for (;;) {
+    for (int d = 0; d < 3; d++) shapes[l][d] = m[d];
+    l++;
+    if (!coarsenable(m) || l >= kMaxLevels) break;
+    for (int d = 0; d < 3; d++) m[d] /= 2;
+  }
+  return l;
+}
+
+struct Plan {
+  size_t mg = 0, mixed = 0, paint = 0, fft = 0, red = 0;
+};
+
+static size_t cells(const size_t* n) { return n[0] * n[1] * n[2]; }
+
+static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags) {
+  Plan p;
+  size_t shapes[kMaxLevels][3];
+  int nl = level_shapes(n, shapes);
+  const size_t w = static_cast<size_t>(dtype);
+  if (flags & PRH_WS_MULTIGRID) {
+    for (int l = 0; l < nl; l++) p.mg += cells(shapes[l]) * w * (l == 0 ? 1 : 3);
+  }
+  if ((flags & PRH_WS_MIXED) && dtype == PRH_F64) {
+    for (int l = 0; l < nl; l++) p.mixed += cells(shapes[l]) * sizeof(float) * 3;
+  }
+  if (flags & PRH_WS_PAINT) {
+    const size_t nc = cells(n);
+    const size_t npart = (nc + 8191) / 8192;
+    p.paint = nc * 4 + (nc + 1) * 4 + (npart + 1) * 4 + chunk * 4 * 3 + chunk * 4 * w + 4 * 4 +
+              (chunk / 32 + 2) * 4;
+  }
+  if (flags & PRH_WS_FFT) {
+    const size_t spec = n[0] * n[1] * (n[2] / 2 + 1) * 2 * w;
+    p.fft = spec + spec;  // spectrum + work area (upper bound used for admission)
+  }
+  p.red = (kReduceBlocks * 2 + 16) * sizeof(double) + 16 * sizeof(uint32_t);
+  return p;
+}
+
+static int fft_setup(prh_workspace* ws) {
+  FFTState& F = ws->fft;
+  const int nx = static_cast<int>(ws->n[0]), ny = static_cast<int>(ws->n[1]),
+            nz = static_cast<int>(ws->n[2]);
+  const bool dbl = ws->dtype == PRH_F64;
+  const size_t spec = ws->n[0] * ws->n[1] * (ws->n[2] / 2 + 1) * 2 * static_cast<size_t>(ws->dtype);
+  PRH_TRY(ws_alloc(ws, &F.spectrum, spec));
+  size_t w1 = 0, w2 = 0;
+  if (hipfftCreate(&F.r2c) != HIPFFT_SUCCESS || hipfftCreate(&F.c2r) != HIPFFT_SUCCESS) {
+    set_error("hipfftCreate failed");
+    return PRH_ERR_FFT;
+  }
+  F.have = true;
+  hipfftSetAutoAllocation(F.r2c, 0);
+  hipfftSetAutoAllocation(F.c2r, 0);
+  if (hipfftMakePlan3d(F.r2c, nx, ny, nz, dbl ? HIPFFT_D2Z : HIPFFT_R2C, &w1) != HIPFFT_SUCCESS ||
+      hipfftMakePlan3d(F.c2r, nx, ny, nz, dbl ? HIPFFT_Z2D : HIPFFT_C2R, &w2) != HIPFFT_SUCCESS) {
+    set_error("hipfftMakePlan3d(%d, %d, %d) failed", nx, ny, nz);
+    return PRH_ERR_FFT;
+  }
+  F.work_bytes = w1 > w2 ? w1 : w2;
+  if (F.work_bytes) {
+    PRH_TRY(ws_alloc(ws, &F.work, F.work_bytes));
+    if (hipfftSetWorkArea(F.r2c, F.work) != HIPFFT_SUCCESS ||
+        hipfftSetWorkArea(F.c2r, F.work) != HIPFFT_SUCCESS) {
+      set_error("hipfftSetWorkArea failed");
+      return PRH_ERR_FFT;
+    }
+  }
+  return PRH_OK;
+}
+
+static int hierarchy_alloc(prh_workspace* ws, Hierarchy& H, size_t elem, bool full_top) {
+  H.nlev = level_shapes(ws->n, H.n);
+  for (int l = 0; l < H.nlev; l++) {
+    const size_t b = cells(H.n[l]) * elem;
+    PRH_TRY(ws_alloc(ws, &H.tmp[l], b));
+    if (l > 0 || full_top) {
+      PRH_TRY(ws_alloc(ws, &H.v[l], b));
+      PRH_TRY(ws_alloc(ws, &H.f[l], b));
+    }
+  }
+  return PRH_OK;
+}
+
+static int paint_alloc(prh_workspace* ws, size_t chunk) {
+  PaintScratch& P = ws->paint;
+  P.chunk = chunk;
+  P.ncell = cells(ws->n);
+  P.npartial = (P.ncell + 8191) / 8192;
+  P.nbig = chunk / 32 + 2;
+  void* p;
+  PRH_TRY(ws_alloc(ws, &p, P.ncell * 4)); P.count = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, (P.ncell + 1) * 4)); P.start = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, (P.npartial + 1) * 4)); P.partial = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.key = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.rank = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.sorted = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &P.payload, chunk * 4 * static_cast<size_t>(ws->dtype)));
+  PRH_TRY(ws_alloc(ws, &p, 4 * 4)); P.misc = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, P.nbig * 4)); P.big = static_cast<uint32_t*>(p);
+  PRH_HIP(hipMemset(P.count, 0, P.ncell * 4));
+  PRH_HIP(hipMemset(P.misc, 0, 16));
+  return PRH_OK;
+}
+
+}  // namespace prh
+
+using namespace prh;
+
+extern "C" {
+
+const char* prh_last_error(void) { return get_error(); }
+
+int prh_version(void) { return 10000; }
+
+int prh_device_count(int* count) {
+  if (!count) return PRH_ERR_ARG;
+  PRH_HIP(hipGetDeviceCount(count));
+  return PRH_OK;
+}
+
+int prh_set_device(int device) {
+  PRH_HIP(hipSetDevice(device));
+  return PRH_OK;
+}
+
+int prh_get_device_name(char* buffer, size_t size) {
+  if (!buffer || size == 0) return PRH_ERR_ARG;
+  int dev = 0;
+  PRH_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  PRH_HIP(hipGetDeviceProperties(&prop, dev));
+  snprintf(buffer, size, "%s|%s|%d", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return PRH_OK;
+}
+
+int prh_synchronize(void* stream) {
+  PRH_HIP(hipStreamSynchronize(as_stream(stream)));
+  return PRH_OK;
+}
+
+int prh_stream_create(void** stream) {
+  if (!stream) return PRH_ERR_ARG;
+  hipStream_t s;
+  PRH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return PRH_OK;
+}
+
+int prh_stream_destroy(void* stream) {
+  if (stream) PRH_HIP(hipStreamDestroy(as_stream(stream)));
+  return PRH_OK;
+}
+
+int prh_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return PRH_ERR_ARG;
+  *ptr = nullptr;
+  if (bytes == 0) return PRH_OK;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    set_error("hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return PRH_ERR_NOMEM;
+  }
+  return PRH_OK;
+}
+
+int prh_free(void* ptr) {
+  if (ptr) PRH_HIP(hipFree(ptr));
+  return PRH_OK;
+}
+
+int prh_host_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return PRH_ERR_ARG;
+  *ptr = nullptr;
+  if (bytes == 0) return PRH_OK;
+  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    set_error("hipHostMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return PRH_ERR_NOMEM;
+  }
+  return PRH_OK;
+}
+
+int prh_host_free(void* ptr) {
+  if (ptr) PRH_HIP(hipHostFree(ptr));
+  return PRH_OK;
+}
+
+int prh_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return PRH_OK;
+  if (!dst || !src) return PRH_ERR_ARG;
+  PRH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+  return PRH_OK;
+}
+
+int prh_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return PRH_OK;
+  if (!dst || !src) return PRH_ERR_ARG;
+  PRH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+  return PRH_OK;
+}
+
+int prh_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return PRH_OK;
+  if (!dst || !src) return PRH_ERR_ARG;
+  PRH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+  return PRH_OK;
+}
+
+int prh_memset(void* dst, int value, size_t bytes, void* stream) {
+  if (bytes == 0) return PRH_OK;
+  if (!dst) return PRH_ERR_ARG;
+  PRH_HIP(hipMemsetAsync(dst, value, bytes, as_stream(stream)));
+  return PRH_OK;
+}
+
+int prh_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return PRH_ERR_ARG;
+  PRH_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return PRH_OK;
+}
+
+int prh_event_create(void** event) {
+  if (!event) return PRH_ERR_ARG;
+  hipEvent_t e;
+  PRH_HIP(hipEventCreate(&e));
+  *event = e;
+  return PRH_OK;
+}
+
+int prh_event_destroy(void* event) {
+  if (event) PRH_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(event)));
+  return PRH_OK;
+}
+
+int prh_event_record(void* event, void* stream) {
+  if (!event) return PRH_ERR_ARG;
+  PRH_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(event), as_stream(stream)));
+  return PRH_OK;
+}
+
+int prh_event_elapsed_ms(void* start, void* stop, float* ms) {
+  if (!start || !stop || !ms) return PRH_ERR_ARG;
+  PRH_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)));
+  PRH_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
+                              reinterpret_cast<hipEvent_t>(stop)));
+  return PRH_OK;
+}
+
+int prh_workspace_footprint(const size_t* nmesh, int dtype, size_t paint_chunk, unsigned flags,
+                            size_t* bytes) {
+  if (!nmesh || !bytes || (dtype != PRH_F32 && dtype != PRH_F64)) return PRH_ERR_ARG;
+  Plan p = plan_bytes(nmesh, dtype, paint_chunk, flags);
+  *bytes = p.mg + p.mixed + p.paint + p.fft + p.red;
+  return PRH_OK;
+}
+
+int prh_workspace_destroy(prh_workspace* ws) {
+  if (!ws) return PRH_OK;
+  auto release = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  for (Hierarchy* H : {&ws->mg, &ws->mixed})
+    for (int l = 0; l < kMaxLevels; l++) {
+      release(H->v[l]);
+      release(H->f[l]);
+      release(H->tmp[l]);
+    }
+  PaintScratch& P = ws->paint;
+  for (void* p : {(void*)P.count, (void*)P.start, (void*)P.partial, (void*)P.key, (void*)P.rank,
+                  (void*)P.sorted, P.payload, (void*)P.misc, (void*)P.big})
+    release(p);
+  if (ws->fft.have) {
+    hipfftDestroy(ws->fft.r2c);
+    hipfftDestroy(ws->fft.c2r);
+  }
+  release(ws->fft.spectrum);
+  release(ws->fft.work);
+  release(ws->red);
+  release(ws->misc);
+  if (ws->red_host) (void)hipHostFree(ws->red_host);
+  if (ws->flag_host) (void)hipHostFree(ws->flag_host);
+  delete ws;
+  return PRH_OK;
+}
+
+int prh_workspace_create(prh_workspace** out, const size_t* nmesh, int dtype, size_t paint_chunk,
+                         unsigned flags) {
+  if (!out || !nmesh) return PRH_ERR_ARG;
+  *out = nullptr;
+  if (dtype != PRH_F32 && dtype != PRH_F64) {
+    set_error("dtype must be PRH_F32 or PRH_F64");
+    return PRH_ERR_ARG;
+  }
+  for (int d = 0; d < 3; d++)
+    if (nmesh[d] < 2 || nmesh[d] > (1u << 16)) {
+      set_error("nmesh[%d] = %zu out of the supported range [2, 65536]", d, nmesh[d]);
+      return PRH_ERR_ARG;
+    }
+  if (cells(nmesh) >= (size_t(1) << 32)) {
+    set_error("mesh of %zu cells exceeds 2^32", cells(nmesh));
+    return PRH_ERR_ARG;
+  }
+  if ((flags & PRH_WS_PAINT) && (paint_chunk == 0 || paint_chunk >= (size_t(1) << 31))) {
+    set_error("paint_chunk must be in [1, 2^31)");
+    return PRH_ERR_ARG;
+  }
+  prh_workspace* ws = new prh_workspace();
+  for (int d = 0; d < 3; d++) ws->n[d] = nmesh[d];
+  ws->dtype = dtype;
+  ws->flags = flags;
+  int s = PRH_OK;
+  if (hipGetDevice(&ws->device) != hipSuccess) {
+    set_error("no HIP device");
+    delete ws;
+    return PRH_ERR_HIP;
+  }
+  if (s == PRH_OK && (flags & PRH_WS_MULTIGRID))
+    s = hierarchy_alloc(ws, ws->mg, static_cast<size_t>(dtype), false);
+  if (s == PRH_OK && (flags & PRH_WS_MIXED) && dtype == PRH_F64)
+    s = hierarchy_alloc(ws, ws->mixed, sizeof(float), true);
+  if (s == PRH_OK && (flags & PRH_WS_PAINT)) s = paint_alloc(ws, paint_chunk);
+  if (s == PRH_OK && (flags & PRH_WS_FFT)) s = fft_setup(ws);
+  if (s == PRH_OK) {
+    void* p;
+    s = ws_alloc(ws, &p, (kReduceBlocks * 2 + 16) * sizeof(double));
+    ws->red = static_cast<double*>(p);
+    ws->nred = kReduceBlocks * 2 + 16;
+    if (s == PRH_OK) s = ws_alloc(ws, &p, 16 * sizeof(uint32_t));
+    ws->misc = static_cast<uint32_t*>(p);
+  }
+  if (s == PRH_OK) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&ws->red_host), 64 * sizeof(double),
+                      hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&ws->flag_host), 64 * sizeof(unsigned),
+                      hipHostMallocDefault) != hipSuccess) {
+      set_error("pinned host allocation failed");
+      s = PRH_ERR_NOMEM;
+    }
+  }
+  if (s != PRH_OK) {
+    prh_workspace_destroy(ws);
+    return s;
+  }
+  *out = ws;
+  return PRH_OK;
+}
+
+int prh_workspace_bytes(const prh_workspace* ws, size_t* bytes) {
+  if (!ws || !bytes) return PRH_ERR_ARG;
+  *bytes = ws->bytes;
+  return PRH_OK;
+}
+
+}  // extern "C"

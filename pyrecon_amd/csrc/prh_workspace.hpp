@@ -1,0 +1,72 @@
+// Internal layout of prh_workspace (opaque in the C ABI).
+#pragma once
+
+#include <hipfft/hipfft.h>
+
+#include "prh_common.hpp"
+
+namespace prh {
+
+constexpr int kMaxLevels = 24;
+
+// One multigrid hierarchy in precision R.  Level 0 borrows the caller's
+// iterate and source; it owns only a second iterate (tmp).  Levels >= 1 own
+// an iterate v, a source f and a second iterate tmp (Jacobi ping-pong and
+// residual storage).  When `full_top` is set, level 0 also owns v and f
+// (the FP32 hierarchy of the mixed-precision solve).
+struct Hierarchy {
+  int nlev = 0;
+  size_t n[kMaxLevels][3] = {};
+  void* v[kMaxLevels] = {};
+  void* f[kMaxLevels] = {};
+  void* tmp[kMaxLevels] = {};
+};
+
+// CIC paint scratch for one chunk of at most `chunk` points.
+struct PaintScratch {
+  size_t chunk = 0;
+  size_t ncell = 0;
+  uint32_t* count = nullptr;    // ncell      per-cell point counts (kept zero between chunks)
+  uint32_t* start = nullptr;    // ncell + 1  exclusive prefix of count
+  uint32_t* partial = nullptr;  // scan block partials
+  size_t npartial = 0;
+  uint32_t* key = nullptr;      // chunk      base cell of each point
+  uint32_t* rank = nullptr;     // chunk      arrival rank within its cell; reused as sort scratch
+  uint32_t* sorted = nullptr;   // chunk      point indices grouped by cell, ascending within a cell
+  void* payload = nullptr;      // chunk      (t_x, t_y, t_z, w) in the mesh precision
+  uint32_t* misc = nullptr;     // [0] error flags, [1] number of long segments
+  uint32_t* big = nullptr;      // long-segment cell list (ncell worst case is bounded by chunk/32)
+  size_t nbig = 0;
+};
+
+struct FFTState {
+  hipfftHandle r2c = 0, c2r = 0;
+  bool have = false;
+  void* spectrum = nullptr;  // n0 * n1 * (n2/2+1) complex
+  void* work = nullptr;
+  size_t work_bytes = 0;
+};
+
+}  // namespace prh
+
+struct prh_workspace {
+  size_t n[3] = {0, 0, 0};
+  int dtype = 0;
+  unsigned flags = 0;
+  size_t bytes = 0;
+  int device = 0;
+  prh::Hierarchy mg;     // workspace precision
+  prh::Hierarchy mixed;  // FP32 hierarchy incl. level 0 (PRH_WS_MIXED)
+  prh::PaintScratch paint;
+  prh::FFTState fft;
+  double* red = nullptr;       // reduction partials (device), then the results
+  uint32_t* misc = nullptr;    // device status flags of the read kernels
+  size_t nred = 0;
+  double* red_host = nullptr;  // pinned host landing for reductions
+  unsigned* flag_host = nullptr;
+};
+
+namespace prh {
+int ws_alloc(prh_workspace* ws, void** p, size_t bytes);
+constexpr unsigned kReduceBlocks = 1024;
+}  // namespace prh

@@ -1,0 +1,321 @@
+"""Mesh geometry and device-resident meshes.
+
+:class:`MeshInfo` restates the reference's geometry rules (pyrecon/mesh.py:322-483):
+``nmesh`` stored as C int, ``boxsize``/``boxcenter`` stored in the mesh dtype,
+box from ``positions`` with ``boxpad`` when not given, ``cellsize`` -> ``nmesh``,
+the recommended-size warning, ``offset``, ``cellsize`` and ``wrap``.
+
+:class:`RealMesh` holds its values on the device; ``.value`` downloads a numpy
+copy (and assigning ``.value`` uploads), so code written against the
+reference's numpy-backed RealMesh keeps working.
+"""
+import warnings
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib, check, MeshError
+from .device import DeviceArray, as_device, stream_handle
+
+DEFAULT_PAINT_CHUNK = 1 << 24
+
+
+def _make_array(value, shape, dtype='f8'):
+    out = np.empty(shape, dtype=dtype)
+    out[...] = value
+    return out
+
+
+class MeshInfo(object):
+    """Mesh geometry (pyrecon/mesh.py:322-483)."""
+
+    _attrs = ['dtype', 'nmesh', 'boxsize', 'boxcenter']
+
+    def __init__(self, nmesh=None, boxsize=None, boxcenter=None, cellsize=None, value=None, positions=None,
+                 boxpad=1.5, select_nmesh=None, dtype=None):
+        if value is not None:
+            value = np.asarray(value)
+            if dtype is None: dtype = value.dtype
+            if nmesh is None: nmesh = value.shape
+        dtype = np.dtype(dtype if dtype is not None else 'f8')
+        if dtype not in (np.dtype('f4'), np.dtype('f8')):
+            raise MeshError('Provide float dtype (f4 or f8)')
+        if boxsize is None or boxcenter is None:
+            if positions is None:
+                raise MeshError('positions must be provided if boxsize or boxcenter are not specified')
+            if isinstance(positions, DeviceArray): positions = positions.numpy()
+            positions = np.asarray(positions)
+            pos_min, pos_max = positions.min(axis=0), positions.max(axis=0)
+            delta = np.abs(pos_max - pos_min)
+            if boxcenter is None: boxcenter = 0.5 * (pos_min + pos_max)
+            if boxsize is None:
+                if cellsize is not None and nmesh is not None:
+                    boxsize = nmesh * cellsize
+                else:
+                    boxsize = delta * boxpad
+            if (boxsize < delta).any(): raise MeshError('boxsize too small to contain all data')
+        boxsize = _make_array(boxsize, 3, dtype='f8')
+        if nmesh is None:
+            if cellsize is not None:
+                cellsize = _make_array(cellsize, 3, dtype='f8')
+                nmesh = boxsize / cellsize
+                nmesh = np.ceil(nmesh).astype('i8')
+                if select_nmesh is not None: nmesh = select_nmesh(nmesh)
+                boxsize = nmesh * cellsize
+            else:
+                raise MeshError('nmesh (or cellsize) must be specified')
+        self.dtype = dtype
+        self.boxsize = boxsize
+        self.boxcenter = boxcenter
+        self.nmesh = nmesh
+        if select_nmesh is not None:
+            recommended = select_nmesh(self.nmesh)
+            if not np.all(recommended == self.nmesh):
+                warnings.warn('Recommended nmesh is {}, provided nmesh is {}'.format(recommended, self.nmesh))
+
+    @property
+    def dtype(self):
+        return self._dtype
+
+    @dtype.setter
+    def dtype(self, dtype):
+        self._dtype = np.dtype(dtype)
+        if hasattr(self, '_boxsize'):
+            self.boxsize = self._boxsize
+            self.boxcenter = self._boxcenter
+
+    @property
+    def boxsize(self):
+        return self._boxsize
+
+    @boxsize.setter
+    def boxsize(self, value):
+        self._boxsize = _make_array(value, 3, dtype=self.dtype)
+
+    @property
+    def boxcenter(self):
+        return self._boxcenter
+
+    @boxcenter.setter
+    def boxcenter(self, value):
+        self._boxcenter = _make_array(value, 3, dtype=self.dtype)
+
+    @property
+    def nmesh(self):
+        return self._nmesh
+
+    @nmesh.setter
+    def nmesh(self, value):
+        self._nmesh = _make_array(value, 3, dtype=np.intc)
+
+    @property
+    def offset(self):
+        """Coordinates of the (0, 0, 0) corner of the box."""
+        return self.boxcenter - self.boxsize / 2.
+
+    @property
+    def cellsize(self):
+        return self.boxsize / self.nmesh
+
+    @property
+    def ndim(self):
+        return 3
+
+    @property
+    def shape(self):
+        return tuple(int(n) for n in self.nmesh)
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    def wrap(self, positions):
+        """Wrap positions into the box (host arrays)."""
+        return ((positions - self.offset) % self.boxsize) + self.offset
+
+    def clone(self, **kwargs):
+        for name in self._attrs: kwargs.setdefault(name, getattr(self, name))
+        return self.__class__(**kwargs)
+
+    def __repr__(self):
+        return 'MeshInfo(dtype={}, nmesh={}, boxsize={}, boxcenter={})'.format(self.dtype, self.nmesh, self.boxsize,
+                                                                               self.boxcenter)
+
+
+class Workspace(object):
+    """Owner of a ``prh_workspace``: multigrid hierarchy, paint scratch, FFT plans."""
+
+    def __init__(self, nmesh, dtype, paint_chunk=DEFAULT_PAINT_CHUNK, flags=_lib.WS_ALL):
+        import ctypes
+        self.nmesh = tuple(int(n) for n in nmesh)
+        self.dtype = np.dtype(dtype)
+        self.paint_chunk = int(paint_chunk)
+        self.flags = int(flags)
+        if self.dtype == np.float32: self.flags &= ~_lib.WS_MIXED
+        handle = ctypes.c_void_p()
+        nm, nm_ptr = _lib.size3(self.nmesh)
+        check(lib.prh_workspace_create(ctypes.byref(handle), nm_ptr, _lib.dtype_code(self.dtype), self.paint_chunk,
+                                       self.flags), 'workspace create')
+        self.handle = handle.value
+        nbytes = ctypes.c_size_t()
+        check(lib.prh_workspace_bytes(self.handle, ctypes.byref(nbytes)), 'workspace bytes')
+        self.nbytes = nbytes.value
+
+    @staticmethod
+    def footprint(nmesh, dtype, paint_chunk=DEFAULT_PAINT_CHUNK, flags=_lib.WS_ALL):
+        import ctypes
+        dtype = np.dtype(dtype)
+        if dtype == np.float32: flags &= ~_lib.WS_MIXED
+        nm, nm_ptr = _lib.size3(nmesh)
+        out = ctypes.c_size_t()
+        check(lib.prh_workspace_footprint(nm_ptr, _lib.dtype_code(dtype), int(paint_chunk), int(flags),
+                                          ctypes.byref(out)), 'workspace footprint')
+        return out.value
+
+    def close(self):
+        handle = getattr(self, 'handle', None)
+        if handle:
+            lib.prh_workspace_destroy(handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_workspace_cache = {}
+
+
+def shared_workspace(nmesh, dtype, flags=_lib.WS_ALL, paint_chunk=DEFAULT_PAINT_CHUNK):
+    key = (tuple(int(n) for n in nmesh), np.dtype(dtype).str, int(flags), int(paint_chunk))
+    ws = _workspace_cache.get(key)
+    if ws is None:
+        ws = _workspace_cache[key] = Workspace(nmesh, dtype, paint_chunk=paint_chunk, flags=flags)
+    return ws
+
+
+def _suffix(dtype):
+    return 'f32' if np.dtype(dtype) == np.float32 else 'f64'
+
+
+class RealMesh(object):
+    """Real mesh whose values live on the device (pyrecon/mesh.py:486-736)."""
+
+    def __init__(self, value=None, info=None, dtype=None, workspace=None, stream=None, **kwargs):
+        if info is None:
+            if dtype is None and (value is None or np.ndim(value) == 0): dtype = 'f8'
+            info = MeshInfo(value=None if isinstance(value, DeviceArray) or np.ndim(value) == 0 else value,
+                            dtype=dtype, **kwargs)
+        self.info = info
+        self.stream = stream
+        self._workspace = workspace
+        self.buffer = None
+        if value is not None: self.value = value
+
+    # geometry passthrough
+    dtype = property(lambda self: self.info.dtype)
+    nmesh = property(lambda self: self.info.nmesh)
+    boxsize = property(lambda self: self.info.boxsize)
+    boxcenter = property(lambda self: self.info.boxcenter)
+    offset = property(lambda self: self.info.offset)
+    cellsize = property(lambda self: self.info.cellsize)
+    shape = property(lambda self: self.info.shape)
+    size = property(lambda self: self.info.size)
+    ndim = property(lambda self: 3)
+
+    @property
+    def workspace(self):
+        if self._workspace is None:
+            self._workspace = shared_workspace(self.shape, self.dtype)
+        return self._workspace
+
+    def _allocate(self, zero=True):
+        if self.buffer is None:
+            self.buffer = DeviceArray(self.shape, self.dtype)
+            if zero: self.buffer.fill_zero(stream=self.stream)
+        return self.buffer
+
+    @property
+    def value(self):
+        if self.buffer is None: return None
+        return self.buffer.numpy(stream=self.stream)
+
+    @value.setter
+    def value(self, value):
+        if value is None:
+            self.buffer = None
+            return
+        if isinstance(value, DeviceArray):
+            if value.size != self.size or value.dtype != self.dtype:
+                raise MeshError('device value must have size {} and dtype {}'.format(self.size, self.dtype))
+            self.buffer = value.reshape(self.shape) if value.shape != self.shape else value
+            return
+        array = np.empty(self.shape, dtype=self.dtype)
+        array[...] = value
+        self._allocate(zero=False).upload(array, stream=self.stream)
+
+    def __array__(self, dtype=None, copy=None):
+        value = self.value
+        if dtype is not None: value = value.astype(dtype)
+        return value
+
+    def __repr__(self):
+        return 'RealMesh(dtype={}, nmesh={}, boxsize={}, boxcenter={})'.format(self.dtype, self.nmesh, self.boxsize,
+                                                                               self.boxcenter)
+
+    def zeros_like(self):
+        new = RealMesh(info=self.info, workspace=self._workspace, stream=self.stream)
+        new._allocate(zero=True)
+        return new
+
+    def deepcopy(self):
+        new = RealMesh(info=self.info.clone(), workspace=self._workspace, stream=self.stream)
+        if self.buffer is not None:
+            new._allocate(zero=False).copy_(self.buffer, stream=self.stream)
+        return new
+
+    def coords(self):
+        return tuple(o + d * np.arange(n) for n, o, d in zip(self.nmesh, self.offset, self.boxsize / self.nmesh))
+
+    # ------------------------------------------------------------------ kernels
+    def assign_cic(self, positions, weights=None, wrap=False):
+        """Paint positions with CIC (pyrecon/mesh.py:529-568); accumulates into the mesh."""
+        pos, _ = as_device(positions, stream=self.stream)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError('positions must have shape (N, 3)')
+        wdev, wcode = None, 0
+        if weights is not None:
+            wdev, _ = as_device(weights, stream=self.stream)
+            if wdev.size != len(pos):
+                raise ValueError('weights must have shape (N,)')
+            wcode = _lib.dtype_code(wdev.dtype)
+        self._allocate(zero=True)
+        nm, nm_ptr = _lib.size3(self.shape)
+        bs, bs_ptr = _lib.vec3(self.boxsize, self.dtype)
+        bc, bc_ptr = _lib.vec3(self.boxcenter, self.dtype)
+        fn = getattr(lib, 'prh_paint_' + _suffix(self.dtype))
+        check(fn(self.workspace.handle, self.buffer.ptr, nm_ptr, bs_ptr, bc_ptr, pos.ptr,
+                 _lib.dtype_code(pos.dtype), wdev.ptr if wdev is not None else None, wcode, len(pos), int(bool(wrap)),
+                 stream_handle(self.stream)), 'assign_cic')
+
+    def smooth_gaussian(self, radius, method='fft', nsigmas=2.5, **kwargs):
+        """Gaussian smoothing in Fourier space (pyrecon/mesh.py:655-680)."""
+        if method != 'fft':
+            raise NotImplementedError('only method="fft" is implemented on the device')
+        if self.buffer is None: raise MeshError('mesh has no value')
+        radius_ = np.empty(3, dtype='f8')
+        radius_[:] = radius
+        nm, nm_ptr = _lib.size3(self.shape)
+        bs, bs_ptr = _lib.vec3(self.boxsize, self.dtype)
+        fn = getattr(lib, 'prh_smooth_gaussian_' + _suffix(self.dtype))
+        check(fn(self.workspace.handle, self.buffer.ptr, nm_ptr, bs_ptr, radius_.ctypes.data,
+                 stream_handle(self.stream)), 'smooth_gaussian')
+
+    def sum(self):
+        import ctypes
+        sums = (ctypes.c_double * 2)()
+        fn = getattr(lib, 'prh_mesh_sums_' + _suffix(self.dtype))
+        check(fn(self.workspace.handle, self.buffer.ptr, None, self.size, sums, stream_handle(self.stream)), 'sum')
+        return sums[0]

@@ -1,0 +1,281 @@
+"""End-to-end parity of the drop-in MultiGridReconstruction with the CPU oracle.
+
+Stage by stage on the configuration-1 catalogs (nmesh=64, 1e5 data, 1e6 randoms):
+- CIC meshes after painting Cartesian positions: bit-exact;
+- density contrast: relative L2 <= 1e-12 (hipFFT R2C vs numpy's c2c FFT);
+- potential (mean removed): relative L2 <= 1e-10;
+- shifted positions: max |diff| <= 1e-8 Mpc/h; and bit-exact when the GPU reads
+  the oracle's own potential.
+Tolerances for FP32 meshes follow the reference's own f4-vs-f8 test
+(pyrecon/tests/test_multigrid.py:94, atol = rtol = 1e-2).
+"""
+import warnings
+
+import numpy as np
+import pytest
+
+from catalogs import make_catalogs, CONFIGS, RECON_KWARGS, SMOOTHING_RADIUS
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view('u{:d}'.format(a.dtype.itemsize))
+
+
+def rel_l2(a, b, remove_mean=False):
+    a, b = np.asarray(a, dtype='f8'), np.asarray(b, dtype='f8')
+    if remove_mean:
+        a, b = a - a.mean(), b - b.mean()
+    return np.sqrt(np.sum((a - b)**2) / np.sum(b**2))
+
+
+@pytest.fixture(scope='module')
+def c1():
+    return make_catalogs('c1')
+
+
+def make_pair(cat, dtype='f8', los=None, wrap=False, **kwargs):
+    from pyrecon_amd import MultiGridReconstruction
+    from oracle.recon_numpy import OracleMultiGridReconstruction
+    cfg = cat['config']
+    geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype=dtype, wrap=wrap)
+    rkw = dict(RECON_KWARGS, los=los)
+    gpu = MultiGridReconstruction(**rkw, **geo, **kwargs)
+    cpu = OracleMultiGridReconstruction(**rkw, **geo)
+    return gpu, cpu
+
+
+def test_pipeline_f8_stages(c1):
+    gpu, cpu = make_pair(c1)
+    for r in (gpu, cpu):
+        r.assign_data(c1['data'], c1['data_weights'])
+        r.assign_randoms(c1['randoms'], c1['randoms_weights'])
+    assert np.array_equal(bits(gpu.mesh_data.value), bits(cpu.mesh_data)), 'data mesh not bit-exact'
+    assert np.array_equal(bits(gpu.mesh_randoms.value), bits(cpu.mesh_randoms)), 'randoms mesh not bit-exact'
+    for r in (gpu, cpu):
+        r.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    delta_gpu, delta_cpu = gpu.mesh_delta.value, cpu.mesh_delta
+    assert rel_l2(delta_gpu, delta_cpu) < 1e-12
+    for r in (gpu, cpu):
+        r.run()
+    phi_gpu, phi_cpu = gpu.mesh_phi.value, cpu.mesh_phi
+    assert rel_l2(phi_gpu, phi_cpu, remove_mean=True) < 1e-10
+    for field in ['disp', 'rsd', 'disp+rsd']:
+        a = gpu.read_shifted_positions(c1['data'], field=field)
+        b = cpu.read_shifted_positions(c1['data'], field=field)
+        assert a.dtype == b.dtype
+        assert np.max(np.abs(a - b)) < 1e-8
+    # with the oracle's own potential the read of the shifts is bit-exact
+    gpu.mesh_phi.value = phi_cpu
+    for field in ['disp', 'rsd', 'disp+rsd']:
+        for pos in [c1['data'], c1['randoms'][:200000]]:
+            a = gpu.read_shifted_positions(pos, field=field)
+            b = cpu.read_shifted_positions(pos, field=field)
+            assert np.array_equal(bits(a), bits(b)), field
+            a = gpu.read_shifts(pos, field=field)
+            b = cpu.read_shifts(pos, field=field)
+            assert np.array_equal(bits(a), bits(b)), field
+
+
+def test_fmg_of_oracle_delta_bitexact(c1):
+    # same density contrast in -> bit-identical potential out (the multigrid is exact)
+    gpu, cpu = make_pair(c1)
+    for r in (gpu, cpu):
+        r.assign_data(c1['data'], c1['data_weights'])
+        r.assign_randoms(c1['randoms'], c1['randoms_weights'])
+        r.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    gpu.mesh_delta.value = cpu.mesh_delta
+    gpu.run()
+    cpu.run()
+    assert np.array_equal(bits(gpu.mesh_phi.value), bits(cpu.mesh_phi))
+
+
+def test_repeatable_bits(c1):
+    outs = []
+    for _ in range(2):
+        gpu, _ = make_pair(c1, paint_chunk=300000)
+        gpu.assign_data(c1['data'], c1['data_weights'])
+        gpu.assign_randoms(c1['randoms'], c1['randoms_weights'])
+        meshes = (gpu.mesh_data.value, gpu.mesh_randoms.value)
+        gpu.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+        gpu.run()
+        outs.append(meshes + (gpu.mesh_phi.value, gpu.read_shifted_positions(c1['data'])))
+    for a, b in zip(*outs):
+        assert np.array_equal(bits(a), bits(b))
+
+
+def test_slab_painting(c1):
+    # assign_data may be called slab by slab (recon.py:175): same bits as one call
+    gpu, cpu = make_pair(c1)
+    n = len(c1['randoms'])
+    for lo, hi in [(0, n // 3), (n // 3, n // 2), (n // 2, n)]:
+        gpu.assign_randoms(c1['randoms'][lo:hi], c1['randoms_weights'][lo:hi])
+    cpu.assign_randoms(c1['randoms'], c1['randoms_weights'])
+    assert np.array_equal(bits(gpu.mesh_randoms.value), bits(cpu.mesh_randoms))
+    assert gpu._size_randoms == n
+
+
+def test_range_errors(c1):
+    from pyrecon_amd import MeshError
+    gpu, cpu = make_pair(c1)
+    gpu.assign_data(c1['data'][:1000])
+    before = gpu.mesh_data.value
+    bad = c1['data'][:2000].copy()
+    bad[-1, 0] += 2000.
+    with pytest.raises(MeshError):
+        gpu.assign_data(bad)
+    assert np.array_equal(bits(gpu.mesh_data.value), bits(before)), 'mesh modified by a failed paint'
+    bad = c1['data'][:2000].copy()
+    bad[5, 1] = np.nan
+    with pytest.raises(MeshError):
+        gpu.assign_data(bad)
+    weights = np.ones(2000)
+    weights[7] = np.inf
+    with pytest.raises(MeshError):
+        gpu.assign_data(c1['data'][:2000], weights)
+    assert np.array_equal(bits(gpu.mesh_data.value), bits(before))
+    # multi-chunk validation happens before any chunk is painted
+    gpu2, _ = make_pair(c1, paint_chunk=500)
+    gpu2.assign_data(c1['data'][:1000])
+    before = gpu2.mesh_data.value
+    bad = c1['data'][:5000].copy()
+    bad[-1, 2] -= 5000.
+    with pytest.raises(MeshError):
+        gpu2.assign_data(bad)
+    assert np.array_equal(bits(gpu2.mesh_data.value), bits(before))
+
+
+def test_wrap(c1):
+    # reference test_wrap (pyrecon/tests/test_multigrid.py:39-64)
+    from pyrecon_amd import MultiGridReconstruction
+    from oracle.recon_numpy import OracleMultiGridReconstruction
+    rng = np.random.RandomState(42)
+    boxsize = 1000.
+    for boxcenter in [-500., 0., 500.]:
+        data = rng.uniform(-boxsize / 2, boxsize / 2, (50000, 3)) + boxcenter
+        data[-1] = boxsize + 1 + boxcenter
+        randoms = rng.uniform(-boxsize / 2, boxsize / 2, (50000, 3)) + boxcenter
+        randoms[-1] = -1 - boxsize + boxcenter
+        kw = dict(f=0.8, bias=2, los='z', boxsize=boxsize, boxcenter=boxcenter, nmesh=64, wrap=True)
+        gpu = MultiGridReconstruction(**kw)
+        cpu = OracleMultiGridReconstruction(**kw)
+        for r in (gpu, cpu):
+            r.assign_data(data)
+            r.assign_randoms(randoms)
+        assert np.array_equal(bits(gpu.mesh_data.value), bits(cpu.mesh_data))
+        for r in (gpu, cpu):
+            r.set_density_contrast()
+            r.run()
+        for field in ['rsd', 'disp', 'disp+rsd']:
+            shifts = gpu.read_shifts(data, field=field)
+            diff = data - shifts
+            positions_rec = (diff - gpu.offset) % gpu.boxsize + gpu.offset
+            assert np.all(positions_rec >= boxcenter - boxsize / 2.) and np.all(positions_rec <= boxcenter + boxsize / 2.)
+            assert np.allclose(gpu.read_shifted_positions(data, field=field), positions_rec)
+        gpu.mesh_phi.value = cpu.mesh_phi
+        for field in ['rsd', 'disp', 'disp+rsd']:
+            assert np.array_equal(bits(gpu.read_shifted_positions(data, field=field)),
+                                  bits(cpu.read_shifted_positions(data, field=field)))
+
+
+@pytest.mark.parametrize('mesh_dtype', ['f4', 'f8'])
+@pytest.mark.parametrize('pos_dtype', ['f4', 'f8'])
+@pytest.mark.parametrize('los', [None, 'x'])
+def test_dtype_combinations(c1, mesh_dtype, pos_dtype, los):
+    # numpy promotion of positions (P) and mesh (R) dtypes through paint and shifts: bit-exact given phi
+    gpu, cpu = make_pair(c1, dtype=mesh_dtype, los=los)
+    data = c1['data'][:20000].astype(pos_dtype)
+    randoms = c1['randoms'][:200000].astype(pos_dtype)
+    for r in (gpu, cpu):
+        r.assign_data(data)
+        r.assign_randoms(randoms)
+    assert np.array_equal(bits(gpu.mesh_data.value), bits(cpu.mesh_data))
+    assert np.array_equal(bits(gpu.mesh_randoms.value), bits(cpu.mesh_randoms))
+    for r in (gpu, cpu):
+        r.set_density_contrast()
+        r.run()
+    gpu.mesh_phi.value = cpu.mesh_phi
+    for field in ['disp', 'rsd', 'disp+rsd']:
+        a, b = gpu.read_shifts(data, field=field), cpu.read_shifts(data, field=field)
+        assert a.dtype == b.dtype and np.array_equal(bits(a), bits(b)), (field, a.dtype, b.dtype)
+        a, b = gpu.read_shifted_positions(data, field=field), cpu.read_shifted_positions(data, field=field)
+        assert a.dtype == b.dtype and np.array_equal(bits(a), bits(b)), (field, a.dtype, b.dtype)
+
+
+def test_f4_vs_f8(c1):
+    # reference test_dtype (pyrecon/tests/test_multigrid.py:67-94)
+    shifts = {}
+    for dtype in ['f4', 'f8']:
+        gpu, _ = make_pair(c1, dtype=dtype, los='x')
+        gpu.assign_data(c1['data'], c1['data_weights'])
+        gpu.assign_randoms(c1['randoms'], c1['randoms_weights'])
+        gpu.set_density_contrast()
+        assert gpu.mesh_delta.dtype.itemsize == int(dtype[1])
+        gpu.run()
+        assert gpu.mesh_phi.dtype.itemsize == int(dtype[1])
+        shifts[dtype] = gpu.read_shifts(c1['data'], field='disp+rsd')
+        assert shifts[dtype].dtype.itemsize == 8
+        assert gpu.read_shifts(c1['data'].astype('f4')).dtype.itemsize == 4
+    assert not np.all(shifts['f4'] == shifts['f8'])
+    assert np.allclose(shifts['f4'], shifts['f8'], atol=1e-2, rtol=1e-2)
+
+
+def test_los_far_box():
+    # reference test_los (pyrecon/tests/test_multigrid.py:124-145)
+    from pyrecon_amd import MultiGridReconstruction
+    rng = np.random.RandomState(42)
+    boxsize = 1000.
+    data = rng.uniform(-boxsize / 2, boxsize / 2, (100000, 3))
+    randoms = rng.uniform(-boxsize / 2, boxsize / 2, (100000, 3))
+    weights = rng.uniform(0.5, 1., 100000)
+    recon = MultiGridReconstruction(f=0.8, bias=2., los='x', boxcenter=0., boxsize=boxsize, nmesh=64, dtype='f8')
+    recon.assign_data(data, weights)
+    recon.assign_randoms(randoms, weights)
+    recon.set_density_contrast()
+    recon.run()
+    shifts_global = recon.read_shifts(data, field='disp+rsd')
+    offset = 1e8
+    data[:, 0] += offset
+    randoms[:, 0] += offset
+    recon = MultiGridReconstruction(f=0.8, bias=2., boxcenter=[offset, 0., 0.], boxsize=boxsize, nmesh=64, dtype='f8')
+    recon.assign_data(data, weights)
+    recon.assign_randoms(randoms, weights)
+    recon.set_density_contrast()
+    recon.run()
+    shifts_local = recon.read_shifts(data, field='disp+rsd')
+    assert np.allclose(shifts_local, shifts_global, rtol=1e-3, atol=1e-3)
+
+
+def test_no_randoms():
+    # reference test_no_nrandoms (pyrecon/tests/test_multigrid.py:26-36)
+    from pyrecon_amd import MultiGridReconstruction
+    rng = np.random.RandomState(42)
+    boxsize = 1000.
+    data = rng.uniform(-boxsize / 2, boxsize / 2, (100000, 3))
+    weights = rng.uniform(0.5, 1., 100000)
+    recon = MultiGridReconstruction(f=0.8, bias=2., los='x', boxcenter=0., boxsize=boxsize, nmesh=8, dtype='f8')
+    recon.assign_data(data, weights)
+    assert not recon.has_randoms
+    recon.set_density_contrast()
+    assert np.allclose(np.mean(recon.mesh_delta), 0.)
+    recon.run(jacobi_niterations=1, vcycle_niterations=1)
+    assert np.all(np.abs(recon.read_shifts(data)) < 2.)
+
+
+def test_mixed_precision(c1):
+    gpu, cpu = make_pair(c1)
+    for r in (gpu, cpu):
+        r.assign_data(c1['data'], c1['data_weights'])
+        r.assign_randoms(c1['randoms'], c1['randoms_weights'])
+        r.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    gpu.run(precision='mixed', nrefine=2, residual_norms=True)
+    cpu.run()
+    norms = gpu.residual_norms
+    assert np.all(np.diff(norms) < 0), norms
+    phi = gpu.mesh_phi.value
+    assert rel_l2(phi, cpu.mesh_phi, remove_mean=True) < 1e-3
+    a = gpu.read_shifted_positions(c1['data'])
+    b = cpu.read_shifted_positions(c1['data'])
+    assert np.max(np.abs(a - b)) <= 1e-3
