@@ -1,0 +1,279 @@
+#!/usr/bin/env python
+"""Benchmark of MultiGridReconstruction on one MI355X.
+
+A step is one full reconstruction of the nmesh=512 survey-like workload
+(configuration 3: boxsize 4000 Mpc/h, boxcenter (0, 0, 3000), 2e7 data and
+1e8 randoms in the shell 1000 < r < 3000 Mpc/h with FKP weights, local line of
+sight, f = 0.8, bias = 2, smoothing radius 15, reference defaults for the
+solver): paint data and randoms, smooth, density contrast, solve, and the
+shifted positions of data and randoms (RecSym).  Catalogs are generated once
+(seeded, synthetic) and uploaded before the timed region, so inputs are
+resident in HBM.  ``value`` is recon/s of the mixed-precision path (FP32
+V-cycles inside FP64 defect correction, FP64 meshes); the plain FP64 path is
+timed in the same run, and the mixed result is compared with it (the plain
+FP64 solve is bit-identical to the reference's C multigrid).
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'tests'))
+
+METRIC = 'recon/s at nmesh=512 (FP32+FP64-refined vs FP64, 1 GPU); p50/p99 job latency'
+HBM_PEAK_GBS = 8000.
+
+
+def log(*args):
+    print('[bench {:7.1f}s]'.format(time.time() - T0), *args, file=sys.stderr, flush=True)
+
+
+T0 = time.time()
+
+
+def parse_args():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=10)
+    p.add_argument('--warmup', type=int, default=2)
+    p.add_argument('--config', default='c3')
+    p.add_argument('--scale', type=float, default=1.0, help='fraction of the catalog sizes (tests only)')
+    p.add_argument('--fp64-steps', type=int, default=None, help='timed steps of the plain FP64 path')
+    p.add_argument('--nrefine', type=int, default=2)
+    p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-accuracy', action='store_true')
+    p.add_argument('--kernel-only', default=None, choices=['jacobi32', 'jacobi64', 'copy'],
+                   help='run only one kernel repeatedly (for rocprofv3 counter runs)')
+    p.add_argument('--workers', type=int, default=min(16, os.cpu_count() or 1))
+    return p.parse_args()
+
+
+def kernel_only(args):
+    """A short run of one kernel at the workload's finest-level shape (for --pmc profiles)."""
+    import pyrecon_amd as P
+    from catalogs import CONFIGS
+    from pyrecon_amd import _lib
+    cfg = CONFIGS[args.config]
+    n = cfg['nmesh']
+    shape = (n, n, n)
+    if args.kernel_only == 'copy':
+        nbytes = n**3 * 4
+        a, b = P.DeviceArray((nbytes,), 'u1'), P.DeviceArray((nbytes,), 'u1')
+        a.fill_zero()
+        for _ in range(args.steps):
+            _lib.check(_lib.lib.prh_copy_kernel(b.ptr, a.ptr, nbytes, None))
+        P.synchronize()
+        log('copy kernel: {} launches of {} bytes'.format(args.steps, nbytes))
+        return
+    dtype = 'f4' if args.kernel_only == 'jacobi32' else 'f8'
+    ws = P.Workspace(shape, dtype, flags=_lib.WS_MULTIGRID)
+    rng = np.random.default_rng(0)
+    v = P.DeviceArray.from_numpy(rng.standard_normal(shape).astype(dtype))
+    f = P.DeviceArray.from_numpy(rng.standard_normal(shape).astype(dtype))
+    nm, nm_ptr = _lib.size3(shape)
+    bs, bs_ptr = _lib.vec3(cfg['boxsize'], dtype)
+    bc, bc_ptr = _lib.vec3(cfg['boxcenter'], dtype)
+    fn = getattr(_lib.lib, 'prh_jacobi_' + ('f32' if dtype == 'f4' else 'f64'))
+    with _lib.KernelTimer(_lib.TIMER_JACOBI_FINEST) as kt:
+        _lib.check(fn(ws.handle, v.ptr, f.ptr, nm_ptr, bs_ptr, bc_ptr, 0.4, 0.4, args.steps, None, None))
+        ms, launches, nbytes = kt.read()
+    log('jacobi {}: {} launches, {:.4f} ms/launch, {:.1f} GB/s algorithmic'.format(
+        dtype, launches, ms / launches, nbytes / (ms * 1e-3) / 1e9))
+
+
+def cpu_baseline(cat, threads):
+    """Reference C kernels (oracle/_ref, built from the reference's own src/*.c) plus the
+    reference's numpy glue, timed on bounded samples of each stage of one reconstruction
+    and scaled to the full workload."""
+    from oracle.cpu import RefKernels, OracleKernels
+    from oracle.recon_numpy import OracleMultiGridReconstruction
+    kind = 'reference'
+    try:
+        kern = RefKernels()
+        kern.set_num_threads(threads)
+    except OSError:
+        kind, kern = 'port', OracleKernels()
+    os.environ['OMP_NUM_THREADS'] = str(threads)
+    cfg = cat['config']
+    n = cfg['nmesh']
+    recon = OracleMultiGridReconstruction(f=0.8, bias=2., los=None, nmesh=n, boxsize=cfg['boxsize'],
+                                          boxcenter=cfg['boxcenter'], dtype='f8', kernels=kern)
+    ntot = len(cat['data']) + len(cat['randoms'])
+    stages = {}
+    # 1. paint (serial C loop + numpy transform/checks): a sample of the randoms
+    ns = min(4000000, len(cat['randoms']))
+    t = time.perf_counter()
+    recon.assign_randoms(cat['randoms'][:ns], cat['randoms_weights'][:ns])
+    stages['paint'] = (time.perf_counter() - t) * ntot / ns
+    # 2. smoothing: numpy c2c transforms; one forward transform timed, 4 transforms per reconstruction
+    mesh = recon.mesh_randoms
+    t = time.perf_counter()
+    spec = np.fft.fftn(mesh)
+    t_fft = time.perf_counter() - t
+    del spec
+    stages['smoothing'] = 4 * t_fft
+    # 3. density contrast: full-mesh numpy passes on a 1/8 slab, scaled
+    slab = mesh[:n // 8]
+    t = time.perf_counter()
+    s_d, s_r = np.sum(slab), np.sum(slab)
+    alpha = s_d * 1. / s_r
+    delta = slab - alpha * slab
+    mask = slab > 0.01 * s_r / ns
+    delta[mask] /= (2. * alpha * slab[mask])
+    delta[~mask] = 0.
+    stages['density'] = (time.perf_counter() - t) * 8
+    # 4. multigrid: one finest-level V-cycle timed; FMG = 6 V-cycles per level, the
+    #    coarser levels add 1/7 of the finest-level cost (cell updates scale as 8^-l)
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((n, n, n))
+    v = np.zeros_like(f)
+    t = time.perf_counter()
+    kern.vcycle(v, f, cfg['boxsize'], cfg['boxcenter'], 0.4, 0.4, 5, None)
+    stages['multigrid'] = (time.perf_counter() - t) * 6 * 8. / 7.
+    del f, v
+    # 5. shifts: finite-difference read + numpy RSD algebra on a sample
+    recon.mesh_phi = mesh
+    t = time.perf_counter()
+    recon.read_shifted_positions(cat['randoms'][:ns])
+    stages['shifts'] = (time.perf_counter() - t) * ntot / ns
+    total = sum(stages.values())
+    return dict(value=1. / total, unit='recon/s', cores=threads, kind=kind,
+                sample=('one reconstruction of the same workload timed stage by stage on samples: paint and shifts '
+                        'of {:d} randoms scaled to {:d} points, one {n}^3 c2c FFT x4, density on a 1/8 slab x8, '
+                        'one {n}^3 V-cycle x6x8/7; reference src/*.c via oracle/_ref with {:d} OpenMP threads, '
+                        'numpy glue single-threaded').format(ns, ntot, threads, n=n),
+                stages_s={k: round(v, 3) for k, v in stages.items()})
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: every GPU box here has one GPU, so --gpus {} is not '
+              'supported (multi-GPU sharding is a design note, not a measured path).'.format(args.gpus),
+              file=sys.stderr)
+        sys.exit(2)
+    if args.kernel_only:
+        return kernel_only(args)
+    import pyrecon_amd as P
+    from pyrecon_amd import _lib
+    from pyrecon_amd.device import device_name
+    from catalogs import make_catalogs, RECON_KWARGS, SMOOTHING_RADIUS
+    log('device', device_name())
+    log('generating catalogs for', args.config)
+    cat = make_catalogs(args.config, scale=args.scale, workers=args.workers)
+    cfg = cat['config']
+    nd, nr = len(cat['data']), len(cat['randoms'])
+    log('catalogs: {:d} data, {:d} randoms'.format(nd, nr))
+    d, dw = P.DeviceArray.from_numpy(cat['data']), P.DeviceArray.from_numpy(cat['data_weights'])
+    r, rw = P.DeviceArray.from_numpy(cat['randoms']), P.DeviceArray.from_numpy(cat['randoms_weights'])
+    shape = (cfg['nmesh'],) * 3
+    ws = P.Workspace(shape, 'f8', paint_chunk=1 << 26)
+    log('workspace {:.2f} GB'.format(ws.nbytes / 1e9))
+    out_d, out_r = P.DeviceArray((nd, 3), 'f8'), P.DeviceArray((nr, 3), 'f8')
+    geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype='f8')
+
+    def step(precision, stages=None):
+        def mark(name):
+            if stages is not None:
+                P.synchronize()
+                stages[name] = stages.get(name, 0.) + time.perf_counter() - mark.t
+                mark.t = time.perf_counter()
+        mark.t = time.perf_counter()
+        recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws)
+        recon.assign_data(d, dw)
+        recon.assign_randoms(r, rw)
+        mark('paint')
+        recon.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+        mark('smoothing+density')
+        recon.run(precision=precision, nrefine=args.nrefine)
+        mark('multigrid')
+        recon.read_shifted_positions(d, out=out_d)
+        recon.read_shifted_positions(r, out=out_r)
+        mark('shifts')
+        return recon
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup): step(precision)
+        P.synchronize()
+        with _lib.KernelTimer(_lib.TIMER_JACOBI_FINEST) as kt:
+            t0 = time.perf_counter()
+            for _ in range(steps): step(precision)
+            P.synchronize()
+            elapsed = time.perf_counter() - t0
+            jac = kt.read()
+        return elapsed, jac
+
+    log('mixed: warmup {:d}, timed {:d}'.format(args.warmup, args.steps))
+    elapsed, jac = timed('mixed', args.steps, args.warmup)
+    value = args.steps / elapsed
+    log('mixed: {:.3f} recon/s ({:.1f} ms/step)'.format(value, 1e3 * elapsed / args.steps))
+    stages_mixed = {}
+    step('mixed', stages=stages_mixed)
+    nfp64 = args.fp64_steps if args.fp64_steps is not None else max(1, args.steps // 2)
+    log('plain fp64: timed {:d}'.format(nfp64))
+    elapsed64, jac64 = timed('plain', nfp64, 1)
+    value64 = nfp64 / elapsed64
+    log('plain fp64: {:.3f} recon/s ({:.1f} ms/step)'.format(value64, 1e3 * elapsed64 / nfp64))
+    stages_plain = {}
+    step('plain', stages=stages_plain)
+
+    accuracy = None
+    if not args.no_accuracy:
+        # mixed vs plain FP64 (the plain FP64 solve is bit-identical to the reference's C multigrid)
+        rp = step('plain')
+        phi_p, xd_p, xr_p = rp.mesh_phi.value, out_d.numpy(), out_r.numpy()
+        rm = step('mixed')
+        phi_m, xd_m, xr_m = rm.mesh_phi.value, out_d.numpy(), out_r.numpy()
+        a, b = phi_m - phi_m.mean(), phi_p - phi_p.mean()
+        dx = np.concatenate([np.abs(xd_m - xd_p).ravel(), np.abs(xr_m - xr_p).ravel()])
+        accuracy = dict(phi_rel_l2=float(np.sqrt(np.sum((a - b)**2) / np.sum(b**2))),
+                        shift_max_mpc_h=float(dx.max()), shift_rms_mpc_h=float(np.sqrt(np.mean(dx**2))),
+                        against='plain FP64 GPU path (bit-identical to the reference FP64 multigrid)')
+        log('accuracy', accuracy)
+        del phi_p, xd_p, xr_p, phi_m, xd_m, xr_m, a, b, dx
+
+    ms_j, nj, bytes_j = jac
+    achieved = bytes_j / (ms_j * 1e-3) / 1e9 if ms_j > 0 else 0.
+    traffic = None
+    tfile = os.path.join(ROOT, 'profiles', 'jacobi_f32_512_traffic.json')
+    if os.path.isfile(tfile):
+        with open(tfile) as fh: traffic = json.load(fh).get('hbm_bytes_per_launch')
+    roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    kernel='k_jacobi<float> damped-Jacobi sweep, finest level 512^3 (mixed path)',
+                    launches=nj, avg_launch_ms=round(ms_j / max(nj, 1), 5),
+                    bytes_per_launch=(bytes_j / nj) if nj else None)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log('cpu baseline')
+        cpu = cpu_baseline(cat, threads=args.workers)
+        log('cpu baseline', cpu)
+    ms_j64, nj64, bytes_j64 = jac64
+    line = dict(metric=METRIC, value=round(value, 4), unit='recon/s', n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=round(1e3 * elapsed / args.steps, 3), higher_is_better=True,
+                scaling='weak', vs_baseline=None, dtype='f32+f64', data='synthetic',
+                config=dict(workload='configuration 3: nmesh=512, boxsize=4000 Mpc/h, boxcenter=(0,0,3000), '
+                                     '{:d} data + {:d} randoms in the 1000<r<3000 shell, FKP weights, local LOS, '
+                                     'f=0.8, bias=2, R=15, 5 sweeps x 6 V-cycles'.format(nd, nr),
+                            nmesh=cfg['nmesh'], ndata=nd, nrandoms=nr, precision='mixed', nrefine=args.nrefine,
+                            inputs='device-resident'),
+                roofline=roofline, cpu_baseline=cpu,
+                fp64_plain=dict(value=round(value64, 4), unit='recon/s', ms_per_step=round(1e3 * elapsed64 / nfp64, 3),
+                                jacobi_avg_launch_ms=round(ms_j64 / max(nj64, 1), 5),
+                                jacobi_achieved_gbs=round(bytes_j64 / (ms_j64 * 1e-3) / 1e9, 1) if ms_j64 else None),
+                speedup_mixed_vs_fp64=round(value / value64, 3), accuracy=accuracy,
+                stages_ms=dict(mixed={k: round(1e3 * v, 2) for k, v in stages_mixed.items()},
+                               plain={k: round(1e3 * v, 2) for k, v in stages_plain.items()}),
+                serving=None)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -231,6 +231,22 @@ int prh_fmg_mixed(prh_workspace *ws, const double *f1h, double *v1h, const size_
                   const double *los, int nrefine, int vcycles_per_refine, double *residual_norms,
                   void *stream);
 
+/* ---- measurement (no reference counterpart) --------------------------------
+ * While a timer is enabled, every launch of that kernel class is bracketed by
+ * HIP events on its own stream (up to 65536 launches); prh_timer_read
+ * synchronizes and returns the summed device time, the number of launches
+ * and the algorithmic bytes they moved (e.g. 3 n^3 words per Jacobi sweep).
+ * Process-wide; not for concurrent use from several threads.              */
+#define PRH_TIMER_JACOBI_FINEST 0 /* damped-Jacobi sweeps on the finest level */
+#define PRH_TIMER_PAINT_GATHER 1  /* CIC paint merge-gather kernel */
+#define PRH_TIMER_READ_SHIFTS 2   /* fused finite-difference read + shift kernel */
+#define PRH_TIMER_COUNT 3
+int prh_timer_enable(int which, int enable);
+int prh_timer_reset(int which);
+int prh_timer_read(int which, double *total_ms, long long *launches, double *bytes);
+/* a plain device-to-device copy kernel (HBM traffic calibration for profiles) */
+int prh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
+
 /* RMS of r = f - A v in FP64 (diagnostics and tests); host result, syncs. */
 int prh_residual_norm_f64(prh_workspace *ws, const double *v, const double *f, const size_t *nmesh,
                           const double *boxsize, const double *boxcenter, double beta,

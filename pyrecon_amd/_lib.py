@@ -25,6 +25,7 @@ ERR_STATE = -8
 
 F32, F64 = 4, 8
 FIELD_DISP, FIELD_RSD, FIELD_DISP_RSD = 0, 1, 2
+TIMER_JACOBI_FINEST, TIMER_PAINT_GATHER, TIMER_READ_SHIFTS = 0, 1, 2
 
 WS_MULTIGRID, WS_PAINT, WS_FFT, WS_MIXED = 1, 2, 4, 8
 WS_ALL = 15
@@ -80,6 +81,10 @@ def _signatures():
         'prh_workspace_destroy': (_i, [_vp]),
         'prh_workspace_bytes': (_i, [_vp, _szp]),
         'prh_workspace_footprint': (_i, [_szp, _i, _sz, ctypes.c_uint, _szp]),
+        'prh_timer_enable': (_i, [_i, _i]),
+        'prh_timer_reset': (_i, [_i]),
+        'prh_timer_read': (_i, [_i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_d)]),
+        'prh_copy_kernel': (_i, [_vp, _vp, _sz, _vp]),
         'prh_fmg_mixed': (_i, [_vp, _vp, _vp, _szp, _vp, _vp, _d, _d, _i, _i, _vp, _i, _i, _vp, _vp]),
         'prh_residual_norm_f64': (_i, [_vp, _vp, _vp, _szp, _vp, _vp, _d, _vp, ctypes.POINTER(_d), _vp]),
     }
@@ -158,3 +163,24 @@ def device_count():
     status = lib.prh_device_count(ctypes.byref(n))
     if status != OK: return 0
     return n.value
+
+
+class KernelTimer(object):
+    """Device time of one kernel class (``TIMER_*``) over a region, with HIP events."""
+
+    def __init__(self, which):
+        self.which = int(which)
+
+    def __enter__(self):
+        check(lib.prh_timer_enable(self.which, 1), 'timer enable')
+        check(lib.prh_timer_reset(self.which), 'timer reset')
+        return self
+
+    def __exit__(self, *exc):
+        check(lib.prh_timer_enable(self.which, 0), 'timer disable')
+
+    def read(self):
+        """(total device ms, number of launches, algorithmic bytes); synchronizes."""
+        ms, n, nbytes = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        check(lib.prh_timer_read(self.which, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(nbytes)), 'timer read')
+        return ms.value, n.value, nbytes.value

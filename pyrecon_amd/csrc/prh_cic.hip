@@ -596,10 +596,15 @@ static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* p
   k_paint_order_long<<<256, kCicThreads, 0, s>>>(S.start, S.sorted, S.misc, S.big,
                                                  static_cast<uint32_t>(S.nbig), S.rank);
   PRH_LAUNCHED();
+  timer_before(PRH_TIMER_PAINT_GATHER, s);
   k_paint_gather<R><<<gcell, kCicThreads, 0, s>>>(mesh, F.n[0], F.n[1], F.n[2], S.start,
                                                   S.sorted,
                                                   static_cast<const Payload<R>*>(S.payload),
                                                   S.misc);
+  // algorithmic: cell starts + 8 contributions' payload per point + mesh read/write
+  timer_after(PRH_TIMER_PAINT_GATHER, s,
+              static_cast<double>(ncell) * (4 + 2 * sizeof(R)) +
+                  static_cast<double>(m) * (4 + sizeof(Payload<R>)));
   PRH_LAUNCHED();
   return PRH_OK;
 }
@@ -763,9 +768,14 @@ static int launch_read_shifts(const R* phi, const Frame<R>& F, const R cell[3], 
                               size_t n, double f, const R* los, int field, int wrap, int shifted,
                               void* out, uint32_t* misc, hipStream_t s) {
   const R l0 = los ? los[0] : R(0), l1 = los ? los[1] : R(0), l2 = los ? los[2] : R(0);
+  timer_before(PRH_TIMER_READ_SHIFTS, s);
   k_read_shifts<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(
       phi, F, cell[0], cell[1], cell[2], pos, n, f, l0, l1, l2, los ? 0 : 1, field, wrap,
       shifted, static_cast<O*>(out), misc);
+  // algorithmic: positions in, shifted positions out, the potential read once
+  timer_after(PRH_TIMER_READ_SHIFTS, s,
+              static_cast<double>(n) * 3 * (sizeof(P) + sizeof(O)) +
+                  static_cast<double>(F.n[0]) * F.n[1] * F.n[2] * sizeof(R));
   PRH_LAUNCHED();
   return PRH_OK;
 }

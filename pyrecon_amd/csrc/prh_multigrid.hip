@@ -186,13 +186,16 @@ static size_t ncells(const size_t* n) { return n[0] * n[1] * n[2]; }
 
 template <typename R>
 static int launch_jacobi(const R* v, const R* f, R* out, const LevelGeom<R>& G, R omega, R keep,
-                         bool zero_in, hipStream_t s) {
+                         bool zero_in, hipStream_t s, int timer = -1) {
   const size_t total = static_cast<size_t>(G.n[0]) * G.n[1] * G.n[2];
   const dim3 grid(ceil_div(total, kThreads));
+  timer_before(timer, s);
   if (zero_in)
     k_jacobi<R, true><<<grid, kThreads, 0, s>>>(v, f, out, G, omega, keep);
   else
     k_jacobi<R, false><<<grid, kThreads, 0, s>>>(v, f, out, G, omega, keep);
+  // algorithmic traffic: read v (unless zero) and f, write the new iterate
+  timer_after(timer, s, static_cast<double>(total) * sizeof(R) * (zero_in ? 2 : 3));
   PRH_LAUNCHED();
   return PRH_OK;
 }
@@ -265,7 +268,8 @@ struct Solver {
       return PRH_OK;
     }
     for (int it = 0; it < niter; it++) {
-      PRH_TRY(launch_jacobi(cur, src, alt, G[l], omega, keep, zero_start && it == 0, s));
+      PRH_TRY(launch_jacobi(cur, src, alt, G[l], omega, keep, zero_start && it == 0, s,
+                            l == 0 ? PRH_TIMER_JACOBI_FINEST : -1));
       std::swap(cur, alt);
     }
     return PRH_OK;

@@ -1,7 +1,9 @@
 // Runtime plumbing of the C ABI: errors, device memory, streams, events,
 // and the workspace (allocation of every buffer the hot calls use).
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
+#include <vector>
 
 #include "prh_workspace.hpp"
 
@@ -17,6 +19,44 @@ void set_error(const char* fmt, ...) {
 }
 
 const char* get_error() { return g_error; }
+
+// ---- opt-in kernel timers ---------------------------------------------------
+struct KernelTimer {
+  bool on = false;
+  size_t used = 0;
+  double bytes = 0.0;
+  std::vector<hipEvent_t> start, stop;
+};
+static KernelTimer g_timer[PRH_TIMER_COUNT];
+constexpr size_t kTimerPool = 16384;
+
+bool timer_on(int which) { return which >= 0 && which < PRH_TIMER_COUNT && g_timer[which].on; }
+
+void timer_before(int which, hipStream_t s) {
+  if (!timer_on(which)) return;
+  KernelTimer& T = g_timer[which];
+  if (T.used < T.start.size()) (void)hipEventRecord(T.start[T.used], s);
+}
+
+void timer_after(int which, hipStream_t s, double bytes) {
+  if (!timer_on(which)) return;
+  KernelTimer& T = g_timer[which];
+  if (T.used < T.start.size()) {
+    (void)hipEventRecord(T.stop[T.used], s);
+    T.used++;
+    T.bytes += bytes;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
+                                                uint4* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_copy1(const unsigned char* __restrict__ src,
+                                               unsigned char* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) dst[i] = src[i];
+}
 
 int ws_alloc(prh_workspace* ws, void** p, size_t bytes) {
   *p = nullptr;
@@ -393,6 +433,63 @@ int prh_workspace_create(prh_workspace** out, const size_t* nmesh, int dtype, si
 int prh_workspace_bytes(const prh_workspace* ws, size_t* bytes) {
   if (!ws || !bytes) return PRH_ERR_ARG;
   *bytes = ws->bytes;
+  return PRH_OK;
+}
+
+int prh_timer_enable(int which, int enable) {
+  if (which < 0 || which >= PRH_TIMER_COUNT) return PRH_ERR_ARG;
+  KernelTimer& T = g_timer[which];
+  if (enable && T.start.empty()) {
+    T.start.resize(kTimerPool);
+    T.stop.resize(kTimerPool);
+    for (size_t i = 0; i < kTimerPool; i++) {
+      PRH_HIP(hipEventCreate(&T.start[i]));
+      PRH_HIP(hipEventCreate(&T.stop[i]));
+    }
+  }
+  T.on = enable != 0;
+  return PRH_OK;
+}
+
+int prh_timer_reset(int which) {
+  if (which < 0 || which >= PRH_TIMER_COUNT) return PRH_ERR_ARG;
+  g_timer[which].used = 0;
+  g_timer[which].bytes = 0.0;
+  return PRH_OK;
+}
+
+int prh_timer_read(int which, double* total_ms, long long* launches, double* bytes) {
+  if (which < 0 || which >= PRH_TIMER_COUNT || !total_ms || !launches || !bytes) return PRH_ERR_ARG;
+  KernelTimer& T = g_timer[which];
+  double total = 0.0;
+  if (T.used) PRH_HIP(hipEventSynchronize(T.stop[T.used - 1]));
+  for (size_t i = 0; i < T.used; i++) {
+    float ms = 0.f;
+    PRH_HIP(hipEventElapsedTime(&ms, T.start[i], T.stop[i]));
+    total += ms;
+  }
+  *total_ms = total;
+  *launches = static_cast<long long>(T.used);
+  *bytes = T.bytes;
+  return PRH_OK;
+}
+
+int prh_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return PRH_OK;
+  if (!dst || !src) return PRH_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  const bool aligned = (reinterpret_cast<uintptr_t>(dst) % 16 == 0) &&
+                       (reinterpret_cast<uintptr_t>(src) % 16 == 0) && (bytes % 16 == 0);
+  if (aligned) {
+    const size_t n = bytes / 16;
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 65536));
+    k_copy16<<<grid, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+  } else {
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>((bytes + 255) / 256, 65536));
+    k_copy1<<<grid, 256, 0, s>>>(static_cast<const unsigned char*>(src),
+                                 static_cast<unsigned char*>(dst), bytes);
+  }
+  PRH_LAUNCHED();
   return PRH_OK;
 }
 

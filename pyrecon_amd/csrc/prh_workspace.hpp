@@ -69,4 +69,11 @@ struct prh_workspace {
 namespace prh {
 int ws_alloc(prh_workspace* ws, void** p, size_t bytes);
 constexpr unsigned kReduceBlocks = 1024;
+
+// Opt-in event timing of one kernel class (PRH_TIMER_*): when enabled, the
+// launch sites bracket each launch with a pair of HIP events on the launch
+// stream; prh_timer_read sums the elapsed times.
+bool timer_on(int which);
+void timer_before(int which, hipStream_t s);
+void timer_after(int which, hipStream_t s, double bytes);
 }  // namespace prh

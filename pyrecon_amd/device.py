@@ -45,6 +45,48 @@ def synchronize(stream=None):
     check(lib.prh_synchronize(stream_handle(stream)), 'synchronize')
 
 
+class _BlockCache(object):
+    """Exact-size cache of freed device blocks, so that repeated reconstructions of the
+    same shapes reuse their allocations instead of calling hipMalloc/hipFree each time.
+    Blocks are reused only after the owning array is released; callers that release
+    memory still used by queued kernels must synchronize first (the classes here
+    release arrays only after the work that reads them has been waited for)."""
+
+    def __init__(self, limit_bytes=64 << 30):
+        self.blocks = {}
+        self.cached = 0
+        self.limit = limit_bytes
+
+    def take(self, nbytes):
+        stack = self.blocks.get(nbytes)
+        if stack:
+            self.cached -= nbytes
+            return stack.pop()
+        return None
+
+    def give(self, ptr, nbytes):
+        if self.cached + nbytes > self.limit:
+            lib.prh_free(ptr)
+            return
+        self.blocks.setdefault(nbytes, []).append(ptr)
+        self.cached += nbytes
+
+    def empty(self):
+        for stack in self.blocks.values():
+            for ptr in stack: lib.prh_free(ptr)
+        self.blocks.clear()
+        self.cached = 0
+
+
+_cache = _BlockCache()
+
+
+def empty_cache():
+    """Return every cached device block to the HIP runtime."""
+    synchronize()
+    _cache.empty()
+
+
 class DeviceArray(object):
     """C-contiguous device array.
 
@@ -61,9 +103,16 @@ class DeviceArray(object):
         self.dtype = np.dtype(dtype)
         self.base = base
         if ptr is None:
-            p = ctypes.c_void_p()
-            check(lib.prh_malloc(ctypes.byref(p), self.nbytes), 'device allocation of {} bytes'.format(self.nbytes))
-            self.ptr = p.value or 0
+            cached = _cache.take(self.nbytes) if self.nbytes else None
+            if cached is None:
+                p = ctypes.c_void_p()
+                status = lib.prh_malloc(ctypes.byref(p), self.nbytes)
+                if status == _lib.ERR_NOMEM and _cache.cached:
+                    _cache.empty()
+                    status = lib.prh_malloc(ctypes.byref(p), self.nbytes)
+                check(status, 'device allocation of {} bytes'.format(self.nbytes))
+                cached = p.value or 0
+            self.ptr = cached
             self.owner = True if owner is None else owner
         else:
             self.ptr = int(ptr)
@@ -147,7 +196,7 @@ class DeviceArray(object):
 
     def free(self):
         if self.owner and self.ptr:
-            lib.prh_free(self.ptr)
+            _cache.give(self.ptr, self.nbytes)
         self.ptr = 0
         self.owner = False
 

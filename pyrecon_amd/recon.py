@@ -174,7 +174,7 @@ class BaseReconstruction(object):
             out = np.result_type(out, self.info.dtype)
         return np.dtype(out)
 
-    def _read(self, positions, field, shifted):
+    def _read(self, positions, field, shifted, out=None):
         field = field.lower()
         allowed_fields = ['disp', 'rsd', 'disp+rsd']
         if field not in allowed_fields:
@@ -189,7 +189,10 @@ class BaseReconstruction(object):
             pos, is_host = as_device(np.asarray(positions, dtype='f8'), stream=self.stream)
             pdtype = np.dtype('f8')
         odtype = self._shift_out_dtype(pdtype, field, shifted)
-        out = DeviceArray((len(pos), 3), odtype)
+        if out is None:
+            out = DeviceArray((len(pos), 3), odtype)
+        elif not (isinstance(out, DeviceArray) and out.shape == (len(pos), 3) and out.dtype == odtype):
+            raise ValueError('out must be a DeviceArray of shape ({:d}, 3) and dtype {}'.format(len(pos), odtype))
         code = {'disp': _lib.FIELD_DISP, 'rsd': _lib.FIELD_RSD, 'disp+rsd': _lib.FIELD_DISP_RSD}[field]
         dtype = self.info.dtype
         nm, nm_ptr = _lib.size3(self.info.shape)
@@ -209,13 +212,14 @@ class BaseReconstruction(object):
             warnings.warn('Some positions are NaN/inf')
         return out.numpy(stream=self.stream) if is_host else out
 
-    def read_shifts(self, positions, field='disp+rsd'):
-        """Displacements at ``positions`` (multigrid.py:137-158); numpy in -> numpy out, device in -> device out."""
-        return self._read(positions, field, shifted=False)
+    def read_shifts(self, positions, field='disp+rsd', out=None):
+        """Displacements at ``positions`` (multigrid.py:137-158); numpy in -> numpy out, device in -> device
+        out (written into ``out`` if given)."""
+        return self._read(positions, field, shifted=False, out=out)
 
-    def read_shifted_positions(self, positions, field='disp+rsd'):
+    def read_shifted_positions(self, positions, field='disp+rsd', out=None):
         """``positions - read_shifts(positions)``, wrapped if :attr:`wrap` (recon.py:304-325)."""
-        return self._read(positions, field, shifted=True)
+        return self._read(positions, field, shifted=True, out=out)
 
 
 def _make_property(name):

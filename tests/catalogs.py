@@ -91,20 +91,32 @@ def in_shell_box(x, boxsize, boxcenter, rmin=1000., rmax=3000.):
     return (r > rmin) & (r < rmax) & np.all((x >= lo) & (x < hi), axis=-1), r
 
 
-def shell_randoms(size, boxsize, boxcenter, seed=RANDOMS_SEED, batch=1 << 24):
-    """``size`` points uniform in the shell intersected with the box, thinned by n(r)/max n."""
-    rng = np.random.default_rng(seed)
+def shell_randoms(size, boxsize, boxcenter, seed=RANDOMS_SEED, batch=1 << 22, workers=8):
+    """``size`` points uniform in the shell intersected with the box, thinned by n(r)/max n.
+
+    Candidates are drawn in batches, each with its own generator spawned from ``seed``,
+    and accepted points are concatenated in batch order: the result does not depend on
+    the number of worker threads."""
+    from concurrent.futures import ThreadPoolExecutor
     offset = np.asarray(boxcenter, dtype='f8') - boxsize / 2.
-    out = np.empty((int(size), 3), dtype='f8')
-    filled = 0
-    while filled < size:
+    seeds = np.random.SeedSequence(seed)
+
+    def one(child):
+        rng = np.random.default_rng(child)
         x = rng.random((batch, 3)) * boxsize + offset
         keep, r = in_shell_box(x, boxsize, boxcenter)
         keep &= rng.random(batch) < shell_nbar(r) / 3e-4
-        x = x[keep]
-        take = min(len(x), size - filled)
-        out[filled:filled + take] = x[:take]
-        filled += take
+        return x[keep]
+
+    out = np.empty((int(size), 3), dtype='f8')
+    filled = 0
+    with ThreadPoolExecutor(max(int(workers), 1)) as pool:
+        while filled < size:
+            for x in pool.map(one, seeds.spawn(max(int(workers), 1))):
+                take = min(len(x), size - filled)
+                out[filled:filled + take] = x[:take]
+                filled += take
+                if filled >= size: break
     return out
 
 
@@ -150,7 +162,7 @@ def make_catalogs(name, scale=1., workers=8, with_randoms=True):
     out = dict(config=cfg, data=data, data_weights=data_weights)
     if with_randoms:
         if cfg['geometry'] == 'shell':
-            randoms = shell_randoms(nrandoms, boxsize, boxcenter)
+            randoms = shell_randoms(nrandoms, boxsize, boxcenter, workers=workers)
             randoms_weights = fkp_weights(np.sqrt(np.sum(randoms**2, axis=-1)))
         else:
             randoms = uniform_box(nrandoms, boxsize, boxcenter)
