@@ -46,7 +46,7 @@ def parse_args():
     p.add_argument('--config', default='c3')
     p.add_argument('--scale', type=float, default=1.0, help='fraction of the catalog sizes (tests only)')
     p.add_argument('--fp64-steps', type=int, default=None, help='timed steps of the plain FP64 path')
-    p.add_argument('--nrefine', type=int, default=2)
+    p.add_argument('--nrefine', type=int, default=0, help='extra FP64 corrections of the mixed path')
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-accuracy', action='store_true')
     p.add_argument('--kernel-only', default=None, choices=['jacobi32', 'jacobi64', 'copy'],

@@ -219,12 +219,16 @@ int prh_read_shifts_f64(prh_workspace *ws, const double *phi, const size_t *nmes
                         int shifted, void *out, int out_dtype, int *nonfinite, void *stream);
 
 /* Mixed-precision solve (no reference counterpart; same contract as
- * prh_fmg_f64): an FP32 full multigrid gives the first iterate, then
- * nrefine defect-correction steps r64 = f - A v (FP64), e = V-cycles in FP32
- * on r64 (vcycles_per_refine of them, zero start), v += e.  Requires a
- * workspace built with PRH_WS_MIXED.  If residual_norms is not NULL it
- * receives nrefine+1 FP64 RMS residual norms (host) and the call
- * synchronizes.                                                            */
+ * prh_fmg_f64, which it tracks).  The coarse stages of the full multigrid
+ * (levels below the finest) run in FP32; the vcycle_niterations
+ * finest-level V-cycles run as FP64 defect corrections: r = f - A v (FP64,
+ * stored FP32), e = vcycles_per_refine FP32 V-cycles on r from zero (fast
+ * FP32 kernels), v += e (FP64).  Since a V-cycle is affine in (v, f), this
+ * reproduces the FP64 iterate up to FP32 rounding of the corrections.
+ * nrefine extra corrections may follow.  Requires PRH_WS_MIXED.  If
+ * residual_norms is not NULL it receives vcycle_niterations + nrefine + 1
+ * FP64 RMS residual norms (host), one before each correction and one at the
+ * end, and the call synchronizes.                                          */
 int prh_fmg_mixed(prh_workspace *ws, const double *f1h, double *v1h, const size_t *nmesh,
                   const double *boxsize, const double *boxcenter, double beta,
                   double jacobi_damping_factor, int jacobi_niterations, int vcycle_niterations,

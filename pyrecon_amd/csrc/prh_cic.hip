@@ -208,30 +208,41 @@ __global__ __launch_bounds__(kCicThreads) void k_scan_partials(uint32_t* __restr
   if (threadIdx.x == 0) partial[np] = carry;
 }
 
+// Coalesced loads into LDS (padded by one word every 32 so that each thread's
+// 32 consecutive cells sit in distinct banks), a per-thread serial scan of
+// its 32 cells, a block scan of the thread totals, coalesced stores.
 __global__ __launch_bounds__(kCicThreads) void k_scan_final(uint32_t* __restrict__ count,
                                                             size_t n,
                                                             const uint32_t* __restrict__ partial,
                                                             uint32_t* __restrict__ start) {
+  __shared__ uint32_t tile[kScanBlock + kScanBlock / 32];
   __shared__ uint32_t lds[kCicThreads];
   const size_t base = blockIdx.x * static_cast<size_t>(kScanBlock);
-  // each thread owns kScanItems consecutive cells
-  uint32_t c[kScanItems];
-  uint32_t acc = 0;
-  const size_t mine = base + static_cast<size_t>(threadIdx.x) * kScanItems;
+  auto pad = [](int e) { return e + (e >> 5); };
   for (int k = 0; k < kScanItems; k++) {
-    const size_t i = mine + k;
-    c[k] = i < n ? count[i] : 0u;
-    acc += c[k];
+    const int e = k * kCicThreads + threadIdx.x;
+    const size_t i = base + e;
+    tile[pad(e)] = i < n ? count[i] : 0u;
   }
+  __syncthreads();
+  uint32_t acc = 0;
+  const int mine = threadIdx.x * kScanItems;
+  for (int k = 0; k < kScanItems; k++) acc += tile[pad(mine + k)];
   uint32_t total;
   uint32_t run = block_exclusive_scan(acc, lds, total) + partial[blockIdx.x];
   for (int k = 0; k < kScanItems; k++) {
-    const size_t i = mine + k;
+    const uint32_t c = tile[pad(mine + k)];
+    tile[pad(mine + k)] = run;
+    run += c;
+  }
+  __syncthreads();
+  for (int k = 0; k < kScanItems; k++) {
+    const int e = k * kCicThreads + threadIdx.x;
+    const size_t i = base + e;
     if (i < n) {
-      start[i] = run;
+      start[i] = tile[pad(e)];
       count[i] = 0u;
     }
-    run += c[k];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = partial[gridDim.x];
 }
@@ -322,13 +333,23 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_order_long(
   }
 }
 
+// Paint step 4c: payload into cell-grouped order, so that step 5 reads each
+// group contiguously.
+template <typename R>
+__global__ __launch_bounds__(kCicThreads) void k_paint_permute(const uint32_t* __restrict__ sorted,
+                                                               const Payload<R>* __restrict__ payload,
+                                                               Payload<R>* __restrict__ spay, size_t m) {
+  const size_t s = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (s < m) spay[s] = payload[sorted[s]];
+}
+
 // Paint step 5: per target cell, merge the groups of its 8 source base cells
 // by point index and accumulate.  Source k = (a, b, c) is the base cell at
 // (x - a, y - b, z - c): the target is that point's corner (a, b, c).
 template <typename R>
 __global__ __launch_bounds__(kCicThreads) void k_paint_gather(
     R* __restrict__ mesh, int n0, int n1, int n2, const uint32_t* __restrict__ start,
-    const uint32_t* __restrict__ sorted, const Payload<R>* __restrict__ payload,
+    const uint32_t* __restrict__ sorted, const Payload<R>* __restrict__ spay,
     const uint32_t* __restrict__ misc) {
   if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
   const size_t total = static_cast<size_t>(n0) * n1 * n2;
@@ -364,7 +385,7 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_gather(
       acc = mesh[c];
       any = true;
     }
-    const Payload<R> q = payload[best];
+    const Payload<R> q = spay[beg[bk]];
     const R fx = (bk & 4) ? q.t[0] : R(1) - q.t[0];
     const R fy = (bk & 2) ? q.t[1] : R(1) - q.t[1];
     const R fz = (bk & 1) ? q.t[2] : R(1) - q.t[2];
@@ -501,11 +522,12 @@ template <typename R, typename P, typename O>
 __global__ __launch_bounds__(kCicThreads) void k_read_shifts(
     const R* __restrict__ phi, Frame<R> F, R c0, R c1, R c2, const P* __restrict__ pos, size_t m,
     double fgrowth, R l0, R l1, R l2, int local, int field, int wrap, int shifted,
-    O* __restrict__ out, uint32_t* __restrict__ misc) {
+    O* __restrict__ out, uint32_t* __restrict__ misc, const uint32_t* __restrict__ order) {
   using Q = promote_t<P, R>;
-  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  const size_t t = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   uint32_t err = 0;
-  if (i < m) {
+  if (t < m) {
+    const size_t i = order ? static_cast<size_t>(order[t]) : t;
     const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
     R p[3];
     to_mesh_units<R, P>(x, F, wrap, p, err);
@@ -565,6 +587,33 @@ __global__ __launch_bounds__(kCicThreads) void k_read_shifts(
   }
 }
 
+// Read binning: a counting sort of the points by 8^3-cell tile (the order
+// inside a tile is irrelevant: every point is evaluated independently and
+// written at its own index), so that the reads of one wavefront touch a small
+// neighbourhood of the potential and hit in cache.
+template <typename R, typename P>
+__global__ __launch_bounds__(kCicThreads) void k_bin_key(const P* __restrict__ pos, size_t m,
+                                                         Frame<R> F, int wrap, int t1, int t2,
+                                                         uint32_t* __restrict__ key,
+                                                         uint32_t* __restrict__ rank,
+                                                         uint32_t* __restrict__ count) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (i >= m) return;
+  const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  R p[3];
+  uint32_t err = 0;
+  to_mesh_units<R, P>(x, F, wrap, p, err);
+  int t[3];
+  for (int d = 0; d < 3; d++) {
+    int ip = static_cast<int>(p[d]);
+    ip = ip < 0 ? 0 : (ip >= F.n[d] ? F.n[d] - 1 : ip);
+    t[d] = ip >> 3;
+  }
+  const uint32_t k = (static_cast<uint32_t>(t[0]) * t1 + t[1]) * t2 + t[2];
+  key[i] = k;
+  rank[i] = atomicAdd(&count[k], 1u);
+}
+
 // ---------------------------------------------------------------------------
 // Host drivers
 // ---------------------------------------------------------------------------
@@ -596,10 +645,14 @@ static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* p
   k_paint_order_long<<<256, kCicThreads, 0, s>>>(S.start, S.sorted, S.misc, S.big,
                                                  static_cast<uint32_t>(S.nbig), S.rank);
   PRH_LAUNCHED();
+  k_paint_permute<R><<<grid, kCicThreads, 0, s>>>(S.sorted,
+                                                  static_cast<const Payload<R>*>(S.payload),
+                                                  static_cast<Payload<R>*>(S.spay), m);
+  PRH_LAUNCHED();
   timer_before(PRH_TIMER_PAINT_GATHER, s);
   k_paint_gather<R><<<gcell, kCicThreads, 0, s>>>(mesh, F.n[0], F.n[1], F.n[2], S.start,
                                                   S.sorted,
-                                                  static_cast<const Payload<R>*>(S.payload),
+                                                  static_cast<const Payload<R>*>(S.spay),
                                                   S.misc);
   // algorithmic: cell starts + 8 contributions' payload per point + mesh read/write
   timer_after(PRH_TIMER_PAINT_GATHER, s,
@@ -766,17 +819,49 @@ static int do_read_fd(const R* mesh, const size_t* nmesh, const R* boxsize, cons
 template <typename R, typename P, typename O>
 static int launch_read_shifts(const R* phi, const Frame<R>& F, const R cell[3], const P* pos,
                               size_t n, double f, const R* los, int field, int wrap, int shifted,
-                              void* out, uint32_t* misc, hipStream_t s) {
+                              void* out, uint32_t* misc, hipStream_t s, const uint32_t* order) {
   const R l0 = los ? los[0] : R(0), l1 = los ? los[1] : R(0), l2 = los ? los[2] : R(0);
   timer_before(PRH_TIMER_READ_SHIFTS, s);
   k_read_shifts<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(
       phi, F, cell[0], cell[1], cell[2], pos, n, f, l0, l1, l2, los ? 0 : 1, field, wrap,
-      shifted, static_cast<O*>(out), misc);
+      shifted, static_cast<O*>(out), misc, order);
   // algorithmic: positions in, shifted positions out, the potential read once
   timer_after(PRH_TIMER_READ_SHIFTS, s,
               static_cast<double>(n) * 3 * (sizeof(P) + sizeof(O)) +
                   static_cast<double>(F.n[0]) * F.n[1] * F.n[2] * sizeof(R));
   PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+// Shifts of n points, binned by tile in chunks of the paint scratch size when
+// the workspace has one for this mesh, directly otherwise.
+template <typename R, typename P, typename O>
+static int read_shifts_all(prh_workspace* ws, const R* phi, const Frame<R>& F, const R cell[3],
+                           const P* pos, size_t n, double f, const R* los, int field, int wrap,
+                           int shifted, O* out, uint32_t* misc, hipStream_t s) {
+  const bool binned = (ws->flags & PRH_WS_PAINT) && n >= 65536 && ws->n[0] == size_t(F.n[0]) &&
+                      ws->n[1] == size_t(F.n[1]) && ws->n[2] == size_t(F.n[2]);
+  if (!binned) return launch_read_shifts<R, P, O>(phi, F, cell, pos, n, f, los, field, wrap, shifted, out, misc, s, nullptr);
+  PaintScratch& S = ws->paint;
+  const int t0 = (F.n[0] + 7) / 8, t1 = (F.n[1] + 7) / 8, t2 = (F.n[2] + 7) / 8;
+  const size_t ntile = static_cast<size_t>(t0) * t1 * t2;
+  const unsigned nscan = ceil_div(ntile, kScanBlock);
+  for (size_t o = 0; o < n; o += S.chunk) {
+    const size_t m = n - o < S.chunk ? n - o : S.chunk;
+    const unsigned grid = ceil_div(m, kCicThreads);
+    k_bin_key<R, P><<<grid, kCicThreads, 0, s>>>(pos + 3 * o, m, F, wrap, t1, t2, S.key, S.rank, S.count);
+    PRH_LAUNCHED();
+    k_scan_reduce<<<nscan, kCicThreads, 0, s>>>(S.count, ntile, S.partial);
+    PRH_LAUNCHED();
+    k_scan_partials<<<1, kCicThreads, 0, s>>>(S.partial, static_cast<int>(nscan));
+    PRH_LAUNCHED();
+    k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.count, ntile, S.partial, S.start);
+    PRH_LAUNCHED();
+    k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
+    PRH_LAUNCHED();
+    PRH_TRY((launch_read_shifts<R, P, O>(phi, F, cell, pos + 3 * o, m, f, los, field, wrap, shifted,
+                                         out + 3 * o, misc, s, S.sorted)));
+  }
   return PRH_OK;
 }
 
@@ -814,14 +899,14 @@ static int do_read_shifts(prh_workspace* ws, const R* phi, const size_t* nmesh, 
   fd_cells(nmesh, boxsize, cell);
   int st;
   if (pdt == PRH_F64) {
-    st = launch_read_shifts<R, double, double>(phi, F, cell, static_cast<const double*>(pos), n, f,
-                                               los, field, wrap, shifted, out, misc, s);
+    st = read_shifts_all<R, double, double>(ws, phi, F, cell, static_cast<const double*>(pos), n, f,
+                                            los, field, wrap, shifted, static_cast<double*>(out), misc, s);
   } else if (odt == PRH_F32) {
-    st = launch_read_shifts<R, float, float>(phi, F, cell, static_cast<const float*>(pos), n, f,
-                                             los, field, wrap, shifted, out, misc, s);
+    st = read_shifts_all<R, float, float>(ws, phi, F, cell, static_cast<const float*>(pos), n, f,
+                                          los, field, wrap, shifted, static_cast<float*>(out), misc, s);
   } else {
-    st = launch_read_shifts<R, float, double>(phi, F, cell, static_cast<const float*>(pos), n, f,
-                                              los, field, wrap, shifted, out, misc, s);
+    st = read_shifts_all<R, float, double>(ws, phi, F, cell, static_cast<const float*>(pos), n, f,
+                                           los, field, wrap, shifted, static_cast<double*>(out), misc, s);
   }
   PRH_TRY(st);
   uint32_t* host = ws->flag_host;

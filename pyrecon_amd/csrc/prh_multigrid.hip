@@ -10,6 +10,7 @@
 // as one pass per sweep into a second iterate (ping-pong) instead of the
 // reference's two passes through a temporary; the arithmetic per cell is the
 // same, so results are bit-identical.
+#include <type_traits>
 #include <utility>
 
 #include "prh_reduce.hpp"
@@ -26,15 +27,18 @@ __device__ __forceinline__ void unflatten(size_t i, int n1, int n2, int& ix, int
   ix = static_cast<int>(t / n1);
 }
 
+// Generic kernels: one thread per cell on a 3D grid (x = blockIdx.z), for any
+// shape; used for the coarse levels and for shapes the tiled kernel does not
+// cover.
 template <typename R, bool ZERO_IN>
 __global__ __launch_bounds__(kThreads) void k_jacobi(const R* __restrict__ v,
                                                      const R* __restrict__ f, R* __restrict__ out,
                                                      LevelGeom<R> G, R omega, R keep) {
-  const size_t total = static_cast<size_t>(G.n[0]) * G.n[1] * G.n[2];
-  const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
-  if (i >= total) return;
-  int ix, iy, iz;
-  unflatten(i, G.n[1], G.n[2], ix, iy, iz);
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int ix = blockIdx.z;
+  if (iz >= G.n[2] || iy >= G.n[1]) return;
+  const size_t i = (static_cast<size_t>(ix) * G.n[1] + iy) * G.n[2] + iz;
   const Coef<R> k = cell_coef(G, ix, iy, iz);
   const Hood<R> h = ZERO_IN ? zero_hood<R>() : gather_hood(v, G.n[0], G.n[1], G.n[2], ix, iy, iz);
   out[i] = jacobi_update(k, h, f[i], omega, keep, G.local);
@@ -44,14 +48,236 @@ template <typename R, typename O>
 __global__ __launch_bounds__(kThreads) void k_residual(const R* __restrict__ v,
                                                        const R* __restrict__ f,
                                                        O* __restrict__ r, LevelGeom<R> G) {
-  const size_t total = static_cast<size_t>(G.n[0]) * G.n[1] * G.n[2];
-  const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
-  if (i >= total) return;
-  int ix, iy, iz;
-  unflatten(i, G.n[1], G.n[2], ix, iy, iz);
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  const int ix = blockIdx.z;
+  if (iz >= G.n[2] || iy >= G.n[1]) return;
+  const size_t i = (static_cast<size_t>(ix) * G.n[1] + iy) * G.n[2] + iz;
   const Coef<R> k = cell_coef(G, ix, iy, iz);
   const Hood<R> h = gather_hood(v, G.n[0], G.n[1], G.n[2], ix, iy, iz);
   r[i] = static_cast<O>(residual_value(k, h, f[i], G.local));
+}
+
+// 2.5D-blocked kernel for large levels (n2 % 64 == 0, n1 % TY == 0).  A block
+// owns a TY x 64 tile of (y, z) and walks a run of x planes.  Planes x-1, x,
+// x+1 of the input (with a one-cell halo in y and z, periodic) sit in a ring
+// of 4 LDS slots; plane x+2 is fetched into registers while plane x is
+// computed, then stored into the free slot, so each input value is read from
+// HBM about once and every stencil read hits LDS.  256 threads: 64 along z,
+// 4 along y, TY/4 rows each.  The arithmetic per cell is the generic
+// kernels' (same bits).
+enum StencilOp { kOpJacobi = 0, kOpResidual = 1 };
+constexpr int kTileZ = 64;
+
+// Coefficients of a level passed to the blocked kernels: the geometry for a
+// local line of sight, the precomputed constants for a global one.
+template <typename R>
+struct TileCoef {
+  LevelGeom<R> G;
+  GlobalCoef<R> K;
+  R dinv;  // 1 / K.d (fast variant only)
+};
+
+template <typename R, typename O, int OP, int TY, bool LOCAL, bool FAST>
+__global__ __launch_bounds__(256) void k_stencil_tiled(const R* __restrict__ v,
+                                                       const R* __restrict__ f,
+                                                       O* __restrict__ out, TileCoef<R> T,
+                                                       R omega, R keep, int xlen) {
+  constexpr int W = kTileZ + 2, H = TY + 2, PLANE = W * H, KY = TY / 4;
+  constexpr int PER = (PLANE + 255) / 256;
+  __shared__ R lds[4 * PLANE];
+  const LevelGeom<R>& G = T.G;
+  const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+  const int tid = threadIdx.x, tz = tid & 63, ty = tid >> 6;
+  const int z0 = blockIdx.x * kTileZ, y0 = blockIdx.y * TY;
+  const int xb = blockIdx.z * xlen;
+  const int xe = xb + xlen < n0 ? xb + xlen : n0;
+  const size_t s0 = static_cast<size_t>(n1) * n2;
+  int off[PER];
+  bool has[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + 256 * q;
+    has[q] = e < PLANE;
+    const int row = e / W, col = e - (e / W) * W;
+    int gy = y0 - 1 + row;
+    gy += gy < 0 ? n1 : 0;
+    gy -= gy >= n1 ? n1 : 0;
+    int gz = z0 - 1 + col;
+    gz += gz < 0 ? n2 : 0;
+    gz -= gz >= n2 ? n2 : 0;
+    off[q] = gy * n2 + gz;
+  }
+  auto wrapx = [n0](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
+  auto slot = [](int x) { return (x + 4) & 3; };
+  for (int p = -1; p <= 1; p++) {
+    const R* src = v + static_cast<size_t>(wrapx(xb + p)) * s0;
+    R* dst = lds + slot(xb + p) * PLANE;
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+      if (has[q]) dst[tid + 256 * q] = src[off[q]];
+  }
+  // per-thread constants: rows iy_k = y0 + ty + 4k, column iz
+  const int iz = z0 + tz;
+  int rowoff[KY], lo[KY];
+  R py[KY], B[KY], pypz[KY];
+  R pz = R(0), C = R(0);
+  if (LOCAL) {
+    pz = static_cast<R>(iz) + G.org[2];
+    C = G.h2[2] * pz * pz;
+  }
+#pragma unroll
+  for (int k = 0; k < KY; k++) {
+    const int iy = y0 + ty + 4 * k;
+    rowoff[k] = iy * n2 + iz;
+    lo[k] = (ty + 4 * k + 1) * W + tz + 1;
+    if (LOCAL) {
+      py[k] = static_cast<R>(iy) + G.org[1];
+      B[k] = G.h2[1] * py[k] * py[k];
+      pypz[k] = py[k] * pz;
+    } else {
+      py[k] = R(0);
+      B[k] = R(0);
+      pypz[k] = T.K.pypz;
+    }
+  }
+  __syncthreads();
+  for (int x = xb; x < xe; x++) {
+    const bool more = x + 1 < xe;
+    R pre[PER];
+    if (more) {
+      const R* src = v + static_cast<size_t>(wrapx(x + 2)) * s0;
+#pragma unroll
+      for (int q = 0; q < PER; q++)
+        if (has[q]) pre[q] = src[off[q]];
+    }
+    const R* M = lds + slot(x - 1) * PLANE;
+    const R* P = lds + slot(x) * PLANE;
+    const R* N = lds + slot(x + 1) * PLANE;
+    const R* fx = f + static_cast<size_t>(x) * s0;
+    O* ox = out + static_cast<size_t>(x) * s0;
+    R px = R(0), A = R(0);
+    if (LOCAL) {
+      px = static_cast<R>(x) + G.org[0];
+      A = G.h2[0] * px * px;
+    }
+#pragma unroll
+    for (int k = 0; k < KY; k++) {
+      const int o = lo[k];
+      Hood<R> h;
+      h.c = P[o];
+      h.xp = N[o];
+      h.xm = M[o];
+      h.yp = P[o + W];
+      h.ym = P[o - W];
+      h.zp = P[o + 1];
+      h.zm = P[o - 1];
+      h.xpyp = N[o + W];
+      h.xmym = M[o - W];
+      h.xmyp = M[o + W];
+      h.xpym = N[o - W];
+      h.xpzp = N[o + 1];
+      h.xmzm = M[o - 1];
+      h.xmzp = M[o + 1];
+      h.xpzm = N[o - 1];
+      h.ypzp = P[o + W + 1];
+      h.ymzm = P[o - W - 1];
+      h.ymzp = P[o - W + 1];
+      h.ypzm = P[o + W - 1];
+      const R fv = fx[rowoff[k]];
+      R g, g2, cx, cy, cz, d, pxpy, pxpz, res;
+      if (LOCAL) {
+        pxpy = px * py[k];
+        pxpz = px * pz;
+        if constexpr (FAST) {
+          g = G.beta * rcp_fast(A + B[k] + C);
+          cx = fmaf(g * px, px, G.ih2[0]);
+          cy = fmaf(g * py[k], py[k], G.ih2[1]);
+          cz = fmaf(g * pz, pz, G.ih2[2]);
+          g2 = g * R(0.5);
+          d = R(2) * (cx + cy + cz);
+        } else {
+          g = G.beta / (A + B[k] + C);
+          cx = G.ih2[0] + g * px * px;
+          cy = G.ih2[1] + g * py[k] * py[k];
+          cz = G.ih2[2] + g * pz * pz;
+          g2 = g / R(2);
+          d = R(2) * (cx + cy + cz);
+        }
+      } else {
+        g = T.K.g;
+        g2 = T.K.g2;
+        cx = T.K.cx;
+        cy = T.K.cy;
+        cz = T.K.cz;
+        d = T.K.d;
+        pxpy = T.K.pxpy;
+        pxpz = T.K.pxpz;
+      }
+      if constexpr (FAST) {
+        if (OP == kOpJacobi) {
+          const R dinv = LOCAL ? rcp_fast(d) : T.dinv;
+          res = jacobi_fast<LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx, cy, cz,
+                                   dinv, omega, keep);
+        } else {
+          res = residual_fast<LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx, cy, cz,
+                                     d);
+        }
+      } else {
+        if (OP == kOpJacobi)
+          res = jacobi_split<R, LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx, cy,
+                                       cz, d, omega, keep);
+        else
+          res = residual_split<R, LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx,
+                                         cy, cz, d);
+      }
+      ox[rowoff[k]] = static_cast<O>(res);
+    }
+    if (more) {
+      R* dst = lds + slot(x + 2) * PLANE;
+#pragma unroll
+      for (int q = 0; q < PER; q++)
+        if (has[q]) dst[tid + 256 * q] = pre[q];
+    }
+    __syncthreads();
+  }
+}
+
+constexpr int kTileY = 16;
+
+inline bool tiled_ok(const int* n) { return n[2] % kTileZ == 0 && n[1] % kTileY == 0 && n[0] >= 1; }
+
+// x-run per block so that the grid has roughly 2048+ blocks
+inline int tiled_xlen(const int* n) {
+  const long tiles = static_cast<long>(n[2] / kTileZ) * (n[1] / kTileY);
+  long chunks = (2048 + tiles - 1) / tiles;
+  if (chunks > n[0]) chunks = n[0];
+  int xlen = static_cast<int>((n[0] + chunks - 1) / chunks);
+  return xlen < 4 ? (n[0] < 4 ? n[0] : 4) : xlen;
+}
+
+template <typename R, typename O, int OP, bool FAST>
+static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, R omega, R keep,
+                         hipStream_t s) {
+  const int xlen = tiled_xlen(G.n);
+  const dim3 grid(G.n[2] / kTileZ, G.n[1] / kTileY, (G.n[0] + xlen - 1) / xlen);
+  TileCoef<R> T;
+  T.G = G;
+  T.K = G.local ? GlobalCoef<R>() : GlobalCoef<R>::make(G);
+  T.dinv = G.local ? R(0) : R(1) / T.K.d;
+  if (G.local)
+    k_stencil_tiled<R, O, OP, kTileY, true, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
+  else
+    k_stencil_tiled<R, O, OP, kTileY, false, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
+}
+
+// Block shape of the generic kernels: up to 64 threads along z, the rest along y.
+inline void generic_shape(const int* n, dim3& block, dim3& grid) {
+  int bz = 1;
+  while (bz < n[2] && bz < 64) bz <<= 1;
+  const int by = kThreads / bz;
+  block = dim3(bz, by, 1);
+  grid = dim3((n[2] + bz - 1) / bz, (n[1] + by - 1) / by, n[0]);
 }
 
 // Residual with block partial sums of r^2 (deterministic) -- optionally also
@@ -79,9 +305,9 @@ __global__ __launch_bounds__(kReduceThreads) void k_residual_norm(const R* __res
 
 // Full weighting (src/multigrid.c:236-262): centre x8, faces x4, edges x2,
 // corners x1, summed in the reference's order, divided by 64.
-template <typename R>
+template <typename R, typename RO>
 __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
-                                                       R* __restrict__ c, int f0, int f1,
+                                                       RO* __restrict__ c, int f0, int f1,
                                                        int f2) {
   const int c0 = f0 / 2, c1 = f1 / 2, c2 = f2 / 2;
   const size_t total = static_cast<size_t>(c0) * c1 * c2;
@@ -111,14 +337,14 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
   s += u[xp + ym + zm];
   s += u[xm + ym + zm];
   // reference divides by the double 64.0; an exact power-of-two scaling
-  c[j] = s * R(0.015625);
+  c[j] = static_cast<RO>(s * R(0.015625));
 }
 
 // Trilinear prolongation from the coarse mesh (c0, c1, c2) to the fine mesh,
 // one thread per fine cell (src/multigrid.c:177-205), either overwriting the
 // fine value or adding to it (the V-cycle correction, src/multigrid.c:296).
-template <typename R, bool ADD>
-__global__ __launch_bounds__(kThreads) void k_prolong(const R* __restrict__ u,
+template <typename RC, typename R, bool ADD>
+__global__ __launch_bounds__(kThreads) void k_prolong(const RC* __restrict__ uc,
                                                       R* __restrict__ fine, int c0, int c1,
                                                       int c2) {
   const int F1 = 2 * c1, F2 = 2 * c2;
@@ -132,19 +358,20 @@ __global__ __launch_bounds__(kThreads) void k_prolong(const R* __restrict__ u,
   const size_t x0 = jx * s0, xp = (jx + 1 == c0 ? 0 : jx + 1) * s0;
   const size_t y0 = jy * s1, yp = (jy + 1 == c1 ? 0 : jy + 1) * s1;
   const size_t z0 = jz, zp = (jz + 1 == c2 ? 0 : jz + 1);
-  const R a = u[x0 + y0 + z0];
+  auto u = [uc](size_t idx) { return static_cast<R>(uc[idx]); };
+  const R a = u(x0 + y0 + z0);
   R val;
   switch (((X & 1) << 2) | ((Y & 1) << 1) | (Z & 1)) {
     case 0: val = a; break;
-    case 4: val = (a + u[xp + y0 + z0]) / R(2); break;
-    case 2: val = (a + u[x0 + yp + z0]) / R(2); break;
-    case 1: val = (a + u[x0 + y0 + zp]) / R(2); break;
-    case 6: val = (a + u[xp + y0 + z0] + u[x0 + yp + z0] + u[xp + yp + z0]) / R(4); break;
-    case 3: val = (a + u[x0 + yp + z0] + u[x0 + y0 + zp] + u[x0 + yp + zp]) / R(4); break;
-    case 5: val = (a + u[xp + y0 + z0] + u[x0 + y0 + zp] + u[xp + y0 + zp]) / R(4); break;
+    case 4: val = (a + u(xp + y0 + z0)) / R(2); break;
+    case 2: val = (a + u(x0 + yp + z0)) / R(2); break;
+    case 1: val = (a + u(x0 + y0 + zp)) / R(2); break;
+    case 6: val = (a + u(xp + y0 + z0) + u(x0 + yp + z0) + u(xp + yp + z0)) / R(4); break;
+    case 3: val = (a + u(x0 + yp + z0) + u(x0 + y0 + zp) + u(x0 + yp + zp)) / R(4); break;
+    case 5: val = (a + u(xp + y0 + z0) + u(x0 + y0 + zp) + u(xp + y0 + zp)) / R(4); break;
     default:
-      val = (a + u[xp + y0 + z0] + u[x0 + yp + z0] + u[x0 + y0 + zp] + u[xp + yp + z0] +
-             u[xp + y0 + zp] + u[x0 + yp + zp] + u[xp + yp + zp]) /
+      val = (a + u(xp + y0 + z0) + u(x0 + yp + z0) + u(x0 + y0 + zp) + u(xp + yp + z0) +
+             u(xp + y0 + zp) + u(x0 + yp + zp) + u(xp + yp + zp)) /
             R(8);
       break;
   }
@@ -186,14 +413,19 @@ static size_t ncells(const size_t* n) { return n[0] * n[1] * n[2]; }
 
 template <typename R>
 static int launch_jacobi(const R* v, const R* f, R* out, const LevelGeom<R>& G, R omega, R keep,
-                         bool zero_in, hipStream_t s, int timer = -1) {
+                         bool zero_in, hipStream_t s, int timer = -1, bool fast = false) {
   const size_t total = static_cast<size_t>(G.n[0]) * G.n[1] * G.n[2];
-  const dim3 grid(ceil_div(total, kThreads));
+  dim3 block, grid;
+  generic_shape(G.n, block, grid);
   timer_before(timer, s);
   if (zero_in)
-    k_jacobi<R, true><<<grid, kThreads, 0, s>>>(v, f, out, G, omega, keep);
+    k_jacobi<R, true><<<grid, block, 0, s>>>(v, f, out, G, omega, keep);
+  else if (tiled_ok(G.n) && fast)
+    launch_tiled<R, R, kOpJacobi, std::is_same<R, float>::value>(v, f, out, G, omega, keep, s);
+  else if (tiled_ok(G.n))
+    launch_tiled<R, R, kOpJacobi, false>(v, f, out, G, omega, keep, s);
   else
-    k_jacobi<R, false><<<grid, kThreads, 0, s>>>(v, f, out, G, omega, keep);
+    k_jacobi<R, false><<<grid, block, 0, s>>>(v, f, out, G, omega, keep);
   // algorithmic traffic: read v (unless zero) and f, write the new iterate
   timer_after(timer, s, static_cast<double>(total) * sizeof(R) * (zero_in ? 2 : 3));
   PRH_LAUNCHED();
@@ -201,31 +433,40 @@ static int launch_jacobi(const R* v, const R* f, R* out, const LevelGeom<R>& G, 
 }
 
 template <typename R, typename O>
-static int launch_residual(const R* v, const R* f, O* r, const LevelGeom<R>& G, hipStream_t s) {
-  const size_t total = static_cast<size_t>(G.n[0]) * G.n[1] * G.n[2];
-  k_residual<R, O><<<ceil_div(total, kThreads), kThreads, 0, s>>>(v, f, r, G);
+static int launch_residual(const R* v, const R* f, O* r, const LevelGeom<R>& G, hipStream_t s,
+                           bool fast = false) {
+  if (tiled_ok(G.n) && fast) {
+    launch_tiled<R, O, kOpResidual, std::is_same<R, float>::value && std::is_same<O, float>::value>(
+        v, f, r, G, R(0), R(0), s);
+  } else if (tiled_ok(G.n)) {
+    launch_tiled<R, O, kOpResidual, false>(v, f, r, G, R(0), R(0), s);
+  } else {
+    dim3 block, grid;
+    generic_shape(G.n, block, grid);
+    k_residual<R, O><<<grid, block, 0, s>>>(v, f, r, G);
+  }
   PRH_LAUNCHED();
   return PRH_OK;
 }
 
-template <typename R>
-static int launch_restrict(const R* fine, R* coarse, const size_t* nf, hipStream_t s) {
+template <typename R, typename RO>
+static int launch_restrict(const R* fine, RO* coarse, const size_t* nf, hipStream_t s) {
   const size_t total = (nf[0] / 2) * (nf[1] / 2) * (nf[2] / 2);
-  k_restrict<R><<<ceil_div(total, kThreads), kThreads, 0, s>>>(
+  k_restrict<R, RO><<<ceil_div(total, kThreads), kThreads, 0, s>>>(
       fine, coarse, static_cast<int>(nf[0]), static_cast<int>(nf[1]), static_cast<int>(nf[2]));
   PRH_LAUNCHED();
   return PRH_OK;
 }
 
-template <typename R>
-static int launch_prolong(const R* coarse, R* fine, const size_t* nc, bool add, hipStream_t s) {
+template <typename RC, typename R>
+static int launch_prolong(const RC* coarse, R* fine, const size_t* nc, bool add, hipStream_t s) {
   const size_t total = 8 * ncells(nc);
   const int c0 = static_cast<int>(nc[0]), c1 = static_cast<int>(nc[1]),
             c2 = static_cast<int>(nc[2]);
   if (add)
-    k_prolong<R, true><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
+    k_prolong<RC, R, true><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
   else
-    k_prolong<R, false><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
+    k_prolong<RC, R, false><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
   PRH_LAUNCHED();
   return PRH_OK;
 }
@@ -245,6 +486,7 @@ struct Solver {
   R omega, keep;
   int niter;
   hipStream_t s;
+  bool fast = false;  // fast FP32 kernels on the blocked levels (mixed-precision corrections)
 
   R* v(int l) { return static_cast<R*>(H->v[l]); }
   R* f(int l) { return static_cast<R*>(H->f[l]); }
@@ -269,7 +511,7 @@ struct Solver {
     }
     for (int it = 0; it < niter; it++) {
       PRH_TRY(launch_jacobi(cur, src, alt, G[l], omega, keep, zero_start && it == 0, s,
-                            l == 0 ? PRH_TIMER_JACOBI_FINEST : -1));
+                            l == 0 ? PRH_TIMER_JACOBI_FINEST : -1, fast));
       std::swap(cur, alt);
     }
     return PRH_OK;
@@ -279,7 +521,7 @@ struct Solver {
   int vcycle(int l, R*& cur, R*& alt, const R* src, bool zero_start) {
     PRH_TRY(smooth(l, cur, alt, src, zero_start));
     if (l + 1 < H->nlev) {
-      PRH_TRY((launch_residual<R, R>(cur, src, alt, G[l], s)));
+      PRH_TRY((launch_residual<R, R>(cur, src, alt, G[l], s, fast)));
       PRH_TRY(launch_restrict(alt, f(l + 1), H->n[l], s));
       R* cc = v(l + 1);
       R* ca = tmp(l + 1);
@@ -289,35 +531,37 @@ struct Solver {
     return smooth(l, cur, alt, src, false);
   }
 
-  // src/multigrid.c:303-328.  v0 is the caller's iterate (used as the start
-  // only when no coarser level exists), f0 the caller's source.
-  int fmg(const R* f0, R* v0, int ncycle) {
+  // src/multigrid.c:303-328, with level `top` as the finest level.  v0 is the
+  // caller's iterate (used as the start only when no coarser level exists),
+  // f0 the caller's source.
+  int fmg(const R* f0, R* v0, int ncycle, int top = 0) {
     const int nl = H->nlev;
     const R* src[kMaxLevels];
-    src[0] = f0;
-    for (int l = 1; l < nl; l++) {
+    src[top] = f0;
+    for (int l = top + 1; l < nl; l++) {
       PRH_TRY(launch_restrict(src[l - 1], f(l), H->n[l - 1], s));
       src[l] = f(l);
     }
     R* cur;
     R* alt;
-    if (nl == 1) {
+    if (nl == top + 1) {
       cur = v0;
-      alt = tmp(0);
+      alt = tmp(top);
     } else {
       cur = v(nl - 1);
       alt = tmp(nl - 1);
       PRH_TRY(launch_fill_zero(cur, size(nl - 1), s));
     }
     for (int c = 0; c < ncycle; c++) PRH_TRY(vcycle(nl - 1, cur, alt, src[nl - 1], false));
-    for (int l = nl - 2; l >= 0; l--) {
-      R* dst = (l == 0) ? v0 : v(l);
+    for (int l = nl - 2; l >= top; l--) {
+      R* dst = (l == top) ? v0 : v(l);
       PRH_TRY(launch_prolong(cur, dst, H->n[l + 1], false, s));
       cur = dst;
       alt = tmp(l);
       for (int c = 0; c < ncycle; c++) PRH_TRY(vcycle(l, cur, alt, src[l], false));
     }
-    if (cur != v0) PRH_HIP(hipMemcpyAsync(v0, cur, size(0) * sizeof(R), hipMemcpyDeviceToDevice, s));
+    if (cur != v0)
+      PRH_HIP(hipMemcpyAsync(v0, cur, size(top) * sizeof(R), hipMemcpyDeviceToDevice, s));
     return PRH_OK;
   }
 };
@@ -411,16 +655,24 @@ static int residual_rms(prh_workspace* ws, const double* v, const double* f, flo
   return PRH_OK;
 }
 
+// Mixed precision.  Every V-cycle of the multigrid is affine in (v, f), so
+// V(v, f) = v + V(0, f - A v): the FP64 iterate of the reference's full
+// multigrid can be advanced by an FP64 residual, a V-cycle on it in FP32 from
+// a zero start, and an FP64 update.  The coarse FMG stages (levels >= 1)
+// run entirely in FP32 (their errors are damped by the finest-level cycles);
+// the ncycle finest-level cycles run as FP64 defect corrections with FP32
+// V-cycles (fast kernels on the blocked levels); nextra further corrections
+// may follow.  The iterate tracks the FP64 multigrid's to FP32 rounding of
+// the corrections.
 static int do_fmg_mixed(prh_workspace* ws, const double* f, double* v, const size_t* nmesh,
                         const double* boxsize, const double* boxcenter, double beta, double damping,
-                        int niter, int ncycle, const double* los, int nrefine, int vpr,
+                        int niter, int ncycle, const double* los, int nextra, int vpr,
                         double* norms, void* stream) {
   PRH_TRY(check_ws(ws, nmesh, PRH_F64, PRH_WS_MULTIGRID | PRH_WS_MIXED));
-  if (!v || !f || !boxsize || !boxcenter || nrefine < 0 || vpr < 0) return PRH_ERR_ARG;
+  if (!v || !f || !boxsize || !boxcenter || nextra < 0 || vpr < 1 || ncycle < 0) return PRH_ERR_ARG;
   hipStream_t s = as_stream(stream);
   const size_t total = ncells(nmesh);
   const unsigned grid = ceil_div(total, kThreads);
-  // FP32 operator: the geometry cast to float, as the reference's FP32 build sees it
   float bs32[3], bc32[3], los32[3];
   for (int d = 0; d < 3; d++) {
     bs32[d] = static_cast<float>(boxsize[d]);
@@ -430,31 +682,31 @@ static int do_fmg_mixed(prh_workspace* ws, const double* f, double* v, const siz
   Solver<float> S;
   S.setup(&ws->mixed, bs32, bc32, static_cast<float>(beta), static_cast<float>(damping), niter,
           los ? los32 : nullptr, s);
+  S.fast = true;
   const LevelGeom<double> G64 = LevelGeom<double>::make(nmesh, boxsize, boxcenter, beta, los);
-  float* f32 = S.f(0);
+  float* r32 = S.f(0);
   float* e32 = S.v(0);
-  // 1. FP32 full multigrid on the FP32 source
-  k_cast_f64_f32<<<grid, kThreads, 0, s>>>(f, f32, total);
-  PRH_LAUNCHED();
-  PRH_TRY(launch_fill_zero(e32, total, s));
-  PRH_TRY(S.fmg(f32, e32, ncycle));
-  k_cast_f32_f64<<<grid, kThreads, 0, s>>>(e32, v, total);
-  PRH_LAUNCHED();
-  // 2. defect correction: r = f - A v in FP64 (stored as FP32), e = V-cycles(r), v += e
-  for (int it = 0; it < nrefine; it++) {
+  // 1. coarse FMG stages in FP32, prolonged into the FP64 iterate
+  if (S.H->nlev > 1) {
+    PRH_TRY((launch_restrict<double, float>(f, S.f(1), nmesh, s)));
+    PRH_TRY(S.fmg(S.f(1), S.v(1), ncycle, 1));
+    PRH_TRY((launch_prolong<float, double>(S.v(1), v, S.H->n[1], false, s)));
+  }
+  // 2. finest-level cycles as FP64 defect corrections
+  const int ncorr = ncycle + nextra;
+  for (int it = 0; it < ncorr; it++) {
     if (norms) {
-      PRH_TRY(residual_rms(ws, v, f, f32, G64, &norms[it], s));
+      PRH_TRY(residual_rms(ws, v, f, r32, G64, &norms[it], s));
     } else {
-      PRH_TRY((launch_residual<double, float>(v, f, f32, G64, s)));
+      PRH_TRY((launch_residual<double, float>(v, f, r32, G64, s)));
     }
     float* cur = e32;
     float* alt = S.tmp(0);
-    for (int c = 0; c < vpr; c++) PRH_TRY(S.vcycle(0, cur, alt, f32, c == 0));
-    if (vpr == 0) PRH_TRY(launch_fill_zero(cur, total, s));
+    for (int c = 0; c < vpr; c++) PRH_TRY(S.vcycle(0, cur, alt, r32, c == 0));
     k_add_f32_f64<<<grid, kThreads, 0, s>>>(v, cur, total);
     PRH_LAUNCHED();
   }
-  if (norms) PRH_TRY(residual_rms(ws, v, f, nullptr, G64, &norms[nrefine], s));
+  if (norms) PRH_TRY(residual_rms(ws, v, f, nullptr, G64, &norms[ncorr], s));
   return PRH_OK;
 }
 
@@ -490,20 +742,20 @@ int prh_residual_f64(const double* v, const double* f, double* r, const size_t* 
 
 int prh_prolong_f32(const float* v2h, float* v1h, const size_t* nmesh, void* stream) {
   if (!v2h || !v1h || !nmesh) return PRH_ERR_ARG;
-  return launch_prolong<float>(v2h, v1h, nmesh, false, as_stream(stream));
+  return launch_prolong<float, float>(v2h, v1h, nmesh, false, as_stream(stream));
 }
 int prh_prolong_f64(const double* v2h, double* v1h, const size_t* nmesh, void* stream) {
   if (!v2h || !v1h || !nmesh) return PRH_ERR_ARG;
-  return launch_prolong<double>(v2h, v1h, nmesh, false, as_stream(stream));
+  return launch_prolong<double, double>(v2h, v1h, nmesh, false, as_stream(stream));
 }
 
 int prh_reduce_f32(const float* v1h, float* v2h, const size_t* nmesh, void* stream) {
   if (!v1h || !v2h || !nmesh) return PRH_ERR_ARG;
-  return launch_restrict<float>(v1h, v2h, nmesh, as_stream(stream));
+  return launch_restrict<float, float>(v1h, v2h, nmesh, as_stream(stream));
 }
 int prh_reduce_f64(const double* v1h, double* v2h, const size_t* nmesh, void* stream) {
   if (!v1h || !v2h || !nmesh) return PRH_ERR_ARG;
-  return launch_restrict<double>(v1h, v2h, nmesh, as_stream(stream));
+  return launch_restrict<double, double>(v1h, v2h, nmesh, as_stream(stream));
 }
 
 int prh_vcycle_f32(prh_workspace* ws, float* v, const float* f, const size_t* nmesh,

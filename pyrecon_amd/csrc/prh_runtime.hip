@@ -106,7 +106,7 @@ static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags)
   if (flags & PRH_WS_PAINT) {
     const size_t nc = cells(n);
     const size_t npart = (nc + 8191) / 8192;
-    p.paint = nc * 4 + (nc + 1) * 4 + (npart + 1) * 4 + chunk * 4 * 3 + chunk * 4 * w + 4 * 4 +
+    p.paint = nc * 4 + (nc + 1) * 4 + (npart + 1) * 4 + chunk * 4 * 3 + 2 * chunk * 4 * w + 4 * 4 +
               (chunk / 32 + 2) * 4;
   }
   if (flags & PRH_WS_FFT) {
@@ -130,14 +130,22 @@ static int fft_setup(prh_workspace* ws) {
     return PRH_ERR_FFT;
   }
   F.have = true;
-  hipfftSetAutoAllocation(F.r2c, 0);
-  hipfftSetAutoAllocation(F.c2r, 0);
+  const char* policy = getenv("PRH_FFT_WORK");
+  const bool autoalloc = policy && !strcmp(policy, "auto");
+  if (!autoalloc) {
+    hipfftSetAutoAllocation(F.r2c, 0);
+    hipfftSetAutoAllocation(F.c2r, 0);
+  }
   if (hipfftMakePlan3d(F.r2c, nx, ny, nz, dbl ? HIPFFT_D2Z : HIPFFT_R2C, &w1) != HIPFFT_SUCCESS ||
       hipfftMakePlan3d(F.c2r, nx, ny, nz, dbl ? HIPFFT_Z2D : HIPFFT_C2R, &w2) != HIPFFT_SUCCESS) {
     set_error("hipfftMakePlan3d(%d, %d, %d) failed", nx, ny, nz);
     return PRH_ERR_FFT;
   }
+  if (getenv("PRH_FFT_DEBUG"))
+    fprintf(stderr, "fft plan %d %d %d dtype %d work %zu %zu\n", nx, ny, nz, ws->dtype, w1, w2);
+  if (autoalloc) return PRH_OK;
   F.work_bytes = w1 > w2 ? w1 : w2;
+  if (policy && !strcmp(policy, "big") && F.work_bytes < 2 * spec) F.work_bytes = 2 * spec;
   if (F.work_bytes) {
     PRH_TRY(ws_alloc(ws, &F.work, F.work_bytes));
     if (hipfftSetWorkArea(F.r2c, F.work) != HIPFFT_SUCCESS ||
@@ -176,6 +184,7 @@ static int paint_alloc(prh_workspace* ws, size_t chunk) {
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.rank = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.sorted = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &P.payload, chunk * 4 * static_cast<size_t>(ws->dtype)));
+  PRH_TRY(ws_alloc(ws, &P.spay, chunk * 4 * static_cast<size_t>(ws->dtype)));
   PRH_TRY(ws_alloc(ws, &p, 4 * 4)); P.misc = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, P.nbig * 4)); P.big = static_cast<uint32_t*>(p);
   PRH_HIP(hipMemset(P.count, 0, P.ncell * 4));
@@ -352,7 +361,7 @@ int prh_workspace_destroy(prh_workspace* ws) {
     }
   PaintScratch& P = ws->paint;
   for (void* p : {(void*)P.count, (void*)P.start, (void*)P.partial, (void*)P.key, (void*)P.rank,
-                  (void*)P.sorted, P.payload, (void*)P.misc, (void*)P.big})
+                  (void*)P.sorted, P.payload, P.spay, (void*)P.misc, (void*)P.big})
     release(p);
   if (ws->fft.have) {
     hipfftDestroy(ws->fft.r2c);

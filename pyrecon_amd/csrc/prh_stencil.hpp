@@ -86,6 +86,106 @@ __device__ __forceinline__ R residual_value(const Coef<R>& k, const Hood<R>& h, 
   return f - av;
 }
 
+// ---- split-coefficient forms used by the 2.5D-blocked kernels -------------
+// The per-cell coefficients factor into parts that depend on x only (px,
+// A = (h0^2 px) px), on y only (py, B), on z only (pz, C) and on (y, z)
+// (py pz).  g = beta / ((A + B) + C) and c_i = 1/h_i^2 + (g p_i) p_i then
+// round exactly like cell_coef().  For a global line of sight every
+// coefficient is a constant of the level, computed once on the host with the
+// same expressions (GlobalCoef::make).
+template <typename R>
+struct GlobalCoef {
+  R g, g2, cx, cy, cz, d, pxpy, pxpz, pypz;
+  static GlobalCoef make(const LevelGeom<R>& G) {
+    GlobalCoef k;
+    const R px = G.losc[0], py = G.losc[1], pz = G.losc[2];
+    k.g = G.beta / (G.h2[0] * px * px + G.h2[1] * py * py + G.h2[2] * pz * pz);
+    k.cx = G.ih2[0] + k.g * px * px;
+    k.cy = G.ih2[1] + k.g * py * py;
+    k.cz = G.ih2[2] + k.g * pz * pz;
+    k.g2 = k.g / R(2);
+    k.d = R(2) * (k.cx + k.cy + k.cz);
+    k.pxpy = px * py;
+    k.pxpz = px * pz;
+    k.pypz = py * pz;
+    return k;
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ void edge_sums(const Hood<R>& h, R& e1, R& e2, R& e3) {
+  e1 = h.xpyp + h.xmym - h.xmyp - h.xpym;
+  e2 = h.xpzp + h.xmzm - h.xmzp - h.xpzm;
+  e3 = h.ypzp + h.ymzm - h.ymzp - h.ypzm;
+}
+
+// Exact forms (same rounding as jacobi_update / residual_value).
+template <typename R, bool LOCAL>
+__device__ __forceinline__ R jacobi_split(const Hood<R>& h, R f, R px, R py, R pz, R pxpy, R pxpz,
+                                          R pypz, R g, R g2, R cx, R cy, R cz, R d, R omega,
+                                          R keep) {
+  R e1, e2, e3;
+  edge_sums(h, e1, e2, e3);
+  const R t0 = cx * (h.xp + h.xm);
+  const R t1 = cy * (h.yp + h.ym);
+  const R t2 = cz * (h.zp + h.zm);
+  const R t3 = g2 * (pxpy * e1 + pxpz * e2 + pypz * e3);
+  R jac = f + t0 + t1 + t2 + t3;
+  if (LOCAL) jac += g * (px * (h.xp - h.xm) + py * (h.yp - h.ym) + pz * (h.zp - h.zm));
+  jac /= d;
+  return keep * h.c + omega * jac;
+}
+
+template <typename R, bool LOCAL>
+__device__ __forceinline__ R residual_split(const Hood<R>& h, R f, R px, R py, R pz, R pxpy,
+                                            R pxpz, R pypz, R g, R g2, R cx, R cy, R cz, R d) {
+  R e1, e2, e3;
+  edge_sums(h, e1, e2, e3);
+  const R t0 = cx * (h.xp + h.xm);
+  const R t1 = cy * (h.yp + h.ym);
+  const R t2 = cz * (h.zp + h.zm);
+  const R t3 = g2 * (pxpy * e1 + pxpz * e2 + pypz * e3);
+  R av = d * h.c - (t0 + t1 + t2 + t3);
+  if (LOCAL) av -= g * (px * (h.xp - h.xm) + py * (h.yp - h.ym) + pz * (h.zp - h.zm));
+  return f - av;
+}
+
+// Fast FP32 forms for the correction solves of the mixed-precision path:
+// fused multiply-adds and the hardware reciprocal (1 ulp) instead of
+// correctly rounded division.  Not bit-compatible with the reference; the
+// FP64 defect-correction loop around them absorbs their rounding.
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <bool LOCAL>
+__device__ __forceinline__ float jacobi_fast(const Hood<float>& h, float f, float px, float py,
+                                             float pz, float pxpy, float pxpz, float pypz, float g,
+                                             float g2, float cx, float cy, float cz, float dinv,
+                                             float omega, float keep) {
+  float e1, e2, e3;
+  edge_sums(h, e1, e2, e3);
+  float s = fmaf(cx, h.xp + h.xm, f);
+  s = fmaf(cy, h.yp + h.ym, s);
+  s = fmaf(cz, h.zp + h.zm, s);
+  s = fmaf(g2, fmaf(pxpy, e1, fmaf(pxpz, e2, pypz * e3)), s);
+  if (LOCAL) s = fmaf(g, fmaf(px, h.xp - h.xm, fmaf(py, h.yp - h.ym, pz * (h.zp - h.zm))), s);
+  return fmaf(keep, h.c, omega * (s * dinv));
+}
+
+template <bool LOCAL>
+__device__ __forceinline__ float residual_fast(const Hood<float>& h, float f, float px, float py,
+                                               float pz, float pxpy, float pxpz, float pypz,
+                                               float g, float g2, float cx, float cy, float cz,
+                                               float d) {
+  float e1, e2, e3;
+  edge_sums(h, e1, e2, e3);
+  float s = fmaf(cx, h.xp + h.xm, f);
+  s = fmaf(cy, h.yp + h.ym, s);
+  s = fmaf(cz, h.zp + h.zm, s);
+  s = fmaf(g2, fmaf(pxpy, e1, fmaf(pxpz, e2, pypz * e3)), s);
+  if (LOCAL) s = fmaf(g, fmaf(px, h.xp - h.xm, fmaf(py, h.yp - h.ym, pz * (h.zp - h.zm))), s);
+  return fmaf(-d, h.c, s);
+}
+
 // Gather the neighbourhood of (ix, iy, iz) from a periodic C-order mesh.
 template <typename R>
 __device__ __forceinline__ Hood<R> gather_hood(const R* __restrict__ v, int n0, int n1, int n2,

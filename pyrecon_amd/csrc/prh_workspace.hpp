@@ -33,7 +33,8 @@ struct PaintScratch {
   uint32_t* key = nullptr;      // chunk      base cell of each point
   uint32_t* rank = nullptr;     // chunk      arrival rank within its cell; reused as sort scratch
   uint32_t* sorted = nullptr;   // chunk      point indices grouped by cell, ascending within a cell
-  void* payload = nullptr;      // chunk      (t_x, t_y, t_z, w) in the mesh precision
+  void* payload = nullptr;      // chunk      (t_x, t_y, t_z, w) in the mesh precision, point order
+  void* spay = nullptr;         // chunk      the same payload in cell-grouped order
   uint32_t* misc = nullptr;     // [0] error flags, [1] number of long segments
   uint32_t* big = nullptr;      // long-segment cell list (ncell worst case is bounded by chunk/32)
   size_t nbig = 0;

@@ -42,17 +42,18 @@ class OriginalMultiGridReconstruction(BaseReconstruction):
         super(OriginalMultiGridReconstruction, self).__init__(*args, **kwargs)
 
     def run(self, jacobi_damping_factor=0.4, jacobi_niterations=5, vcycle_niterations=6, precision='plain',
-            nrefine=2, vcycles_per_refine=1, residual_norms=False):
+            nrefine=0, vcycles_per_refine=1, residual_norms=False):
         """Solve for the displacement potential :attr:`mesh_phi` (pyrecon/multigrid.py:99-135).
 
         Parameters
         ----------
         jacobi_damping_factor, jacobi_niterations, vcycle_niterations : as the reference.
         precision : 'plain' (the reference's algorithm in the mesh dtype) or 'mixed'
-            (FP64 meshes: FP32 full multigrid, then ``nrefine`` FP64 defect corrections of
-            ``vcycles_per_refine`` FP32 V-cycles each).
-        residual_norms : if True (mixed only), store the FP64 RMS residual after each stage in
-            :attr:`residual_norms` (costs one synchronisation per stage).
+            (FP64 meshes: the coarse full-multigrid stages in FP32, then the
+            ``vcycle_niterations`` finest-level V-cycles as FP64 defect corrections with FP32
+            V-cycles, tracking the plain FP64 iterate; ``nrefine`` extra corrections may follow).
+        residual_norms : if True (mixed only), store the FP64 RMS residual before each finest-level
+            correction and at the end in :attr:`residual_norms` (one synchronisation each).
         """
         if getattr(self, 'mesh_delta', None) is None:
             raise ReconstructionError('set the density contrast before running')
@@ -75,7 +76,7 @@ class OriginalMultiGridReconstruction(BaseReconstruction):
         elif precision == 'mixed':
             if dtype != np.float64:
                 raise ValueError('precision="mixed" needs an FP64 mesh (dtype="f8")')
-            norms = (ctypes.c_double * (int(nrefine) + 1))() if residual_norms else None
+            norms = (ctypes.c_double * (int(vcycle_niterations) + int(nrefine) + 1))() if residual_norms else None
             check(lib.prh_fmg_mixed(self.workspace.handle, delta.ptr, phi.buffer.ptr, nm_ptr, bs_ptr, bc_ptr,
                                     float(self.beta), float(jacobi_damping_factor), int(jacobi_niterations),
                                     int(vcycle_niterations), los_ptr, int(nrefine), int(vcycles_per_refine),

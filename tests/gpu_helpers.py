@@ -30,7 +30,7 @@ class GPU(object):
             raise RuntimeError('no HIP device visible: GPU tests need an MI355X')
         self._ws = {}
 
-    def workspace(self, shape, dtype, flags=_lib.WS_ALL, chunk=1 << 16):
+    def workspace(self, shape, dtype, flags=_lib.WS_MULTIGRID, chunk=1 << 16):
         key = (tuple(shape), np.dtype(dtype).str, flags, chunk)
         if key not in self._ws:
             self._ws[key] = Workspace(shape, dtype, paint_chunk=chunk, flags=flags)
@@ -103,11 +103,11 @@ class GPU(object):
         return v
 
     def fmg_mixed(self, f, boxsize, boxcenter, beta, damping_factor=0.4, niterations=5, vcycle_niterations=6,
-                  los=None, nrefine=2, vcycles_per_refine=1):
-        ws = self.workspace(f.shape, np.float64)
+                  los=None, nrefine=0, vcycles_per_refine=1):
+        ws = self.workspace(f.shape, np.float64, flags=_lib.WS_MULTIGRID | _lib.WS_MIXED)
         dv, df = DeviceArray.zeros(f.shape, np.float64), DeviceArray.from_numpy(f)
         keep, nm, bs, bc, lp = self._geom(f.shape, boxsize, boxcenter, los, np.float64)
-        norms = (ctypes.c_double * (nrefine + 1))()
+        norms = (ctypes.c_double * (vcycle_niterations + nrefine + 1))()
         check(lib.prh_fmg_mixed(ws.handle, df.ptr, dv.ptr, nm, bs, bc, beta, damping_factor, int(niterations),
                                 int(vcycle_niterations), lp, int(nrefine), int(vcycles_per_refine), norms, None),
               'fmg_mixed')
@@ -124,7 +124,7 @@ class GPU(object):
 
     def assign_cic(self, mesh, positions, weights, chunk=1 << 16):
         dtype = mesh.dtype
-        ws = self.workspace(mesh.shape, dtype, chunk=chunk)
+        ws = self.workspace(mesh.shape, dtype, flags=_lib.WS_PAINT, chunk=chunk)
         dm = DeviceArray.from_numpy(mesh)
         dp = DeviceArray.from_numpy(np.ascontiguousarray(positions, dtype=dtype))
         dw = DeviceArray.from_numpy(np.ascontiguousarray(weights, dtype=dtype))

@@ -11,7 +11,9 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(8, 8, 8), (8, 12, 16), (16, 16, 16), (6, 10, 12), (32, 32, 32), (24, 40, 20)]
+SHAPES = [(8, 8, 8), (8, 12, 16), (16, 16, 16), (6, 10, 12), (32, 32, 32), (24, 40, 20),
+          # shapes taking the 2.5D-blocked kernel (n2 % 64 == 0, n1 % 16 == 0), incl. a partial x run
+          (6, 16, 64), (10, 32, 128), (64, 64, 64)]
 LOS = [None, [0., 0., 1.], [0.3, 0.4, np.sqrt(1 - 0.25)]]
 BOX = dict(boxsize=np.array([1000., 1200., 900.]), boxcenter=np.array([0., 100., 1500.]))
 
