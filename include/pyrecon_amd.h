@@ -69,6 +69,8 @@ int prh_event_create(void **event);
 int prh_event_destroy(void *event);
 int prh_event_record(void *event, void *stream);
 int prh_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronizes on stop */
+int prh_event_synchronize(void *event);
+int prh_stream_wait_event(void *stream, void *event); /* later work on stream waits for event */
 
 /* ---- workspace ------------------------------------------------------------
  * Owns everything a reconstruction of one mesh shape needs besides the
@@ -198,6 +200,65 @@ int prh_density_contrast_f32(const float *data, const float *randoms, float *del
 int prh_density_contrast_f64(const double *data, const double *randoms, double *delta,
                              size_t size, double alpha, double threshold, double bias_alpha,
                              double bias, void *stream);
+
+/* replaces the sums of OriginalMultiGridReconstruction.set_density_contrast
+ * (multigrid.py:79-83): sums[0] = sum of data over the cells that
+ * 0 < randoms < ran_min does not zero, sums[1] = sum of randoms >= ran_min.
+ * Deterministic FP64; sums: host double[2]; synchronizes stream.           */
+int prh_martin_sums_f32(prh_workspace *ws, const float *data, const float *randoms, size_t size,
+                        float ran_min, double *sums, void *stream);
+int prh_martin_sums_f64(prh_workspace *ws, const double *data, const double *randoms,
+                        size_t size, double ran_min, double *sums, void *stream);
+
+/* multigrid.py:84-95: delta = (data / (alpha * randoms) - 1) / bias where
+ * randoms >= ran_min, 0 / bias elsewhere (delta may alias data); sums[0] =
+ * FP64 sum of the non-zero delta, sums[1] = their count.  Host double[2];
+ * synchronizes stream.                                                     */
+int prh_martin_contrast_f32(prh_workspace *ws, const float *data, const float *randoms,
+                            float *delta, size_t size, float ran_min, float alpha, float bias,
+                            double *sums, void *stream);
+int prh_martin_contrast_f64(prh_workspace *ws, const double *data, const double *randoms,
+                            double *delta, size_t size, double ran_min, double alpha,
+                            double bias, double *sums, void *stream);
+
+/* multigrid.py:95-96: delta[delta != 0] -= mean.                           */
+int prh_subtract_nonzero_mean_f32(float *delta, size_t size, float mean, void *stream);
+int prh_subtract_nonzero_mean_f64(double *delta, size_t size, double mean, void *stream);
+
+/* Grouped full multigrid (no reference counterpart; for serving several
+ * reconstructions at once): count (1..8) independent problems of the same
+ * nmesh and precision, job j with its own workspace ws[j], source f1h[j],
+ * iterate v1h[j], geometry boxsize[3j..3j+2], boxcenter[3j..3j+2], beta[j]
+ * and line of sight los[j] (los == NULL or los[j] == NULL: local).  The
+ * levels handled by the generic kernels are launched once for all jobs; every
+ * job's result is bit-identical to prh_fmg_* (or prh_fmg_mixed) on it alone. */
+int prh_fmg_group_f32(prh_workspace *const *ws, int count, const float *const *f1h,
+                      float *const *v1h, const size_t *nmesh, const float *boxsize,
+                      const float *boxcenter, const float *beta, float jacobi_damping_factor,
+                      int jacobi_niterations, int vcycle_niterations, const float *const *los,
+                      void *stream);
+int prh_fmg_group_f64(prh_workspace *const *ws, int count, const double *const *f1h,
+                      double *const *v1h, const size_t *nmesh, const double *boxsize,
+                      const double *boxcenter, const double *beta, double jacobi_damping_factor,
+                      int jacobi_niterations, int vcycle_niterations, const double *const *los,
+                      void *stream);
+int prh_fmg_mixed_group(prh_workspace *const *ws, int count, const double *const *f1h,
+                        double *const *v1h, const size_t *nmesh, const double *boxsize,
+                        const double *boxcenter, const double *beta, double jacobi_damping_factor,
+                        int jacobi_niterations, int vcycle_niterations, const double *const *los,
+                        int nrefine, int vcycles_per_refine, void *stream);
+
+/* Synthetic catalog on the device (no reference counterpart; load generation
+ * for serving and benchmarks).  positions (n, 3) and weights (n,) (may be
+ * NULL) of dtype PRH_F32/PRH_F64; point i depends on (seed, i) only.  With
+ * nmodes == 0 points are uniform in the box; otherwise uniform points are
+ * displaced by psi = sum_m modes[6m+3] sin(k_m . q + modes[6m+4]) k_m with
+ * k_m = modes[6m..6m+2] (device array of 6 * nmodes doubles), shifted by
+ * f (psi . rhat) rhat along the local line of sight and wrapped into the box.
+ * Weights are uniform in [wlo, whi).                                       */
+int prh_synth_points(void *positions, void *weights, int dtype, size_t n, uint64_t seed,
+                     const double *boxsize, const double *boxcenter, const double *modes,
+                     int nmodes, double f, double wlo, double whi, void *stream);
 
 /* replaces read_shifts / read_shifted_positions of MultiGridReconstruction
  * (multigrid.py:137-158, recon.py:304-325, mesh.py:612-653): finite-

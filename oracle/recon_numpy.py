@@ -10,6 +10,8 @@ A restatement of the reference's Python layer with numpy, calling the C oracle
   812-820, 827-844, 974-988, pyrecon/recon.py:117;
 - the density contrast: pyrecon/recon.py:198-249 (used by
   MultiGridReconstruction, pyrecon/multigrid.py:165-167);
+- Martin White's density contrast of OriginalMultiGridReconstruction:
+  pyrecon/multigrid.py:69-97;
 - the solve: pyrecon/multigrid.py:99-135 (``beta = f / bias``, recon.py:122-125);
 - the shifts: pyrecon/multigrid.py:137-158 and pyrecon/recon.py:304-325.
 
@@ -203,3 +205,30 @@ class OracleMultiGridReconstruction(object):
         positions = positions - shifts
         if self.wrap: positions = self.wrap_positions(positions)
         return positions
+
+
+class OracleOriginalMultiGridReconstruction(OracleMultiGridReconstruction):
+    """The reference's OriginalMultiGridReconstruction: Martin White's density contrast."""
+
+    def set_density_contrast(self, ran_min=0.75, smoothing_radius=15.):
+        # pyrecon/multigrid.py:69-97
+        data = np.array(self.mesh_data, copy=True)
+        if not self.has_randoms:
+            delta = data / np.mean(data) - 1.
+            delta /= self.bias
+        else:
+            randoms = self.mesh_randoms
+            mask = randoms >= ran_min
+            data[(randoms > 0) & (randoms < ran_min)] = 0.
+            alpha = np.sum(data) / np.sum(randoms[mask])
+            data[mask] /= alpha * randoms[mask]
+            delta = data
+            delta -= 1
+            delta[~mask] = 0.
+            delta /= self.bias
+            nonzero = delta != 0.
+            delta[nonzero] -= np.mean(delta[nonzero])
+            self.alpha, self.mask_count = alpha, int(mask.sum())
+        self.mesh_delta = self.smooth_gaussian(delta, smoothing_radius)
+        self.mesh_data = self.mesh_randoms = None
+

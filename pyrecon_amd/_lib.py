@@ -76,6 +76,8 @@ def _signatures():
         'prh_event_create': (_i, [ctypes.POINTER(_vp)]),
         'prh_event_destroy': (_i, [_vp]),
         'prh_event_record': (_i, [_vp, _vp]),
+        'prh_event_synchronize': (_i, [_vp]),
+        'prh_stream_wait_event': (_i, [_vp, _vp]),
         'prh_event_elapsed_ms': (_i, [_vp, _vp, ctypes.POINTER(_f)]),
         'prh_workspace_create': (_i, [ctypes.POINTER(_vp), _szp, _i, _sz, ctypes.c_uint]),
         'prh_workspace_destroy': (_i, [_vp]),
@@ -86,6 +88,8 @@ def _signatures():
         'prh_timer_read': (_i, [_i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_d)]),
         'prh_copy_kernel': (_i, [_vp, _vp, _sz, _vp]),
         'prh_fmg_mixed': (_i, [_vp, _vp, _vp, _szp, _vp, _vp, _d, _d, _i, _i, _vp, _i, _i, _vp, _vp]),
+        'prh_synth_points': (_i, [_vp, _vp, _i, _sz, ctypes.c_uint64, _vp, _vp, _vp, _i, _d, _d, _d, _vp]),
+        'prh_fmg_mixed_group': (_i, [_vp, _i, _vp, _vp, _szp, _vp, _vp, _vp, _d, _i, _i, _vp, _i, _i, _vp]),
         'prh_residual_norm_f64': (_i, [_vp, _vp, _vp, _szp, _vp, _vp, _d, _vp, ctypes.POINTER(_d), _vp]),
     }
     for suffix, real in [('f32', _f), ('f64', _d)]:
@@ -103,6 +107,10 @@ def _signatures():
             'prh_smooth_gaussian_' + suffix: (_i, [_vp, _vp, _szp, _vp, _vp, _vp]),
             'prh_mesh_sums_' + suffix: (_i, [_vp, _vp, _vp, _sz, _vp, _vp]),
             'prh_density_contrast_' + suffix: (_i, [_vp, _vp, _vp, _sz, real, real, real, real, _vp]),
+            'prh_fmg_group_' + suffix: (_i, [_vp, _i, _vp, _vp, _szp, _vp, _vp, _vp, real, _i, _i, _vp, _vp]),
+            'prh_martin_sums_' + suffix: (_i, [_vp, _vp, _vp, _sz, real, _vp, _vp]),
+            'prh_martin_contrast_' + suffix: (_i, [_vp, _vp, _vp, _vp, _sz, real, real, real, _vp, _vp]),
+            'prh_subtract_nonzero_mean_' + suffix: (_i, [_vp, _sz, real, _vp]),
             'prh_read_shifts_' + suffix: (_i, [_vp, _vp, _szp, _vp, _vp, _vp, _i, _sz, _d, _vp, _i, _i, _i,
                                                 _vp, _i, ctypes.POINTER(_i), _vp]),
         })

@@ -91,6 +91,60 @@ __global__ __launch_bounds__(kDenThreads) void k_density(const R* __restrict__ d
   }
 }
 
+// Martin White's density contrast (pyrecon/multigrid.py:79-83): the data
+// summed over all cells except those that 0 < randoms < ran_min sets to zero,
+// and the randoms summed where randoms >= ran_min.
+template <typename R>
+__global__ __launch_bounds__(kReduceThreads) void k_martin_sums(const R* __restrict__ data,
+                                                                const R* __restrict__ randoms,
+                                                                size_t n, R ran_min,
+                                                                double* __restrict__ partial) {
+  block_reduce_range<2>(
+      n,
+      [&](size_t i, double* out) {
+        const R r = randoms[i];
+        out[0] = (r > R(0) && r < ran_min) ? 0.0 : static_cast<double>(data[i]);
+        out[1] = (r >= ran_min) ? static_cast<double>(r) : 0.0;
+      },
+      partial, kReduceBlocks);
+}
+
+// multigrid.py:84-95: inside the mask delta = (data / (alpha * randoms) - 1) / bias,
+// outside 0 / bias; sums the non-zero delta and counts them for the mean removal.
+template <typename R>
+__global__ __launch_bounds__(kReduceThreads) void k_martin_contrast(
+    const R* __restrict__ data, const R* __restrict__ randoms, R* __restrict__ delta, size_t n,
+    R ran_min, R alpha, R bias, double* __restrict__ partial) {
+  block_reduce_range<2>(
+      n,
+      [&](size_t i, double* out) {
+        const R r = randoms[i];
+        R x;
+        if (r >= ran_min) {
+          x = data[i] / (alpha * r);
+          x = x - R(1);
+          x = x / bias;
+        } else {
+          x = R(0) / bias;
+        }
+        delta[i] = x;
+        const bool nz = x != R(0);
+        out[0] = nz ? static_cast<double>(x) : 0.0;
+        out[1] = nz ? 1.0 : 0.0;
+      },
+      partial, kReduceBlocks);
+}
+
+// multigrid.py:95-96: delta[delta != 0] -= mean.
+template <typename R>
+__global__ __launch_bounds__(kDenThreads) void k_subtract_nonzero(R* __restrict__ delta, size_t n,
+                                                                  R mean) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kDenThreads) + threadIdx.x;
+  if (i >= n) return;
+  const R x = delta[i];
+  if (x != R(0)) delta[i] = x - mean;
+}
+
 static int fft_status(hipfftResult r, const char* what) {
   if (r != HIPFFT_SUCCESS) {
     set_error("%s failed with hipfftResult %d", what, static_cast<int>(r));
@@ -168,15 +222,11 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
   return PRH_OK;
 }
 
-template <typename R>
-static int do_sums(prh_workspace* ws, const R* a, const R* b, size_t n, double* sums,
-                   void* stream) {
-  if (!ws || !a || !sums) return PRH_ERR_ARG;
-  hipStream_t s = as_stream(stream);
-  k_sums<R><<<kReduceBlocks, kReduceThreads, 0, s>>>(a, b, n, ws->red);
-  PRH_LAUNCHED();
+// Sums the nv partial rows left in ws->red by a block_reduce_range kernel and
+// copies the nv results to sums (host); synchronizes the stream.
+static int finish_sums(prh_workspace* ws, int nv, double* sums, hipStream_t s) {
   double* out = ws->red + 2 * kReduceBlocks;  // past the partials
-  if (b) {
+  if (nv == 2) {
     k_reduce_final<2><<<1, 1024, 0, s>>>(ws->red, kReduceBlocks, kReduceBlocks, out);
   } else {
     k_reduce_final<1><<<1, 1024, 0, s>>>(ws->red, kReduceBlocks, kReduceBlocks, out);
@@ -185,7 +235,49 @@ static int do_sums(prh_workspace* ws, const R* a, const R* b, size_t n, double* 
   PRH_HIP(hipMemcpyAsync(ws->red_host, out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   PRH_HIP(hipStreamSynchronize(s));
   sums[0] = ws->red_host[0];
-  sums[1] = b ? ws->red_host[1] : 0.0;
+  sums[1] = nv == 2 ? ws->red_host[1] : 0.0;
+  return PRH_OK;
+}
+
+template <typename R>
+static int do_sums(prh_workspace* ws, const R* a, const R* b, size_t n, double* sums,
+                   void* stream) {
+  if (!ws || !a || !sums) return PRH_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  k_sums<R><<<kReduceBlocks, kReduceThreads, 0, s>>>(a, b, n, ws->red);
+  PRH_LAUNCHED();
+  return finish_sums(ws, b ? 2 : 1, sums, s);
+}
+
+template <typename R>
+static int do_martin_sums(prh_workspace* ws, const R* data, const R* randoms, size_t n,
+                          R ran_min, double* sums, void* stream) {
+  if (!ws || !data || !randoms || !sums) return PRH_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  k_martin_sums<R><<<kReduceBlocks, kReduceThreads, 0, s>>>(data, randoms, n, ran_min, ws->red);
+  PRH_LAUNCHED();
+  return finish_sums(ws, 2, sums, s);
+}
+
+template <typename R>
+static int do_martin_contrast(prh_workspace* ws, const R* data, const R* randoms, R* delta,
+                              size_t n, R ran_min, R alpha, R bias, double* sums,
+                              void* stream) {
+  if (!ws || !data || !randoms || !delta || !sums) return PRH_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  k_martin_contrast<R><<<kReduceBlocks, kReduceThreads, 0, s>>>(data, randoms, delta, n,
+                                                                 ran_min, alpha, bias, ws->red);
+  PRH_LAUNCHED();
+  return finish_sums(ws, 2, sums, s);
+}
+
+template <typename R>
+static int do_subtract_nonzero(R* delta, size_t n, R mean, void* stream) {
+  if (n && !delta) return PRH_ERR_ARG;
+  if (n == 0) return PRH_OK;
+  k_subtract_nonzero<R><<<ceil_div(n, kDenThreads), kDenThreads, 0, as_stream(stream)>>>(
+      delta, n, mean);
+  PRH_LAUNCHED();
   return PRH_OK;
 }
 
@@ -234,6 +326,35 @@ int prh_density_contrast_f64(const double* data, const double* randoms, double* 
                              double bias, void* stream) {
   return do_density<double>(data, randoms, delta, size, alpha, threshold, bias_alpha, bias,
                             stream);
+}
+
+int prh_martin_sums_f32(prh_workspace* ws, const float* data, const float* randoms, size_t size,
+                        float ran_min, double* sums, void* stream) {
+  return do_martin_sums<float>(ws, data, randoms, size, ran_min, sums, stream);
+}
+int prh_martin_sums_f64(prh_workspace* ws, const double* data, const double* randoms,
+                        size_t size, double ran_min, double* sums, void* stream) {
+  return do_martin_sums<double>(ws, data, randoms, size, ran_min, sums, stream);
+}
+
+int prh_martin_contrast_f32(prh_workspace* ws, const float* data, const float* randoms,
+                            float* delta, size_t size, float ran_min, float alpha, float bias,
+                            double* sums, void* stream) {
+  return do_martin_contrast<float>(ws, data, randoms, delta, size, ran_min, alpha, bias, sums,
+                                   stream);
+}
+int prh_martin_contrast_f64(prh_workspace* ws, const double* data, const double* randoms,
+                            double* delta, size_t size, double ran_min, double alpha,
+                            double bias, double* sums, void* stream) {
+  return do_martin_contrast<double>(ws, data, randoms, delta, size, ran_min, alpha, bias, sums,
+                                    stream);
+}
+
+int prh_subtract_nonzero_mean_f32(float* delta, size_t size, float mean, void* stream) {
+  return do_subtract_nonzero<float>(delta, size, mean, stream);
+}
+int prh_subtract_nonzero_mean_f64(double* delta, size_t size, double mean, void* stream) {
+  return do_subtract_nonzero<double>(delta, size, mean, stream);
 }
 
 }  // extern "C"

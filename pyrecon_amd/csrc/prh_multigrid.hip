@@ -306,9 +306,8 @@ __global__ __launch_bounds__(kReduceThreads) void k_residual_norm(const R* __res
 // Full weighting (src/multigrid.c:236-262): centre x8, faces x4, edges x2,
 // corners x1, summed in the reference's order, divided by 64.
 template <typename R, typename RO>
-__global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
-                                                       RO* __restrict__ c, int f0, int f1,
-                                                       int f2) {
+__device__ __forceinline__ void restrict_body(const R* __restrict__ u, RO* __restrict__ c, int f0,
+                                              int f1, int f2) {
   const int c0 = f0 / 2, c1 = f1 / 2, c2 = f2 / 2;
   const size_t total = static_cast<size_t>(c0) * c1 * c2;
   const size_t j = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
@@ -340,13 +339,19 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
   c[j] = static_cast<RO>(s * R(0.015625));
 }
 
+template <typename R, typename RO>
+__global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
+                                                       RO* __restrict__ c, int f0, int f1,
+                                                       int f2) {
+  restrict_body<R, RO>(u, c, f0, f1, f2);
+}
+
 // Trilinear prolongation from the coarse mesh (c0, c1, c2) to the fine mesh,
 // one thread per fine cell (src/multigrid.c:177-205), either overwriting the
 // fine value or adding to it (the V-cycle correction, src/multigrid.c:296).
 template <typename RC, typename R, bool ADD>
-__global__ __launch_bounds__(kThreads) void k_prolong(const RC* __restrict__ uc,
-                                                      R* __restrict__ fine, int c0, int c1,
-                                                      int c2) {
+__device__ __forceinline__ void prolong_body(const RC* __restrict__ uc, R* __restrict__ fine,
+                                             int c0, int c1, int c2) {
   const int F1 = 2 * c1, F2 = 2 * c2;
   const size_t total = static_cast<size_t>(2 * c0) * F1 * F2;
   const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
@@ -381,6 +386,13 @@ __global__ __launch_bounds__(kThreads) void k_prolong(const RC* __restrict__ uc,
     fine[i] = val;
 }
 
+template <typename RC, typename R, bool ADD>
+__global__ __launch_bounds__(kThreads) void k_prolong(const RC* __restrict__ uc,
+                                                      R* __restrict__ fine, int c0, int c1,
+                                                      int c2) {
+  prolong_body<RC, R, ADD>(uc, fine, c0, c1, c2);
+}
+
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_fill_zero(R* __restrict__ a, size_t n) {
   const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
@@ -403,6 +415,72 @@ __global__ __launch_bounds__(kThreads) void k_add_f32_f64(double* __restrict__ v
                                                           const float* __restrict__ e, size_t n) {
   const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
   if (i < n) v[i] += static_cast<double>(e[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches: one launch applies the same operation at the same level
+// shape to up to kMaxGroup independent problems (serving several
+// reconstructions at once), each with its own buffers and geometry.  The job
+// index is folded into the grid; per cell the arithmetic is the single-job
+// kernels', so each job's bits do not depend on its company.
+// ---------------------------------------------------------------------------
+constexpr int kMaxGroup = 8;
+
+template <typename RI, typename RO, typename RG>
+struct GroupArgs {
+  const RI* in[kMaxGroup];
+  const RI* src[kMaxGroup];
+  RO* out[kMaxGroup];
+  LevelGeom<RG> G[kMaxGroup];
+};
+
+template <typename R, bool ZERO_IN>
+__global__ __launch_bounds__(kThreads) void k_jacobi_group(GroupArgs<R, R, R> A, R omega, R keep,
+                                                           int n0) {
+  const int job = blockIdx.z / n0;
+  const int ix = blockIdx.z - job * n0;
+  const LevelGeom<R>& G = A.G[job];
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (iz >= G.n[2] || iy >= G.n[1]) return;
+  const size_t i = (static_cast<size_t>(ix) * G.n[1] + iy) * G.n[2] + iz;
+  const Coef<R> k = cell_coef(G, ix, iy, iz);
+  const Hood<R> h =
+      ZERO_IN ? zero_hood<R>() : gather_hood(A.in[job], G.n[0], G.n[1], G.n[2], ix, iy, iz);
+  A.out[job][i] = jacobi_update(k, h, A.src[job][i], omega, keep, G.local);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_residual_group(GroupArgs<R, R, R> A, int n0) {
+  const int job = blockIdx.z / n0;
+  const int ix = blockIdx.z - job * n0;
+  const LevelGeom<R>& G = A.G[job];
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (iz >= G.n[2] || iy >= G.n[1]) return;
+  const size_t i = (static_cast<size_t>(ix) * G.n[1] + iy) * G.n[2] + iz;
+  const Coef<R> k = cell_coef(G, ix, iy, iz);
+  const Hood<R> h = gather_hood(A.in[job], G.n[0], G.n[1], G.n[2], ix, iy, iz);
+  A.out[job][i] = residual_value(k, h, A.src[job][i], G.local);
+}
+
+// restriction / prolongation / zero fill of every job: blockIdx.y is the job
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_restrict_group(GroupArgs<R, R, R> A, int f0, int f1,
+                                                             int f2) {
+  restrict_body<R, R>(A.in[blockIdx.y], A.out[blockIdx.y], f0, f1, f2);
+}
+
+template <typename R, bool ADD>
+__global__ __launch_bounds__(kThreads) void k_prolong_group(GroupArgs<R, R, R> A, int c0, int c1,
+                                                            int c2) {
+  prolong_body<R, R, ADD>(A.in[blockIdx.y], A.out[blockIdx.y], c0, c1, c2);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_fill_zero_group(GroupArgs<R, R, R> A, size_t n) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
+  if (i < n) A.out[blockIdx.y][i] = R(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,55 +556,185 @@ static int launch_fill_zero(R* a, size_t n, hipStream_t s) {
   return PRH_OK;
 }
 
-// State of one solve over a hierarchy.
+// Per-job buffers of a (grouped) solve.
+template <typename R>
+struct Ptrs {
+  R* p[kMaxGroup];
+  R*& operator[](int j) { return p[j]; }
+  R* operator[](int j) const { return p[j]; }
+};
+
+template <typename R>
+static void swap_all(Ptrs<R>& a, Ptrs<R>& b, int count) {
+  for (int j = 0; j < count; j++) std::swap(a.p[j], b.p[j]);
+}
+
+// State of one solve over `count` hierarchies of the same shape (count = 1
+// for the reference-shaped entry points).  Every level runs the schedule of
+// src/multigrid.c:270-328 for all jobs in lock step.  Levels that the blocked
+// kernels cover are launched once per job (they fill the device alone); the
+// smaller levels, which the generic kernels handle and whose launches are
+// latency-bound, are launched once for the whole group.
 template <typename R>
 struct Solver {
-  Hierarchy* H;
-  LevelGeom<R> G[kMaxLevels];
+  int count = 1;
+  Hierarchy* H[kMaxGroup];
+  LevelGeom<R> G[kMaxGroup][kMaxLevels];
   R omega, keep;
   int niter;
   hipStream_t s;
   bool fast = false;  // fast FP32 kernels on the blocked levels (mixed-precision corrections)
 
-  R* v(int l) { return static_cast<R*>(H->v[l]); }
-  R* f(int l) { return static_cast<R*>(H->f[l]); }
-  R* tmp(int l) { return static_cast<R*>(H->tmp[l]); }
-  size_t size(int l) const { return ncells(H->n[l]); }
+  int nlev() const { return H[0]->nlev; }
+  const size_t* n(int l) const { return H[0]->n[l]; }
+  size_t size(int l) const { return ncells(H[0]->n[l]); }
+  Ptrs<R> v(int l) const { return level_ptrs(&Hierarchy::v, l); }
+  Ptrs<R> f(int l) const { return level_ptrs(&Hierarchy::f, l); }
+  Ptrs<R> tmp(int l) const { return level_ptrs(&Hierarchy::tmp, l); }
+  Ptrs<R> level_ptrs(void* (Hierarchy::*m)[kMaxLevels], int l) const {
+    Ptrs<R> r;
+    for (int j = 0; j < count; j++) r.p[j] = static_cast<R*>((H[j]->*m)[l]);
+    return r;
+  }
+  bool grouped(int l) const {
+    int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]), static_cast<int>(n(l)[2])};
+    return count > 1 && !tiled_ok(nn);
+  }
 
   void setup(Hierarchy* h, const R* boxsize, const R* boxcenter, R beta, R damping, int nit,
              const R* los, hipStream_t stream) {
-    H = h;
-    for (int l = 0; l < H->nlev; l++) G[l] = LevelGeom<R>::make(H->n[l], boxsize, boxcenter, beta, los);
+    count = 1;
+    H[0] = h;
+    geometry(0, boxsize, boxcenter, beta, los);
+    params(damping, nit, stream);
+  }
+  void geometry(int j, const R* boxsize, const R* boxcenter, R beta, const R* los) {
+    for (int l = 0; l < H[j]->nlev; l++)
+      G[j][l] = LevelGeom<R>::make(H[j]->n[l], boxsize, boxcenter, beta, los);
+  }
+  void params(R damping, int nit, hipStream_t stream) {
     omega = damping;
     keep = R(1) - damping;
     niter = nit;
     s = stream;
   }
 
+  GroupArgs<R, R, R> args(int l, const Ptrs<R>& in, const Ptrs<R>& src, const Ptrs<R>& out) const {
+    GroupArgs<R, R, R> A;
+    for (int j = 0; j < kMaxGroup; j++) {
+      const int q = j < count ? j : 0;
+      A.in[j] = in.p[q];
+      A.src[j] = src.p[q];
+      A.out[j] = out.p[q];
+      A.G[j] = G[q][l];
+    }
+    return A;
+  }
+
+  int jacobi(int l, const Ptrs<R>& in, const Ptrs<R>& src, const Ptrs<R>& out, bool zero_in) {
+    if (grouped(l)) {
+      const int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]),
+                         static_cast<int>(n(l)[2])};
+      dim3 block, grid;
+      generic_shape(nn, block, grid);
+      grid.z *= count;
+      if (zero_in)
+        k_jacobi_group<R, true><<<grid, block, 0, s>>>(args(l, in, src, out), omega, keep, nn[0]);
+      else
+        k_jacobi_group<R, false><<<grid, block, 0, s>>>(args(l, in, src, out), omega, keep, nn[0]);
+      PRH_LAUNCHED();
+      return PRH_OK;
+    }
+    for (int j = 0; j < count; j++)
+      PRH_TRY(launch_jacobi(in[j], src[j], out[j], G[j][l], omega, keep, zero_in, s,
+                            l == 0 ? PRH_TIMER_JACOBI_FINEST : -1, fast));
+    return PRH_OK;
+  }
+
+  int residual(int l, const Ptrs<R>& in, const Ptrs<R>& src, const Ptrs<R>& out) {
+    if (grouped(l)) {
+      const int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]),
+                         static_cast<int>(n(l)[2])};
+      dim3 block, grid;
+      generic_shape(nn, block, grid);
+      grid.z *= count;
+      k_residual_group<R><<<grid, block, 0, s>>>(args(l, in, src, out), nn[0]);
+      PRH_LAUNCHED();
+      return PRH_OK;
+    }
+    for (int j = 0; j < count; j++)
+      PRH_TRY((launch_residual<R, R>(in[j], src[j], out[j], G[j][l], s, fast)));
+    return PRH_OK;
+  }
+
+  // fine level l -> coarse level l + 1
+  int restrict_to(int l, const Ptrs<R>& fine, const Ptrs<R>& coarse) {
+    if (grouped(l + 1)) {
+      const size_t total = size(l + 1);
+      const dim3 grid(ceil_div(total, kThreads), count);
+      k_restrict_group<R><<<grid, kThreads, 0, s>>>(
+          args(l + 1, fine, fine, coarse), static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]),
+          static_cast<int>(n(l)[2]));
+      PRH_LAUNCHED();
+      return PRH_OK;
+    }
+    for (int j = 0; j < count; j++) PRH_TRY((launch_restrict<R, R>(fine[j], coarse[j], n(l), s)));
+    return PRH_OK;
+  }
+
+  // coarse level l + 1 -> fine level l, added to or overwriting the fine values
+  int prolong_to(int l, const Ptrs<R>& coarse, const Ptrs<R>& fine, bool add) {
+    if (grouped(l + 1)) {
+      const size_t total = 8 * size(l + 1);
+      const dim3 grid(ceil_div(total, kThreads), count);
+      const int c0 = static_cast<int>(n(l + 1)[0]), c1 = static_cast<int>(n(l + 1)[1]),
+                c2 = static_cast<int>(n(l + 1)[2]);
+      if (add)
+        k_prolong_group<R, true><<<grid, kThreads, 0, s>>>(args(l + 1, coarse, coarse, fine), c0, c1, c2);
+      else
+        k_prolong_group<R, false><<<grid, kThreads, 0, s>>>(args(l + 1, coarse, coarse, fine), c0, c1, c2);
+      PRH_LAUNCHED();
+      return PRH_OK;
+    }
+    for (int j = 0; j < count; j++) PRH_TRY((launch_prolong<R, R>(coarse[j], fine[j], n(l + 1), add, s)));
+    return PRH_OK;
+  }
+
+  int fill_zero(int l, const Ptrs<R>& a) {
+    if (grouped(l)) {
+      const dim3 grid(ceil_div(size(l), kThreads), count);
+      k_fill_zero_group<R><<<grid, kThreads, 0, s>>>(args(l, a, a, a), size(l));
+      PRH_LAUNCHED();
+      return PRH_OK;
+    }
+    for (int j = 0; j < count; j++) PRH_TRY(launch_fill_zero(a[j], size(l), s));
+    return PRH_OK;
+  }
+
   // niter damped-Jacobi sweeps; the iterate moves between cur and alt.
-  int smooth(int l, R*& cur, R*& alt, const R* src, bool zero_start) {
+  int smooth(int l, Ptrs<R>& cur, Ptrs<R>& alt, const Ptrs<R>& src, bool zero_start) {
     if (niter <= 0) {
-      if (zero_start) PRH_TRY(launch_fill_zero(cur, size(l), s));
+      if (zero_start) PRH_TRY(fill_zero(l, cur));
       return PRH_OK;
     }
     for (int it = 0; it < niter; it++) {
-      PRH_TRY(launch_jacobi(cur, src, alt, G[l], omega, keep, zero_start && it == 0, s,
-                            l == 0 ? PRH_TIMER_JACOBI_FINEST : -1, fast));
-      std::swap(cur, alt);
+      PRH_TRY(jacobi(l, cur, src, alt, zero_start && it == 0));
+      swap_all(cur, alt, count);
     }
     return PRH_OK;
   }
 
   // src/multigrid.c:270-300
-  int vcycle(int l, R*& cur, R*& alt, const R* src, bool zero_start) {
+  int vcycle(int l, Ptrs<R>& cur, Ptrs<R>& alt, const Ptrs<R>& src, bool zero_start) {
     PRH_TRY(smooth(l, cur, alt, src, zero_start));
-    if (l + 1 < H->nlev) {
-      PRH_TRY((launch_residual<R, R>(cur, src, alt, G[l], s, fast)));
-      PRH_TRY(launch_restrict(alt, f(l + 1), H->n[l], s));
-      R* cc = v(l + 1);
-      R* ca = tmp(l + 1);
-      PRH_TRY(vcycle(l + 1, cc, ca, f(l + 1), true));
-      PRH_TRY(launch_prolong(cc, cur, H->n[l + 1], true, s));
+    if (l + 1 < nlev()) {
+      PRH_TRY(residual(l, cur, src, alt));
+      const Ptrs<R> fc = f(l + 1);
+      PRH_TRY(restrict_to(l, alt, fc));
+      Ptrs<R> cc = v(l + 1);
+      Ptrs<R> ca = tmp(l + 1);
+      PRH_TRY(vcycle(l + 1, cc, ca, fc, true));
+      PRH_TRY(prolong_to(l, cc, cur, true));
     }
     return smooth(l, cur, alt, src, false);
   }
@@ -534,37 +742,44 @@ struct Solver {
   // src/multigrid.c:303-328, with level `top` as the finest level.  v0 is the
   // caller's iterate (used as the start only when no coarser level exists),
   // f0 the caller's source.
-  int fmg(const R* f0, R* v0, int ncycle, int top = 0) {
-    const int nl = H->nlev;
-    const R* src[kMaxLevels];
+  int fmg(const Ptrs<R>& f0, const Ptrs<R>& v0, int ncycle, int top = 0) {
+    const int nl = nlev();
+    Ptrs<R> src[kMaxLevels];
     src[top] = f0;
     for (int l = top + 1; l < nl; l++) {
-      PRH_TRY(launch_restrict(src[l - 1], f(l), H->n[l - 1], s));
       src[l] = f(l);
+      PRH_TRY(restrict_to(l - 1, src[l - 1], src[l]));
     }
-    R* cur;
-    R* alt;
+    Ptrs<R> cur, alt;
     if (nl == top + 1) {
       cur = v0;
       alt = tmp(top);
     } else {
       cur = v(nl - 1);
       alt = tmp(nl - 1);
-      PRH_TRY(launch_fill_zero(cur, size(nl - 1), s));
+      PRH_TRY(fill_zero(nl - 1, cur));
     }
     for (int c = 0; c < ncycle; c++) PRH_TRY(vcycle(nl - 1, cur, alt, src[nl - 1], false));
     for (int l = nl - 2; l >= top; l--) {
-      R* dst = (l == top) ? v0 : v(l);
-      PRH_TRY(launch_prolong(cur, dst, H->n[l + 1], false, s));
+      const Ptrs<R> dst = (l == top) ? v0 : v(l);
+      PRH_TRY(prolong_to(l, cur, dst, false));
       cur = dst;
       alt = tmp(l);
       for (int c = 0; c < ncycle; c++) PRH_TRY(vcycle(l, cur, alt, src[l], false));
     }
-    if (cur != v0)
-      PRH_HIP(hipMemcpyAsync(v0, cur, size(top) * sizeof(R), hipMemcpyDeviceToDevice, s));
+    for (int j = 0; j < count; j++)
+      if (cur[j] != v0[j])
+        PRH_HIP(hipMemcpyAsync(v0[j], cur[j], size(top) * sizeof(R), hipMemcpyDeviceToDevice, s));
     return PRH_OK;
   }
 };
+
+template <typename R>
+static Ptrs<R> one(R* p) {
+  Ptrs<R> r;
+  r.p[0] = p;
+  return r;
+}
 
 static int check_ws(const prh_workspace* ws, const size_t* nmesh, int dtype, unsigned need) {
   if (!ws || !nmesh) {
@@ -595,11 +810,10 @@ static int do_jacobi(prh_workspace* ws, R* v, const R* f, const size_t* nmesh, c
   if (!v || !f || !boxsize || !boxcenter) return PRH_ERR_ARG;
   Solver<R> S;
   S.setup(&ws->mg, boxsize, boxcenter, beta, damping, niter, los, as_stream(stream));
-  R* cur = v;
-  R* alt = S.tmp(0);
-  PRH_TRY(S.smooth(0, cur, alt, f, false));
-  if (cur != v)
-    PRH_HIP(hipMemcpyAsync(v, cur, S.size(0) * sizeof(R), hipMemcpyDeviceToDevice, S.s));
+  Ptrs<R> cur = one(v), alt = S.tmp(0);
+  PRH_TRY(S.smooth(0, cur, alt, one(const_cast<R*>(f)), false));
+  if (cur[0] != v)
+    PRH_HIP(hipMemcpyAsync(v, cur[0], S.size(0) * sizeof(R), hipMemcpyDeviceToDevice, S.s));
   return PRH_OK;
 }
 
@@ -610,11 +824,10 @@ static int do_vcycle(prh_workspace* ws, R* v, const R* f, const size_t* nmesh, c
   if (!v || !f || !boxsize || !boxcenter) return PRH_ERR_ARG;
   Solver<R> S;
   S.setup(&ws->mg, boxsize, boxcenter, beta, damping, niter, los, as_stream(stream));
-  R* cur = v;
-  R* alt = S.tmp(0);
-  PRH_TRY(S.vcycle(0, cur, alt, f, false));
-  if (cur != v)
-    PRH_HIP(hipMemcpyAsync(v, cur, S.size(0) * sizeof(R), hipMemcpyDeviceToDevice, S.s));
+  Ptrs<R> cur = one(v), alt = S.tmp(0);
+  PRH_TRY(S.vcycle(0, cur, alt, one(const_cast<R*>(f)), false));
+  if (cur[0] != v)
+    PRH_HIP(hipMemcpyAsync(v, cur[0], S.size(0) * sizeof(R), hipMemcpyDeviceToDevice, S.s));
   return PRH_OK;
 }
 
@@ -626,7 +839,55 @@ static int do_fmg(prh_workspace* ws, const R* f, R* v, const size_t* nmesh, cons
   if (!v || !f || !boxsize || !boxcenter) return PRH_ERR_ARG;
   Solver<R> S;
   S.setup(&ws->mg, boxsize, boxcenter, beta, damping, niter, los, as_stream(stream));
-  return S.fmg(f, v, ncycle);
+  return S.fmg(one(const_cast<R*>(f)), one(v), ncycle);
+}
+
+// Checks a group of jobs: 1..kMaxGroup workspaces of the same shape and
+// precision, each with its buffers and geometry.
+template <typename T>
+static int check_group(prh_workspace* const* ws, int count, const size_t* nmesh, int dtype,
+                       unsigned need, const T* const* f, T* const* v, const T* boxsize,
+                       const T* boxcenter, const T* beta) {
+  if (!ws || !f || !v || !nmesh || !boxsize || !boxcenter || !beta) {
+    set_error("group solve: ws, f, v, nmesh, boxsize, boxcenter and beta are required");
+    return PRH_ERR_ARG;
+  }
+  if (count < 1 || count > kMaxGroup) {
+    set_error("group solve: count %d out of [1, %d]", count, kMaxGroup);
+    return PRH_ERR_ARG;
+  }
+  for (int j = 0; j < count; j++) {
+    PRH_TRY(check_ws(ws[j], nmesh, dtype, need));
+    if (!f[j] || !v[j]) {
+      set_error("group solve: job %d has no source or iterate", j);
+      return PRH_ERR_ARG;
+    }
+    for (int q = 0; q < j; q++)
+      if (ws[q] == ws[j]) {
+        set_error("group solve: jobs %d and %d share a workspace", q, j);
+        return PRH_ERR_ARG;
+      }
+  }
+  return PRH_OK;
+}
+
+template <typename R>
+static int do_fmg_group(prh_workspace* const* ws, int count, const R* const* f, R* const* v,
+                        const size_t* nmesh, const R* boxsize, const R* boxcenter, const R* beta,
+                        R damping, int niter, int ncycle, const R* const* los, void* stream) {
+  PRH_TRY(check_group<R>(ws, count, nmesh, Tag<R>::code, PRH_WS_MULTIGRID, f, v, boxsize,
+                         boxcenter, beta));
+  Solver<R> S;
+  S.count = count;
+  Ptrs<R> f0, v0;
+  for (int j = 0; j < count; j++) {
+    S.H[j] = &ws[j]->mg;
+    S.geometry(j, boxsize + 3 * j, boxcenter + 3 * j, beta[j], los ? los[j] : nullptr);
+    f0[j] = const_cast<R*>(f[j]);
+    v0[j] = v[j];
+  }
+  S.params(damping, niter, as_stream(stream));
+  return S.fmg(f0, v0, ncycle);
 }
 
 template <typename R>
@@ -663,51 +924,74 @@ static int residual_rms(prh_workspace* ws, const double* v, const double* f, flo
 // the ncycle finest-level cycles run as FP64 defect corrections with FP32
 // V-cycles (fast kernels on the blocked levels); nextra further corrections
 // may follow.  The iterate tracks the FP64 multigrid's to FP32 rounding of
-// the corrections.
-static int do_fmg_mixed(prh_workspace* ws, const double* f, double* v, const size_t* nmesh,
-                        const double* boxsize, const double* boxcenter, double beta, double damping,
-                        int niter, int ncycle, const double* los, int nextra, int vpr,
-                        double* norms, void* stream) {
-  PRH_TRY(check_ws(ws, nmesh, PRH_F64, PRH_WS_MULTIGRID | PRH_WS_MIXED));
-  if (!v || !f || !boxsize || !boxcenter || nextra < 0 || vpr < 1 || ncycle < 0) return PRH_ERR_ARG;
+// the corrections.  `count` problems of one shape advance in lock step (the
+// FP32 levels are grouped as in Solver); norms only for a single problem.
+static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double* const* f,
+                              double* const* v, const size_t* nmesh, const double* boxsize,
+                              const double* boxcenter, const double* beta, double damping,
+                              int niter, int ncycle, const double* const* los, int nextra, int vpr,
+                              double* norms, void* stream) {
+  PRH_TRY(check_group<double>(ws, count, nmesh, PRH_F64, PRH_WS_MULTIGRID | PRH_WS_MIXED, f, v,
+                              boxsize, boxcenter, beta));
+  if (nextra < 0 || vpr < 1 || ncycle < 0 || (norms && count != 1)) return PRH_ERR_ARG;
   hipStream_t s = as_stream(stream);
   const size_t total = ncells(nmesh);
   const unsigned grid = ceil_div(total, kThreads);
-  float bs32[3], bc32[3], los32[3];
-  for (int d = 0; d < 3; d++) {
-    bs32[d] = static_cast<float>(boxsize[d]);
-    bc32[d] = static_cast<float>(boxcenter[d]);
-    if (los) los32[d] = static_cast<float>(los[d]);
-  }
   Solver<float> S;
-  S.setup(&ws->mixed, bs32, bc32, static_cast<float>(beta), static_cast<float>(damping), niter,
-          los ? los32 : nullptr, s);
+  S.count = count;
+  LevelGeom<double> G64[kMaxGroup];
+  for (int j = 0; j < count; j++) {
+    const double* lj = los ? los[j] : nullptr;
+    float bs32[3], bc32[3], los32[3];
+    for (int d = 0; d < 3; d++) {
+      bs32[d] = static_cast<float>(boxsize[3 * j + d]);
+      bc32[d] = static_cast<float>(boxcenter[3 * j + d]);
+      if (lj) los32[d] = static_cast<float>(lj[d]);
+    }
+    S.H[j] = &ws[j]->mixed;
+    S.geometry(j, bs32, bc32, static_cast<float>(beta[j]), lj ? los32 : nullptr);
+    G64[j] = LevelGeom<double>::make(nmesh, boxsize + 3 * j, boxcenter + 3 * j, beta[j], lj);
+  }
+  S.params(static_cast<float>(damping), niter, s);
   S.fast = true;
-  const LevelGeom<double> G64 = LevelGeom<double>::make(nmesh, boxsize, boxcenter, beta, los);
-  float* r32 = S.f(0);
-  float* e32 = S.v(0);
+  const Ptrs<float> r32 = S.f(0), e32 = S.v(0);
   // 1. coarse FMG stages in FP32, prolonged into the FP64 iterate
-  if (S.H->nlev > 1) {
-    PRH_TRY((launch_restrict<double, float>(f, S.f(1), nmesh, s)));
-    PRH_TRY(S.fmg(S.f(1), S.v(1), ncycle, 1));
-    PRH_TRY((launch_prolong<float, double>(S.v(1), v, S.H->n[1], false, s)));
+  if (S.nlev() > 1) {
+    const Ptrs<float> f1 = S.f(1), v1 = S.v(1);
+    for (int j = 0; j < count; j++) PRH_TRY((launch_restrict<double, float>(f[j], f1[j], nmesh, s)));
+    PRH_TRY(S.fmg(f1, v1, ncycle, 1));
+    for (int j = 0; j < count; j++)
+      PRH_TRY((launch_prolong<float, double>(v1[j], v[j], S.n(1), false, s)));
   }
   // 2. finest-level cycles as FP64 defect corrections
   const int ncorr = ncycle + nextra;
   for (int it = 0; it < ncorr; it++) {
-    if (norms) {
-      PRH_TRY(residual_rms(ws, v, f, r32, G64, &norms[it], s));
-    } else {
-      PRH_TRY((launch_residual<double, float>(v, f, r32, G64, s)));
+    for (int j = 0; j < count; j++) {
+      if (norms) {
+        PRH_TRY(residual_rms(ws[j], v[j], f[j], r32[j], G64[j], &norms[it], s));
+      } else {
+        PRH_TRY((launch_residual<double, float>(v[j], f[j], r32[j], G64[j], s)));
+      }
     }
-    float* cur = e32;
-    float* alt = S.tmp(0);
+    Ptrs<float> cur = e32, alt = S.tmp(0);
     for (int c = 0; c < vpr; c++) PRH_TRY(S.vcycle(0, cur, alt, r32, c == 0));
-    k_add_f32_f64<<<grid, kThreads, 0, s>>>(v, cur, total);
-    PRH_LAUNCHED();
+    for (int j = 0; j < count; j++) {
+      k_add_f32_f64<<<grid, kThreads, 0, s>>>(v[j], cur[j], total);
+      PRH_LAUNCHED();
+    }
   }
-  if (norms) PRH_TRY(residual_rms(ws, v, f, nullptr, G64, &norms[ncorr], s));
+  if (norms) PRH_TRY(residual_rms(ws[0], v[0], f[0], nullptr, G64[0], &norms[ncorr], s));
   return PRH_OK;
+}
+
+// One problem: the reference-shaped entry point of the mixed-precision solve.
+static int do_fmg_mixed(prh_workspace* ws, const double* f, double* v, const size_t* nmesh,
+                        const double* boxsize, const double* boxcenter, double beta, double damping,
+                        int niter, int ncycle, const double* los, int nextra, int vpr,
+                        double* norms, void* stream) {
+  if (!v || !f || !boxsize || !boxcenter) return PRH_ERR_ARG;
+  return do_fmg_mixed_group(&ws, 1, &f, &v, nmesh, boxsize, boxcenter, &beta, damping, niter,
+                            ncycle, &los, nextra, vpr, norms, stream);
 }
 
 }  // namespace prh
@@ -794,6 +1078,34 @@ int prh_fmg_mixed(prh_workspace* ws, const double* f1h, double* v1h, const size_
   return do_fmg_mixed(ws, f1h, v1h, nmesh, boxsize, boxcenter, beta, jacobi_damping_factor,
                       jacobi_niterations, vcycle_niterations, los, nrefine, vcycles_per_refine,
                       residual_norms, stream);
+}
+
+int prh_fmg_group_f32(prh_workspace* const* ws, int count, const float* const* f1h,
+                      float* const* v1h, const size_t* nmesh, const float* boxsize,
+                      const float* boxcenter, const float* beta, float jacobi_damping_factor,
+                      int jacobi_niterations, int vcycle_niterations, const float* const* los,
+                      void* stream) {
+  return do_fmg_group<float>(ws, count, f1h, v1h, nmesh, boxsize, boxcenter, beta,
+                             jacobi_damping_factor, jacobi_niterations, vcycle_niterations, los,
+                             stream);
+}
+int prh_fmg_group_f64(prh_workspace* const* ws, int count, const double* const* f1h,
+                      double* const* v1h, const size_t* nmesh, const double* boxsize,
+                      const double* boxcenter, const double* beta, double jacobi_damping_factor,
+                      int jacobi_niterations, int vcycle_niterations, const double* const* los,
+                      void* stream) {
+  return do_fmg_group<double>(ws, count, f1h, v1h, nmesh, boxsize, boxcenter, beta,
+                              jacobi_damping_factor, jacobi_niterations, vcycle_niterations, los,
+                              stream);
+}
+int prh_fmg_mixed_group(prh_workspace* const* ws, int count, const double* const* f1h,
+                        double* const* v1h, const size_t* nmesh, const double* boxsize,
+                        const double* boxcenter, const double* beta, double jacobi_damping_factor,
+                        int jacobi_niterations, int vcycle_niterations, const double* const* los,
+                        int nrefine, int vcycles_per_refine, void* stream) {
+  return do_fmg_mixed_group(ws, count, f1h, v1h, nmesh, boxsize, boxcenter, beta,
+                            jacobi_damping_factor, jacobi_niterations, vcycle_niterations, los,
+                            nrefine, vcycles_per_refine, nullptr, stream);
 }
 
 int prh_residual_norm_f64(prh_workspace* ws, const double* v, const double* f, const size_t* nmesh,

@@ -332,6 +332,18 @@ int prh_event_record(void* event, void* stream) {
   return PRH_OK;
 }
 
+int prh_event_synchronize(void* event) {
+  if (!event) return PRH_ERR_ARG;
+  PRH_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)));
+  return PRH_OK;
+}
+
+int prh_stream_wait_event(void* stream, void* event) {
+  if (!event) return PRH_ERR_ARG;
+  PRH_HIP(hipStreamWaitEvent(as_stream(stream), reinterpret_cast<hipEvent_t>(event), 0));
+  return PRH_OK;
+}
+
 int prh_event_elapsed_ms(void* start, void* stop, float* ms) {
   if (!start || !stop || !ms) return PRH_ERR_ARG;
   PRH_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)));

@@ -7,6 +7,7 @@ pointer through ``data_ptr()`` (e.g. PyTorch-ROCm tensors) can be wrapped
 without a copy with :func:`as_device`.
 """
 import ctypes
+import threading
 
 import numpy as np
 
@@ -46,36 +47,44 @@ def synchronize(stream=None):
 
 
 class _BlockCache(object):
-    """Exact-size cache of freed device blocks, so that repeated reconstructions of the
-    same shapes reuse their allocations instead of calling hipMalloc/hipFree each time.
-    Blocks are reused only after the owning array is released; callers that release
-    memory still used by queued kernels must synchronize first (the classes here
-    release arrays only after the work that reads them has been waited for)."""
+    """Exact-size, stream-ordered cache of freed device blocks, so that repeated
+    reconstructions of the same shapes reuse their allocations instead of calling
+    hipMalloc/hipFree each time.
+
+    A block is returned to the cache of the stream its array was allocated for and is
+    handed out again only for that stream: work queued on the stream before the release
+    completes before any work queued after the reuse, so an array may be released while
+    kernels that read it are still queued on its own stream.  Work on other streams that
+    uses an array must be ordered before its release by the caller (events)."""
 
     def __init__(self, limit_bytes=64 << 30):
         self.blocks = {}
         self.cached = 0
         self.limit = limit_bytes
+        self.lock = threading.Lock()  # arrays are created and released from several threads when serving
 
-    def take(self, nbytes):
-        stack = self.blocks.get(nbytes)
-        if stack:
-            self.cached -= nbytes
-            return stack.pop()
+    def take(self, nbytes, stream=None):
+        with self.lock:
+            stack = self.blocks.get((nbytes, stream))
+            if stack:
+                self.cached -= nbytes
+                return stack.pop()
         return None
 
-    def give(self, ptr, nbytes):
-        if self.cached + nbytes > self.limit:
-            lib.prh_free(ptr)
-            return
-        self.blocks.setdefault(nbytes, []).append(ptr)
-        self.cached += nbytes
+    def give(self, ptr, nbytes, stream=None):
+        with self.lock:
+            if self.cached + nbytes <= self.limit:
+                self.blocks.setdefault((nbytes, stream), []).append(ptr)
+                self.cached += nbytes
+                return
+        lib.prh_free(ptr)
 
     def empty(self):
-        for stack in self.blocks.values():
-            for ptr in stack: lib.prh_free(ptr)
-        self.blocks.clear()
-        self.cached = 0
+        with self.lock:
+            ptrs = [ptr for stack in self.blocks.values() for ptr in stack]
+            self.blocks.clear()
+            self.cached = 0
+        for ptr in ptrs: lib.prh_free(ptr)
 
 
 _cache = _BlockCache()
@@ -96,14 +105,17 @@ class DeviceArray(object):
     dtype : numpy dtype
     ptr : int, optional
         Existing device address (not owned unless ``owner``).
+    stream : Stream, optional
+        Stream whose work uses the array: a released block is reused for the same stream only.
     """
 
-    def __init__(self, shape, dtype, ptr=None, owner=None, base=None):
+    def __init__(self, shape, dtype, ptr=None, owner=None, base=None, stream=None):
         self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if np.ndim(shape) else (int(shape),)
         self.dtype = np.dtype(dtype)
         self.base = base
+        self.stream = stream_handle(stream)
         if ptr is None:
-            cached = _cache.take(self.nbytes) if self.nbytes else None
+            cached = _cache.take(self.nbytes, self.stream) if self.nbytes else None
             if cached is None:
                 p = ctypes.c_void_p()
                 status = lib.prh_malloc(ctypes.byref(p), self.nbytes)
@@ -137,19 +149,19 @@ class DeviceArray(object):
         return 'DeviceArray(shape={}, dtype={}, ptr=0x{:x})'.format(self.shape, self.dtype, self.ptr or 0)
 
     @classmethod
-    def empty(cls, shape, dtype):
-        return cls(shape, dtype)
+    def empty(cls, shape, dtype, stream=None):
+        return cls(shape, dtype, stream=stream)
 
     @classmethod
     def zeros(cls, shape, dtype, stream=None):
-        arr = cls(shape, dtype)
+        arr = cls(shape, dtype, stream=stream)
         arr.fill_zero(stream=stream)
         return arr
 
     @classmethod
     def from_numpy(cls, array, stream=None):
         array = np.ascontiguousarray(array)
-        arr = cls(array.shape, array.dtype)
+        arr = cls(array.shape, array.dtype, stream=stream)
         arr.upload(array, stream=stream)
         return arr
 
@@ -196,7 +208,7 @@ class DeviceArray(object):
 
     def free(self):
         if self.owner and self.ptr:
-            _cache.give(self.ptr, self.nbytes)
+            _cache.give(self.ptr, self.nbytes, self.stream)
         self.ptr = 0
         self.owner = False
 
@@ -230,6 +242,13 @@ class Event(object):
 
     def record(self, stream=None):
         check(lib.prh_event_record(self.handle, stream_handle(stream)), 'event record')
+
+    def synchronize(self):
+        check(lib.prh_event_synchronize(self.handle), 'event synchronize')
+
+    def wait(self, stream=None):
+        """Make later work on ``stream`` wait until this event has completed (no host wait)."""
+        check(lib.prh_stream_wait_event(stream_handle(stream), self.handle), 'stream wait event')
 
     def elapsed_ms(self, stop):
         ms = ctypes.c_float()

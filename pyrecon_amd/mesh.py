@@ -233,7 +233,7 @@ class RealMesh(object):
 
     def _allocate(self, zero=True):
         if self.buffer is None:
-            self.buffer = DeviceArray(self.shape, self.dtype)
+            self.buffer = DeviceArray(self.shape, self.dtype, stream=self.stream)
             if zero: self.buffer.fill_zero(stream=self.stream)
         return self.buffer
 

@@ -16,6 +16,12 @@ from .device import DeviceArray, as_device, stream_handle
 from .mesh import MeshInfo, RealMesh, Workspace, DEFAULT_PAINT_CHUNK
 
 
+def numpy_mean(rtype, total, count):
+    """np.mean's arithmetic (numpy/_core/_methods.py): the sum in the array's type divided by an
+    intp count, cast back to the array's type."""
+    return rtype(rtype(total) / np.intp(count))
+
+
 class ReconstructionError(Exception):
     """Error raised when issue with reconstruction (pyrecon/recon.py:10-12)."""
 
@@ -152,7 +158,7 @@ class BaseReconstruction(object):
             self.alpha, self.threshold = alpha, threshold
         else:
             s_data, _ = self._mesh_sums(data.buffer)
-            mean = rtype(s_data) / self.info.size
+            mean = numpy_mean(rtype, s_data, self.info.size)
             check_status = fn(data.buffer.ptr, None, data.buffer.ptr, self.info.size, float(mean), 0., 0.,
                               float(rtype(self.bias)), stream_handle(self.stream))
         _lib.check(check_status, 'density contrast')
@@ -190,7 +196,7 @@ class BaseReconstruction(object):
             pdtype = np.dtype('f8')
         odtype = self._shift_out_dtype(pdtype, field, shifted)
         if out is None:
-            out = DeviceArray((len(pos), 3), odtype)
+            out = DeviceArray((len(pos), 3), odtype, stream=self.stream)
         elif not (isinstance(out, DeviceArray) and out.shape == (len(pos), 3) and out.dtype == odtype):
             raise ValueError('out must be a DeviceArray of shape ({:d}, 3) and dtype {}'.format(len(pos), odtype))
         code = {'disp': _lib.FIELD_DISP, 'rsd': _lib.FIELD_RSD, 'disp+rsd': _lib.FIELD_DISP_RSD}[field]

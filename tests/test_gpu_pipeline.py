@@ -36,14 +36,18 @@ def c1():
     return make_catalogs('c1')
 
 
-def make_pair(cat, dtype='f8', los=None, wrap=False, **kwargs):
-    from pyrecon_amd import MultiGridReconstruction
-    from oracle.recon_numpy import OracleMultiGridReconstruction
+def make_pair(cat, dtype='f8', los=None, wrap=False, original=False, **kwargs):
+    import pyrecon_amd
+    from oracle import recon_numpy
     cfg = cat['config']
     geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype=dtype, wrap=wrap)
     rkw = dict(RECON_KWARGS, los=los)
-    gpu = MultiGridReconstruction(**rkw, **geo, **kwargs)
-    cpu = OracleMultiGridReconstruction(**rkw, **geo)
+    if original:
+        gpu = pyrecon_amd.OriginalMultiGridReconstruction(**rkw, **geo, **kwargs)
+        cpu = recon_numpy.OracleOriginalMultiGridReconstruction(**rkw, **geo)
+    else:
+        gpu = pyrecon_amd.MultiGridReconstruction(**rkw, **geo, **kwargs)
+        cpu = recon_numpy.OracleMultiGridReconstruction(**rkw, **geo)
     return gpu, cpu
 
 
@@ -262,6 +266,30 @@ def test_no_randoms():
     assert np.allclose(np.mean(recon.mesh_delta), 0.)
     recon.run(jacobi_niterations=1, vcycle_niterations=1)
     assert np.all(np.abs(recon.read_shifts(data)) < 2.)
+
+
+@pytest.mark.parametrize('dtype', ['f8', 'f4'])
+@pytest.mark.parametrize('randoms', [True, False])
+def test_original_martin_density(c1, dtype, randoms):
+    # OriginalMultiGridReconstruction: Martin White's density rule (pyrecon/multigrid.py:69-97)
+    # ahead of the same multigrid; the meshes are bit-exact, the sums FP64 on the device
+    gpu, cpu = make_pair(c1, dtype=dtype, original=True)
+    for r in (gpu, cpu):
+        r.assign_data(c1['data'], c1['data_weights'])
+        if randoms: r.assign_randoms(c1['randoms'], c1['randoms_weights'])
+    for r in (gpu, cpu):
+        r.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    if randoms:
+        assert np.isclose(gpu.alpha, cpu.alpha, rtol=1e-14 if dtype == 'f8' else 1e-6, atol=0)
+    tol = 1e-12 if dtype == 'f8' else 1e-5
+    assert rel_l2(gpu.mesh_delta.value, cpu.mesh_delta) < tol
+    gpu.mesh_delta.value = cpu.mesh_delta
+    for r in (gpu, cpu):
+        r.run()
+    assert np.array_equal(bits(gpu.mesh_phi.value), bits(cpu.mesh_phi))
+    a = gpu.read_shifted_positions(c1['data'])
+    b = cpu.read_shifted_positions(c1['data'])
+    assert np.array_equal(bits(a), bits(b))
 
 
 def test_mixed_precision(c1):

@@ -107,3 +107,28 @@ def test_cosmo():
     r.set_cosmo(beta=0.5)
     assert np.isclose(r.f, 1.0) and np.isclose(r.bias, 2.)
     assert issubclass(MultiGridReconstruction, BaseReconstruction)
+
+
+def test_serving_footprint_and_modes():
+    # admission sizes a job by its workspace (hierarchy, paint scratch, FFT) plus meshes, catalogs and outputs
+    from pyrecon_amd.serving import ReconJob, zeldovich_modes
+    from pyrecon_amd.mesh import Workspace
+    job = ReconJob((np.zeros((1000, 3)), None), (np.zeros((10000, 3)), None), nmesh=64, boxsize=500., boxcenter=0.,
+                   shifted=('data', 'randoms'))
+    ws, other = job.footprint(1 << 16)
+    assert ws == Workspace.footprint((64, 64, 64), 'f8', paint_chunk=1 << 16, flags=job.flags())
+    assert other == 3 * 64**3 * 8 + 11000 * 32 + 11000 * 24
+    mixed = ReconJob((np.zeros((10, 3)), None), nmesh=64, boxsize=500., boxcenter=0., precision='mixed')
+    assert mixed.footprint(1 << 16)[0] > ws  # the FP32 hierarchy of the mixed solve
+    with pytest.raises(ValueError):
+        ReconJob(lambda s: None, nmesh=64, boxsize=500., boxcenter=0.)
+    # random Fourier modes: every mode carries an equal share of the displacement variance
+    # sigma^2 = int P(k) dk / (2 pi^2); the per-component variance of psi is sigma^2 / 3
+    modes = zeldovich_modes([1000.] * 3, [128] * 3, seed=3, nmodes=512)
+    k = np.sqrt(np.sum(modes[:, :3]**2, axis=-1))
+    assert np.all(k >= 2 * np.pi / 1000. - 1e-12) and np.all(k <= np.pi * 128 / 1000. + 1e-12)
+    grid = np.linspace(2 * np.pi / 1000., np.pi * 128 / 1000., 200001)
+    pk = 2e4 * (grid / 0.1)**-1.5 * np.exp(-(grid / 0.3)**2)
+    sigma2 = np.trapezoid(pk, grid) / (2 * np.pi**2)
+    var = np.sum((modes[:, 3] * k)**2 / 2.)
+    assert np.isclose(var, sigma2, rtol=1e-3)

@@ -7,6 +7,10 @@
 - Density contrast: the rule of pyrecon/recon.py:227-247 on hand-built meshes
   (mask below threshold -> 0, ratio above), and mean zero without randoms
   (test_multigrid.py:33).
+- Martin White's density contrast (pyrecon/multigrid.py:69-97) on a hand-built
+  mesh whose answer is worked out in fractions: the cell with 0 < randoms <
+  ran_min loses its data, the empty-randoms cell keeps its data in alpha, the
+  non-zero cells end with mean zero.
 - Mesh-unit transform and wrap: numpy's promotion for f4/f8 and Python's
   modulo sign convention.
 - Shift algebra: 'disp+rsd' = 'disp' + 'rsd', rsd parallel to the line of
@@ -16,7 +20,7 @@
 import numpy as np
 import pytest
 
-from oracle.recon_numpy import OracleMultiGridReconstruction, OracleError, distance
+from oracle.recon_numpy import OracleMultiGridReconstruction, OracleOriginalMultiGridReconstruction, OracleError, distance
 
 
 def make(dtype='f8', los=None, nmesh=16, boxsize=800., boxcenter=(0., 0., 1000.), wrap=False, f=0.8, bias=2.):
@@ -63,6 +67,33 @@ def test_density_rule():
     assert r.mesh_delta[0, 0, 0] == 0.
     assert np.allclose(r.mesh_delta.ravel()[1:], expect)
     assert r.mask_count == randoms.size - 1
+
+
+def test_martin_density_rule():
+    r = OracleOriginalMultiGridReconstruction(f=0.8, bias=2., nmesh=(2, 2, 2), boxsize=800., boxcenter=0.)
+    randoms = np.array([1., 2., 4., 8., 0.5, 0., 0.5, 0.]).reshape(r.shape)
+    data = np.array([1., 1., 1., 1., 3., 5., 0., 0.]).reshape(r.shape)
+    r.mesh_data, r.mesh_randoms, r._size_randoms = data, randoms, 100
+    r.smooth_gaussian = lambda value, radius: value  # isolate the rule from the smoothing
+    r.set_density_contrast()
+    # alpha = (1 + 1 + 1 + 1 + 5) / (1 + 2 + 4 + 8): the r = 0.5 cell's data is zeroed, the r = 0 cell's is kept
+    assert np.isclose(r.alpha, 0.6)
+    expect = np.array([85., 5., -35., -55., 0., 0., 0., 0.]) / 192.
+    assert np.allclose(r.mesh_delta.ravel(), expect, rtol=0, atol=1e-15)
+    assert r.mask_count == 4
+
+
+def test_martin_density_no_randoms():
+    r = OracleOriginalMultiGridReconstruction(f=0.8, bias=2., nmesh=8, boxsize=800., boxcenter=0.)
+    rng = np.random.RandomState(3)
+    data = rng.uniform(0.5, 1.5, r.shape)
+    r.mesh_data = data.copy()
+    r.smooth_gaussian = lambda value, radius: value
+    r.set_density_contrast()
+    # 1 + bias delta is the data in units of its mean: ratios of cells are ratios of data
+    ratio = (1. + 2. * r.mesh_delta) / (1. + 2. * r.mesh_delta.ravel()[0])
+    assert np.allclose(ratio, data / data.ravel()[0], rtol=1e-14)
+    assert np.isclose(np.mean(r.mesh_delta), 0., atol=1e-15)
 
 
 def test_density_no_randoms():
