@@ -52,7 +52,99 @@ def parse_args():
     p.add_argument('--kernel-only', default=None, choices=['jacobi32', 'jacobi64', 'copy'],
                    help='run only one kernel repeatedly (for rocprofv3 counter runs)')
     p.add_argument('--workers', type=int, default=min(16, os.cpu_count() or 1))
+    p.add_argument('--serving-jobs', type=int, default=256, help='jobs of the serving stream (0: skip it)')
+    p.add_argument('--serving-rate', type=float, default=4.0, help='Poisson arrival rate of the serving stream (jobs/s)')
+    p.add_argument('--serving-only', action='store_true', help='run only the serving stream')
     return p.parse_args()
+
+
+def serving_stream(njobs, rate, seed=2024):
+    """Configuration 5: Poisson arrivals; nmesh in {128, 256, 512} with probabilities
+    {0.5, 0.35, 0.15}; boxsize = 7.8 nmesh Mpc/h centred at (0, 0, boxsize); data count
+    lognormal with median 1e6 and sigma(ln N) = 1, clipped to [1e5, 2e7]; randoms 10x."""
+    rng = np.random.RandomState(seed)
+    arrivals = np.cumsum(rng.exponential(1. / rate, njobs))
+    nmesh = rng.choice([128, 256, 512], size=njobs, p=[0.5, 0.35, 0.15])
+    ndata = np.clip(np.exp(np.log(1e6) + rng.normal(size=njobs)), 1e5, 2e7).astype(np.int64)
+    return [dict(arrival=float(a), nmesh=int(n), ndata=int(d), seed=int(seed * 1000 + i))
+            for i, (a, n, d) in enumerate(zip(arrivals, nmesh, ndata))]
+
+
+def serving_bench(args, precision, stream_spec, keep=None, reference=None):
+    """Run the serving stream through one ReconServer; returns the report and, for the jobs in
+    ``keep``, the shifted data positions (host); with ``reference``, errors against it."""
+    from pyrecon_amd.serving import ReconServer, ReconJob, synthetic_catalog, zeldovich_modes
+    from catalogs import RECON_KWARGS, SMOOTHING_RADIUS
+    keep = set(keep or [])
+    kept, errors = {}, []
+
+    def make_job(i, spec):
+        n, nd = spec['nmesh'], spec['ndata']
+        boxsize = 7.8 * n
+        boxcenter = (0., 0., boxsize)
+        modes = zeldovich_modes([boxsize] * 3, [n] * 3, spec['seed'])
+
+        def data(stream):
+            return synthetic_catalog(nd, boxsize, boxcenter, 2 * spec['seed'], modes=modes, f=RECON_KWARGS['f'],
+                                     weights=(0.5, 1.5), stream=stream)
+
+        def randoms(stream):
+            return synthetic_catalog(10 * nd, boxsize, boxcenter, 2 * spec['seed'] + 1, weights=(1., 1.),
+                                     stream=stream)
+
+        return ReconJob(data, randoms, ndata=nd, nrandoms=10 * nd, nmesh=n, boxsize=boxsize, boxcenter=boxcenter,
+                        f=RECON_KWARGS['f'], bias=RECON_KWARGS['bias'], los=RECON_KWARGS['los'], dtype='f8',
+                        smoothing_radius=SMOOTHING_RADIUS, precision=precision, shifted=('data', 'randoms'),
+                        field='disp+rsd', randoms_field='disp+rsd', to_host=False, name=str(i))
+
+    def done(i, fut):
+        res = fut.result()
+        if i in keep:
+            kept[i] = res.shifted_data.numpy()
+        res.shifted_data = res.shifted_randoms = None
+
+    server = ReconServer(budget_bytes=200 * 10**9, slots=4, paint_chunk=1 << 23)
+    futures = []
+    t0 = time.perf_counter()
+    for i, spec in enumerate(stream_spec):
+        delay = t0 + spec['arrival'] - time.perf_counter()
+        if delay > 0: time.sleep(delay)
+        fut = server.submit(make_job(i, spec))
+        fut.add_done_callback(lambda f, i=i: done(i, f))
+        futures.append(fut)
+        if (i + 1) % 32 == 0:
+            log('serving {}: submitted {:d} jobs'.format(precision, i + 1))
+    for fut in futures: fut.result()
+    report = server.report()
+    per_mesh = {}
+    for res in server.results:
+        per_mesh.setdefault(res.job.nmesh[0], []).append(res.latency)
+    report['p50_s_by_nmesh'] = {str(k): round(float(np.percentile(v, 50)), 4) for k, v in sorted(per_mesh.items())}
+    server.close()
+    if reference is not None:
+        dx = np.concatenate([np.abs(kept[i] - reference[i]).ravel() for i in sorted(kept)])
+        report['shift_max_mpc_h'] = float(dx.max())
+        report['shift_rms_mpc_h'] = float(np.sqrt(np.mean(dx**2)))
+        report['error_jobs'] = len(kept)
+    return report, kept
+
+
+def run_serving(args):
+    spec = serving_stream(args.serving_jobs, args.serving_rate)
+    keep = [i for i, s in enumerate(spec) if s['ndata'] <= 4 * 10**6][::8]
+    log('serving stream: {:d} jobs at {:.1f}/s'.format(len(spec), args.serving_rate))
+    plain, ref = serving_bench(args, 'plain', spec, keep=keep)
+    log('serving plain', plain)
+    mixed, _ = serving_bench(args, 'mixed', spec, keep=keep, reference=ref)
+    log('serving mixed', mixed)
+    return dict(workload='configuration 5: {:d} jobs, Poisson {:.1f}/s, nmesh {{128,256,512}} p={{.5,.35,.15}}, '
+                         'boxsize 7.8 nmesh, data lognormal median 1e6 [1e5, 2e7], randoms 10x, '
+                         '200 GB budget, 4 slots, grouped solves'.format(len(spec), args.serving_rate),
+                catalogs='generated on the device per job from its seed (uniform randoms; data Zel\'dovich-displaced '
+                         'by 256 random Fourier modes of P(k)), inside the job latency',
+                mixed={k: (round(v, 5) if isinstance(v, float) else v) for k, v in mixed.items()},
+                plain={k: (round(v, 5) if isinstance(v, float) else v) for k, v in plain.items()},
+                error_against='plain FP64 serving run of the same jobs (subset)')
 
 
 def kernel_only(args):
@@ -161,6 +253,9 @@ def main():
         sys.exit(2)
     if args.kernel_only:
         return kernel_only(args)
+    if args.serving_only:
+        print(json.dumps(dict(metric=METRIC, serving=run_serving(args))), flush=True)
+        return
     import pyrecon_amd as P
     from pyrecon_amd import _lib
     from pyrecon_amd.device import device_name
@@ -255,6 +350,12 @@ def main():
         log('cpu baseline')
         cpu = cpu_baseline(cat, threads=args.workers)
         log('cpu baseline', cpu)
+    serving = None
+    if args.serving_jobs > 0:
+        del d, dw, r, rw, out_d, out_r
+        ws.close()
+        P.empty_cache()
+        serving = run_serving(args)
     ms_j64, nj64, bytes_j64 = jac64
     line = dict(metric=METRIC, value=round(value, 4), unit='recon/s', n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=round(1e3 * elapsed / args.steps, 3), higher_is_better=True,
@@ -271,7 +372,7 @@ def main():
                 speedup_mixed_vs_fp64=round(value / value64, 3), accuracy=accuracy,
                 stages_ms=dict(mixed={k: round(1e3 * v, 2) for k, v in stages_mixed.items()},
                                plain={k: round(1e3 * v, 2) for k, v in stages_plain.items()}),
-                serving=None)
+                serving=serving)
     print(json.dumps(line), flush=True)
 
 

@@ -45,6 +45,7 @@ extern "C" {
 #define PRH_FIELD_DISP 0
 #define PRH_FIELD_RSD 1
 #define PRH_FIELD_DISP_RSD 2
+#define PRH_FIELD_VALUE 3 /* CIC value of the mesh (RealMesh.read_cic), out (N,) */
 
 /* ---- runtime plumbing (no reference counterpart: device memory is what
  *      replaces the reference's numpy buffers) ---------------------------- */
@@ -269,7 +270,11 @@ int prh_synth_points(void *positions, void *weights, int dtype, size_t n, uint64
  * PRH_ERR_RANGE if wrap == 0 and a position is outside the box; sets
  * *nonfinite (host int, may be NULL) when positions hold NaN/inf (the
  * reference only warns, mesh.py:637-639).  ws holds the device status
- * flags (any workspace; one stream at a time).  Synchronizes stream.       */
+ * flags (any workspace; one stream at a time).  Synchronizes stream.
+ * field == PRH_FIELD_DISP with shifted == 0 is RealMesh.
+ * read_finite_difference_cic (mesh.py:612-653) of the mesh phi; field ==
+ * PRH_FIELD_VALUE (shifted == 0) is RealMesh.read_cic (mesh.py:570-610): out
+ * is (N,) in the positions dtype.                                          */
 int prh_read_shifts_f32(prh_workspace *ws, const float *phi, const size_t *nmesh, const float *boxsize,
                         const float *boxcenter, const void *positions, int pos_dtype,
                         size_t npositions, double f, const float *los, int field, int wrap,

@@ -16,11 +16,12 @@ ImportError if the library has not been built.  There is no CPU fallback.
 __version__ = '0.1.0'
 
 from ._lib import MeshError, DeviceError
-from .device import DeviceArray, Stream, Event, synchronize
+from .device import DeviceArray, Stream, Event, synchronize, empty_cache
 from .mesh import MeshInfo, RealMesh, Workspace
 from .recon import BaseReconstruction, ReconstructionError
-from .multigrid import MultiGridReconstruction, OriginalMultiGridReconstruction
+from .multigrid import MultiGridReconstruction, OriginalMultiGridReconstruction, run_group
+from .serving import ReconServer, ReconJob, AdmissionError
 
 __all__ = ['MultiGridReconstruction', 'OriginalMultiGridReconstruction', 'BaseReconstruction', 'ReconstructionError',
            'MeshError', 'DeviceError', 'MeshInfo', 'RealMesh', 'Workspace', 'DeviceArray', 'Stream', 'Event',
-           'synchronize']
+           'synchronize', 'empty_cache', 'run_group', 'ReconServer', 'ReconJob', 'AdmissionError']

@@ -24,7 +24,7 @@ ERR_CAPACITY = -7
 ERR_STATE = -8
 
 F32, F64 = 4, 8
-FIELD_DISP, FIELD_RSD, FIELD_DISP_RSD = 0, 1, 2
+FIELD_DISP, FIELD_RSD, FIELD_DISP_RSD, FIELD_VALUE = 0, 1, 2, 3
 TIMER_JACOBI_FINEST, TIMER_PAINT_GATHER, TIMER_READ_SHIFTS = 0, 1, 2
 
 WS_MULTIGRID, WS_PAINT, WS_FFT, WS_MIXED = 1, 2, 4, 8

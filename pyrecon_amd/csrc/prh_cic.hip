@@ -531,6 +531,10 @@ __global__ __launch_bounds__(kCicThreads) void k_read_shifts(
     const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
     R p[3];
     to_mesh_units<R, P>(x, F, wrap, p, err);
+    if (field == PRH_FIELD_VALUE) {
+      // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
+      out[i] = static_cast<O>(static_cast<P>(cic_value(phi, F.n[0], F.n[1], F.n[2], p)));
+    } else {
     const R cell[3] = {c0, c1, c2};
     R g[3];
     fd_gradient(phi, F.n[0], F.n[1], F.n[2], cell, p, g);
@@ -578,6 +582,7 @@ __global__ __launch_bounds__(kCicThreads) void k_read_shifts(
     out[3 * i] = res[0];
     out[3 * i + 1] = res[1];
     out[3 * i + 2] = res[2];
+    }
   }
   const uint64_t bad = __ballot(err != 0);
   if (bad) {
@@ -859,8 +864,9 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const Frame<R>& F, c
     PRH_LAUNCHED();
     k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
     PRH_LAUNCHED();
+    const size_t ostride = field == PRH_FIELD_VALUE ? 1 : 3;
     PRH_TRY((launch_read_shifts<R, P, O>(phi, F, cell, pos + 3 * o, m, f, los, field, wrap, shifted,
-                                         out + 3 * o, misc, s, S.sorted)));
+                                         out + ostride * o, misc, s, S.sorted)));
   }
   return PRH_OK;
 }
@@ -871,8 +877,13 @@ static int do_read_shifts(prh_workspace* ws, const R* phi, const size_t* nmesh, 
                           int wrap, int shifted, void* out, int odt, int* nonfinite,
                           void* stream) {
   if (!ws || !nmesh || !boxsize || !boxcenter || (n && (!phi || !pos || !out))) return PRH_ERR_ARG;
-  if (field != PRH_FIELD_DISP && field != PRH_FIELD_RSD && field != PRH_FIELD_DISP_RSD) {
+  if (field != PRH_FIELD_DISP && field != PRH_FIELD_RSD && field != PRH_FIELD_DISP_RSD &&
+      field != PRH_FIELD_VALUE) {
     set_error("unknown field %d", field);
+    return PRH_ERR_ARG;
+  }
+  if (field == PRH_FIELD_VALUE && shifted) {
+    set_error("PRH_FIELD_VALUE reads mesh values: shifted must be 0");
     return PRH_ERR_ARG;
   }
   if ((pdt != PRH_F32 && pdt != PRH_F64) || (odt != PRH_F32 && odt != PRH_F64)) {

@@ -300,6 +300,43 @@ class RealMesh(object):
                  _lib.dtype_code(pos.dtype), wdev.ptr if wdev is not None else None, wcode, len(pos), int(bool(wrap)),
                  stream_handle(self.stream)), 'assign_cic')
 
+    def _read(self, positions, field, wrap):
+        import ctypes
+        import warnings
+        if self.buffer is None: raise MeshError('mesh has no value')
+        pos, is_host = as_device(positions, stream=self.stream)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError('positions must have shape (N, 3)')
+        if pos.dtype not in (np.dtype('f4'), np.dtype('f8')):
+            pos, is_host = as_device(np.asarray(positions, dtype='f8'), stream=self.stream)
+        # values come back in the positions dtype (mesh.py:610, 653)
+        out = DeviceArray((len(pos),) if field == _lib.FIELD_VALUE else (len(pos), 3), pos.dtype, stream=self.stream)
+        nm, nm_ptr = _lib.size3(self.shape)
+        bs, bs_ptr = _lib.vec3(self.boxsize, self.dtype)
+        bc, bc_ptr = _lib.vec3(self.boxcenter, self.dtype)
+        nonfinite = ctypes.c_int(0)
+        fn = getattr(lib, 'prh_read_shifts_' + _suffix(self.dtype))
+        status = fn(self.workspace.handle, self.buffer.ptr, nm_ptr, bs_ptr, bc_ptr, pos.ptr, _lib.dtype_code(pos.dtype),
+                    len(pos), 0., None, field, int(bool(wrap)), 0, out.ptr, _lib.dtype_code(pos.dtype),
+                    ctypes.byref(nonfinite), stream_handle(self.stream))
+        if status == _lib.ERR_RANGE:
+            low, high = self.boxcenter - self.boxsize / 2., self.boxcenter + self.boxsize / 2.
+            raise MeshError('positions not in box range {} - {}'.format(low, high))
+        check(status, 'read')
+        if nonfinite.value:
+            warnings.warn('Some positions are NaN/inf')
+        return out.numpy(stream=self.stream) if is_host else out
+
+    def read_cic(self, positions, wrap=False):
+        """Mesh values at ``positions`` by CIC interpolation (pyrecon/mesh.py:570-610), in the positions
+        dtype; numpy in -> numpy out, device in -> device out."""
+        return self._read(positions, _lib.FIELD_VALUE, wrap)
+
+    def read_finite_difference_cic(self, positions, wrap=False):
+        """Finite-difference gradient of the mesh at ``positions`` (pyrecon/mesh.py:612-653), shape (N, 3),
+        in the positions dtype."""
+        return self._read(positions, _lib.FIELD_DISP, wrap)
+
     def smooth_gaussian(self, radius, method='fft', nsigmas=2.5, **kwargs):
         """Gaussian smoothing in Fourier space (pyrecon/mesh.py:655-680)."""
         if method != 'fft':

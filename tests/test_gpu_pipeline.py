@@ -208,6 +208,33 @@ def test_dtype_combinations(c1, mesh_dtype, pos_dtype, los):
         assert a.dtype == b.dtype and np.array_equal(bits(a), bits(b)), (field, a.dtype, b.dtype)
 
 
+@pytest.mark.parametrize('mesh_dtype', ['f4', 'f8'])
+@pytest.mark.parametrize('pos_dtype', ['f4', 'f8'])
+@pytest.mark.parametrize('wrap', [False, True])
+def test_realmesh_reads(c1, mesh_dtype, pos_dtype, wrap):
+    # RealMesh.read_cic / read_finite_difference_cic (pyrecon/mesh.py:570-653) with Cartesian positions:
+    # bit-exact, values in the positions dtype, box range checked unless wrapped
+    from pyrecon_amd import RealMesh, MeshError
+    gpu, cpu = make_pair(c1, dtype=mesh_dtype, wrap=wrap)
+    rng = np.random.RandomState(11)
+    value = rng.normal(size=gpu.info.shape).astype(mesh_dtype)
+    mesh = RealMesh(info=gpu.info)
+    mesh.value = value
+    pos = c1['data'][:70000].astype(pos_dtype)
+    if wrap:
+        pos = pos + np.array([1.5, -2.5, 0.5], dtype=pos_dtype) * gpu.info.boxsize.astype(pos_dtype)
+    a, b = mesh.read_cic(pos, wrap=wrap), cpu.read_cic(value, pos)
+    assert a.shape == (len(pos),) and a.dtype == b.dtype == np.dtype(pos_dtype)
+    assert np.array_equal(bits(a), bits(b))
+    cpu.mesh_phi = value
+    a, b = mesh.read_finite_difference_cic(pos, wrap=wrap), cpu.read_finite_difference_cic(pos)
+    assert a.shape == (len(pos), 3) and a.dtype == b.dtype
+    assert np.array_equal(bits(a), bits(b))
+    if not wrap:
+        with pytest.raises(MeshError):
+            mesh.read_cic(pos + np.array([0., 0., 1e4], dtype=pos_dtype))
+
+
 def test_f4_vs_f8(c1):
     # reference test_dtype (pyrecon/tests/test_multigrid.py:67-94)
     shifts = {}
