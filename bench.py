@@ -55,6 +55,8 @@ def parse_args():
     p.add_argument('--serving-jobs', type=int, default=256, help='jobs of the serving stream (0: skip it)')
     p.add_argument('--serving-rate', type=float, default=4.0, help='Poisson arrival rate of the serving stream (jobs/s)')
     p.add_argument('--serving-only', action='store_true', help='run only the serving stream')
+    p.add_argument('--catalog-cache', default=None,
+                   help='directory where the generated catalogs are saved and, on later runs, loaded from')
     return p.parse_args()
 
 
@@ -244,6 +246,26 @@ def cpu_baseline(cat, threads):
                 stages_s={k: round(v, 3) for k, v in stages.items()})
 
 
+def load_catalogs(args, make_catalogs):
+    """The seeded catalogs of the configuration, generated or read back from --catalog-cache."""
+    keys = ['data', 'data_weights', 'randoms', 'randoms_weights']
+    path = None
+    if args.catalog_cache:
+        path = os.path.join(args.catalog_cache, '{}_{:g}'.format(args.config, args.scale))
+        if all(os.path.isfile(os.path.join(path, k + '.npy')) for k in keys):
+            log('loading catalogs from', path)
+            from catalogs import CONFIGS
+            cat = {k: np.load(os.path.join(path, k + '.npy')) for k in keys}
+            cat['config'] = dict(CONFIGS[args.config])
+            return cat
+    log('generating catalogs for', args.config)
+    cat = make_catalogs(args.config, scale=args.scale, workers=args.workers, progress=log)
+    if path:
+        os.makedirs(path, exist_ok=True)
+        for k in keys: np.save(os.path.join(path, k + '.npy'), cat[k])
+    return cat
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -261,8 +283,7 @@ def main():
     from pyrecon_amd.device import device_name
     from catalogs import make_catalogs, RECON_KWARGS, SMOOTHING_RADIUS
     log('device', device_name())
-    log('generating catalogs for', args.config)
-    cat = make_catalogs(args.config, scale=args.scale, workers=args.workers)
+    cat = load_catalogs(args, make_catalogs)
     cfg = cat['config']
     nd, nr = len(cat['data']), len(cat['randoms'])
     log('catalogs: {:d} data, {:d} randoms'.format(nd, nr))

@@ -306,12 +306,9 @@ __global__ __launch_bounds__(kReduceThreads) void k_residual_norm(const R* __res
 // Full weighting (src/multigrid.c:236-262): centre x8, faces x4, edges x2,
 // corners x1, summed in the reference's order, divided by 64.
 template <typename R, typename RO>
-__device__ __forceinline__ void restrict_body(const R* __restrict__ u, RO* __restrict__ c, int f0,
-                                              int f1, int f2) {
-  const int c0 = f0 / 2, c1 = f1 / 2, c2 = f2 / 2;
-  const size_t total = static_cast<size_t>(c0) * c1 * c2;
-  const size_t j = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
-  if (j >= total) return;
+__device__ __forceinline__ void restrict_at(const R* __restrict__ u, RO* __restrict__ c, int f0,
+                                            int f1, int f2, size_t j) {
+  const int c1 = f1 / 2, c2 = f2 / 2;
   int jx, jy, jz;
   unflatten(j, c1, c2, jx, jy, jz);
   const size_t s0 = static_cast<size_t>(f1) * f2, s1 = static_cast<size_t>(f2);
@@ -340,6 +337,14 @@ __device__ __forceinline__ void restrict_body(const R* __restrict__ u, RO* __res
 }
 
 template <typename R, typename RO>
+__device__ __forceinline__ void restrict_body(const R* __restrict__ u, RO* __restrict__ c, int f0,
+                                              int f1, int f2) {
+  const size_t total = static_cast<size_t>(f0 / 2) * (f1 / 2) * (f2 / 2);
+  const size_t j = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
+  if (j < total) restrict_at<R, RO>(u, c, f0, f1, f2, j);
+}
+
+template <typename R, typename RO>
 __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
                                                        RO* __restrict__ c, int f0, int f1,
                                                        int f2) {
@@ -350,12 +355,9 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
 // one thread per fine cell (src/multigrid.c:177-205), either overwriting the
 // fine value or adding to it (the V-cycle correction, src/multigrid.c:296).
 template <typename RC, typename R, bool ADD>
-__device__ __forceinline__ void prolong_body(const RC* __restrict__ uc, R* __restrict__ fine,
-                                             int c0, int c1, int c2) {
+__device__ __forceinline__ void prolong_at(const RC* __restrict__ uc, R* __restrict__ fine,
+                                           int c0, int c1, int c2, size_t i) {
   const int F1 = 2 * c1, F2 = 2 * c2;
-  const size_t total = static_cast<size_t>(2 * c0) * F1 * F2;
-  const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
-  if (i >= total) return;
   int X, Y, Z;
   unflatten(i, F1, F2, X, Y, Z);
   const int jx = X >> 1, jy = Y >> 1, jz = Z >> 1;
@@ -384,6 +386,14 @@ __device__ __forceinline__ void prolong_body(const RC* __restrict__ uc, R* __res
     fine[i] += val;
   else
     fine[i] = val;
+}
+
+template <typename RC, typename R, bool ADD>
+__device__ __forceinline__ void prolong_body(const RC* __restrict__ uc, R* __restrict__ fine,
+                                             int c0, int c1, int c2) {
+  const size_t total = static_cast<size_t>(2 * c0) * (2 * c1) * (2 * c2);
+  const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
+  if (i < total) prolong_at<RC, R, ADD>(uc, fine, c0, c1, c2, i);
 }
 
 template <typename RC, typename R, bool ADD>
@@ -481,6 +491,126 @@ template <typename R>
 __global__ __launch_bounds__(kThreads) void k_fill_zero_group(GroupArgs<R, R, R> A, size_t n) {
   const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
   if (i < n) A.out[blockIdx.y][i] = R(0);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse levels in one workgroup.  Below kCoarseCells cells the launches of a
+// level are latency, not work: a V-cycle from such a level down to the
+// coarsest runs in one 1024-thread workgroup per problem, every step sweeping
+// all cells of its level and ending with a barrier (the levels sit in L2).
+// Per cell the arithmetic is the generic kernels', so the bits are the same.
+// ---------------------------------------------------------------------------
+constexpr int kCoarseThreads = 1024;
+constexpr size_t kCoarseCells = 4096;
+constexpr int kMaxCoarse = 4;
+constexpr int kCoarseGroup = 4;
+
+template <typename R>
+struct CoarseLevel {
+  LevelGeom<R> G;
+  R* v;
+  R* f;
+  R* tmp;
+};
+
+template <typename R>
+struct CoarseArgs {
+  int nlev, niter, zero_start;
+  R omega, keep;
+  CoarseLevel<R> L[kCoarseGroup][kMaxCoarse];  // relative level 0 is the V-cycle's top level
+  R* cur0[kCoarseGroup];
+  R* alt0[kCoarseGroup];
+  const R* src0[kCoarseGroup];
+};
+
+template <typename R>
+__device__ __forceinline__ void cw_jacobi(const LevelGeom<R>& G, const R* in, const R* src, R* out,
+                                          R omega, R keep, bool zero_in) {
+  const int n1 = G.n[1], n2 = G.n[2];
+  const int total = G.n[0] * n1 * n2;
+  for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
+    int ix, iy, iz;
+    unflatten(i, n1, n2, ix, iy, iz);
+    const Coef<R> k = cell_coef(G, ix, iy, iz);
+    const Hood<R> h = zero_in ? zero_hood<R>() : gather_hood(in, G.n[0], n1, n2, ix, iy, iz);
+    out[i] = jacobi_update(k, h, src[i], omega, keep, G.local);
+  }
+  __syncthreads();
+}
+
+template <typename R>
+__device__ __forceinline__ void cw_smooth(const LevelGeom<R>& G, R*& cur, R*& alt, const R* src,
+                                          int niter, R omega, R keep, bool zero_start) {
+  if (niter <= 0) {
+    if (zero_start) {
+      const int total = G.n[0] * G.n[1] * G.n[2];
+      for (int i = threadIdx.x; i < total; i += kCoarseThreads) cur[i] = R(0);
+      __syncthreads();
+    }
+    return;
+  }
+  for (int it = 0; it < niter; it++) {
+    cw_jacobi(G, cur, src, alt, omega, keep, zero_start && it == 0);
+    R* t = cur;
+    cur = alt;
+    alt = t;
+  }
+}
+
+// src/multigrid.c:270-300 from relative level 0 to the coarsest, unrolled over
+// the (at most kMaxCoarse) levels so the level pointers stay in registers.
+template <typename R>
+__global__ __launch_bounds__(kCoarseThreads) void k_vcycle_coarse(CoarseArgs<R> A) {
+  const int job = blockIdx.x;
+  const int nl = A.nlev;
+  R* cur[kMaxCoarse];
+  R* alt[kMaxCoarse];
+  const R* src[kMaxCoarse];
+#pragma unroll
+  for (int m = 0; m < kMaxCoarse; m++) {
+    if (m >= nl) break;
+    const CoarseLevel<R>& Lm = A.L[job][m];
+    if (m == 0) {
+      cur[0] = A.cur0[job];
+      alt[0] = A.alt0[job];
+      src[0] = A.src0[job];
+    } else {
+      cur[m] = Lm.v;
+      alt[m] = Lm.tmp;
+      src[m] = Lm.f;
+    }
+    cw_smooth(Lm.G, cur[m], alt[m], src[m], A.niter, A.omega, A.keep, m > 0 || A.zero_start);
+    if (m + 1 < nl) {
+      const LevelGeom<R>& G = Lm.G;
+      const int total = G.n[0] * G.n[1] * G.n[2];
+      for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
+        int ix, iy, iz;
+        unflatten(i, G.n[1], G.n[2], ix, iy, iz);
+        const Coef<R> k = cell_coef(G, ix, iy, iz);
+        const Hood<R> h = gather_hood(cur[m], G.n[0], G.n[1], G.n[2], ix, iy, iz);
+        alt[m][i] = residual_value(k, h, src[m][i], G.local);
+      }
+      __syncthreads();
+      const int ctotal = (G.n[0] / 2) * (G.n[1] / 2) * (G.n[2] / 2);
+      R* fc = A.L[job][m + 1].f;
+      for (int j = threadIdx.x; j < ctotal; j += kCoarseThreads)
+        restrict_at<R, R>(alt[m], fc, G.n[0], G.n[1], G.n[2], j);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int m = kMaxCoarse - 1; m >= 0; m--) {
+    if (m >= nl) continue;
+    const CoarseLevel<R>& Lm = A.L[job][m];
+    if (m + 1 < nl) {
+      const LevelGeom<R>& Gc = A.L[job][m + 1].G;
+      const int ftotal = 8 * Gc.n[0] * Gc.n[1] * Gc.n[2];
+      for (int i = threadIdx.x; i < ftotal; i += kCoarseThreads)
+        prolong_at<R, R, true>(cur[m + 1], cur[m], Gc.n[0], Gc.n[1], Gc.n[2], i);
+      __syncthreads();
+    }
+    cw_smooth(Lm.G, cur[m], alt[m], src[m], A.niter, A.omega, A.keep, false);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -724,8 +854,49 @@ struct Solver {
     return PRH_OK;
   }
 
+  // first level handled by the single-workgroup coarse kernel
+  int coarse_level() const {
+    for (int l = 0; l < nlev(); l++)
+      if (size(l) <= kCoarseCells && nlev() - l <= kMaxCoarse) return l;
+    return nlev();
+  }
+
+  // V-cycle from level l (>= coarse_level()) to the coarsest in one launch per
+  // kCoarseGroup problems; cur and alt end where they started (2 niter swaps
+  // per level).
+  int vcycle_coarse(int l, const Ptrs<R>& cur, const Ptrs<R>& alt, const Ptrs<R>& src,
+                    bool zero_start) {
+    const int nl = nlev() - l;
+    for (int j0 = 0; j0 < count; j0 += kCoarseGroup) {
+      const int nj = count - j0 < kCoarseGroup ? count - j0 : kCoarseGroup;
+      CoarseArgs<R> A;
+      A.nlev = nl;
+      A.niter = niter;
+      A.zero_start = zero_start ? 1 : 0;
+      A.omega = omega;
+      A.keep = keep;
+      for (int q = 0; q < kCoarseGroup; q++) {
+        const int j = j0 + (q < nj ? q : 0);
+        A.cur0[q] = cur[j];
+        A.alt0[q] = alt[j];
+        A.src0[q] = src[j];
+        for (int m = 0; m < kMaxCoarse; m++) {
+          const int lm = l + (m < nl ? m : 0);
+          A.L[q][m].G = G[j][lm];
+          A.L[q][m].v = static_cast<R*>(H[j]->v[lm]);
+          A.L[q][m].f = static_cast<R*>(H[j]->f[lm]);
+          A.L[q][m].tmp = static_cast<R*>(H[j]->tmp[lm]);
+        }
+      }
+      k_vcycle_coarse<R><<<nj, kCoarseThreads, 0, s>>>(A);
+      PRH_LAUNCHED();
+    }
+    return PRH_OK;
+  }
+
   // src/multigrid.c:270-300
   int vcycle(int l, Ptrs<R>& cur, Ptrs<R>& alt, const Ptrs<R>& src, bool zero_start) {
+    if (l >= coarse_level()) return vcycle_coarse(l, cur, alt, src, zero_start);
     PRH_TRY(smooth(l, cur, alt, src, zero_start));
     if (l + 1 < nlev()) {
       PRH_TRY(residual(l, cur, src, alt));

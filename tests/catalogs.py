@@ -133,16 +133,19 @@ RECON_KWARGS = dict(f=0.8, bias=2.0, los=None)
 SMOOTHING_RADIUS = 15.
 
 
-def make_catalogs(name, scale=1., workers=8, with_randoms=True):
+def make_catalogs(name, scale=1., workers=8, with_randoms=True, progress=None):
     """Data/randoms positions and weights of a configuration.
 
-    ``scale`` < 1 shrinks the point counts (not the mesh) for quick tests.
+    ``scale`` < 1 shrinks the point counts (not the mesh) for quick tests; ``progress`` is
+    called with a short message after each generation step.
     """
+    progress = progress or (lambda msg: None)
     cfg = dict(CONFIGS[name])
     ndata = int(cfg['ndata'] * scale)
     nrandoms = int(cfg['nrandoms'] * scale)
     boxsize, boxcenter = cfg['boxsize'], cfg['boxcenter']
     x = zeldovich_catalog(cfg['nlattice'], boxsize, boxcenter, f=RECON_KWARGS['f'], workers=workers)
+    progress('displaced lattice of {:d} points'.format(len(x)))
     rng = np.random.default_rng(SAMPLE_SEED)
     if cfg['geometry'] == 'shell':
         keep, r = in_shell_box(x, boxsize, boxcenter)
@@ -160,6 +163,7 @@ def make_catalogs(name, scale=1., workers=8, with_randoms=True):
     else:
         data_weights = fkp_weights(np.sqrt(np.sum(data**2, axis=-1)))
     out = dict(config=cfg, data=data, data_weights=data_weights)
+    progress('data catalog')
     if with_randoms:
         if cfg['geometry'] == 'shell':
             randoms = shell_randoms(nrandoms, boxsize, boxcenter, workers=workers)
