@@ -346,6 +346,19 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_permute(const uint32_t* _
 // Paint step 5: per target cell, merge the groups of its 8 source base cells
 // by point index and accumulate.  Source k = (a, b, c) is the base cell at
 // (x - a, y - b, z - c): the target is that point's corner (a, b, c).
+// The merge is software-pipelined: the next index of every group is fetched
+// ahead (the choice of the next point never waits for memory unless one group
+// wins twice in a row), and the payload of a chosen point is consumed one step
+// after its load was issued, with two payload slots alternating, so two loads
+// are in flight per thread.  The accumulation order is the merge order.
+template <typename R>
+__device__ __forceinline__ R corner_weight(const Payload<R>& q, int k) {
+  const R fx = (k & 4) ? q.t[0] : R(1) - q.t[0];
+  const R fy = (k & 2) ? q.t[1] : R(1) - q.t[1];
+  const R fz = (k & 1) ? q.t[2] : R(1) - q.t[2];
+  return fx * fy * fz * q.w;
+}
+
 template <typename R>
 __global__ __launch_bounds__(kCicThreads) void k_paint_gather(
     R* __restrict__ mesh, int n0, int n1, int n2, const uint32_t* __restrict__ start,
@@ -360,18 +373,27 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_gather(
   const int y = static_cast<int>(t % n1), x = static_cast<int>(t / n1);
   const int xm = x == 0 ? n0 - 1 : x - 1, ym = y == 0 ? n1 - 1 : y - 1,
             zm = z == 0 ? n2 - 1 : z - 1;
-  uint32_t beg[8], end[8], cur[8];
+  constexpr uint32_t kNone = 0xffffffffu;
+  uint32_t beg[8], end[8], cur[8], nxt[8];
+  bool any = false;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int sx = (k & 4) ? xm : x, sy = (k & 2) ? ym : y, sz = (k & 1) ? zm : z;
     const size_t sc = (static_cast<size_t>(sx) * n1 + sy) * n2 + sz;
     beg[k] = start[sc];
     end[k] = start[sc + 1];
-    cur[k] = beg[k] < end[k] ? sorted[beg[k]] : 0xffffffffu;
+    any |= beg[k] < end[k];
   }
-  bool any = false;
-  R acc = R(0);
-  for (;;) {
+  if (!any) return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    cur[k] = beg[k] < end[k] ? sorted[beg[k]] : kNone;
+    nxt[k] = beg[k] + 1 < end[k] ? sorted[beg[k] + 1] : kNone;
+  }
+  R acc = mesh[c];
+  // chooses the group with the smallest next index, returns its payload slot
+  // position and advances the group; -1 when all groups are exhausted
+  auto choose = [&](uint32_t& pos) -> int {
     uint32_t best = cur[0];
     int bk = 0;
 #pragma unroll
@@ -380,24 +402,33 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_gather(
         best = cur[k];
         bk = k;
       }
-    if (best == 0xffffffffu) break;
-    if (!any) {
-      acc = mesh[c];
-      any = true;
-    }
-    const Payload<R> q = spay[beg[bk]];
-    const R fx = (bk & 4) ? q.t[0] : R(1) - q.t[0];
-    const R fy = (bk & 2) ? q.t[1] : R(1) - q.t[1];
-    const R fz = (bk & 1) ? q.t[2] : R(1) - q.t[2];
-    acc += fx * fy * fz * q.w;
+    if (best == kNone) return -1;
 #pragma unroll
     for (int k = 0; k < 8; k++)
       if (k == bk) {
+        pos = beg[k];
         beg[k]++;
-        cur[k] = beg[k] < end[k] ? sorted[beg[k]] : 0xffffffffu;
+        cur[k] = nxt[k];
+        nxt[k] = beg[k] + 1 < end[k] ? sorted[beg[k] + 1] : kNone;
       }
+    return bk;
+  };
+  Payload<R> qa, qb;
+  int kb = -1;
+  for (;;) {
+    uint32_t pa = 0;
+    const int ka = choose(pa);
+    if (ka < 0) break;
+    qa = spay[pa];
+    if (kb >= 0) acc += corner_weight(qb, kb);
+    uint32_t pb = 0;
+    kb = choose(pb);
+    if (kb >= 0) qb = spay[pb];
+    acc += corner_weight(qa, ka);
+    if (kb < 0) break;
   }
-  if (any) mesh[c] = acc;
+  if (kb >= 0) acc += corner_weight(qb, kb);
+  mesh[c] = acc;
 }
 
 __global__ void k_zero_misc(uint32_t* misc) {
