@@ -229,6 +229,6 @@ class OracleOriginalMultiGridReconstruction(OracleMultiGridReconstruction):
             nonzero = delta != 0.
             delta[nonzero] -= np.mean(delta[nonzero])
             self.alpha, self.mask_count = alpha, int(mask.sum())
-        self.mesh_delta = self.smooth_gaussian(delta, smoothing_radius)
+        self.mesh_delta = np.ascontiguousarray(self.smooth_gaussian(delta, smoothing_radius))
         self.mesh_data = self.mesh_randoms = None
 

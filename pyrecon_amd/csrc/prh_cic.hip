@@ -343,14 +343,8 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_permute(const uint32_t* _
   if (s < m) spay[s] = payload[sorted[s]];
 }
 
-// Paint step 5: per target cell, merge the groups of its 8 source base cells
-// by point index and accumulate.  Source k = (a, b, c) is the base cell at
-// (x - a, y - b, z - c): the target is that point's corner (a, b, c).
-// The merge is software-pipelined: the next index of every group is fetched
-// ahead (the choice of the next point never waits for memory unless one group
-// wins twice in a row), and the payload of a chosen point is consumed one step
-// after its load was issued, with two payload slots alternating, so two loads
-// are in flight per thread.  The accumulation order is the merge order.
+// CIC weight of a point at corner k = (a, b, c) of its base cell, in the
+// reference's product order (src/mesh.c:45-60).
 template <typename R>
 __device__ __forceinline__ R corner_weight(const Payload<R>& q, int k) {
   const R fx = (k & 4) ? q.t[0] : R(1) - q.t[0];
@@ -359,41 +353,23 @@ __device__ __forceinline__ R corner_weight(const Payload<R>& q, int k) {
   return fx * fy * fz * q.w;
 }
 
+// Merge of the (at most 8) groups of one target cell in point-index order,
+// accumulating onto `acc`; groups are [beg[k], end[k]) of `idx` / `pay`
+// (in LDS or global memory).  Returns whether any point contributed.
 template <typename R>
-__global__ __launch_bounds__(kCicThreads) void k_paint_gather(
-    R* __restrict__ mesh, int n0, int n1, int n2, const uint32_t* __restrict__ start,
-    const uint32_t* __restrict__ sorted, const Payload<R>* __restrict__ spay,
-    const uint32_t* __restrict__ misc) {
-  if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
-  const size_t total = static_cast<size_t>(n0) * n1 * n2;
-  const size_t c = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
-  if (c >= total) return;
-  const int z = static_cast<int>(c % n2);
-  const size_t t = c / n2;
-  const int y = static_cast<int>(t % n1), x = static_cast<int>(t / n1);
-  const int xm = x == 0 ? n0 - 1 : x - 1, ym = y == 0 ? n1 - 1 : y - 1,
-            zm = z == 0 ? n2 - 1 : z - 1;
+__device__ __forceinline__ bool merge_groups(uint32_t beg[8], const uint32_t end[8],
+                                             const uint32_t* __restrict__ idx,
+                                             const Payload<R>* __restrict__ pay, R& acc) {
   constexpr uint32_t kNone = 0xffffffffu;
-  uint32_t beg[8], end[8], cur[8], nxt[8];
+  uint32_t cur[8];
   bool any = false;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const int sx = (k & 4) ? xm : x, sy = (k & 2) ? ym : y, sz = (k & 1) ? zm : z;
-    const size_t sc = (static_cast<size_t>(sx) * n1 + sy) * n2 + sz;
-    beg[k] = start[sc];
-    end[k] = start[sc + 1];
+    cur[k] = beg[k] < end[k] ? idx[beg[k]] : kNone;
     any |= beg[k] < end[k];
   }
-  if (!any) return;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    cur[k] = beg[k] < end[k] ? sorted[beg[k]] : kNone;
-    nxt[k] = beg[k] + 1 < end[k] ? sorted[beg[k] + 1] : kNone;
-  }
-  R acc = mesh[c];
-  // chooses the group with the smallest next index, returns its payload slot
-  // position and advances the group; -1 when all groups are exhausted
-  auto choose = [&](uint32_t& pos) -> int {
+  if (!any) return false;
+  for (;;) {
     uint32_t best = cur[0];
     int bk = 0;
 #pragma unroll
@@ -402,33 +378,121 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_gather(
         best = cur[k];
         bk = k;
       }
-    if (best == kNone) return -1;
+    if (best == kNone) break;
+    uint32_t pos = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++)
       if (k == bk) {
-        pos = beg[k];
-        beg[k]++;
-        cur[k] = nxt[k];
-        nxt[k] = beg[k] + 1 < end[k] ? sorted[beg[k] + 1] : kNone;
+        pos = beg[k]++;
+        cur[k] = beg[k] < end[k] ? idx[beg[k]] : kNone;
       }
-    return bk;
-  };
-  Payload<R> qa, qb;
-  int kb = -1;
-  for (;;) {
-    uint32_t pa = 0;
-    const int ka = choose(pa);
-    if (ka < 0) break;
-    qa = spay[pa];
-    if (kb >= 0) acc += corner_weight(qb, kb);
-    uint32_t pb = 0;
-    kb = choose(pb);
-    if (kb >= 0) qb = spay[pb];
-    acc += corner_weight(qa, ka);
-    if (kb < 0) break;
+    acc += corner_weight(pay[pos], bk);
   }
-  if (kb >= 0) acc += corner_weight(qb, kb);
-  mesh[c] = acc;
+  return true;
+}
+
+// Paint step 5, tiled: a workgroup owns an 8^3 tile of target cells; the
+// groups of the 9^3 base cells that reach it (the tile and its lower
+// neighbours) are staged in LDS (indices and payloads, read once, coalesced),
+// and each thread merges its cell's 8 groups from LDS.  A tile whose groups
+// exceed the LDS capacity merges from global memory instead.  Source k =
+// (a, b, c) of a target cell is the base cell at (x - a, y - b, z - c).
+constexpr int kGT = 8;                      // tile edge
+constexpr int kGE = kGT + 1;                // staged region edge
+constexpr int kGCells = kGE * kGE * kGE;    // 729 staged base cells
+constexpr int kGThreads = kGT * kGT * kGT;  // one thread per target cell
+template <typename R>
+struct GatherCap {
+  static constexpr int value = sizeof(R) == 8 ? 1792 : 3072;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
+    R* __restrict__ mesh, int n0, int n1, int n2, int tiles1, int tiles2,
+    const uint32_t* __restrict__ start, const uint32_t* __restrict__ sorted,
+    const Payload<R>* __restrict__ spay, const uint32_t* __restrict__ misc) {
+  constexpr int kCap = GatherCap<R>::value;
+  __shared__ uint32_t s_gbeg[kGCells];  // global start of each staged base cell
+  __shared__ uint32_t s_cnt[kGCells];   // its number of points
+  __shared__ uint32_t s_off[kGCells];   // its first slot in the staged arrays
+  __shared__ uint32_t s_row[kGE * kGE];
+  __shared__ uint32_t s_idx[kCap];
+  __shared__ Payload<R> s_pay[kCap];
+  if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
+  const int tid = threadIdx.x;
+  const int bz = blockIdx.x % tiles2;
+  const int by = (blockIdx.x / tiles2) % tiles1;
+  const int bx = blockIdx.x / (tiles2 * tiles1);
+  const int x0 = bx * kGT, y0 = by * kGT, z0 = bz * kGT;
+  auto wrap = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+  for (int e = tid; e < kGCells; e += kGThreads) {
+    const int ex = e / (kGE * kGE), ey = (e / kGE) % kGE, ez = e % kGE;
+    const size_t bc = (static_cast<size_t>(wrap(x0 - 1 + ex, n0)) * n1 + wrap(y0 - 1 + ey, n1)) * n2 +
+                      wrap(z0 - 1 + ez, n2);
+    const uint32_t g0 = start[bc];
+    s_gbeg[e] = g0;
+    s_cnt[e] = start[bc + 1] - g0;
+  }
+  __syncthreads();
+  // offsets: row sums, then a serial scan over the 81 rows, then within rows
+  if (tid < kGE * kGE) {
+    uint32_t sum = 0;
+    for (int q = 0; q < kGE; q++) sum += s_cnt[tid * kGE + q];
+    s_row[tid] = sum;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int r = 0; r < kGE * kGE; r++) {
+      const uint32_t v = s_row[r];
+      s_row[r] = run;
+      run += v;
+    }
+    s_off[0] = run;  // total, read below before the per-row pass rewrites s_off
+  }
+  __syncthreads();
+  const uint32_t total = s_off[0];
+  __syncthreads();
+  if (total == 0) return;
+  if (tid < kGE * kGE) {
+    uint32_t run = s_row[tid];
+    for (int q = 0; q < kGE; q++) {
+      s_off[tid * kGE + q] = run;
+      run += s_cnt[tid * kGE + q];
+    }
+  }
+  __syncthreads();
+  const bool staged = total <= static_cast<uint32_t>(kCap);
+  if (staged) {
+    for (uint32_t p = tid; p < total; p += kGThreads) {
+      // staged cell holding slot p: binary search over the offsets
+      int lo = 0, hi = kGCells - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_off[mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      const uint32_t g = s_gbeg[lo] + (p - s_off[lo]);
+      s_idx[p] = sorted[g];
+      s_pay[p] = spay[g];
+    }
+    __syncthreads();
+  }
+  const int lz = tid % kGT, ly = (tid / kGT) % kGT, lx = tid / (kGT * kGT);
+  const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+  if (x >= n0 || y >= n1 || z >= n2) return;
+  uint32_t beg[8], end[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int e = (lx + 1 - ((k >> 2) & 1)) * kGE * kGE + (ly + 1 - ((k >> 1) & 1)) * kGE +
+                  (lz + 1 - (k & 1));
+    beg[k] = staged ? s_off[e] : s_gbeg[e];
+    end[k] = beg[k] + s_cnt[e];
+  }
+  const size_t c = (static_cast<size_t>(x) * n1 + y) * n2 + z;
+  R acc = mesh[c];
+  const bool any = staged ? merge_groups<R>(beg, end, s_idx, s_pay, acc)
+                          : merge_groups<R>(beg, end, sorted, spay, acc);
+  if (any) mesh[c] = acc;
 }
 
 __global__ void k_zero_misc(uint32_t* misc) {
@@ -686,10 +750,14 @@ static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* p
                                                   static_cast<Payload<R>*>(S.spay), m);
   PRH_LAUNCHED();
   timer_before(PRH_TIMER_PAINT_GATHER, s);
-  k_paint_gather<R><<<gcell, kCicThreads, 0, s>>>(mesh, F.n[0], F.n[1], F.n[2], S.start,
-                                                  S.sorted,
-                                                  static_cast<const Payload<R>*>(S.spay),
-                                                  S.misc);
+  {
+    const int t0 = (F.n[0] + kGT - 1) / kGT, t1 = (F.n[1] + kGT - 1) / kGT,
+              t2 = (F.n[2] + kGT - 1) / kGT;
+    const unsigned gtiles = static_cast<unsigned>(static_cast<size_t>(t0) * t1 * t2);
+    k_paint_gather_tiled<R><<<gtiles, kGThreads, 0, s>>>(
+        mesh, F.n[0], F.n[1], F.n[2], t1, t2, S.start, S.sorted,
+        static_cast<const Payload<R>*>(S.spay), S.misc);
+  }
   // algorithmic: cell starts + 8 contributions' payload per point + mesh read/write
   timer_after(PRH_TIMER_PAINT_GATHER, s,
               static_cast<double>(ncell) * (4 + 2 * sizeof(R)) +

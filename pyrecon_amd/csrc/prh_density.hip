@@ -1,58 +1,16 @@
-// Gaussian smoothing and density contrast on the device.
+// Density contrast on the device.
 //
-// Smoothing replaces RealMesh.smooth_gaussian(method='fft')
-// (pyrecon/mesh.py:655-680): the reference multiplies the full complex
-// spectrum by exp(-sum_i (r k_i)^2 / 2) with k_i = 2 pi fftfreq(n_i, L_i/n_i),
-// using numpy's complex-to-complex transforms.  The field is real, so the
-// half spectrum of a real-to-complex transform carries the same information
-// (the reference's own test_hermitian checks that both agree); hipFFT does
-// the transforms and one kernel applies the Gaussian and numpy's 1/N
-// normalisation of the inverse transform.
-//
-// The density contrast replaces the arithmetic of
-// BaseReconstruction.set_density_contrast (pyrecon/recon.py:227-247).  The
-// two sums are deterministic FP64 reductions (the reference uses numpy's
-// pairwise sums in the mesh precision); the scalar algebra is done by the
-// caller with numpy's scalar types, and the per-cell rule by one kernel.
+// Replaces the arithmetic of BaseReconstruction.set_density_contrast
+// (pyrecon/recon.py:227-247) and of OriginalMultiGridReconstruction's
+// (pyrecon/multigrid.py:69-97).  The sums are deterministic FP64 reductions
+// (the reference uses numpy's pairwise sums in the mesh precision); the scalar
+// algebra is done by the caller with numpy's scalar types, and the per-cell
+// rules by one kernel each.  The smoothing is in prh_smooth.hip.
 #include "prh_reduce.hpp"
 
 namespace prh {
 
 constexpr int kDenThreads = 256;
-
-template <typename R>
-struct Cplx {
-  R re, im;
-};
-
-template <typename R>
-__global__ __launch_bounds__(kDenThreads) void k_gauss_multiply(Cplx<R>* __restrict__ spec, int n0,
-                                                                int n1, int n2, int nh, double v0,
-                                                                double v1, double v2, double r0,
-                                                                double r1, double r2,
-                                                                double norm) {
-  const size_t total = static_cast<size_t>(n0) * n1 * nh;
-  const size_t i = blockIdx.x * static_cast<size_t>(kDenThreads) + threadIdx.x;
-  if (i >= total) return;
-  const int iz = static_cast<int>(i % nh);
-  const size_t t = i / nh;
-  const int iy = static_cast<int>(t % n1), ix = static_cast<int>(t / n1);
-  // numpy.fft.fftfreq ordering: i < (n-1)//2 + 1 -> i, else i - n
-  const int fx = ix < (n0 - 1) / 2 + 1 ? ix : ix - n0;
-  const int fy = iy < (n1 - 1) / 2 + 1 ? iy : iy - n1;
-  const int fz = iz < (n2 - 1) / 2 + 1 ? iz : iz - n2;
-  constexpr double twopi = 2 * 3.141592653589793;
-  const double kx = twopi * (static_cast<double>(fx) * v0);
-  const double ky = twopi * (static_cast<double>(fy) * v1);
-  const double kz = twopi * (static_cast<double>(fz) * v2);
-  const double ax = r0 * kx, ay = r1 * ky, az = r2 * kz;
-  const double e = -0.5 * (ax * ax) + -0.5 * (ay * ay) + -0.5 * (az * az);
-  const double g = exp(e) * norm;
-  Cplx<R> c = spec[i];
-  c.re = static_cast<R>(static_cast<double>(c.re) * g);
-  c.im = static_cast<R>(static_cast<double>(c.im) * g);
-  spec[i] = c;
-}
 
 template <typename R>
 __global__ __launch_bounds__(kReduceThreads) void k_sums(const R* __restrict__ a,
@@ -145,83 +103,6 @@ __global__ __launch_bounds__(kDenThreads) void k_subtract_nonzero(R* __restrict_
   if (x != R(0)) delta[i] = x - mean;
 }
 
-static int fft_status(hipfftResult r, const char* what) {
-  if (r != HIPFFT_SUCCESS) {
-    set_error("%s failed with hipfftResult %d", what, static_cast<int>(r));
-    return PRH_ERR_FFT;
-  }
-  return PRH_OK;
-}
-
-template <typename R>
-static int exec_r2c(prh_workspace* ws, R* mesh);
-template <>
-int exec_r2c<double>(prh_workspace* ws, double* mesh) {
-  return fft_status(hipfftExecD2Z(ws->fft.r2c, mesh,
-                                  static_cast<hipfftDoubleComplex*>(ws->fft.spectrum)),
-                    "hipfftExecD2Z");
-}
-template <>
-int exec_r2c<float>(prh_workspace* ws, float* mesh) {
-  return fft_status(
-      hipfftExecR2C(ws->fft.r2c, mesh, static_cast<hipfftComplex*>(ws->fft.spectrum)),
-      "hipfftExecR2C");
-}
-template <typename R>
-static int exec_c2r(prh_workspace* ws, R* mesh);
-template <>
-int exec_c2r<double>(prh_workspace* ws, double* mesh) {
-  return fft_status(
-      hipfftExecZ2D(ws->fft.c2r, static_cast<hipfftDoubleComplex*>(ws->fft.spectrum), mesh),
-      "hipfftExecZ2D");
-}
-template <>
-int exec_c2r<float>(prh_workspace* ws, float* mesh) {
-  return fft_status(hipfftExecC2R(ws->fft.c2r, static_cast<hipfftComplex*>(ws->fft.spectrum), mesh),
-                    "hipfftExecC2R");
-}
-
-template <typename R>
-static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* boxsize,
-                     const double* radius, void* stream) {
-  if (!ws || !mesh || !nmesh || !boxsize || !radius) return PRH_ERR_ARG;
-  if (!(ws->flags & PRH_WS_FFT) || !ws->fft.have) {
-    set_error("workspace has no FFT plans");
-    return PRH_ERR_STATE;
-  }
-  if (ws->dtype != Tag<R>::code) {
-    set_error("workspace precision does not match the mesh");
-    return PRH_ERR_ARG;
-  }
-  for (int d = 0; d < 3; d++)
-    if (ws->n[d] != nmesh[d]) {
-      set_error("workspace mesh shape does not match nmesh");
-      return PRH_ERR_CAPACITY;
-    }
-  hipStream_t s = as_stream(stream);
-  PRH_TRY(fft_status(hipfftSetStream(ws->fft.r2c, s), "hipfftSetStream"));
-  PRH_TRY(fft_status(hipfftSetStream(ws->fft.c2r, s), "hipfftSetStream"));
-  PRH_TRY(exec_r2c<R>(ws, mesh));
-  double val[3], rad[3];
-  for (int d = 0; d < 3; d++) {
-    // d_i = boxsize / nmesh (promoted to float64), val = 1 / (n d) as in numpy.fft.fftfreq
-    const double cell = static_cast<double>(boxsize[d]) / static_cast<double>(static_cast<int>(nmesh[d]));
-    val[d] = 1.0 / (static_cast<double>(nmesh[d]) * cell);
-    rad[d] = static_cast<double>(static_cast<R>(radius[d]));  // radius_ is in the mesh dtype
-  }
-  const int n0 = static_cast<int>(nmesh[0]), n1 = static_cast<int>(nmesh[1]),
-            n2 = static_cast<int>(nmesh[2]);
-  const int nh = n2 / 2 + 1;
-  const size_t total = static_cast<size_t>(n0) * n1 * nh;
-  const double norm = 1.0 / (static_cast<double>(n0) * n1 * n2);
-  k_gauss_multiply<R><<<ceil_div(total, kDenThreads), kDenThreads, 0, s>>>(
-      static_cast<Cplx<R>*>(ws->fft.spectrum), n0, n1, n2, nh, val[0], val[1], val[2], rad[0],
-      rad[1], rad[2], norm);
-  PRH_LAUNCHED();
-  PRH_TRY(exec_c2r<R>(ws, mesh));
-  return PRH_OK;
-}
-
 // Sums the nv partial rows left in ws->red by a block_reduce_range kernel and
 // copies the nv results to sums (host); synchronizes the stream.
 static int finish_sums(prh_workspace* ws, int nv, double* sums, hipStream_t s) {
@@ -297,15 +178,6 @@ static int do_density(const R* data, const R* randoms, R* delta, size_t n, R alp
 using namespace prh;
 
 extern "C" {
-
-int prh_smooth_gaussian_f32(prh_workspace* ws, float* mesh, const size_t* nmesh,
-                            const float* boxsize, const double* radius, void* stream) {
-  return do_smooth<float>(ws, mesh, nmesh, boxsize, radius, stream);
-}
-int prh_smooth_gaussian_f64(prh_workspace* ws, double* mesh, const size_t* nmesh,
-                            const double* boxsize, const double* radius, void* stream) {
-  return do_smooth<double>(ws, mesh, nmesh, boxsize, radius, stream);
-}
 
 int prh_mesh_sums_f32(prh_workspace* ws, const float* a, const float* b, size_t size,
                       double* sums, void* stream) {

@@ -109,52 +109,9 @@ static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags)
     p.paint = nc * 4 + (nc + 1) * 4 + (npart + 1) * 4 + chunk * 4 * 3 + 2 * chunk * 4 * w + 4 * 4 +
               (chunk / 32 + 2) * 4;
   }
-  if (flags & PRH_WS_FFT) {
-    const size_t spec = n[0] * n[1] * (n[2] / 2 + 1) * 2 * w;
-    p.fft = spec + spec;  // spectrum + work area (upper bound used for admission)
-  }
+  if (flags & PRH_WS_FFT) p.fft = smooth_bytes(n, dtype);
   p.red = (kReduceBlocks * 2 + 16) * sizeof(double) + 16 * sizeof(uint32_t);
   return p;
-}
-
-static int fft_setup(prh_workspace* ws) {
-  FFTState& F = ws->fft;
-  const int nx = static_cast<int>(ws->n[0]), ny = static_cast<int>(ws->n[1]),
-            nz = static_cast<int>(ws->n[2]);
-  const bool dbl = ws->dtype == PRH_F64;
-  const size_t spec = ws->n[0] * ws->n[1] * (ws->n[2] / 2 + 1) * 2 * static_cast<size_t>(ws->dtype);
-  PRH_TRY(ws_alloc(ws, &F.spectrum, spec));
-  size_t w1 = 0, w2 = 0;
-  if (hipfftCreate(&F.r2c) != HIPFFT_SUCCESS || hipfftCreate(&F.c2r) != HIPFFT_SUCCESS) {
-    set_error("hipfftCreate failed");
-    return PRH_ERR_FFT;
-  }
-  F.have = true;
-  const char* policy = getenv("PRH_FFT_WORK");
-  const bool autoalloc = policy && !strcmp(policy, "auto");
-  if (!autoalloc) {
-    hipfftSetAutoAllocation(F.r2c, 0);
-    hipfftSetAutoAllocation(F.c2r, 0);
-  }
-  if (hipfftMakePlan3d(F.r2c, nx, ny, nz, dbl ? HIPFFT_D2Z : HIPFFT_R2C, &w1) != HIPFFT_SUCCESS ||
-      hipfftMakePlan3d(F.c2r, nx, ny, nz, dbl ? HIPFFT_Z2D : HIPFFT_C2R, &w2) != HIPFFT_SUCCESS) {
-    set_error("hipfftMakePlan3d(%d, %d, %d) failed", nx, ny, nz);
-    return PRH_ERR_FFT;
-  }
-  if (getenv("PRH_FFT_DEBUG"))
-    fprintf(stderr, "fft plan %d %d %d dtype %d work %zu %zu\n", nx, ny, nz, ws->dtype, w1, w2);
-  if (autoalloc) return PRH_OK;
-  F.work_bytes = w1 > w2 ? w1 : w2;
-  if (policy && !strcmp(policy, "big") && F.work_bytes < 2 * spec) F.work_bytes = 2 * spec;
-  if (F.work_bytes) {
-    PRH_TRY(ws_alloc(ws, &F.work, F.work_bytes));
-    if (hipfftSetWorkArea(F.r2c, F.work) != HIPFFT_SUCCESS ||
-        hipfftSetWorkArea(F.c2r, F.work) != HIPFFT_SUCCESS) {
-      set_error("hipfftSetWorkArea failed");
-      return PRH_ERR_FFT;
-    }
-  }
-  return PRH_OK;
 }
 
 static int hierarchy_alloc(prh_workspace* ws, Hierarchy& H, size_t elem, bool full_top) {
@@ -375,12 +332,10 @@ int prh_workspace_destroy(prh_workspace* ws) {
   for (void* p : {(void*)P.count, (void*)P.start, (void*)P.partial, (void*)P.key, (void*)P.rank,
                   (void*)P.sorted, P.payload, P.spay, (void*)P.misc, (void*)P.big})
     release(p);
-  if (ws->fft.have) {
-    hipfftDestroy(ws->fft.r2c);
-    hipfftDestroy(ws->fft.c2r);
+  for (int d = 0; d < 3; d++) {
+    release(ws->smooth.tw[d]);
+    release(ws->smooth.gain[d]);
   }
-  release(ws->fft.spectrum);
-  release(ws->fft.work);
   release(ws->red);
   release(ws->misc);
   if (ws->red_host) (void)hipHostFree(ws->red_host);
@@ -425,7 +380,7 @@ int prh_workspace_create(prh_workspace** out, const size_t* nmesh, int dtype, si
   if (s == PRH_OK && (flags & PRH_WS_MIXED) && dtype == PRH_F64)
     s = hierarchy_alloc(ws, ws->mixed, sizeof(float), true);
   if (s == PRH_OK && (flags & PRH_WS_PAINT)) s = paint_alloc(ws, paint_chunk);
-  if (s == PRH_OK && (flags & PRH_WS_FFT)) s = fft_setup(ws);
+  if (s == PRH_OK && (flags & PRH_WS_FFT)) s = smooth_setup(ws);
   if (s == PRH_OK) {
     void* p;
     s = ws_alloc(ws, &p, (kReduceBlocks * 2 + 16) * sizeof(double));

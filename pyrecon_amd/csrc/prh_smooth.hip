@@ -1,0 +1,338 @@
+// Gaussian smoothing of a real mesh in Fourier space, on the device, without
+// a 3-D FFT library.
+//
+// The reference (RealMesh.smooth_gaussian(method='fft'), pyrecon/mesh.py:
+// 655-680) multiplies the 3-D spectrum by exp(-sum_i (r_i k_i)^2 / 2) with
+// k_i = 2 pi fftfreq(n_i, L_i / n_i) and transforms back.  The filter is the
+// product of one factor per axis, and the 3-D DFT is the product of 1-D DFTs
+// along the axes, so the same field results from filtering along each axis in
+// turn: for every line along the axis, forward DFT, multiply by
+// exp(-(r k)^2 / 2) / n, inverse DFT.  Each pass reads and writes the mesh
+// once.  The mesh is real and the 1-D filter is real and even, so two real
+// lines are filtered at once as the real and imaginary parts of one complex
+// line.
+//
+// The 1-D transforms are mixed-radix Stockham FFTs (radix 2, 3, 4, 5, 7) on
+// lines held in LDS, ping-ponging between two buffers; twiddles come from a
+// per-axis table exp(-2 pi i k / n) built on the host in double precision.
+#include <cmath>
+#include <vector>
+
+#include "prh_workspace.hpp"
+
+namespace prh {
+
+constexpr int kSmoothThreads = 256;
+constexpr size_t kSmoothLds = 65536;  // bytes of the two line buffers of a block
+
+template <typename R>
+struct Cx {
+  R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// One Stockham pass of radix RAD over LP lines of length n (a -> b).
+template <typename R, int RAD>
+__device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>* __restrict__ b,
+                                              int n, int LP, int Ns,
+                                              const Cx<R>* __restrict__ tw, bool inverse) {
+  const int m = n / RAD;
+  const int span = n / (Ns * RAD);  // twiddle index step
+  const int rstep = n / RAD;        // index step of the RAD-th roots of unity
+  for (int t = threadIdx.x; t < LP * m; t += kSmoothThreads) {
+    const int l = t / m;
+    const int j = t - l * m;
+    const int k = j % Ns;
+    const Cx<R>* al = a + l * n;
+    Cx<R> v[RAD];
+#pragma unroll
+    for (int r = 0; r < RAD; r++) v[r] = al[j + r * m];
+#pragma unroll
+    for (int r = 1; r < RAD; r++) {
+      Cx<R> w = tw[(r * k * span) % n];
+      if (inverse) w.im = -w.im;
+      v[r] = cmul(v[r], w);
+    }
+    Cx<R> o[RAD];
+    if constexpr (RAD == 2) {
+      o[0] = {v[0].re + v[1].re, v[0].im + v[1].im};
+      o[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
+    } else if constexpr (RAD == 4) {
+      const Cx<R> s02 = {v[0].re + v[2].re, v[0].im + v[2].im};
+      const Cx<R> d02 = {v[0].re - v[2].re, v[0].im - v[2].im};
+      const Cx<R> s13 = {v[1].re + v[3].re, v[1].im + v[3].im};
+      const Cx<R> d13 = {v[1].re - v[3].re, v[1].im - v[3].im};
+      // forward: -i * d13; inverse: +i * d13
+      const Cx<R> rot = inverse ? Cx<R>{-d13.im, d13.re} : Cx<R>{d13.im, -d13.re};
+      o[0] = {s02.re + s13.re, s02.im + s13.im};
+      o[2] = {s02.re - s13.re, s02.im - s13.im};
+      o[1] = {d02.re + rot.re, d02.im + rot.im};
+      o[3] = {d02.re - rot.re, d02.im - rot.im};
+    } else {
+      // direct DFT with the RAD-th roots of unity taken from the table
+#pragma unroll
+      for (int q = 0; q < RAD; q++) {
+        Cx<R> acc = v[0];
+#pragma unroll
+        for (int r = 1; r < RAD; r++) {
+          Cx<R> w = tw[((q * r) % RAD) * rstep];
+          if (inverse) w.im = -w.im;
+          const Cx<R> p = cmul(v[r], w);
+          acc.re += p.re;
+          acc.im += p.im;
+        }
+        o[q] = acc;
+      }
+    }
+    Cx<R>* bl = b + l * n;
+    const int d = (j / Ns) * Ns * RAD + k;
+#pragma unroll
+    for (int r = 0; r < RAD; r++) bl[d + r * Ns] = o[r];
+  }
+}
+
+struct AxisPlan {
+  int n, npass;
+  int radix[24];
+};
+
+// Runs the passes of `P` on the LP lines in buf[0]; returns the buffer holding
+// the result (natural order).
+template <typename R>
+__device__ Cx<R>* fft_lines(Cx<R>* b0, Cx<R>* b1, int LP, const AxisPlan& P,
+                            const Cx<R>* __restrict__ tw, bool inverse) {
+  Cx<R>* a = b0;
+  Cx<R>* b = b1;
+  int Ns = 1;
+  for (int p = 0; p < P.npass; p++) {
+    const int rad = P.radix[p];
+    switch (rad) {
+      case 2: stockham_pass<R, 2>(a, b, P.n, LP, Ns, tw, inverse); break;
+      case 3: stockham_pass<R, 3>(a, b, P.n, LP, Ns, tw, inverse); break;
+      case 4: stockham_pass<R, 4>(a, b, P.n, LP, Ns, tw, inverse); break;
+      case 5: stockham_pass<R, 5>(a, b, P.n, LP, Ns, tw, inverse); break;
+      default: stockham_pass<R, 7>(a, b, P.n, LP, Ns, tw, inverse); break;
+    }
+    __syncthreads();
+    Cx<R>* t = a;
+    a = b;
+    b = t;
+    Ns *= rad;
+  }
+  return a;
+}
+
+// Real lines of one axis: line q starts at (q / I) * ostride + (q % I) * istride
+// and has n elements `es` apart.  Block b filters complex lines
+// [b * LP, b * LP + LP): complex line p is real line 2p (+ i real line 2p + 1).
+// CONTIG: the elements of a line are contiguous (the z axis); otherwise
+// consecutive real lines are (loads iterate over lines fastest).
+template <typename R, bool CONTIG>
+__global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
+    R* __restrict__ mesh, AxisPlan P, long long nreal, int I, long long ostride,
+    long long istride, long long es, int LP, const Cx<R>* __restrict__ tw,
+    const R* __restrict__ gain) {
+  extern __shared__ double smem[];
+  Cx<R>* b0 = reinterpret_cast<Cx<R>*>(smem);
+  Cx<R>* b1 = b0 + static_cast<size_t>(LP) * P.n;
+  const int n = P.n;
+  const long long q0 = 2LL * blockIdx.x * LP;  // first real line of the block
+  auto base = [&](long long q) { return (q / I) * ostride + (q % I) * istride; };
+  const int nel = 2 * LP * n;
+  for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
+    int lp, e;
+    if (CONTIG) {
+      lp = t / n;
+      e = t - lp * n;
+    } else {
+      e = t / (2 * LP);
+      lp = t - e * (2 * LP);
+    }
+    const long long q = q0 + lp;
+    const R val = q < nreal ? mesh[base(q) + e * es] : R(0);
+    R* slot = reinterpret_cast<R*>(&b0[(lp >> 1) * n + e]);
+    slot[lp & 1] = val;
+  }
+  __syncthreads();
+  Cx<R>* spec = fft_lines<R>(b0, b1, LP, P, tw, false);
+  for (int t = threadIdx.x; t < LP * n; t += kSmoothThreads) {
+    const R g = gain[t % n];
+    spec[t].re *= g;
+    spec[t].im *= g;
+  }
+  __syncthreads();
+  Cx<R>* other = spec == b0 ? b1 : b0;
+  Cx<R>* res = fft_lines<R>(spec, other, LP, P, tw, true);
+  for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
+    int lp, e;
+    if (CONTIG) {
+      lp = t / n;
+      e = t - lp * n;
+    } else {
+      e = t / (2 * LP);
+      lp = t - e * (2 * LP);
+    }
+    const long long q = q0 + lp;
+    if (q < nreal) {
+      const R* slot = reinterpret_cast<const R*>(&res[(lp >> 1) * n + e]);
+      mesh[base(q) + e * es] = slot[lp & 1];
+    }
+  }
+}
+
+// gain[k] = exp(-(r k_phys)^2 / 2) / n with k_phys = 2 pi fftfreq(n, L / n)[k]
+// (numpy's ordering: k < (n - 1) / 2 + 1 -> k, else k - n), in double.
+template <typename R>
+__global__ void k_axis_gain(R* __restrict__ gain, int n, double val, double rad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int f = k < (n - 1) / 2 + 1 ? k : k - n;
+  constexpr double twopi = 2 * 3.141592653589793;
+  const double kp = twopi * (static_cast<double>(f) * val);
+  const double a = rad * kp;
+  gain[k] = static_cast<R>(exp(-0.5 * (a * a)) / static_cast<double>(n));
+}
+
+static bool factor_plan(int n, AxisPlan& P) {
+  P.n = n;
+  P.npass = 0;
+  int m = n;
+  for (int q : {7, 5, 3})
+    while (m % q == 0) {
+      P.radix[P.npass++] = q;
+      m /= q;
+    }
+  while (m % 4 == 0) {
+    P.radix[P.npass++] = 4;
+    m /= 4;
+  }
+  while (m % 2 == 0) {
+    P.radix[P.npass++] = 2;
+    m /= 2;
+  }
+  return m == 1;
+}
+
+int smooth_setup(prh_workspace* ws) {
+  SmoothState& S = ws->smooth;
+  S.have = true;
+  const size_t w = static_cast<size_t>(ws->dtype);
+  for (int d = 0; d < 3; d++) {
+    const int n = static_cast<int>(ws->n[d]);
+    AxisPlan P;
+    if (!factor_plan(n, P) || 4 * static_cast<size_t>(n) * w > kSmoothLds) {
+      S.have = false;  // smoothing refused at call time for this shape
+      continue;
+    }
+    PRH_TRY(ws_alloc(ws, &S.tw[d], 2 * w * n));
+    PRH_TRY(ws_alloc(ws, &S.gain[d], w * n));
+    std::vector<double> tw(2 * static_cast<size_t>(n));
+    for (int k = 0; k < n; k++) {
+      const double ang = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
+      tw[2 * k] = std::cos(ang);
+      tw[2 * k + 1] = std::sin(ang);
+    }
+    if (ws->dtype == PRH_F64) {
+      PRH_HIP(hipMemcpy(S.tw[d], tw.data(), 2 * w * n, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> tf(tw.begin(), tw.end());
+      PRH_HIP(hipMemcpy(S.tw[d], tf.data(), 2 * w * n, hipMemcpyHostToDevice));
+    }
+  }
+  return PRH_OK;
+}
+
+size_t smooth_bytes(const size_t* n, int dtype) {
+  size_t b = 0;
+  for (int d = 0; d < 3; d++) b += 3 * static_cast<size_t>(dtype) * n[d];
+  return b;
+}
+
+template <typename R>
+static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* boxsize,
+                     const double* radius, void* stream) {
+  if (!ws || !mesh || !nmesh || !boxsize || !radius) return PRH_ERR_ARG;
+  if (!(ws->flags & PRH_WS_FFT)) {
+    set_error("workspace was built without smoothing tables (PRH_WS_FFT)");
+    return PRH_ERR_STATE;
+  }
+  if (!ws->smooth.have) {
+    set_error("smoothing supports mesh sizes whose prime factors are 2, 3, 5 and 7, up to %zu per "
+              "dimension", kSmoothLds / (4 * sizeof(R)));
+    return PRH_ERR_ARG;
+  }
+  if (ws->dtype != Tag<R>::code) {
+    set_error("workspace precision does not match the mesh");
+    return PRH_ERR_ARG;
+  }
+  for (int d = 0; d < 3; d++)
+    if (ws->n[d] != nmesh[d]) {
+      set_error("workspace mesh does not match nmesh");
+      return PRH_ERR_CAPACITY;
+    }
+  hipStream_t s = as_stream(stream);
+  const long long n0 = static_cast<long long>(nmesh[0]), n1 = static_cast<long long>(nmesh[1]),
+                  n2 = static_cast<long long>(nmesh[2]);
+  for (int d = 0; d < 3; d++) {
+    // d_i = boxsize / nmesh (float64), val = 1 / (n d) as in numpy.fft.fftfreq; the radius is
+    // held in the mesh dtype (mesh.py:664-666)
+    const double cell = static_cast<double>(boxsize[d]) / static_cast<double>(static_cast<int>(nmesh[d]));
+    const double val = 1.0 / (static_cast<double>(nmesh[d]) * cell);
+    const double rad = static_cast<double>(static_cast<R>(radius[d]));
+    const int n = static_cast<int>(nmesh[d]);
+    k_axis_gain<R><<<ceil_div(n, 256), 256, 0, s>>>(static_cast<R*>(ws->smooth.gain[d]), n, val, rad);
+    PRH_LAUNCHED();
+  }
+  // axis order z, y, x: the z pass reads whole rows
+  for (int d = 2; d >= 0; d--) {
+    AxisPlan P;
+    factor_plan(static_cast<int>(nmesh[d]), P);
+    const int n = P.n;
+    int LP = static_cast<int>(kSmoothLds / (4 * sizeof(R) * static_cast<size_t>(n)));
+    if (LP > 16) LP = 16;
+    if (LP < 1) LP = 1;
+    long long nreal, ostride, istride, es;
+    int I;
+    if (d == 2) {
+      nreal = n0 * n1; I = static_cast<int>(n1); ostride = n1 * n2; istride = n2; es = 1;
+    } else if (d == 1) {
+      nreal = n0 * n2; I = static_cast<int>(n2); ostride = n1 * n2; istride = 1; es = n2;
+    } else {
+      nreal = n1 * n2; I = static_cast<int>(n2); ostride = n2; istride = 1; es = n1 * n2;
+    }
+    const long long ncomplex = (nreal + 1) / 2;
+    const unsigned grid = static_cast<unsigned>((ncomplex + LP - 1) / LP);
+    const size_t lds = 2 * static_cast<size_t>(LP) * n * sizeof(Cx<R>);
+    const Cx<R>* tw = static_cast<const Cx<R>*>(ws->smooth.tw[d]);
+    const R* gain = static_cast<const R*>(ws->smooth.gain[d]);
+    if (d == 2)
+      k_smooth_axis<R, true><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride, istride,
+                                                               es, LP, tw, gain);
+    else
+      k_smooth_axis<R, false><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride,
+                                                                istride, es, LP, tw, gain);
+    PRH_LAUNCHED();
+  }
+  return PRH_OK;
+}
+
+}  // namespace prh
+
+using namespace prh;
+
+extern "C" {
+
+int prh_smooth_gaussian_f32(prh_workspace* ws, float* mesh, const size_t* nmesh,
+                            const float* boxsize, const double* radius, void* stream) {
+  return do_smooth<float>(ws, mesh, nmesh, boxsize, radius, stream);
+}
+int prh_smooth_gaussian_f64(prh_workspace* ws, double* mesh, const size_t* nmesh,
+                            const double* boxsize, const double* radius, void* stream) {
+  return do_smooth<double>(ws, mesh, nmesh, boxsize, radius, stream);
+}
+
+}  // extern "C"

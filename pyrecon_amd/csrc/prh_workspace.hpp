@@ -1,8 +1,6 @@
 // Internal layout of prh_workspace (opaque in the C ABI).
 #pragma once
 
-#include <hipfft/hipfft.h>
-
 #include "prh_common.hpp"
 
 namespace prh {
@@ -40,12 +38,12 @@ struct PaintScratch {
   size_t nbig = 0;
 };
 
-struct FFTState {
-  hipfftHandle r2c = 0, c2r = 0;
-  bool have = false;
-  void* spectrum = nullptr;  // n0 * n1 * (n2/2+1) complex
-  void* work = nullptr;
-  size_t work_bytes = 0;
+// Tables of the separable Fourier-space smoothing: per axis, the twiddles
+// exp(-2 pi i k / n) and the per-call filter gains, in the mesh precision.
+struct SmoothState {
+  bool have = false;  // false when a dimension has a prime factor > 7 or is too long
+  void* tw[3] = {};
+  void* gain[3] = {};
 };
 
 }  // namespace prh
@@ -59,7 +57,7 @@ struct prh_workspace {
   prh::Hierarchy mg;     // workspace precision
   prh::Hierarchy mixed;  // FP32 hierarchy incl. level 0 (PRH_WS_MIXED)
   prh::PaintScratch paint;
-  prh::FFTState fft;
+  prh::SmoothState smooth;
   double* red = nullptr;       // reduction partials (device), then the results
   uint32_t* misc = nullptr;    // device status flags of the read kernels
   size_t nred = 0;
@@ -69,6 +67,8 @@ struct prh_workspace {
 
 namespace prh {
 int ws_alloc(prh_workspace* ws, void** p, size_t bytes);
+int smooth_setup(prh_workspace* ws);
+size_t smooth_bytes(const size_t* n, int dtype);
 constexpr unsigned kReduceBlocks = 1024;
 
 // Opt-in event timing of one kernel class (PRH_TIMER_*): when enabled, the

@@ -349,4 +349,6 @@ def test_mixed_extra_refinement(c1):
     gpu.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
     gpu.run(precision='mixed', nrefine=3, vcycles_per_refine=2, residual_norms=True)
     norms = gpu.residual_norms
-    assert len(norms) == 10 and norms[-1] < norms[6] < norms[0], norms
+    # extra FP64 corrections never increase the residual; it levels off at the part of the source
+    # the periodic operator cannot reach (its mean), which the plain FMG leaves as well
+    assert len(norms) == 10 and norms[6] < norms[0] and norms[-1] <= norms[6] * (1 + 1e-6), norms
