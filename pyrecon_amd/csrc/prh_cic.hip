@@ -434,24 +434,33 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
     s_cnt[e] = start[bc + 1] - g0;
   }
   __syncthreads();
-  // offsets: row sums, then a serial scan over the 81 rows, then within rows
+  // offsets: row sums, an exclusive scan over the 81 rows (first wave, by
+  // shuffles), then within the rows
   if (tid < kGE * kGE) {
     uint32_t sum = 0;
     for (int q = 0; q < kGE; q++) sum += s_cnt[tid * kGE + q];
     s_row[tid] = sum;
   }
   __syncthreads();
-  if (tid == 0) {
-    uint32_t run = 0;
-    for (int r = 0; r < kGE * kGE; r++) {
-      const uint32_t v = s_row[r];
-      s_row[r] = run;
-      run += v;
+  if (tid < 64) {
+    uint32_t v0 = s_row[tid];
+    uint32_t v1 = tid + 64 < kGE * kGE ? s_row[tid + 64] : 0u;
+    uint32_t i0 = v0, i1 = v1;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u0 = __shfl_up(i0, off), u1 = __shfl_up(i1, off);
+      if (tid >= off) {
+        i0 += u0;
+        i1 += u1;
+      }
     }
-    s_off[0] = run;  // total, read below before the per-row pass rewrites s_off
+    const uint32_t t0 = __shfl(i0, 63);
+    s_row[tid] = i0 - v0;
+    if (tid + 64 < kGE * kGE) s_row[tid + 64] = t0 + i1 - v1;
+    if (tid == 63) s_off[kGCells - 1] = t0 + __shfl(i1, 63);  // total, rewritten below
   }
   __syncthreads();
-  const uint32_t total = s_off[0];
+  const uint32_t total = s_off[kGCells - 1];
   __syncthreads();
   if (total == 0) return;
   if (tid < kGE * kGE) {
@@ -464,16 +473,14 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
   __syncthreads();
   const bool staged = total <= static_cast<uint32_t>(kCap);
   if (staged) {
-    for (uint32_t p = tid; p < total; p += kGThreads) {
-      // staged cell holding slot p: binary search over the offsets
-      int lo = 0, hi = kGCells - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_off[mid] <= p) lo = mid; else hi = mid - 1;
+    // one thread per staged cell copies its group (neighbouring cells are
+    // neighbouring in memory, so a wave's copies stay close together)
+    for (int e = tid; e < kGCells; e += kGThreads) {
+      const uint32_t g0 = s_gbeg[e], o0 = s_off[e], c = s_cnt[e];
+      for (uint32_t j = 0; j < c; j++) {
+        s_idx[o0 + j] = sorted[g0 + j];
+        s_pay[o0 + j] = spay[g0 + j];
       }
-      const uint32_t g = s_gbeg[lo] + (p - s_off[lo]);
-      s_idx[p] = sorted[g];
-      s_pay[p] = spay[g];
     }
     __syncthreads();
   }
@@ -503,22 +510,19 @@ __global__ void k_zero_misc(uint32_t* misc) {
 // Reads
 // ---------------------------------------------------------------------------
 
-// src/mesh.c:277-306
+// Base cell and offsets of a position in mesh units (src/mesh.c:205-214, 277-283).
 template <typename R>
-__device__ __forceinline__ R cic_value(const R* __restrict__ mesh, int n0, int n1, int n2,
-                                       const R p[3]) {
-  R t[3];
-  int b[3];
-  const int nn[3] = {n0, n1, n2};
+__device__ __forceinline__ void cic_base(const R p[3], const int nn[3], int b[3], R t[3]) {
   for (int d = 0; d < 3; d++) {
     const int ip = static_cast<int>(p[d]);
     t[d] = p[d] - static_cast<R>(ip);
     b[d] = base_index(ip, nn[d]);
   }
-  const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
-  const size_t X[2] = {b[0] * s0, (b[0] + 1 == n0 ? 0 : b[0] + 1) * s0};
-  const size_t Y[2] = {b[1] * s1, (b[1] + 1 == n1 ? 0 : b[1] + 1) * s1};
-  const size_t Z[2] = {static_cast<size_t>(b[2]), static_cast<size_t>(b[2] + 1 == n2 ? 0 : b[2] + 1)};
+}
+
+// src/mesh.c:277-306.  at(a, b, c) is the value at base + (a, b, c), a, b, c in {0, 1}.
+template <typename R, typename A>
+__device__ __forceinline__ R cic_core(A at, const R t[3]) {
   R acc = R(0);
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -526,7 +530,7 @@ __device__ __forceinline__ R cic_value(const R* __restrict__ mesh, int n0, int n
     const R wx = a ? t[0] : R(1) - t[0];
     const R wy = bb ? t[1] : R(1) - t[1];
     const R wz = e ? t[2] : R(1) - t[2];
-    const R term = mesh[X[a] + Y[bb] + Z[e]] * wx * wy * wz;
+    const R term = at(a, bb, e) * wx * wy * wz;
     acc = (k == 0) ? term : acc + term;
   }
   return acc;
@@ -534,25 +538,9 @@ __device__ __forceinline__ R cic_value(const R* __restrict__ mesh, int n0, int n
 
 // src/mesh.c:205-262: central differences at the 8 corners, corner order
 // 000, 100, 010, 001, 110, 101, 011, 111; divided by cell = 2 L / n.
-template <typename R>
-__device__ __forceinline__ void fd_gradient(const R* __restrict__ mesh, int n0, int n1, int n2,
-                                            const R cell[3], const R p[3], R g[3]) {
-  R t[3];
-  int b[3];
-  const int nn[3] = {n0, n1, n2};
-  for (int d = 0; d < 3; d++) {
-    const int ip = static_cast<int>(p[d]);
-    t[d] = p[d] - static_cast<R>(ip);
-    b[d] = base_index(ip, nn[d]);
-  }
-  const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
-  size_t X[4], Y[4], Z[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    X[k] = static_cast<size_t>((b[0] + n0 - 1 + k) % n0) * s0;
-    Y[k] = static_cast<size_t>((b[1] + n1 - 1 + k) % n1) * s1;
-    Z[k] = static_cast<size_t>((b[2] + n2 - 1 + k) % n2);
-  }
+// at(i, j, k) is the value at base + (i, j, k) - 1, i, j, k in [0, 4).
+template <typename R, typename A>
+__device__ __forceinline__ void fd_core(A at, const R t[3], const R cell[3], R g[3]) {
   constexpr int corner[8] = {0, 4, 2, 1, 6, 5, 3, 7};  // bit 2: x, bit 1: y, bit 0: z
   R gx = R(0), gy = R(0), gz = R(0);
 #pragma unroll
@@ -562,10 +550,9 @@ __device__ __forceinline__ void fd_gradient(const R* __restrict__ mesh, int n0, 
     const R wy = bb ? t[1] : R(1) - t[1];
     const R wz = e ? t[2] : R(1) - t[2];
     const R wt = wx * wy * wz;
-    const size_t cx = X[1 + a], cy = Y[1 + bb], cz = Z[1 + e];
-    const R dx = (mesh[X[2 + a] + cy + cz] - mesh[X[a] + cy + cz]) * wt;
-    const R dy = (mesh[cx + Y[2 + bb] + cz] - mesh[cx + Y[bb] + cz]) * wt;
-    const R dz = (mesh[cx + cy + Z[2 + e]] - mesh[cx + cy + Z[e]]) * wt;
+    const R dx = (at(2 + a, 1 + bb, 1 + e) - at(a, 1 + bb, 1 + e)) * wt;
+    const R dy = (at(1 + a, 2 + bb, 1 + e) - at(1 + a, bb, 1 + e)) * wt;
+    const R dz = (at(1 + a, 1 + bb, 2 + e) - at(1 + a, 1 + bb, e)) * wt;
     if (q == 0) {
       gx = dx;
       gy = dy;
@@ -579,6 +566,38 @@ __device__ __forceinline__ void fd_gradient(const R* __restrict__ mesh, int n0, 
   g[0] = gx / cell[0];
   g[1] = gy / cell[1];
   g[2] = gz / cell[2];
+}
+
+template <typename R>
+__device__ __forceinline__ R cic_value(const R* __restrict__ mesh, int n0, int n1, int n2,
+                                       const R p[3]) {
+  R t[3];
+  int b[3];
+  const int nn[3] = {n0, n1, n2};
+  cic_base(p, nn, b, t);
+  const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
+  const size_t X[2] = {b[0] * s0, (b[0] + 1 == n0 ? 0 : b[0] + 1) * s0};
+  const size_t Y[2] = {b[1] * s1, (b[1] + 1 == n1 ? 0 : b[1] + 1) * s1};
+  const size_t Z[2] = {static_cast<size_t>(b[2]), static_cast<size_t>(b[2] + 1 == n2 ? 0 : b[2] + 1)};
+  return cic_core<R>([&](int a, int bb, int e) { return mesh[X[a] + Y[bb] + Z[e]]; }, t);
+}
+
+template <typename R>
+__device__ __forceinline__ void fd_gradient(const R* __restrict__ mesh, int n0, int n1, int n2,
+                                            const R cell[3], const R p[3], R g[3]) {
+  R t[3];
+  int b[3];
+  const int nn[3] = {n0, n1, n2};
+  cic_base(p, nn, b, t);
+  const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
+  size_t X[4], Y[4], Z[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    X[k] = static_cast<size_t>((b[0] + n0 - 1 + k) % n0) * s0;
+    Y[k] = static_cast<size_t>((b[1] + n1 - 1 + k) % n1) * s1;
+    Z[k] = static_cast<size_t>((b[2] + n2 - 1 + k) % n2);
+  }
+  fd_core<R>([&](int i, int j, int k) { return mesh[X[i] + Y[j] + Z[k]]; }, t, cell, g);
 }
 
 template <typename R>
@@ -608,83 +627,167 @@ __global__ __launch_bounds__(kCicThreads) void k_read_fd(const R* __restrict__ m
   out[3 * i + 2] = g[2];
 }
 
+// Parameters of a fused read shared by all points.
+template <typename R>
+struct ReadArgs {
+  Frame<R> F;
+  R cell[3];
+  double fgrowth;
+  R los[3];
+  int local, field, wrap, shifted;
+};
+
+constexpr int kRT = 8;         // read tile edge (cells), the binning's tile
+constexpr int kRE = kRT + 3;   // staged edge: base - 1 .. base + 2
+constexpr int kRThreads = 256;
+
 // Fused read of the shifts and shifted positions (multigrid.py:137-158,
 // recon.py:304-325), with numpy's dtypes: the gradient is cast to the
 // positions dtype P; the local line of sight x/|x| is in P, computed from
 // the unwrapped input; with a global line of sight (mesh dtype R) the RSD term
-// is in Q = promote(P, R).  O is the output dtype.
+// is in Q = promote(P, R).  O is the output dtype.  Point i reads the mesh
+// through `tile` (the staged kRE^3 region whose origin is torg - 1) when its
+// base cell lies in [torg, torg + kRT), from `phi` otherwise; the values, and
+// so the bits, are the same either way.
 template <typename R, typename P, typename O>
-__global__ __launch_bounds__(kCicThreads) void k_read_shifts(
-    const R* __restrict__ phi, Frame<R> F, R c0, R c1, R c2, const P* __restrict__ pos, size_t m,
-    double fgrowth, R l0, R l1, R l2, int local, int field, int wrap, int shifted,
-    O* __restrict__ out, uint32_t* __restrict__ misc, const uint32_t* __restrict__ order) {
+__device__ __forceinline__ uint32_t read_point(size_t i, const R* __restrict__ phi,
+                                               const ReadArgs<R>& A, const P* __restrict__ pos,
+                                               O* __restrict__ out, const R* tile,
+                                               const int torg[3]) {
   using Q = promote_t<P, R>;
-  const size_t t = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  const Frame<R>& F = A.F;
   uint32_t err = 0;
-  if (t < m) {
-    const size_t i = order ? static_cast<size_t>(order[t]) : t;
-    const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-    R p[3];
-    to_mesh_units<R, P>(x, F, wrap, p, err);
-    if (field == PRH_FIELD_VALUE) {
-      // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
-      out[i] = static_cast<O>(static_cast<P>(cic_value(phi, F.n[0], F.n[1], F.n[2], p)));
-    } else {
-    const R cell[3] = {c0, c1, c2};
-    R g[3];
-    fd_gradient(phi, F.n[0], F.n[1], F.n[2], cell, p, g);
-    P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
-    // shifts in P (disp, disp+rsd, local rsd) or in Q (global rsd)
-    O res[3];
-    if (field == PRH_FIELD_DISP) {
-      for (int d = 0; d < 3; d++) res[d] = static_cast<O>(s[d]);
-      if (shifted)
-        for (int d = 0; d < 3; d++) res[d] = static_cast<O>(x[d] - s[d]);
-    } else if (local) {
-      const P dist = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-      const P l[3] = {x[0] / dist, x[1] / dist, x[2] / dist};
-      const P dot = s[0] * l[0] + s[1] * l[1] + s[2] * l[2];
-      const P fd = static_cast<P>(fgrowth) * dot;
-      P sh[3];
-      for (int d = 0; d < 3; d++) {
-        const P rsd = fd * l[d];
-        sh[d] = (field == PRH_FIELD_RSD) ? rsd : static_cast<P>(s[d] + rsd);
-      }
-      for (int d = 0; d < 3; d++)
-        res[d] = static_cast<O>(shifted ? static_cast<P>(x[d] - sh[d]) : sh[d]);
-    } else {
-      const Q l[3] = {static_cast<Q>(l0), static_cast<Q>(l1), static_cast<Q>(l2)};
-      const Q dot = static_cast<Q>(s[0]) * l[0] + static_cast<Q>(s[1]) * l[1] +
-                    static_cast<Q>(s[2]) * l[2];
-      const Q fd = static_cast<Q>(fgrowth) * dot;
-      for (int d = 0; d < 3; d++) {
-        const Q rsd = fd * l[d];
-        if (field == PRH_FIELD_RSD) {
-          res[d] = static_cast<O>(shifted ? static_cast<Q>(static_cast<Q>(x[d]) - rsd) : rsd);
-        } else {
-          const P sh = static_cast<P>(static_cast<Q>(s[d]) + rsd);
-          res[d] = static_cast<O>(shifted ? static_cast<P>(x[d] - sh) : sh);
-        }
-      }
+  const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  R p[3];
+  to_mesh_units<R, P>(x, F, A.wrap, p, err);
+  R t[3];
+  int b[3];
+  cic_base(p, F.n, b, t);
+  int rb[3];
+  bool in_tile = tile != nullptr;
+  for (int d = 0; d < 3; d++) {
+    rb[d] = b[d] - (tile ? torg[d] : 0);
+    in_tile = in_tile && rb[d] >= 0 && rb[d] < kRT;
+  }
+  if (A.field == PRH_FIELD_VALUE) {
+    // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
+    const R v = in_tile ? cic_core<R>(
+                              [&](int a, int bb, int e) {
+                                return tile[((rb[0] + 1 + a) * kRE + rb[1] + 1 + bb) * kRE + rb[2] + 1 + e];
+                              },
+                              t)
+                        : cic_value(phi, F.n[0], F.n[1], F.n[2], p);
+    out[i] = static_cast<O>(static_cast<P>(v));
+    return err;
+  }
+  R g[3];
+  if (in_tile)
+    fd_core<R>([&](int u, int v, int w) { return tile[((rb[0] + u) * kRE + rb[1] + v) * kRE + rb[2] + w]; },
+               t, A.cell, g);
+  else
+    fd_gradient(phi, F.n[0], F.n[1], F.n[2], A.cell, p, g);
+  P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
+  const int field = A.field, shifted = A.shifted;
+  // shifts in P (disp, disp+rsd, local rsd) or in Q (global rsd)
+  O res[3];
+  if (field == PRH_FIELD_DISP) {
+    for (int d = 0; d < 3; d++) res[d] = static_cast<O>(s[d]);
+    if (shifted)
+      for (int d = 0; d < 3; d++) res[d] = static_cast<O>(x[d] - s[d]);
+  } else if (A.local) {
+    const P dist = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const P l[3] = {x[0] / dist, x[1] / dist, x[2] / dist};
+    const P dot = s[0] * l[0] + s[1] * l[1] + s[2] * l[2];
+    const P fd = static_cast<P>(A.fgrowth) * dot;
+    P sh[3];
+    for (int d = 0; d < 3; d++) {
+      const P rsd = fd * l[d];
+      sh[d] = (field == PRH_FIELD_RSD) ? rsd : static_cast<P>(s[d] + rsd);
     }
-    if (shifted && wrap) {
-      using QO = promote_t<O, R>;
-      for (int d = 0; d < 3; d++)
-        res[d] = static_cast<O>(np_mod(static_cast<QO>(res[d]) - static_cast<QO>(F.off[d]),
-                                       static_cast<QO>(F.bs[d])) +
-                                static_cast<QO>(F.off[d]));
-    }
-    out[3 * i] = res[0];
-    out[3 * i + 1] = res[1];
-    out[3 * i + 2] = res[2];
+    for (int d = 0; d < 3; d++)
+      res[d] = static_cast<O>(shifted ? static_cast<P>(x[d] - sh[d]) : sh[d]);
+  } else {
+    const Q l[3] = {static_cast<Q>(A.los[0]), static_cast<Q>(A.los[1]), static_cast<Q>(A.los[2])};
+    const Q dot = static_cast<Q>(s[0]) * l[0] + static_cast<Q>(s[1]) * l[1] +
+                  static_cast<Q>(s[2]) * l[2];
+    const Q fd = static_cast<Q>(A.fgrowth) * dot;
+    for (int d = 0; d < 3; d++) {
+      const Q rsd = fd * l[d];
+      if (field == PRH_FIELD_RSD) {
+        res[d] = static_cast<O>(shifted ? static_cast<Q>(static_cast<Q>(x[d]) - rsd) : rsd);
+      } else {
+        const P sh = static_cast<P>(static_cast<Q>(s[d]) + rsd);
+        res[d] = static_cast<O>(shifted ? static_cast<P>(x[d] - sh) : sh);
+      }
     }
   }
+  if (shifted && A.wrap) {
+    using QO = promote_t<O, R>;
+    for (int d = 0; d < 3; d++)
+      res[d] = static_cast<O>(np_mod(static_cast<QO>(res[d]) - static_cast<QO>(F.off[d]),
+                                     static_cast<QO>(F.bs[d])) +
+                              static_cast<QO>(F.off[d]));
+  }
+  out[3 * i] = res[0];
+  out[3 * i + 1] = res[1];
+  out[3 * i + 2] = res[2];
+  return err;
+}
+
+// Wave-level OR of the status flags, one atomic per wave.
+__device__ __forceinline__ void flag_errors(uint32_t err, uint32_t* misc) {
   const uint64_t bad = __ballot(err != 0);
   if (bad) {
     uint32_t e = err;
     for (int off = 32; off > 0; off >>= 1) e |= __shfl_xor(e, off);
     if ((threadIdx.x & 63) == 0) atomicOr(&misc[0], e);
   }
+}
+
+// Points in input order, reading the mesh from global memory.
+template <typename R, typename P, typename O>
+__global__ __launch_bounds__(kCicThreads) void k_read_shifts(const R* __restrict__ phi,
+                                                             ReadArgs<R> A,
+                                                             const P* __restrict__ pos, size_t m,
+                                                             O* __restrict__ out,
+                                                             uint32_t* __restrict__ misc) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  uint32_t err = 0;
+  const int torg[3] = {0, 0, 0};
+  if (i < m) err = read_point<R, P, O>(i, phi, A, pos, out, nullptr, torg);
+  flag_errors(err, misc);
+}
+
+// Points binned by 8^3 tile: a workgroup per tile stages the mesh values the
+// tile's points need (an 11^3 region, periodic) in LDS, then reads them there.
+template <typename R, typename P, typename O>
+__global__ __launch_bounds__(kRThreads) void k_read_tiled(const R* __restrict__ phi, ReadArgs<R> A,
+                                                          const P* __restrict__ pos,
+                                                          O* __restrict__ out,
+                                                          uint32_t* __restrict__ misc,
+                                                          const uint32_t* __restrict__ order,
+                                                          const uint32_t* __restrict__ tstart,
+                                                          int tiles1, int tiles2) {
+  __shared__ R tile[kRE * kRE * kRE];
+  const uint32_t p0 = tstart[blockIdx.x], p1 = tstart[blockIdx.x + 1];
+  if (p0 == p1) return;
+  const int tz = blockIdx.x % tiles2, ty = (blockIdx.x / tiles2) % tiles1,
+            tx = blockIdx.x / (tiles2 * tiles1);
+  const int torg[3] = {tx * kRT, ty * kRT, tz * kRT};
+  const Frame<R>& F = A.F;
+  const size_t s0 = static_cast<size_t>(F.n[1]) * F.n[2];
+  for (int e = threadIdx.x; e < kRE * kRE * kRE; e += kRThreads) {
+    const int ex = e / (kRE * kRE), ey = (e / kRE) % kRE, ez = e % kRE;
+    auto wrapc = [](int v, int n) { return ((v % n) + n) % n; };
+    const size_t gx = wrapc(torg[0] - 1 + ex, F.n[0]), gy = wrapc(torg[1] - 1 + ey, F.n[1]),
+                 gz = wrapc(torg[2] - 1 + ez, F.n[2]);
+    tile[e] = phi[gx * s0 + gy * F.n[2] + gz];
+  }
+  __syncthreads();
+  uint32_t err = 0;
+  for (uint32_t q = p0 + threadIdx.x; q < p1; q += kRThreads)
+    err |= read_point<R, P, O>(order[q], phi, A, pos, out, tile, torg);
+  flag_errors(err, misc);
 }
 
 // Read binning: a counting sort of the points by 8^3-cell tile (the order
@@ -920,40 +1023,34 @@ static int do_read_fd(const R* mesh, const size_t* nmesh, const R* boxsize, cons
   return PRH_OK;
 }
 
+// Shifts of n points.  With a workspace holding paint scratch for this mesh,
+// the points are binned by 8^3 tile (a counting sort, in chunks of the paint
+// scratch size) and read through LDS tiles; otherwise in input order.
 template <typename R, typename P, typename O>
-static int launch_read_shifts(const R* phi, const Frame<R>& F, const R cell[3], const P* pos,
-                              size_t n, double f, const R* los, int field, int wrap, int shifted,
-                              void* out, uint32_t* misc, hipStream_t s, const uint32_t* order) {
-  const R l0 = los ? los[0] : R(0), l1 = los ? los[1] : R(0), l2 = los ? los[2] : R(0);
-  timer_before(PRH_TIMER_READ_SHIFTS, s);
-  k_read_shifts<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(
-      phi, F, cell[0], cell[1], cell[2], pos, n, f, l0, l1, l2, los ? 0 : 1, field, wrap,
-      shifted, static_cast<O*>(out), misc, order);
-  // algorithmic: positions in, shifted positions out, the potential read once
-  timer_after(PRH_TIMER_READ_SHIFTS, s,
-              static_cast<double>(n) * 3 * (sizeof(P) + sizeof(O)) +
-                  static_cast<double>(F.n[0]) * F.n[1] * F.n[2] * sizeof(R));
-  PRH_LAUNCHED();
-  return PRH_OK;
-}
-
-// Shifts of n points, binned by tile in chunks of the paint scratch size when
-// the workspace has one for this mesh, directly otherwise.
-template <typename R, typename P, typename O>
-static int read_shifts_all(prh_workspace* ws, const R* phi, const Frame<R>& F, const R cell[3],
-                           const P* pos, size_t n, double f, const R* los, int field, int wrap,
-                           int shifted, O* out, uint32_t* misc, hipStream_t s) {
+static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A, const P* pos,
+                           size_t n, O* out, uint32_t* misc, hipStream_t s) {
+  const Frame<R>& F = A.F;
+  const size_t ostride = A.field == PRH_FIELD_VALUE ? 1 : 3;
+  // algorithmic: positions in, results out, the potential read once
+  const double bytes = static_cast<double>(n) * (3 * sizeof(P) + ostride * sizeof(O)) +
+                       static_cast<double>(F.n[0]) * F.n[1] * F.n[2] * sizeof(R);
   const bool binned = (ws->flags & PRH_WS_PAINT) && n >= 65536 && ws->n[0] == size_t(F.n[0]) &&
                       ws->n[1] == size_t(F.n[1]) && ws->n[2] == size_t(F.n[2]);
-  if (!binned) return launch_read_shifts<R, P, O>(phi, F, cell, pos, n, f, los, field, wrap, shifted, out, misc, s, nullptr);
+  if (!binned) {
+    timer_before(PRH_TIMER_READ_SHIFTS, s);
+    k_read_shifts<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, pos, n, out, misc);
+    timer_after(PRH_TIMER_READ_SHIFTS, s, bytes);
+    PRH_LAUNCHED();
+    return PRH_OK;
+  }
   PaintScratch& S = ws->paint;
-  const int t0 = (F.n[0] + 7) / 8, t1 = (F.n[1] + 7) / 8, t2 = (F.n[2] + 7) / 8;
+  const int t0 = (F.n[0] + kRT - 1) / kRT, t1 = (F.n[1] + kRT - 1) / kRT, t2 = (F.n[2] + kRT - 1) / kRT;
   const size_t ntile = static_cast<size_t>(t0) * t1 * t2;
   const unsigned nscan = ceil_div(ntile, kScanBlock);
   for (size_t o = 0; o < n; o += S.chunk) {
     const size_t m = n - o < S.chunk ? n - o : S.chunk;
     const unsigned grid = ceil_div(m, kCicThreads);
-    k_bin_key<R, P><<<grid, kCicThreads, 0, s>>>(pos + 3 * o, m, F, wrap, t1, t2, S.key, S.rank, S.count);
+    k_bin_key<R, P><<<grid, kCicThreads, 0, s>>>(pos + 3 * o, m, F, A.wrap, t1, t2, S.key, S.rank, S.count);
     PRH_LAUNCHED();
     k_scan_reduce<<<nscan, kCicThreads, 0, s>>>(S.count, ntile, S.partial);
     PRH_LAUNCHED();
@@ -963,9 +1060,13 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const Frame<R>& F, c
     PRH_LAUNCHED();
     k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
     PRH_LAUNCHED();
-    const size_t ostride = field == PRH_FIELD_VALUE ? 1 : 3;
-    PRH_TRY((launch_read_shifts<R, P, O>(phi, F, cell, pos + 3 * o, m, f, los, field, wrap, shifted,
-                                         out + ostride * o, misc, s, S.sorted)));
+    timer_before(PRH_TIMER_READ_SHIFTS, s);
+    k_read_tiled<R, P, O><<<static_cast<unsigned>(ntile), kRThreads, 0, s>>>(
+        phi, A, pos + 3 * o, out + ostride * o, misc, S.sorted, S.start, t1, t2);
+    timer_after(PRH_TIMER_READ_SHIFTS, s,
+                static_cast<double>(m) * (3 * sizeof(P) + ostride * sizeof(O)) +
+                    static_cast<double>(F.n[0]) * F.n[1] * F.n[2] * sizeof(R));
+    PRH_LAUNCHED();
   }
   return PRH_OK;
 }
@@ -1004,19 +1105,25 @@ static int do_read_shifts(prh_workspace* ws, const R* phi, const size_t* nmesh, 
   hipStream_t s = as_stream(stream);
   k_zero_misc<<<1, 64, 0, s>>>(misc);
   PRH_LAUNCHED();
-  const Frame<R> F = make_frame<R>(nmesh, boxsize, boxcenter);
-  R cell[3];
-  fd_cells(nmesh, boxsize, cell);
+  ReadArgs<R> A;
+  A.F = make_frame<R>(nmesh, boxsize, boxcenter);
+  fd_cells(nmesh, boxsize, A.cell);
+  A.fgrowth = f;
+  for (int d = 0; d < 3; d++) A.los[d] = los ? los[d] : R(0);
+  A.local = los ? 0 : 1;
+  A.field = field;
+  A.wrap = wrap;
+  A.shifted = shifted;
   int st;
   if (pdt == PRH_F64) {
-    st = read_shifts_all<R, double, double>(ws, phi, F, cell, static_cast<const double*>(pos), n, f,
-                                            los, field, wrap, shifted, static_cast<double*>(out), misc, s);
+    st = read_shifts_all<R, double, double>(ws, phi, A, static_cast<const double*>(pos), n,
+                                            static_cast<double*>(out), misc, s);
   } else if (odt == PRH_F32) {
-    st = read_shifts_all<R, float, float>(ws, phi, F, cell, static_cast<const float*>(pos), n, f,
-                                          los, field, wrap, shifted, static_cast<float*>(out), misc, s);
+    st = read_shifts_all<R, float, float>(ws, phi, A, static_cast<const float*>(pos), n,
+                                          static_cast<float*>(out), misc, s);
   } else {
-    st = read_shifts_all<R, float, double>(ws, phi, F, cell, static_cast<const float*>(pos), n, f,
-                                           los, field, wrap, shifted, static_cast<double*>(out), misc, s);
+    st = read_shifts_all<R, float, double>(ws, phi, A, static_cast<const float*>(pos), n,
+                                           static_cast<double*>(out), misc, s);
   }
   PRH_TRY(st);
   uint32_t* host = ws->flag_host;

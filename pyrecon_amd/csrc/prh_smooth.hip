@@ -35,25 +35,48 @@ __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// One Stockham pass of radix RAD over LP lines of length n (a -> b).
+// Parameters of one Stockham pass: radix, span Ns of the sub-transforms done
+// so far (a power of two except after a second odd radix), butterflies per
+// line m = n / radix (looped over a power-of-two range 2^lm >= m, so that the
+// (line, butterfly) split of a thread index is a shift), twiddle step.
+struct Pass {
+  int rad, ns, ns_shift;  // ns_shift = log2(ns), or -1 when ns is not a power of two
+  int m, lm, span;
+};
+
+struct AxisPlan {
+  int n, npass, ln;  // 2^ln >= n: padded element range of the load/store loops
+  Pass pass[24];
+};
+
+// One Stockham pass of radix RAD over LP = 2^llp lines of length n (a -> b).
 template <typename R, int RAD>
 __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>* __restrict__ b,
-                                              int n, int LP, int Ns,
+                                              int n, int llp, const Pass& P,
                                               const Cx<R>* __restrict__ tw, bool inverse) {
-  const int m = n / RAD;
-  const int span = n / (Ns * RAD);  // twiddle index step
-  const int rstep = n / RAD;        // index step of the RAD-th roots of unity
-  for (int t = threadIdx.x; t < LP * m; t += kSmoothThreads) {
-    const int l = t / m;
-    const int j = t - l * m;
-    const int k = j % Ns;
+  const int m = P.m, Ns = P.ns;
+  const int rstep = n / RAD;  // index step of the RAD-th roots of unity
+  const int total = 1 << (llp + P.lm);
+  for (int t = threadIdx.x; t < total; t += kSmoothThreads) {
+    const int l = t >> P.lm;
+    const int j = t & ((1 << P.lm) - 1);
+    if (j >= m) continue;
+    int k, jq;
+    if (P.ns_shift >= 0) {
+      k = j & (Ns - 1);
+      jq = j >> P.ns_shift;
+    } else {
+      jq = j / Ns;
+      k = j - jq * Ns;
+    }
     const Cx<R>* al = a + l * n;
     Cx<R> v[RAD];
 #pragma unroll
     for (int r = 0; r < RAD; r++) v[r] = al[j + r * m];
+    const int tstep = k * P.span;  // r * tstep < n
 #pragma unroll
     for (int r = 1; r < RAD; r++) {
-      Cx<R> w = tw[(r * k * span) % n];
+      Cx<R> w = tw[r * tstep];
       if (inverse) w.im = -w.im;
       v[r] = cmul(v[r], w);
     }
@@ -89,68 +112,70 @@ __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>
       }
     }
     Cx<R>* bl = b + l * n;
-    const int d = (j / Ns) * Ns * RAD + k;
+    const int d = jq * Ns * RAD + k;
 #pragma unroll
     for (int r = 0; r < RAD; r++) bl[d + r * Ns] = o[r];
   }
 }
 
-struct AxisPlan {
-  int n, npass;
-  int radix[24];
-};
-
-// Runs the passes of `P` on the LP lines in buf[0]; returns the buffer holding
+// Runs the passes of `P` on the 2^llp lines in b0; returns the buffer holding
 // the result (natural order).
 template <typename R>
-__device__ Cx<R>* fft_lines(Cx<R>* b0, Cx<R>* b1, int LP, const AxisPlan& P,
+__device__ Cx<R>* fft_lines(Cx<R>* b0, Cx<R>* b1, int llp, const AxisPlan& P,
                             const Cx<R>* __restrict__ tw, bool inverse) {
   Cx<R>* a = b0;
   Cx<R>* b = b1;
-  int Ns = 1;
   for (int p = 0; p < P.npass; p++) {
-    const int rad = P.radix[p];
-    switch (rad) {
-      case 2: stockham_pass<R, 2>(a, b, P.n, LP, Ns, tw, inverse); break;
-      case 3: stockham_pass<R, 3>(a, b, P.n, LP, Ns, tw, inverse); break;
-      case 4: stockham_pass<R, 4>(a, b, P.n, LP, Ns, tw, inverse); break;
-      case 5: stockham_pass<R, 5>(a, b, P.n, LP, Ns, tw, inverse); break;
-      default: stockham_pass<R, 7>(a, b, P.n, LP, Ns, tw, inverse); break;
+    const Pass& S = P.pass[p];
+    switch (S.rad) {
+      case 2: stockham_pass<R, 2>(a, b, P.n, llp, S, tw, inverse); break;
+      case 3: stockham_pass<R, 3>(a, b, P.n, llp, S, tw, inverse); break;
+      case 4: stockham_pass<R, 4>(a, b, P.n, llp, S, tw, inverse); break;
+      case 5: stockham_pass<R, 5>(a, b, P.n, llp, S, tw, inverse); break;
+      default: stockham_pass<R, 7>(a, b, P.n, llp, S, tw, inverse); break;
     }
     __syncthreads();
     Cx<R>* t = a;
     a = b;
     b = t;
-    Ns *= rad;
   }
   return a;
 }
 
 // Real lines of one axis: line q starts at (q / I) * ostride + (q % I) * istride
-// and has n elements `es` apart.  Block b filters complex lines
-// [b * LP, b * LP + LP): complex line p is real line 2p (+ i real line 2p + 1).
+// and has n elements `es` apart.  Block b filters the 2^llp complex lines
+// starting at complex line b * 2^llp (remapped so that consecutive bundles run
+// on one XCD); complex line p is real line 2p (+ i real line 2p + 1).
 // CONTIG: the elements of a line are contiguous (the z axis); otherwise
 // consecutive real lines are (loads iterate over lines fastest).
 template <typename R, bool CONTIG>
 __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     R* __restrict__ mesh, AxisPlan P, long long nreal, int I, long long ostride,
-    long long istride, long long es, int LP, const Cx<R>* __restrict__ tw,
+    long long istride, long long es, int llp, const Cx<R>* __restrict__ tw,
     const R* __restrict__ gain) {
   extern __shared__ double smem[];
+  const int LP = 1 << llp;
   Cx<R>* b0 = reinterpret_cast<Cx<R>*>(smem);
   Cx<R>* b1 = b0 + static_cast<size_t>(LP) * P.n;
   const int n = P.n;
-  const long long q0 = 2LL * blockIdx.x * LP;  // first real line of the block
+  // XCD-aware order: the 8 XCDs take blocks round-robin; give consecutive
+  // bundles (which share cache lines in the strided passes) to one XCD
+  const unsigned nb = gridDim.x, bid = blockIdx.x;
+  const unsigned per = nb / 8;
+  const unsigned blk = bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+  const long long q0 = 2LL * blk * LP;  // first real line of the block
   auto base = [&](long long q) { return (q / I) * ostride + (q % I) * istride; };
-  const int nel = 2 * LP * n;
+  const int lpl = llp + 1;  // log2 of the real lines of the block
+  const int nel = CONTIG ? (2 * LP) << P.ln : (2 * LP) * n;
   for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
     int lp, e;
     if (CONTIG) {
-      lp = t / n;
-      e = t - lp * n;
+      lp = t >> P.ln;
+      e = t & ((1 << P.ln) - 1);
+      if (e >= n) continue;
     } else {
-      e = t / (2 * LP);
-      lp = t - e * (2 * LP);
+      e = t >> lpl;
+      lp = t & ((1 << lpl) - 1);
     }
     const long long q = q0 + lp;
     const R val = q < nreal ? mesh[base(q) + e * es] : R(0);
@@ -158,23 +183,27 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     slot[lp & 1] = val;
   }
   __syncthreads();
-  Cx<R>* spec = fft_lines<R>(b0, b1, LP, P, tw, false);
-  for (int t = threadIdx.x; t < LP * n; t += kSmoothThreads) {
-    const R g = gain[t % n];
-    spec[t].re *= g;
-    spec[t].im *= g;
+  Cx<R>* spec = fft_lines<R>(b0, b1, llp, P, tw, false);
+  for (int t = threadIdx.x; t < (LP << P.ln); t += kSmoothThreads) {
+    const int e = t & ((1 << P.ln) - 1);
+    if (e >= n) continue;
+    const int l = t >> P.ln;
+    const R g = gain[e];
+    spec[l * n + e].re *= g;
+    spec[l * n + e].im *= g;
   }
   __syncthreads();
   Cx<R>* other = spec == b0 ? b1 : b0;
-  Cx<R>* res = fft_lines<R>(spec, other, LP, P, tw, true);
+  Cx<R>* res = fft_lines<R>(spec, other, llp, P, tw, true);
   for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
     int lp, e;
     if (CONTIG) {
-      lp = t / n;
-      e = t - lp * n;
+      lp = t >> P.ln;
+      e = t & ((1 << P.ln) - 1);
+      if (e >= n) continue;
     } else {
-      e = t / (2 * LP);
-      lp = t - e * (2 * LP);
+      e = t >> lpl;
+      lp = t & ((1 << lpl) - 1);
     }
     const long long q = q0 + lp;
     if (q < nreal) {
@@ -197,24 +226,45 @@ __global__ void k_axis_gain(R* __restrict__ gain, int n, double val, double rad)
   gain[k] = static_cast<R>(exp(-0.5 * (a * a)) / static_cast<double>(n));
 }
 
+static int ceil_log2(int v) {
+  int l = 0;
+  while ((1 << l) < v) l++;
+  return l;
+}
+
+// Radix sequence 4, ..., 4, (2), then the odd factors 3, 5, 7, so that the
+// span of every power-of-two pass, and of the first odd one, is a power of two.
 static bool factor_plan(int n, AxisPlan& P) {
   P.n = n;
   P.npass = 0;
-  int m = n;
-  for (int q : {7, 5, 3})
-    while (m % q == 0) {
-      P.radix[P.npass++] = q;
-      m /= q;
-    }
+  P.ln = ceil_log2(n);
+  int rads[24], nr = 0, m = n;
   while (m % 4 == 0) {
-    P.radix[P.npass++] = 4;
+    rads[nr++] = 4;
     m /= 4;
   }
   while (m % 2 == 0) {
-    P.radix[P.npass++] = 2;
+    rads[nr++] = 2;
     m /= 2;
   }
-  return m == 1;
+  for (int q : {3, 5, 7})
+    while (m % q == 0 && nr < 24) {
+      rads[nr++] = q;
+      m /= q;
+    }
+  if (m != 1) return false;
+  int ns = 1;
+  for (int i = 0; i < nr; i++) {
+    Pass& S = P.pass[P.npass++];
+    S.rad = rads[i];
+    S.ns = ns;
+    S.ns_shift = (ns & (ns - 1)) == 0 ? ceil_log2(ns) : -1;
+    S.m = n / rads[i];
+    S.lm = ceil_log2(S.m);
+    S.span = n / (ns * rads[i]);
+    ns *= rads[i];
+  }
+  return true;
 }
 
 int smooth_setup(prh_workspace* ws) {
@@ -292,9 +342,10 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     AxisPlan P;
     factor_plan(static_cast<int>(nmesh[d]), P);
     const int n = P.n;
-    int LP = static_cast<int>(kSmoothLds / (4 * sizeof(R) * static_cast<size_t>(n)));
-    if (LP > 16) LP = 16;
-    if (LP < 1) LP = 1;
+    // lines per block: the largest power of two whose two buffers fit kSmoothLds
+    int llp = 0;
+    while (llp < 4 && (size_t(4) << (llp + 1)) * sizeof(R) * static_cast<size_t>(n) <= kSmoothLds) llp++;
+    const int LP = 1 << llp;
     long long nreal, ostride, istride, es;
     int I;
     if (d == 2) {
@@ -311,10 +362,10 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     const R* gain = static_cast<const R*>(ws->smooth.gain[d]);
     if (d == 2)
       k_smooth_axis<R, true><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride, istride,
-                                                               es, LP, tw, gain);
+                                                               es, llp, tw, gain);
     else
       k_smooth_axis<R, false><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride,
-                                                                istride, es, LP, tw, gain);
+                                                                istride, es, llp, tw, gain);
     PRH_LAUNCHED();
   }
   return PRH_OK;

@@ -19,6 +19,9 @@ from .mesh import RealMesh
 from .recon import BaseReconstruction, ReconstructionError, _suffix, numpy_mean
 
 
+_OWN = object()  # "this reconstruction's own stream"
+
+
 class OriginalMultiGridReconstruction(BaseReconstruction):
     """Multigrid solve of Martin J. White's reconstruction (pyrecon/multigrid.py:13-158)."""
 
@@ -132,9 +135,12 @@ class OriginalMultiGridReconstruction(BaseReconstruction):
             raise ReconstructionError('set the density contrast before running')
         return self.mesh_delta.buffer
 
-    def _new_phi(self):
+    def _new_phi(self, zero_stream=_OWN):
+        """Zero-filled potential mesh; the fill is queued on ``zero_stream`` (default: this
+        reconstruction's stream), the stream of the solve that writes it."""
         phi = RealMesh(info=self.info, workspace=self.workspace, stream=self.stream)
-        phi._allocate(zero=True)
+        phi._allocate(zero=False)
+        phi.buffer.fill_zero(stream=self.stream if zero_stream is _OWN else zero_stream)
         return phi
 
 
@@ -168,7 +174,7 @@ def run_group(recons, jacobi_damping_factor=0.4, jacobi_niterations=5, vcycle_ni
     if precision not in ('plain', 'mixed'):
         raise ValueError('precision must be "plain" or "mixed"')
     deltas = [r._solve_source() for r in recons]
-    phis = [r._new_phi() for r in recons]
+    phis = [r._new_phi(zero_stream=stream) for r in recons]
     real = ctypes.c_float if dtype == np.float32 else ctypes.c_double
     ws = (ctypes.c_void_p * count)(*[r.workspace.handle for r in recons])
     f = (ctypes.c_void_p * count)(*[d.ptr for d in deltas])
