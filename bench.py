@@ -49,7 +49,7 @@ def parse_args():
     p.add_argument('--nrefine', type=int, default=0, help='extra FP64 corrections of the mixed path')
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-accuracy', action='store_true')
-    p.add_argument('--kernel-only', default=None, choices=['jacobi32', 'jacobi64', 'copy'],
+    p.add_argument('--kernel-only', default=None, choices=['jacobi32', 'jacobi64', 'copy', 'fmg_mixed', 'fmg64'],
                    help='run only one kernel repeatedly (for rocprofv3 counter runs)')
     p.add_argument('--workers', type=int, default=min(16, os.cpu_count() or 1))
     p.add_argument('--serving-jobs', type=int, default=256, help='jobs of the serving stream (0: skip it)')
@@ -165,6 +165,31 @@ def kernel_only(args):
             _lib.check(_lib.lib.prh_copy_kernel(b.ptr, a.ptr, nbytes, None))
         P.synchronize()
         log('copy kernel: {} launches of {} bytes'.format(args.steps, nbytes))
+        return
+    if args.kernel_only in ('fmg_mixed', 'fmg64'):
+        # the multigrid solve alone on a random source (finest-level kernels at the workload's shape)
+        mixed = args.kernel_only == 'fmg_mixed'
+        ws = P.Workspace(shape, 'f8', flags=_lib.WS_MULTIGRID | (_lib.WS_MIXED if mixed else 0))
+        rng = np.random.default_rng(0)
+        f = P.DeviceArray.from_numpy(rng.standard_normal(shape))
+        v = P.DeviceArray.zeros(shape, 'f8')
+        nm, nm_ptr = _lib.size3(shape)
+        bs, bs_ptr = _lib.vec3(cfg['boxsize'], 'f8')
+        bc, bc_ptr = _lib.vec3(cfg['boxcenter'], 'f8')
+        with _lib.KernelTimer(_lib.TIMER_JACOBI_FINEST) as kt:
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                if mixed:
+                    _lib.check(_lib.lib.prh_fmg_mixed(ws.handle, f.ptr, v.ptr, nm_ptr, bs_ptr, bc_ptr, 0.4, 0.4, 5, 6,
+                                                      None, 0, 1, None, None))
+                else:
+                    _lib.check(_lib.lib.prh_fmg_f64(ws.handle, f.ptr, v.ptr, nm_ptr, bs_ptr, bc_ptr, 0.4, 0.4, 5, 6,
+                                                    None, None))
+            P.synchronize()
+            elapsed = time.perf_counter() - t0
+            ms, launches, nbytes = kt.read()
+        log('{}: {:.2f} ms per solve; finest-level sweeps {:.4f} ms/launch, {:.1f} GB/s algorithmic'.format(
+            args.kernel_only, 1e3 * elapsed / args.steps, ms / launches, nbytes / (ms * 1e-3) / 1e9))
         return
     dtype = 'f4' if args.kernel_only == 'jacobi32' else 'f8'
     ws = P.Workspace(shape, dtype, flags=_lib.WS_MULTIGRID)

@@ -637,55 +637,30 @@ struct ReadArgs {
   int local, field, wrap, shifted;
 };
 
-constexpr int kRT = 8;         // read tile edge (cells), the binning's tile
-constexpr int kRE = kRT + 3;   // staged edge: base - 1 .. base + 2
-constexpr int kRThreads = 256;
+constexpr int kRT = 8;  // read binning tile edge (cells)
 
 // Fused read of the shifts and shifted positions (multigrid.py:137-158,
 // recon.py:304-325), with numpy's dtypes: the gradient is cast to the
 // positions dtype P; the local line of sight x/|x| is in P, computed from
 // the unwrapped input; with a global line of sight (mesh dtype R) the RSD term
-// is in Q = promote(P, R).  O is the output dtype.  Point i reads the mesh
-// through `tile` (the staged kRE^3 region whose origin is torg - 1) when its
-// base cell lies in [torg, torg + kRT), from `phi` otherwise; the values, and
-// so the bits, are the same either way.
+// is in Q = promote(P, R).  O is the output dtype.
 template <typename R, typename P, typename O>
 __device__ __forceinline__ uint32_t read_point(size_t i, const R* __restrict__ phi,
                                                const ReadArgs<R>& A, const P* __restrict__ pos,
-                                               O* __restrict__ out, const R* tile,
-                                               const int torg[3]) {
+                                               O* __restrict__ out) {
   using Q = promote_t<P, R>;
   const Frame<R>& F = A.F;
   uint32_t err = 0;
   const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
   R p[3];
   to_mesh_units<R, P>(x, F, A.wrap, p, err);
-  R t[3];
-  int b[3];
-  cic_base(p, F.n, b, t);
-  int rb[3];
-  bool in_tile = tile != nullptr;
-  for (int d = 0; d < 3; d++) {
-    rb[d] = b[d] - (tile ? torg[d] : 0);
-    in_tile = in_tile && rb[d] >= 0 && rb[d] < kRT;
-  }
   if (A.field == PRH_FIELD_VALUE) {
     // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
-    const R v = in_tile ? cic_core<R>(
-                              [&](int a, int bb, int e) {
-                                return tile[((rb[0] + 1 + a) * kRE + rb[1] + 1 + bb) * kRE + rb[2] + 1 + e];
-                              },
-                              t)
-                        : cic_value(phi, F.n[0], F.n[1], F.n[2], p);
-    out[i] = static_cast<O>(static_cast<P>(v));
+    out[i] = static_cast<O>(static_cast<P>(cic_value(phi, F.n[0], F.n[1], F.n[2], p)));
     return err;
   }
   R g[3];
-  if (in_tile)
-    fd_core<R>([&](int u, int v, int w) { return tile[((rb[0] + u) * kRE + rb[1] + v) * kRE + rb[2] + w]; },
-               t, A.cell, g);
-  else
-    fd_gradient(phi, F.n[0], F.n[1], F.n[2], A.cell, p, g);
+  fd_gradient(phi, F.n[0], F.n[1], F.n[2], A.cell, p, g);
   P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
   const int field = A.field, shifted = A.shifted;
   // shifts in P (disp, disp+rsd, local rsd) or in Q (global rsd)
@@ -753,40 +728,22 @@ __global__ __launch_bounds__(kCicThreads) void k_read_shifts(const R* __restrict
                                                              uint32_t* __restrict__ misc) {
   const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   uint32_t err = 0;
-  const int torg[3] = {0, 0, 0};
-  if (i < m) err = read_point<R, P, O>(i, phi, A, pos, out, nullptr, torg);
+  if (i < m) err = read_point<R, P, O>(i, phi, A, pos, out);
   flag_errors(err, misc);
 }
 
-// Points binned by 8^3 tile: a workgroup per tile stages the mesh values the
-// tile's points need (an 11^3 region, periodic) in LDS, then reads them there.
+// Points in binned order (`order`: point indices grouped by 8^3 tile), so that
+// the mesh reads of a wave fall in a small neighbourhood and hit in cache.
 template <typename R, typename P, typename O>
-__global__ __launch_bounds__(kRThreads) void k_read_tiled(const R* __restrict__ phi, ReadArgs<R> A,
-                                                          const P* __restrict__ pos,
-                                                          O* __restrict__ out,
-                                                          uint32_t* __restrict__ misc,
-                                                          const uint32_t* __restrict__ order,
-                                                          const uint32_t* __restrict__ tstart,
-                                                          int tiles1, int tiles2) {
-  __shared__ R tile[kRE * kRE * kRE];
-  const uint32_t p0 = tstart[blockIdx.x], p1 = tstart[blockIdx.x + 1];
-  if (p0 == p1) return;
-  const int tz = blockIdx.x % tiles2, ty = (blockIdx.x / tiles2) % tiles1,
-            tx = blockIdx.x / (tiles2 * tiles1);
-  const int torg[3] = {tx * kRT, ty * kRT, tz * kRT};
-  const Frame<R>& F = A.F;
-  const size_t s0 = static_cast<size_t>(F.n[1]) * F.n[2];
-  for (int e = threadIdx.x; e < kRE * kRE * kRE; e += kRThreads) {
-    const int ex = e / (kRE * kRE), ey = (e / kRE) % kRE, ez = e % kRE;
-    auto wrapc = [](int v, int n) { return ((v % n) + n) % n; };
-    const size_t gx = wrapc(torg[0] - 1 + ex, F.n[0]), gy = wrapc(torg[1] - 1 + ey, F.n[1]),
-                 gz = wrapc(torg[2] - 1 + ez, F.n[2]);
-    tile[e] = phi[gx * s0 + gy * F.n[2] + gz];
-  }
-  __syncthreads();
+__global__ __launch_bounds__(kCicThreads) void k_read_binned(const R* __restrict__ phi,
+                                                             ReadArgs<R> A,
+                                                             const P* __restrict__ pos, size_t m,
+                                                             O* __restrict__ out,
+                                                             uint32_t* __restrict__ misc,
+                                                             const uint32_t* __restrict__ order) {
+  const size_t t = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   uint32_t err = 0;
-  for (uint32_t q = p0 + threadIdx.x; q < p1; q += kRThreads)
-    err |= read_point<R, P, O>(order[q], phi, A, pos, out, tile, torg);
+  if (t < m) err = read_point<R, P, O>(order[t], phi, A, pos, out);
   flag_errors(err, misc);
 }
 
@@ -1025,7 +982,7 @@ static int do_read_fd(const R* mesh, const size_t* nmesh, const R* boxsize, cons
 
 // Shifts of n points.  With a workspace holding paint scratch for this mesh,
 // the points are binned by 8^3 tile (a counting sort, in chunks of the paint
-// scratch size) and read through LDS tiles; otherwise in input order.
+// scratch size) and read in binned order; otherwise in input order.
 template <typename R, typename P, typename O>
 static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A, const P* pos,
                            size_t n, O* out, uint32_t* misc, hipStream_t s) {
@@ -1061,8 +1018,8 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
     k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
     PRH_LAUNCHED();
     timer_before(PRH_TIMER_READ_SHIFTS, s);
-    k_read_tiled<R, P, O><<<static_cast<unsigned>(ntile), kRThreads, 0, s>>>(
-        phi, A, pos + 3 * o, out + ostride * o, misc, S.sorted, S.start, t1, t2);
+    k_read_binned<R, P, O><<<grid, kCicThreads, 0, s>>>(phi, A, pos + 3 * o, m, out + ostride * o,
+                                                        misc, S.sorted);
     timer_after(PRH_TIMER_READ_SHIFTS, s,
                 static_cast<double>(m) * (3 * sizeof(P) + ostride * sizeof(O)) +
                     static_cast<double>(F.n[0]) * F.n[1] * F.n[2] * sizeof(R));

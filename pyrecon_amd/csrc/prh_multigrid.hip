@@ -109,7 +109,8 @@ __global__ __launch_bounds__(256) void k_stencil_tiled(const R* __restrict__ v,
     off[q] = gy * n2 + gz;
   }
   auto wrapx = [n0](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
-  auto slot = [](int x) { return (x + 4) & 3; };
+  // ring slot of plane x, relative to the run start (static inside the unrolled loop)
+  auto slot = [xb](int x) { return (x - xb + 4) & 3; };
   for (int p = -1; p <= 1; p++) {
     const R* src = v + static_cast<size_t>(wrapx(xb + p)) * s0;
     R* dst = lds + slot(xb + p) * PLANE;
@@ -141,19 +142,22 @@ __global__ __launch_bounds__(256) void k_stencil_tiled(const R* __restrict__ v,
       pypz[k] = T.K.pypz;
     }
   }
-  __syncthreads();
-  for (int x = xb; x < xe; x++) {
-    const bool more = x + 1 < xe;
-    R pre[PER];
-    if (more) {
-      const R* src = v + static_cast<size_t>(wrapx(x + 2)) * s0;
+  // planes x + 2 and x + 3 are in flight in registers while plane x is computed:
+  // set A holds the planes at an even distance from the run start, set B the odd
+  R preA[PER], preB[PER];
+  auto fetch = [&](R(&dst)[PER], int xp) {
+    const R* src = v + static_cast<size_t>(wrapx(xp)) * s0;
 #pragma unroll
-      for (int q = 0; q < PER; q++)
-        if (has[q]) pre[q] = src[off[q]];
-    }
-    const R* M = lds + slot(x - 1) * PLANE;
-    const R* P = lds + slot(x) * PLANE;
-    const R* N = lds + slot(x + 1) * PLANE;
+    for (int q = 0; q < PER; q++)
+      if (has[q]) dst[q] = src[off[q]];
+  };
+  auto stash = [&](const R(&src)[PER], int sl) {
+    R* dst = lds + sl * PLANE;
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+      if (has[q]) dst[tid + 256 * q] = src[q];
+  };
+  auto compute = [&](int x, const R* M, const R* P, const R* N) {
     const R* fx = f + static_cast<size_t>(x) * s0;
     O* ox = out + static_cast<size_t>(x) * s0;
     R px = R(0), A = R(0);
@@ -233,16 +237,32 @@ __global__ __launch_bounds__(256) void k_stencil_tiled(const R* __restrict__ v,
       }
       ox[rowoff[k]] = static_cast<O>(res);
     }
-    if (more) {
-      R* dst = lds + slot(x + 2) * PLANE;
+  };
+  if (xb + 1 < xe) fetch(preA, xb + 2);
+  __syncthreads();
+  for (int x0 = xb; x0 < xe; x0 += 4) {
 #pragma unroll
-      for (int q = 0; q < PER; q++)
-        if (has[q]) dst[tid + 256 * q] = pre[q];
+    for (int u = 0; u < 4; u++) {
+      const int x = x0 + u;
+      if (x < xe) {
+        if (x + 2 < xe) {
+          if ((u + 3) & 1)
+            fetch(preB, x + 3);
+          else
+            fetch(preA, x + 3);
+        }
+        compute(x, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE, lds + ((u + 1) & 3) * PLANE);
+        if (x + 1 < xe) {
+          if ((u + 2) & 1)
+            stash(preB, (u + 2) & 3);
+          else
+            stash(preA, (u + 2) & 3);
+        }
+        __syncthreads();
+      }
     }
-    __syncthreads();
   }
 }
-
 constexpr int kTileY = 16;
 
 inline bool tiled_ok(const int* n) { return n[2] % kTileZ == 0 && n[1] % kTileY == 0 && n[0] >= 1; }

@@ -23,7 +23,8 @@
 namespace prh {
 
 constexpr int kSmoothThreads = 256;
-constexpr size_t kSmoothLds = 65536;  // bytes of the two line buffers of a block
+constexpr size_t kSmoothLds = 73728;      // target LDS bytes of a block (2 blocks per CU)
+constexpr size_t kSmoothLdsMax = 163840;  // upper limit (one line per block)
 
 template <typename R>
 struct Cx {
@@ -48,6 +49,20 @@ struct AxisPlan {
   int n, npass, ln;  // 2^ln >= n: padded element range of the load/store loops
   Pass pass[24];
 };
+
+// 4-point DFT (forward: exp(-2 pi i / 4) = -i; inverse: +i).
+template <typename R>
+__device__ __forceinline__ void dft4(const Cx<R> c[4], Cx<R> y[4], bool inverse) {
+  const Cx<R> s02 = {c[0].re + c[2].re, c[0].im + c[2].im};
+  const Cx<R> d02 = {c[0].re - c[2].re, c[0].im - c[2].im};
+  const Cx<R> s13 = {c[1].re + c[3].re, c[1].im + c[3].im};
+  const Cx<R> d13 = {c[1].re - c[3].re, c[1].im - c[3].im};
+  const Cx<R> rot = inverse ? Cx<R>{-d13.im, d13.re} : Cx<R>{d13.im, -d13.re};
+  y[0] = {s02.re + s13.re, s02.im + s13.im};
+  y[2] = {s02.re - s13.re, s02.im - s13.im};
+  y[1] = {d02.re + rot.re, d02.im + rot.im};
+  y[3] = {d02.re - rot.re, d02.im - rot.im};
+}
 
 // One Stockham pass of radix RAD over LP = 2^llp lines of length n (a -> b).
 template <typename R, int RAD>
@@ -84,17 +99,29 @@ __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>
     if constexpr (RAD == 2) {
       o[0] = {v[0].re + v[1].re, v[0].im + v[1].im};
       o[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
+    } else if constexpr (RAD == 8) {
+      // 2 x 4 split: a_k = v_k + v_k+4, b_k = (v_k - v_k+4) w8^k, then a 4-point DFT of
+      // each; X_2m from a, X_2m+1 from b
+      constexpr R h = R(0.70710678118654752440);
+      Cx<R> aa[4], bb[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        aa[q] = {v[q].re + v[q + 4].re, v[q].im + v[q + 4].im};
+        bb[q] = {v[q].re - v[q + 4].re, v[q].im - v[q + 4].im};
+      }
+      bb[1] = cmul(bb[1], Cx<R>{h, inverse ? h : -h});
+      bb[2] = inverse ? Cx<R>{-bb[2].im, bb[2].re} : Cx<R>{bb[2].im, -bb[2].re};
+      bb[3] = cmul(bb[3], Cx<R>{-h, inverse ? h : -h});
+      Cx<R> ya[4], yb[4];
+      dft4(aa, ya, inverse);
+      dft4(bb, yb, inverse);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        o[2 * q] = ya[q];
+        o[2 * q + 1] = yb[q];
+      }
     } else if constexpr (RAD == 4) {
-      const Cx<R> s02 = {v[0].re + v[2].re, v[0].im + v[2].im};
-      const Cx<R> d02 = {v[0].re - v[2].re, v[0].im - v[2].im};
-      const Cx<R> s13 = {v[1].re + v[3].re, v[1].im + v[3].im};
-      const Cx<R> d13 = {v[1].re - v[3].re, v[1].im - v[3].im};
-      // forward: -i * d13; inverse: +i * d13
-      const Cx<R> rot = inverse ? Cx<R>{-d13.im, d13.re} : Cx<R>{d13.im, -d13.re};
-      o[0] = {s02.re + s13.re, s02.im + s13.im};
-      o[2] = {s02.re - s13.re, s02.im - s13.im};
-      o[1] = {d02.re + rot.re, d02.im + rot.im};
-      o[3] = {d02.re - rot.re, d02.im - rot.im};
+      dft4(v, o, inverse);
     } else {
       // direct DFT with the RAD-th roots of unity taken from the table
 #pragma unroll
@@ -131,6 +158,7 @@ __device__ Cx<R>* fft_lines(Cx<R>* b0, Cx<R>* b1, int llp, const AxisPlan& P,
       case 2: stockham_pass<R, 2>(a, b, P.n, llp, S, tw, inverse); break;
       case 3: stockham_pass<R, 3>(a, b, P.n, llp, S, tw, inverse); break;
       case 4: stockham_pass<R, 4>(a, b, P.n, llp, S, tw, inverse); break;
+      case 8: stockham_pass<R, 8>(a, b, P.n, llp, S, tw, inverse); break;
       case 5: stockham_pass<R, 5>(a, b, P.n, llp, S, tw, inverse); break;
       default: stockham_pass<R, 7>(a, b, P.n, llp, S, tw, inverse); break;
     }
@@ -151,13 +179,15 @@ __device__ Cx<R>* fft_lines(Cx<R>* b0, Cx<R>* b1, int llp, const AxisPlan& P,
 template <typename R, bool CONTIG>
 __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     R* __restrict__ mesh, AxisPlan P, long long nreal, int I, long long ostride,
-    long long istride, long long es, int llp, const Cx<R>* __restrict__ tw,
+    long long istride, long long es, int llp, const Cx<R>* __restrict__ tw_global,
     const R* __restrict__ gain) {
   extern __shared__ double smem[];
   const int LP = 1 << llp;
   Cx<R>* b0 = reinterpret_cast<Cx<R>*>(smem);
   Cx<R>* b1 = b0 + static_cast<size_t>(LP) * P.n;
+  Cx<R>* stw = b1 + static_cast<size_t>(LP) * P.n;  // twiddle table, staged once per block
   const int n = P.n;
+  for (int k = threadIdx.x; k < n; k += kSmoothThreads) stw[k] = tw_global[k];
   // XCD-aware order: the 8 XCDs take blocks round-robin; give consecutive
   // bundles (which share cache lines in the strided passes) to one XCD
   const unsigned nb = gridDim.x, bid = blockIdx.x;
@@ -183,7 +213,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     slot[lp & 1] = val;
   }
   __syncthreads();
-  Cx<R>* spec = fft_lines<R>(b0, b1, llp, P, tw, false);
+  Cx<R>* spec = fft_lines<R>(b0, b1, llp, P, stw, false);
   for (int t = threadIdx.x; t < (LP << P.ln); t += kSmoothThreads) {
     const int e = t & ((1 << P.ln) - 1);
     if (e >= n) continue;
@@ -194,7 +224,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
   }
   __syncthreads();
   Cx<R>* other = spec == b0 ? b1 : b0;
-  Cx<R>* res = fft_lines<R>(spec, other, llp, P, tw, true);
+  Cx<R>* res = fft_lines<R>(spec, other, llp, P, stw, true);
   for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
     int lp, e;
     if (CONTIG) {
@@ -232,13 +262,17 @@ static int ceil_log2(int v) {
   return l;
 }
 
-// Radix sequence 4, ..., 4, (2), then the odd factors 3, 5, 7, so that the
+// Radix sequence 8, ..., 8, (4), (2), then the odd factors 3, 5, 7, so that the
 // span of every power-of-two pass, and of the first odd one, is a power of two.
 static bool factor_plan(int n, AxisPlan& P) {
   P.n = n;
   P.npass = 0;
   P.ln = ceil_log2(n);
   int rads[24], nr = 0, m = n;
+  while (m % 8 == 0) {
+    rads[nr++] = 8;
+    m /= 8;
+  }
   while (m % 4 == 0) {
     rads[nr++] = 4;
     m /= 4;
@@ -274,7 +308,7 @@ int smooth_setup(prh_workspace* ws) {
   for (int d = 0; d < 3; d++) {
     const int n = static_cast<int>(ws->n[d]);
     AxisPlan P;
-    if (!factor_plan(n, P) || 4 * static_cast<size_t>(n) * w > kSmoothLds) {
+    if (!factor_plan(n, P) || 6 * static_cast<size_t>(n) * w > kSmoothLdsMax) {
       S.have = false;  // smoothing refused at call time for this shape
       continue;
     }
@@ -312,7 +346,7 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
   }
   if (!ws->smooth.have) {
     set_error("smoothing supports mesh sizes whose prime factors are 2, 3, 5 and 7, up to %zu per "
-              "dimension", kSmoothLds / (4 * sizeof(R)));
+              "dimension", kSmoothLdsMax / (6 * sizeof(R)));
     return PRH_ERR_ARG;
   }
   if (ws->dtype != Tag<R>::code) {
@@ -344,7 +378,8 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     const int n = P.n;
     // lines per block: the largest power of two whose two buffers fit kSmoothLds
     int llp = 0;
-    while (llp < 4 && (size_t(4) << (llp + 1)) * sizeof(R) * static_cast<size_t>(n) <= kSmoothLds) llp++;
+    while (llp < 4 && ((size_t(2) << (llp + 1)) + 1) * 2 * sizeof(R) * static_cast<size_t>(n) <= kSmoothLds)
+      llp++;
     const int LP = 1 << llp;
     long long nreal, ostride, istride, es;
     int I;
@@ -357,9 +392,15 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     }
     const long long ncomplex = (nreal + 1) / 2;
     const unsigned grid = static_cast<unsigned>((ncomplex + LP - 1) / LP);
-    const size_t lds = 2 * static_cast<size_t>(LP) * n * sizeof(Cx<R>);
+    const size_t lds = (2 * static_cast<size_t>(LP) + 1) * n * sizeof(Cx<R>);
     const Cx<R>* tw = static_cast<const Cx<R>*>(ws->smooth.tw[d]);
     const R* gain = static_cast<const R*>(ws->smooth.gain[d]);
+    if (lds > 65536) {
+      PRH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_smooth_axis<R, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      PRH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_smooth_axis<R, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    }
     if (d == 2)
       k_smooth_axis<R, true><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride, istride,
                                                                es, llp, tw, gain);
