@@ -315,7 +315,7 @@ def main():
     d, dw = P.DeviceArray.from_numpy(cat['data']), P.DeviceArray.from_numpy(cat['data_weights'])
     r, rw = P.DeviceArray.from_numpy(cat['randoms']), P.DeviceArray.from_numpy(cat['randoms_weights'])
     shape = (cfg['nmesh'],) * 3
-    ws = P.Workspace(shape, 'f8', paint_chunk=1 << 26)
+    ws = P.Workspace(shape, "f8", paint_chunk=1 << 27)  # one chunk for 1.2e8 points: no validation pass
     log('workspace {:.2f} GB'.format(ws.nbytes / 1e9))
     out_d, out_r = P.DeviceArray((nd, 3), 'f8'), P.DeviceArray((nr, 3), 'f8')
     geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype='f8')

@@ -11,7 +11,10 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libpyrecon_amd.so')
+# PYRECON_AMD_LIBRARY may name another build of the same library (e.g. a variant built with
+# different kernel options for a comparison); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get('PYRECON_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                                    'libpyrecon_amd.so')
 
 OK = 0
 ERR_ARG = -1

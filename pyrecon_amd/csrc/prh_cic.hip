@@ -417,6 +417,7 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
   __shared__ uint32_t s_off[kGCells];   // its first slot in the staged arrays
   __shared__ uint32_t s_row[kGE * kGE];
   __shared__ uint32_t s_idx[kCap];
+  __shared__ uint16_t s_map[kCap];  // staged slot -> staged cell
   __shared__ Payload<R> s_pay[kCap];
   if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
   const int tid = threadIdx.x;
@@ -473,13 +474,33 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
   __syncthreads();
   const bool staged = total <= static_cast<uint32_t>(kCap);
   if (staged) {
-    // one thread per staged cell copies its group (neighbouring cells are
-    // neighbouring in memory, so a wave's copies stay close together)
+    // slot -> staged cell map, then the copy over slots in batches whose reads are
+    // all issued before the LDS stores (several reads in flight per thread)
     for (int e = tid; e < kGCells; e += kGThreads) {
-      const uint32_t g0 = s_gbeg[e], o0 = s_off[e], c = s_cnt[e];
-      for (uint32_t j = 0; j < c; j++) {
-        s_idx[o0 + j] = sorted[g0 + j];
-        s_pay[o0 + j] = spay[g0 + j];
+      const uint32_t o0 = s_off[e], c = s_cnt[e];
+      for (uint32_t j = 0; j < c; j++) s_map[o0 + j] = static_cast<uint16_t>(e);
+    }
+    __syncthreads();
+    constexpr int kBatch = 4;
+    for (uint32_t p0 = tid; p0 < total; p0 += kBatch * kGThreads) {
+      uint32_t ix[kBatch];
+      Payload<R> pl[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        // slots past the end re-read the last slot (in bounds; not stored)
+        const uint32_t p = min(p0 + u * kGThreads, total - 1);
+        const int e = s_map[p];
+        const uint32_t g = s_gbeg[e] + (p - s_off[e]);
+        ix[u] = sorted[g];
+        pl[u] = spay[g];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const uint32_t p = p0 + u * kGThreads;
+        if (p < total) {
+          s_idx[p] = ix[u];
+          s_pay[p] = pl[u];
+        }
       }
     }
     __syncthreads();

@@ -78,8 +78,14 @@ struct TileCoef {
   R dinv;  // 1 / K.d (fast variant only)
 };
 
+// Minimum waves per SIMD requested for the blocked kernel (a register cap);
+// 1 leaves the allocation to the compiler.
+#ifndef PRH_TILED_MIN_WAVES
+#define PRH_TILED_MIN_WAVES 1
+#endif
+
 template <typename R, typename O, int OP, int TY, bool LOCAL, bool FAST>
-__global__ __launch_bounds__(256) void k_stencil_tiled(const R* __restrict__ v,
+__global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(const R* __restrict__ v,
                                                        const R* __restrict__ f,
                                                        O* __restrict__ out, TileCoef<R> T,
                                                        R omega, R keep, int xlen) {

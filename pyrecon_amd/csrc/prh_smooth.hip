@@ -197,20 +197,31 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
   auto base = [&](long long q) { return (q / I) * ostride + (q % I) * istride; };
   const int lpl = llp + 1;  // log2 of the real lines of the block
   const int nel = CONTIG ? (2 * LP) << P.ln : (2 * LP) * n;
-  for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
-    int lp, e;
-    if (CONTIG) {
-      lp = t >> P.ln;
-      e = t & ((1 << P.ln) - 1);
-      if (e >= n) continue;
-    } else {
-      e = t >> lpl;
-      lp = t & ((1 << lpl) - 1);
+  // loads in batches of kBatch per thread, all issued before any LDS store, so
+  // that a thread keeps kBatch reads in flight
+  constexpr int kBatch = 8;
+  for (int t0 = threadIdx.x; t0 < nel; t0 += kBatch * kSmoothThreads) {
+    R val[kBatch];
+    int dst[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int t = t0 + u * kSmoothThreads;
+      int lp, e;
+      if (CONTIG) {
+        lp = t >> P.ln;
+        e = t & ((1 << P.ln) - 1);
+      } else {
+        e = t >> lpl;
+        lp = t & ((1 << lpl) - 1);
+      }
+      const long long q = q0 + lp;
+      const bool ok = t < nel && e < n;
+      val[u] = ok && q < nreal ? mesh[base(q) + e * es] : R(0);
+      dst[u] = ok ? 2 * ((lp >> 1) * n + e) + (lp & 1) : -1;
     }
-    const long long q = q0 + lp;
-    const R val = q < nreal ? mesh[base(q) + e * es] : R(0);
-    R* slot = reinterpret_cast<R*>(&b0[(lp >> 1) * n + e]);
-    slot[lp & 1] = val;
+#pragma unroll
+    for (int u = 0; u < kBatch; u++)
+      if (dst[u] >= 0) reinterpret_cast<R*>(b0)[dst[u]] = val[u];
   }
   __syncthreads();
   Cx<R>* spec = fft_lines<R>(b0, b1, llp, P, stw, false);
