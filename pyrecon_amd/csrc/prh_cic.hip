@@ -472,55 +472,73 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
     }
   }
   __syncthreads();
-  const bool staged = total <= static_cast<uint32_t>(kCap);
-  if (staged) {
-    // slot -> staged cell map, then the copy over slots in batches whose reads are
-    // all issued before the LDS stores (several reads in flight per thread)
-    for (int e = tid; e < kGCells; e += kGThreads) {
-      const uint32_t o0 = s_off[e], c = s_cnt[e];
-      for (uint32_t j = 0; j < c; j++) s_map[o0 + j] = static_cast<uint16_t>(e);
-    }
-    __syncthreads();
-    constexpr int kBatch = 4;
-    for (uint32_t p0 = tid; p0 < total; p0 += kBatch * kGThreads) {
-      uint32_t ix[kBatch];
-      Payload<R> pl[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; u++) {
-        // slots past the end re-read the last slot (in bounds; not stored)
-        const uint32_t p = min(p0 + u * kGThreads, total - 1);
-        const int e = s_map[p];
-        const uint32_t g = s_gbeg[e] + (p - s_off[e]);
-        ix[u] = sorted[g];
-        pl[u] = spay[g];
-      }
-#pragma unroll
-      for (int u = 0; u < kBatch; u++) {
-        const uint32_t p = p0 + u * kGThreads;
-        if (p < total) {
-          s_idx[p] = ix[u];
-          s_pay[p] = pl[u];
-        }
-      }
-    }
-    __syncthreads();
-  }
+  // Staged in one part (the whole region) when it fits; otherwise in two halves
+  // along x: target planes [0, 4) need staged planes [0, 5), target planes
+  // [4, 8) staged planes [4, 9).  The cells of consecutive planes hold a
+  // contiguous range of slots.  A part that still does not fit merges from
+  // global memory.
   const int lz = tid % kGT, ly = (tid / kGT) % kGT, lx = tid / (kGT * kGT);
   const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
-  if (x >= n0 || y >= n1 || z >= n2) return;
-  uint32_t beg[8], end[8];
+  const bool valid = x < n0 && y < n1 && z < n2;
+  const int nparts = total <= static_cast<uint32_t>(kCap) ? 1 : 2;
+  constexpr int kPlane = kGE * kGE;
+  for (int part = 0; part < nparts; part++) {
+    const int ex0 = nparts == 1 ? 0 : 4 * part;
+    const int ex1 = nparts == 1 ? kGE : 4 * part + 5;
+    const uint32_t base = s_off[ex0 * kPlane];
+    const uint32_t lim = ex1 == kGE ? total : s_off[ex1 * kPlane];
+    const uint32_t cnt = lim - base;
+    const bool staged = cnt <= static_cast<uint32_t>(kCap);
+    if (staged && cnt > 0) {
+      // slot -> staged cell map, then the copy over slots in batches whose reads are
+      // all issued before the LDS stores (several reads in flight per thread)
+      for (int e = ex0 * kPlane + tid; e < ex1 * kPlane; e += kGThreads) {
+        const uint32_t o0 = s_off[e] - base, c = s_cnt[e];
+        for (uint32_t j = 0; j < c; j++) s_map[o0 + j] = static_cast<uint16_t>(e);
+      }
+      __syncthreads();
+      constexpr int kBatch = 4;
+      for (uint32_t p0 = tid; p0 < cnt; p0 += kBatch * kGThreads) {
+        uint32_t ix[kBatch];
+        Payload<R> pl[kBatch];
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int e = (lx + 1 - ((k >> 2) & 1)) * kGE * kGE + (ly + 1 - ((k >> 1) & 1)) * kGE +
-                  (lz + 1 - (k & 1));
-    beg[k] = staged ? s_off[e] : s_gbeg[e];
-    end[k] = beg[k] + s_cnt[e];
+        for (int u = 0; u < kBatch; u++) {
+          // slots past the end re-read the last slot (in bounds; not stored)
+          const uint32_t p = min(p0 + u * kGThreads, cnt - 1);
+          const int e = s_map[p];
+          const uint32_t g = s_gbeg[e] + (p + base - s_off[e]);
+          ix[u] = sorted[g];
+          pl[u] = spay[g];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+          const uint32_t p = p0 + u * kGThreads;
+          if (p < cnt) {
+            s_idx[p] = ix[u];
+            s_pay[p] = pl[u];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const bool mine = nparts == 1 || (lx >= 4 * part && lx < 4 * part + 4);
+    if (valid && mine) {
+      uint32_t beg[8], end[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int e = (lx + 1 - ((k >> 2) & 1)) * kPlane + (ly + 1 - ((k >> 1) & 1)) * kGE +
+                      (lz + 1 - (k & 1));
+        beg[k] = staged ? s_off[e] - base : s_gbeg[e];
+        end[k] = beg[k] + s_cnt[e];
+      }
+      const size_t c = (static_cast<size_t>(x) * n1 + y) * n2 + z;
+      R acc = mesh[c];
+      const bool any = staged ? merge_groups<R>(beg, end, s_idx, s_pay, acc)
+                              : merge_groups<R>(beg, end, sorted, spay, acc);
+      if (any) mesh[c] = acc;
+    }
+    if (part + 1 < nparts) __syncthreads();  // the next part restages the LDS arrays
   }
-  const size_t c = (static_cast<size_t>(x) * n1 + y) * n2 + z;
-  R acc = mesh[c];
-  const bool any = staged ? merge_groups<R>(beg, end, s_idx, s_pay, acc)
-                          : merge_groups<R>(beg, end, sorted, spay, acc);
-  if (any) mesh[c] = acc;
 }
 
 __global__ void k_zero_misc(uint32_t* misc) {

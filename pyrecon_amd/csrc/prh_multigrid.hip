@@ -95,8 +95,17 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
   const LevelGeom<R>& G = T.G;
   const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
   const int tid = threadIdx.x, tz = tid & 63, ty = tid >> 6;
-  const int z0 = blockIdx.x * kTileZ, y0 = blockIdx.y * TY;
-  const int xb = blockIdx.z * xlen;
+  // XCD-aware tile order: the hardware deals blocks to the 8 XCDs round-robin;
+  // logical tiles are numbered z-tile fastest, then y-tile, then x-run, and each
+  // XCD gets a contiguous range of them, so tiles that share halo rows and
+  // columns run together on one L2
+  const int ztiles = n2 / kTileZ, ytiles = n1 / TY;
+  const unsigned nb = gridDim.x, bid = blockIdx.x, per = nb / 8;
+  const unsigned L = bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+  const int zt = static_cast<int>(L % ztiles), yt = static_cast<int>((L / ztiles) % ytiles);
+  const int xr = static_cast<int>(L / (static_cast<unsigned>(ztiles) * ytiles));
+  const int z0 = zt * kTileZ, y0 = yt * TY;
+  const int xb = xr * xlen;
   const int xe = xb + xlen < n0 ? xb + xlen : n0;
   const size_t s0 = static_cast<size_t>(n1) * n2;
   int off[PER];
@@ -286,7 +295,8 @@ template <typename R, typename O, int OP, bool FAST>
 static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, R omega, R keep,
                          hipStream_t s) {
   const int xlen = tiled_xlen(G.n);
-  const dim3 grid(G.n[2] / kTileZ, G.n[1] / kTileY, (G.n[0] + xlen - 1) / xlen);
+  const unsigned grid = static_cast<unsigned>((G.n[2] / kTileZ) * (G.n[1] / kTileY) *
+                                              ((G.n[0] + xlen - 1) / xlen));
   TileCoef<R> T;
   T.G = G;
   T.K = G.local ? GlobalCoef<R>() : GlobalCoef<R>::make(G);

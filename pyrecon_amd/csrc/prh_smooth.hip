@@ -23,7 +23,7 @@
 namespace prh {
 
 constexpr int kSmoothThreads = 256;
-constexpr size_t kSmoothLds = 73728;      // target LDS bytes of a block (2 blocks per CU)
+constexpr size_t kSmoothLds = 81920;      // target LDS bytes of a block (2 blocks per CU)
 constexpr size_t kSmoothLdsMax = 163840;  // upper limit (one line per block)
 
 template <typename R>
@@ -49,6 +49,12 @@ struct AxisPlan {
   int n, npass, ln;  // 2^ln >= n: padded element range of the load/store loops
   Pass pass[24];
 };
+
+// LDS position of element `idx` of the line buffers: one pad slot every 16
+// elements, so that the strided writes of the early Stockham passes (stride
+// radix elements) spread over the banks instead of hitting a few of them.
+__device__ __forceinline__ int lds_pad(int idx) { return idx + (idx >> 4); }
+__host__ __device__ constexpr size_t padded_len(size_t n) { return n + n / 16 + 1; }
 
 // 4-point DFT (forward: exp(-2 pi i / 4) = -i; inverse: +i).
 template <typename R>
@@ -84,10 +90,10 @@ __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>
       jq = j / Ns;
       k = j - jq * Ns;
     }
-    const Cx<R>* al = a + l * n;
+    const int la = l * n + j;
     Cx<R> v[RAD];
 #pragma unroll
-    for (int r = 0; r < RAD; r++) v[r] = al[j + r * m];
+    for (int r = 0; r < RAD; r++) v[r] = a[lds_pad(la + r * m)];
     const int tstep = k * P.span;  // r * tstep < n
 #pragma unroll
     for (int r = 1; r < RAD; r++) {
@@ -138,10 +144,9 @@ __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>
         o[q] = acc;
       }
     }
-    Cx<R>* bl = b + l * n;
-    const int d = jq * Ns * RAD + k;
+    const int d = l * n + jq * Ns * RAD + k;
 #pragma unroll
-    for (int r = 0; r < RAD; r++) bl[d + r * Ns] = o[r];
+    for (int r = 0; r < RAD; r++) b[lds_pad(d + r * Ns)] = o[r];
   }
 }
 
@@ -183,9 +188,10 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     const R* __restrict__ gain) {
   extern __shared__ double smem[];
   const int LP = 1 << llp;
+  const size_t buf = padded_len(static_cast<size_t>(LP) * P.n);
   Cx<R>* b0 = reinterpret_cast<Cx<R>*>(smem);
-  Cx<R>* b1 = b0 + static_cast<size_t>(LP) * P.n;
-  Cx<R>* stw = b1 + static_cast<size_t>(LP) * P.n;  // twiddle table, staged once per block
+  Cx<R>* b1 = b0 + buf;
+  Cx<R>* stw = b1 + buf;  // twiddle table, staged once per block
   const int n = P.n;
   for (int k = threadIdx.x; k < n; k += kSmoothThreads) stw[k] = tw_global[k];
   // XCD-aware order: the 8 XCDs take blocks round-robin; give consecutive
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
       const long long q = q0 + lp;
       const bool ok = t < nel && e < n;
       val[u] = ok && q < nreal ? mesh[base(q) + e * es] : R(0);
-      dst[u] = ok ? 2 * ((lp >> 1) * n + e) + (lp & 1) : -1;
+      dst[u] = ok ? 2 * lds_pad((lp >> 1) * n + e) + (lp & 1) : -1;
     }
 #pragma unroll
     for (int u = 0; u < kBatch; u++)
@@ -230,8 +236,9 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     if (e >= n) continue;
     const int l = t >> P.ln;
     const R g = gain[e];
-    spec[l * n + e].re *= g;
-    spec[l * n + e].im *= g;
+    const int q = lds_pad(l * n + e);
+    spec[q].re *= g;
+    spec[q].im *= g;
   }
   __syncthreads();
   Cx<R>* other = spec == b0 ? b1 : b0;
@@ -248,7 +255,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     }
     const long long q = q0 + lp;
     if (q < nreal) {
-      const R* slot = reinterpret_cast<const R*>(&res[(lp >> 1) * n + e]);
+      const R* slot = reinterpret_cast<const R*>(&res[lds_pad((lp >> 1) * n + e)]);
       mesh[base(q) + e * es] = slot[lp & 1];
     }
   }
@@ -319,7 +326,7 @@ int smooth_setup(prh_workspace* ws) {
   for (int d = 0; d < 3; d++) {
     const int n = static_cast<int>(ws->n[d]);
     AxisPlan P;
-    if (!factor_plan(n, P) || 6 * static_cast<size_t>(n) * w > kSmoothLdsMax) {
+    if (!factor_plan(n, P) || (2 * padded_len(n) + n) * 2 * w > kSmoothLdsMax) {
       S.have = false;  // smoothing refused at call time for this shape
       continue;
     }
@@ -357,7 +364,7 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
   }
   if (!ws->smooth.have) {
     set_error("smoothing supports mesh sizes whose prime factors are 2, 3, 5 and 7, up to %zu per "
-              "dimension", kSmoothLdsMax / (6 * sizeof(R)));
+              "dimension", kSmoothLdsMax * 8 / (50 * sizeof(R)));
     return PRH_ERR_ARG;
   }
   if (ws->dtype != Tag<R>::code) {
@@ -389,8 +396,10 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     const int n = P.n;
     // lines per block: the largest power of two whose two buffers fit kSmoothLds
     int llp = 0;
-    while (llp < 4 && ((size_t(2) << (llp + 1)) + 1) * 2 * sizeof(R) * static_cast<size_t>(n) <= kSmoothLds)
-      llp++;
+    auto bytes = [&](int l) {
+      return (2 * padded_len((size_t(1) << l) * static_cast<size_t>(n)) + n) * 2 * sizeof(R);
+    };
+    while (llp < 4 && bytes(llp + 1) <= kSmoothLds) llp++;
     const int LP = 1 << llp;
     long long nreal, ostride, istride, es;
     int I;
@@ -403,7 +412,7 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     }
     const long long ncomplex = (nreal + 1) / 2;
     const unsigned grid = static_cast<unsigned>((ncomplex + LP - 1) / LP);
-    const size_t lds = (2 * static_cast<size_t>(LP) + 1) * n * sizeof(Cx<R>);
+    const size_t lds = bytes(llp);
     const Cx<R>* tw = static_cast<const Cx<R>*>(ws->smooth.tw[d]);
     const R* gain = static_cast<const R*>(ws->smooth.gain[d]);
     if (lds > 65536) {

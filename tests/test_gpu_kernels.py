@@ -108,9 +108,11 @@ def test_assign_cic(gpu, oracle, dtype, shape, chunk):
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
 def test_assign_cic_dense_cells(gpu, oracle, dtype):
     # many points per cell: exercises the long-group ordering paths (LDS sort and rank placement)
+    # and the gather's staging: whole 9^3 region, two x-halves, and the global-memory merge
     rng = np.random.RandomState(12)
-    shape = (4, 4, 4)
-    for n in [300, 5000, 40000]:
+    cases = [((4, 4, 4), 300), ((4, 4, 4), 5000), ((4, 4, 4), 40000),
+             ((16, 16, 16), 14000), ((16, 16, 16), 20000), ((16, 16, 16), 30000)]
+    for shape, n in cases:
         pos = (rng.uniform(0., 1., size=(n, 3)) * np.array(shape)).astype(dtype)
         w = rng.uniform(0.5, 1.5, n).astype(dtype)
         a, b = np.zeros(shape, dtype), np.zeros(shape, dtype)
