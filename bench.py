@@ -388,9 +388,10 @@ def main():
         with open(tfile) as fh: traffic = json.load(fh).get('hbm_bytes_per_launch')
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel='k_stencil_tiled<float, fast> damped-Jacobi sweep, finest level 512^3 (mixed path); '
-                           'achieved = 3 x 512^3 x 4 B per launch / average launch time (HIP events); traffic = '
-                           'FETCH_SIZE + WRITE_SIZE per launch, calibrated on a copy kernel',
+                    kernel='k_stencil_rows<float, fast> damped-Jacobi sweep (blocks of 8 whole 512-cell rows), '
+                           'finest level 512^3 (mixed path); achieved = algorithmic bytes (3 x 512^3 x 4 B per '
+                           'sweep, 2 x 512^3 x 4 B for the zero-start sweep) / average launch time (HIP events); '
+                           'traffic = FETCH_SIZE + WRITE_SIZE per launch, calibrated on a copy kernel',
                     launches=nj, avg_launch_ms=round(ms_j / max(nj, 1), 5),
                     bytes_per_launch=(bytes_j / nj) if nj else None)
     cpu = None

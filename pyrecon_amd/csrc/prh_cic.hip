@@ -480,6 +480,9 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
   const int lz = tid % kGT, ly = (tid / kGT) % kGT, lx = tid / (kGT * kGT);
   const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
   const bool valid = x < n0 && y < n1 && z < n2;
+  // the target value is read now, so that its latency overlaps the staging
+  const size_t c = valid ? (static_cast<size_t>(x) * n1 + y) * n2 + z : 0;
+  R acc = valid ? mesh[c] : R(0);
   const int nparts = total <= static_cast<uint32_t>(kCap) ? 1 : 2;
   constexpr int kPlane = kGE * kGE;
   for (int part = 0; part < nparts; part++) {
@@ -531,8 +534,6 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
         beg[k] = staged ? s_off[e] - base : s_gbeg[e];
         end[k] = beg[k] + s_cnt[e];
       }
-      const size_t c = (static_cast<size_t>(x) * n1 + y) * n2 + z;
-      R acc = mesh[c];
       const bool any = staged ? merge_groups<R>(beg, end, s_idx, s_pay, acc)
                               : merge_groups<R>(beg, end, sorted, spay, acc);
       if (any) mesh[c] = acc;

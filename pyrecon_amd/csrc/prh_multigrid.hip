@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(const R* __restrict__ v,
 // HBM about once and every stencil read hits LDS.  256 threads: 64 along z,
 // 4 along y, TY/4 rows each.  The arithmetic per cell is the generic
 // kernels' (same bits).
-enum StencilOp { kOpJacobi = 0, kOpResidual = 1 };
+enum StencilOp { kOpJacobi = 0, kOpResidual = 1, kOpJacobiZero = 2 };
 constexpr int kTileZ = 64;
 
 // Coefficients of a level passed to the blocked kernels: the geometry for a
@@ -77,6 +77,61 @@ struct TileCoef {
   GlobalCoef<R> K;
   R dinv;  // 1 / K.d (fast variant only)
 };
+
+// One cell of a blocked sweep: the coefficients at (px, py, pz) (A, B, C are
+// h2 * p^2 along x, y, z) and the Jacobi update or the residual.
+template <typename R, int OP, bool LOCAL, bool FAST>
+__device__ __forceinline__ R stencil_cell(const TileCoef<R>& T, const Hood<R>& h, R fv, R px, R A,
+                                          R py, R B, R pz, R C, R pypz, R omega, R keep) {
+  const LevelGeom<R>& G = T.G;
+  R g, g2, cx, cy, cz, d, pxpy, pxpz, res;
+  if (LOCAL) {
+    pxpy = px * py;
+    pxpz = px * pz;
+    if constexpr (FAST) {
+      g = G.beta * rcp_fast(A + B + C);
+      cx = fmaf(g * px, px, G.ih2[0]);
+      cy = fmaf(g * py, py, G.ih2[1]);
+      cz = fmaf(g * pz, pz, G.ih2[2]);
+      g2 = g * R(0.5);
+      d = R(2) * (cx + cy + cz);
+    } else {
+      g = G.beta / (A + B + C);
+      cx = G.ih2[0] + g * px * px;
+      cy = G.ih2[1] + g * py * py;
+      cz = G.ih2[2] + g * pz * pz;
+      g2 = g / R(2);
+      d = R(2) * (cx + cy + cz);
+    }
+  } else {
+    g = T.K.g;
+    g2 = T.K.g2;
+    cx = T.K.cx;
+    cy = T.K.cy;
+    cz = T.K.cz;
+    d = T.K.d;
+    pxpy = T.K.pxpy;
+    pxpz = T.K.pxpz;
+  }
+  if constexpr (FAST) {
+    if (OP != kOpResidual) {
+      const R dinv = LOCAL ? rcp_fast(d) : T.dinv;
+      res = jacobi_fast<LOCAL>(h, fv, px, py, pz, pxpy, pxpz, pypz, g, g2, cx, cy, cz,
+                               dinv, omega, keep);
+    } else {
+      res = residual_fast<LOCAL>(h, fv, px, py, pz, pxpy, pxpz, pypz, g, g2, cx, cy, cz,
+                                 d);
+    }
+  } else {
+    if (OP != kOpResidual)
+      res = jacobi_split<R, LOCAL>(h, fv, px, py, pz, pxpy, pxpz, pypz, g, g2, cx, cy,
+                                   cz, d, omega, keep);
+    else
+      res = residual_split<R, LOCAL>(h, fv, px, py, pz, pxpy, pxpz, pypz, g, g2, cx,
+                                     cy, cz, d);
+  }
+  return res;
+}
 
 // Minimum waves per SIMD requested for the blocked kernel (a register cap);
 // 1 leaves the allocation to the compiler.
@@ -126,6 +181,7 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
   auto wrapx = [n0](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
   // ring slot of plane x, relative to the run start (static inside the unrolled loop)
   auto slot = [xb](int x) { return (x - xb + 4) & 3; };
+  if (OP != kOpJacobiZero)
   for (int p = -1; p <= 1; p++) {
     const R* src = v + static_cast<size_t>(wrapx(xb + p)) * s0;
     R* dst = lds + slot(xb + p) * PLANE;
@@ -183,7 +239,8 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
 #pragma unroll
     for (int k = 0; k < KY; k++) {
       const int o = lo[k];
-      Hood<R> h;
+      Hood<R> h = zero_hood<R>();
+      if (OP != kOpJacobiZero) {
       h.c = P[o];
       h.xp = N[o];
       h.xm = M[o];
@@ -203,56 +260,18 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
       h.ymzm = P[o - W - 1];
       h.ymzp = P[o - W + 1];
       h.ypzm = P[o + W - 1];
+      }
       const R fv = fx[rowoff[k]];
-      R g, g2, cx, cy, cz, d, pxpy, pxpz, res;
-      if (LOCAL) {
-        pxpy = px * py[k];
-        pxpz = px * pz;
-        if constexpr (FAST) {
-          g = G.beta * rcp_fast(A + B[k] + C);
-          cx = fmaf(g * px, px, G.ih2[0]);
-          cy = fmaf(g * py[k], py[k], G.ih2[1]);
-          cz = fmaf(g * pz, pz, G.ih2[2]);
-          g2 = g * R(0.5);
-          d = R(2) * (cx + cy + cz);
-        } else {
-          g = G.beta / (A + B[k] + C);
-          cx = G.ih2[0] + g * px * px;
-          cy = G.ih2[1] + g * py[k] * py[k];
-          cz = G.ih2[2] + g * pz * pz;
-          g2 = g / R(2);
-          d = R(2) * (cx + cy + cz);
-        }
-      } else {
-        g = T.K.g;
-        g2 = T.K.g2;
-        cx = T.K.cx;
-        cy = T.K.cy;
-        cz = T.K.cz;
-        d = T.K.d;
-        pxpy = T.K.pxpy;
-        pxpz = T.K.pxpz;
-      }
-      if constexpr (FAST) {
-        if (OP == kOpJacobi) {
-          const R dinv = LOCAL ? rcp_fast(d) : T.dinv;
-          res = jacobi_fast<LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx, cy, cz,
-                                   dinv, omega, keep);
-        } else {
-          res = residual_fast<LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx, cy, cz,
-                                     d);
-        }
-      } else {
-        if (OP == kOpJacobi)
-          res = jacobi_split<R, LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx, cy,
-                                       cz, d, omega, keep);
-        else
-          res = residual_split<R, LOCAL>(h, fv, px, py[k], pz, pxpy, pxpz, pypz[k], g, g2, cx,
-                                         cy, cz, d);
-      }
+      const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, fv, px, A, py[k], B[k], pz, C, pypz[k],
+                                                     omega, keep);
       ox[rowoff[k]] = static_cast<O>(res);
     }
   };
+  if (OP == kOpJacobiZero) {
+    // zero iterate: no neighbourhood to stage, one pass over the run
+    for (int x = xb; x < xe; x++) compute(x, lds, lds, lds);
+    return;
+  }
   if (xb + 1 < xe) fetch(preA, xb + 2);
   __syncthreads();
   for (int x0 = xb; x0 < xe; x0 += 4) {
@@ -280,6 +299,189 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
 }
 constexpr int kTileY = 16;
 
+// Full-row blocked kernel for FP32 levels with n2 = 64 * WPR * CZ and
+// n1 % RY == 0.  A block owns RY complete rows (every z) of a run of x
+// planes: WPR wavefronts per row, each lane CZ cells 64 * WPR apart.  Whole rows
+// make the z neighbours wrap inside the staged row, so besides the tile only
+// the two y-halo rows are read (the 64-wide tiles above also fetch a cache
+// line on each side of every row).  Planes x-1, x, x+1 sit in a ring of 4
+// LDS slots and planes x+2, x+3 are in flight in registers, as above; the
+// arithmetic per cell is stencil_cell()'s.
+constexpr int kRowsY = 8;
+#ifndef PRH_ROWS_BLOCKS
+#define PRH_ROWS_BLOCKS 512
+#endif
+#ifndef PRH_ROWS_MIN_WAVES
+#define PRH_ROWS_MIN_WAVES 1
+#endif
+
+template <typename R, typename O, int OP, int WPR, int CZ, bool LOCAL, bool FAST>
+__global__ __launch_bounds__(64 * kRowsY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_rows(const R* __restrict__ v,
+                                                              const R* __restrict__ f,
+                                                              O* __restrict__ out, TileCoef<R> T,
+                                                              R omega, R keep, int xlen) {
+  constexpr int RY = kRowsY, N2 = 64 * WPR * CZ, PLANE = (RY + 2) * N2, NT = 64 * RY * WPR;
+  constexpr int PER = (PLANE + NT - 1) / NT;
+  __shared__ R lds[4 * PLANE];
+  const LevelGeom<R>& G = T.G;
+  const int n0 = G.n[0], n1 = G.n[1];
+  const int tid = threadIdx.x, lane = tid & 63, ty = (tid >> 6) / WPR;
+  const int zb = lane + 64 * ((tid >> 6) % WPR);  // first column of the lane; then every 64 * WPR
+  // XCD-aware order (see k_stencil_tiled): y-tile fastest, then x-run
+  const int ytiles = n1 / RY;
+  const unsigned nb = gridDim.x, bid = blockIdx.x, per = nb / 8;
+  const unsigned L = bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+  const int yt = static_cast<int>(L % ytiles), xr = static_cast<int>(L / ytiles);
+  const int y0 = yt * RY;
+  const int xb = xr * xlen;
+  const int xe = xb + xlen < n0 ? xb + xlen : n0;
+  const size_t s0 = static_cast<size_t>(n1) * N2;
+  int off[PER];
+  bool has[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + NT * q;
+    has[q] = e < PLANE;
+    const int row = e / N2, col = e % N2;
+    int gy = y0 - 1 + row;
+    gy += gy < 0 ? n1 : 0;
+    gy -= gy >= n1 ? n1 : 0;
+    off[q] = gy * N2 + col;
+  }
+  auto wrapx = [n0](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
+  auto slot = [xb](int x) { return (x - xb + 4) & 3; };
+  if (OP != kOpJacobiZero)
+    for (int p = -1; p <= 1; p++) {
+      const R* src = v + static_cast<size_t>(wrapx(xb + p)) * s0;
+      R* dst = lds + slot(xb + p) * PLANE;
+#pragma unroll
+      for (int q = 0; q < PER; q++)
+        if (has[q]) dst[tid + NT * q] = src[off[q]];
+    }
+  // per-thread constants: row iy = y0 + ty, columns iz = zb + 64 WPR c
+  const int iy = y0 + ty;
+  const int rowoff = iy * N2;
+  R py = R(0), B = R(0);
+  R pz[CZ], C[CZ], pypz[CZ];
+  if (LOCAL) {
+    py = static_cast<R>(iy) + G.org[1];
+    B = G.h2[1] * py * py;
+  }
+#pragma unroll
+  for (int c = 0; c < CZ; c++) {
+    if (LOCAL) {
+      pz[c] = static_cast<R>(zb + 64 * WPR * c) + G.org[2];
+      C[c] = G.h2[2] * pz[c] * pz[c];
+      pypz[c] = py * pz[c];
+    } else {
+      pz[c] = R(0);
+      C[c] = R(0);
+      pypz[c] = T.K.pypz;
+    }
+  }
+  R preA[PER], preB[PER];
+  auto fetch = [&](R(&dst)[PER], int xp) {
+    const R* src = v + static_cast<size_t>(wrapx(xp)) * s0;
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+      if (has[q]) dst[q] = src[off[q]];
+  };
+  auto stash = [&](const R(&src)[PER], int sl) {
+    R* dst = lds + sl * PLANE;
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+      if (has[q]) dst[tid + NT * q] = src[q];
+  };
+  const int rm = ty * N2, r0 = (ty + 1) * N2, rp = (ty + 2) * N2;
+  auto compute = [&](int x, const R* M, const R* P, const R* N) {
+    const R* fx = f + static_cast<size_t>(x) * s0 + rowoff;
+    O* ox = out + static_cast<size_t>(x) * s0 + rowoff;
+    R px = R(0), A = R(0);
+    if (LOCAL) {
+      px = static_cast<R>(x) + G.org[0];
+      A = G.h2[0] * px * px;
+    }
+    R fv[CZ];
+#pragma unroll
+    for (int c = 0; c < CZ; c++) fv[c] = fx[zb + 64 * WPR * c];
+#pragma unroll
+    for (int c = 0; c < CZ; c++) {
+      const int zc = zb + 64 * WPR * c;
+      const int zl = zc == 0 ? N2 - 1 : zc - 1;
+      const int zr = zc == N2 - 1 ? 0 : zc + 1;
+      Hood<R> h = zero_hood<R>();
+      if (OP != kOpJacobiZero) {
+        h.c = P[r0 + zc];
+        h.xp = N[r0 + zc];
+        h.xm = M[r0 + zc];
+        h.yp = P[rp + zc];
+        h.ym = P[rm + zc];
+        h.zp = P[r0 + zr];
+        h.zm = P[r0 + zl];
+        h.xpyp = N[rp + zc];
+        h.xmym = M[rm + zc];
+        h.xmyp = M[rp + zc];
+        h.xpym = N[rm + zc];
+        h.xpzp = N[r0 + zr];
+        h.xmzm = M[r0 + zl];
+        h.xmzp = M[r0 + zr];
+        h.xpzm = N[r0 + zl];
+        h.ypzp = P[rp + zr];
+        h.ymzm = P[rm + zl];
+        h.ymzp = P[rm + zr];
+        h.ypzm = P[rp + zl];
+      }
+      const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, fv[c], px, A, py, B, pz[c], C[c],
+                                                     pypz[c], omega, keep);
+      ox[zc] = static_cast<O>(res);
+    }
+  };
+  if (OP == kOpJacobiZero) {
+    for (int x = xb; x < xe; x++) compute(x, lds, lds, lds);
+    return;
+  }
+  if (xb + 1 < xe) fetch(preA, xb + 2);
+  __syncthreads();
+  for (int x0 = xb; x0 < xe; x0 += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int x = x0 + u;
+      if (x < xe) {
+        if (x + 2 < xe) {
+          if ((u + 3) & 1)
+            fetch(preB, x + 3);
+          else
+            fetch(preA, x + 3);
+        }
+        compute(x, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE, lds + ((u + 1) & 3) * PLANE);
+        if (x + 1 < xe) {
+          if ((u + 2) & 1)
+            stash(preB, (u + 2) & 3);
+          else
+            stash(preA, (u + 2) & 3);
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// row length in units of 64 cells (1, 2, 4 or 8), 0 where the kernel does not apply
+inline int rows_cz(const int* n) {
+  if (n[2] % 64 != 0 || n[1] % kRowsY != 0 || n[0] < 1) return 0;
+  const int cz = n[2] / 64;
+  return (cz == 1 || cz == 2 || cz == 4 || cz == 8) ? cz : 0;
+}
+
+// x-run per block so that the grid has roughly PRH_ROWS_BLOCKS blocks
+inline int rows_xlen(const int* n) {
+  const long tiles = n[1] / kRowsY;
+  long chunks = (PRH_ROWS_BLOCKS + tiles - 1) / tiles;
+  if (chunks > n[0]) chunks = n[0];
+  int xlen = static_cast<int>((n[0] + chunks - 1) / chunks);
+  return xlen < 4 ? (n[0] < 4 ? n[0] : 4) : xlen;
+}
+
 inline bool tiled_ok(const int* n) { return n[2] % kTileZ == 0 && n[1] % kTileY == 0 && n[0] >= 1; }
 
 // x-run per block so that the grid has roughly 2048+ blocks
@@ -289,6 +491,37 @@ inline int tiled_xlen(const int* n) {
   if (chunks > n[0]) chunks = n[0];
   int xlen = static_cast<int>((n[0] + chunks - 1) / chunks);
   return xlen < 4 ? (n[0] < 4 ? n[0] : 4) : xlen;
+}
+
+template <typename R, typename O, int OP, bool FAST, int WPR, int CZ>
+static void launch_rows_cz(const R* v, const R* f, O* out, const TileCoef<R>& T, R omega, R keep,
+                           hipStream_t s) {
+  const int xlen = rows_xlen(T.G.n);
+  const unsigned grid =
+      static_cast<unsigned>((T.G.n[1] / kRowsY) * ((T.G.n[0] + xlen - 1) / xlen));
+  constexpr int nt = 64 * kRowsY * WPR;
+  if (T.G.local)
+    k_stencil_rows<R, O, OP, WPR, CZ, true, FAST><<<grid, nt, 0, s>>>(v, f, out, T, omega, keep, xlen);
+  else
+    k_stencil_rows<R, O, OP, WPR, CZ, false, FAST><<<grid, nt, 0, s>>>(v, f, out, T, omega, keep, xlen);
+}
+
+// The full-row kernel where it applies (FP32 levels with whole rows of 64, 128,
+// 256 or 512 cells); false leaves the level to the 64-wide tiles.
+template <typename R, typename O, int OP, bool FAST>
+static bool launch_rows(const R* v, const R* f, O* out, const TileCoef<R>& T, R omega, R keep,
+                        hipStream_t s) {
+  if constexpr (std::is_same<R, float>::value && PRH_ROWS_BLOCKS > 0) {
+    switch (rows_cz(T.G.n)) {
+      case 1: launch_rows_cz<R, O, OP, FAST, 1, 1>(v, f, out, T, omega, keep, s); return true;
+      case 2: launch_rows_cz<R, O, OP, FAST, 1, 2>(v, f, out, T, omega, keep, s); return true;
+      case 4: launch_rows_cz<R, O, OP, FAST, 1, 4>(v, f, out, T, omega, keep, s); return true;
+      // 512-cell rows: two wavefronts per row keep the registers per lane near those of 256
+      case 8: launch_rows_cz<R, O, OP, FAST, 2, 4>(v, f, out, T, omega, keep, s); return true;
+      default: return false;
+    }
+  }
+  return false;
 }
 
 template <typename R, typename O, int OP, bool FAST>
@@ -301,6 +534,7 @@ static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, 
   T.G = G;
   T.K = G.local ? GlobalCoef<R>() : GlobalCoef<R>::make(G);
   T.dinv = G.local ? R(0) : R(1) / T.K.d;
+  if (launch_rows<R, O, OP, FAST>(v, f, out, T, omega, keep, s)) return;
   if (G.local)
     k_stencil_tiled<R, O, OP, kTileY, true, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
   else
@@ -341,35 +575,44 @@ __global__ __launch_bounds__(kReduceThreads) void k_residual_norm(const R* __res
 
 // Full weighting (src/multigrid.c:236-262): centre x8, faces x4, edges x2,
 // corners x1, summed in the reference's order, divided by 64.
+// The 27-point sum of the full weighting; at(a, b, c) is the fine value at
+// offset (a - 1, b - 1, c - 1) from the centre, a, b, c in [0, 3).
+template <typename R, typename A>
+__device__ __forceinline__ R restrict_sum(A at) {
+  const R faces = at(2, 1, 1) + at(0, 1, 1) + at(1, 2, 1) + at(1, 0, 1) + at(1, 1, 2) + at(1, 1, 0);
+  const R edges = at(2, 2, 1) + at(0, 2, 1) + at(2, 0, 1) + at(0, 0, 1) + at(2, 1, 2) + at(0, 1, 2) +
+                  at(2, 1, 0) + at(0, 1, 0) + at(1, 2, 2) + at(1, 0, 2) + at(1, 2, 0) + at(1, 0, 0);
+  R s = R(8) * at(1, 1, 1) + R(4) * faces + R(2) * edges;
+  s += at(2, 2, 2);
+  s += at(0, 2, 2);
+  s += at(2, 0, 2);
+  s += at(0, 0, 2);
+  s += at(2, 2, 0);
+  s += at(0, 2, 0);
+  s += at(2, 0, 0);
+  s += at(0, 0, 0);
+  // reference divides by the double 64.0; an exact power-of-two scaling
+  return s * R(0.015625);
+}
+
+template <typename R>
+__device__ __forceinline__ R restrict_value(const R* __restrict__ u, int f0, int f1, int f2,
+                                            int jx, int jy, int jz) {
+  const size_t s0 = static_cast<size_t>(f1) * f2, s1 = static_cast<size_t>(f2);
+  const int fx = 2 * jx, fy = 2 * jy, fz = 2 * jz;
+  const size_t xo[3] = {(fx == 0 ? f0 - 1 : fx - 1) * s0, fx * s0, (fx + 1 == f0 ? 0 : fx + 1) * s0};
+  const size_t yo[3] = {(fy == 0 ? f1 - 1 : fy - 1) * s1, fy * s1, (fy + 1 == f1 ? 0 : fy + 1) * s1};
+  const size_t zo[3] = {static_cast<size_t>(fz == 0 ? f2 - 1 : fz - 1), static_cast<size_t>(fz),
+                        static_cast<size_t>(fz + 1 == f2 ? 0 : fz + 1)};
+  return restrict_sum<R>([&](int a, int b, int c) { return u[xo[a] + yo[b] + zo[c]]; });
+}
+
 template <typename R, typename RO>
 __device__ __forceinline__ void restrict_at(const R* __restrict__ u, RO* __restrict__ c, int f0,
                                             int f1, int f2, size_t j) {
-  const int c1 = f1 / 2, c2 = f2 / 2;
   int jx, jy, jz;
-  unflatten(j, c1, c2, jx, jy, jz);
-  const size_t s0 = static_cast<size_t>(f1) * f2, s1 = static_cast<size_t>(f2);
-  const int fx = 2 * jx, fy = 2 * jy, fz = 2 * jz;
-  const size_t x0 = fx * s0, xp = (fx + 1 == f0 ? 0 : fx + 1) * s0,
-               xm = (fx == 0 ? f0 - 1 : fx - 1) * s0;
-  const size_t y0 = fy * s1, yp = (fy + 1 == f1 ? 0 : fy + 1) * s1,
-               ym = (fy == 0 ? f1 - 1 : fy - 1) * s1;
-  const size_t z0 = fz, zp = (fz + 1 == f2 ? 0 : fz + 1), zm = (fz == 0 ? f2 - 1 : fz - 1);
-  const R faces = u[xp + y0 + z0] + u[xm + y0 + z0] + u[x0 + yp + z0] + u[x0 + ym + z0] +
-                  u[x0 + y0 + zp] + u[x0 + y0 + zm];
-  const R edges = u[xp + yp + z0] + u[xm + yp + z0] + u[xp + ym + z0] + u[xm + ym + z0] +
-                  u[xp + y0 + zp] + u[xm + y0 + zp] + u[xp + y0 + zm] + u[xm + y0 + zm] +
-                  u[x0 + yp + zp] + u[x0 + ym + zp] + u[x0 + yp + zm] + u[x0 + ym + zm];
-  R s = R(8) * u[x0 + y0 + z0] + R(4) * faces + R(2) * edges;
-  s += u[xp + yp + zp];
-  s += u[xm + yp + zp];
-  s += u[xp + ym + zp];
-  s += u[xm + ym + zp];
-  s += u[xp + yp + zm];
-  s += u[xm + yp + zm];
-  s += u[xp + ym + zm];
-  s += u[xm + ym + zm];
-  // reference divides by the double 64.0; an exact power-of-two scaling
-  c[j] = static_cast<RO>(s * R(0.015625));
+  unflatten(j, f1 / 2, f2 / 2, jx, jy, jz);
+  c[j] = static_cast<RO>(restrict_value(u, f0, f1, f2, jx, jy, jz));
 }
 
 template <typename R, typename RO>
@@ -385,6 +628,51 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
                                                        RO* __restrict__ c, int f0, int f1,
                                                        int f2) {
   restrict_body<R, RO>(u, c, f0, f1, f2);
+}
+
+template <typename R> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+
+// The same restriction on a 3D grid (coarse z along the block's x, coarse y
+// along its y, coarse x = blockIdx.z).  Each of the 9 fine rows a thread
+// needs is read as one pair (fine z = 2 jz, 2 jz + 1); fine z = 2 jz - 1 is
+// the previous lane's second value, read directly only by the first thread
+// of each block row.  Requires the fine mesh on a pair boundary.
+template <typename R, typename RO>
+__global__ __launch_bounds__(kThreads) void k_restrict_rows(const R* __restrict__ u,
+                                                            RO* __restrict__ c, int f0, int f1,
+                                                            int f2) {
+  using V = typename Pair<R>::type;
+  const int c1 = f1 / 2, c2 = f2 / 2;
+  const int jz0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jy0 = blockIdx.y * blockDim.y + threadIdx.y;
+  const int jx = blockIdx.z;
+  const bool in = jz0 < c2 && jy0 < c1;
+  // lanes past the edge compute a clamped cell (every lane takes part in the shuffles)
+  const int jz = jz0 < c2 ? jz0 : c2 - 1, jy = jy0 < c1 ? jy0 : c1 - 1;
+  const size_t s0 = static_cast<size_t>(f1) * f2, s1 = static_cast<size_t>(f2);
+  const int fx = 2 * jx, fy = 2 * jy, fz = 2 * jz;
+  const size_t xo[3] = {(fx == 0 ? f0 - 1 : fx - 1) * s0, fx * s0, (fx + 1 == f0 ? 0 : fx + 1) * s0};
+  const size_t yo[3] = {(fy == 0 ? f1 - 1 : fy - 1) * s1, fy * s1, (fy + 1 == f1 ? 0 : fy + 1) * s1};
+  const size_t zm = static_cast<size_t>(fz == 0 ? f2 - 1 : fz - 1);
+  const bool first = threadIdx.x == 0;
+  R w[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const R* row = u + xo[a] + yo[b];
+      const V pr = *reinterpret_cast<const V*>(row + fz);
+      const R left = __shfl_up(pr.y, 1);
+      w[a][b][1] = pr.x;
+      w[a][b][2] = pr.y;
+      w[a][b][0] = first ? row[zm] : left;
+    }
+  // fine z = 2 jz + 1 wraps to 0 only when f2 == 2 jz + 1, which cannot happen (f2 even)
+  if (in)
+    c[(static_cast<size_t>(jx) * c1 + jy) * c2 + jz] =
+        static_cast<RO>(restrict_sum<R>([&](int a, int b, int q) { return w[a][b][q]; }));
 }
 
 // Trilinear prolongation from the coarse mesh (c0, c1, c2) to the fine mesh,
@@ -437,6 +725,61 @@ __global__ __launch_bounds__(kThreads) void k_prolong(const RC* __restrict__ uc,
                                                       R* __restrict__ fine, int c0, int c1,
                                                       int c2) {
   prolong_body<RC, R, ADD>(uc, fine, c0, c1, c2);
+}
+
+// The same prolongation with one thread per fine (X, Y) row and coarse z:
+// the thread writes fine cells Z = 2 jz and 2 jz + 1 as one vector (fine
+// rows have even length), from the coarse values that both share.  The
+// parities of X and Y are uniform across a wavefront when c2 >= 64; each
+// value is formed with the operands and order of prolong_at().
+template <typename RC, typename R, bool ADD>
+__global__ __launch_bounds__(kThreads) void k_prolong_rows(const RC* __restrict__ uc,
+                                                           R* __restrict__ fine, int c0, int c1,
+                                                           int c2) {
+  const int F1 = 2 * c1, F2 = 2 * c2;
+  const int jz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int X = blockIdx.z;
+  if (jz >= c2 || Y >= F1) return;
+  const int jx = X >> 1, jy = Y >> 1;
+  const size_t s0 = static_cast<size_t>(c1) * c2, s1 = static_cast<size_t>(c2);
+  const size_t x0 = jx * s0, xp = (jx + 1 == c0 ? 0 : jx + 1) * s0;
+  const size_t y0 = jy * s1, yp = (jy + 1 == c1 ? 0 : jy + 1) * s1;
+  const size_t z0 = jz, zp = (jz + 1 == c2 ? 0 : jz + 1);
+  auto u = [uc](size_t idx) { return static_cast<R>(uc[idx]); };
+  const R a = u(x0 + y0 + z0), az = u(x0 + y0 + zp);
+  R even, odd;
+  if (X & 1) {
+    const R ax = u(xp + y0 + z0), axz = u(xp + y0 + zp);
+    if (Y & 1) {
+      const R ay = u(x0 + yp + z0), ayz = u(x0 + yp + zp);
+      const R axy = u(xp + yp + z0), axyz = u(xp + yp + zp);
+      even = (a + ax + ay + axy) / R(4);
+      odd = (a + ax + ay + az + axy + axz + ayz + axyz) / R(8);
+    } else {
+      even = (a + ax) / R(2);
+      odd = (a + ax + az + axz) / R(4);
+    }
+  } else if (Y & 1) {
+    const R ay = u(x0 + yp + z0), ayz = u(x0 + yp + zp);
+    even = (a + ay) / R(2);
+    odd = (a + ay + az + ayz) / R(4);
+  } else {
+    even = a;
+    odd = (a + az) / R(2);
+  }
+  using V = typename Pair<R>::type;
+  V* dst = reinterpret_cast<V*>(fine + (static_cast<size_t>(X) * F1 + Y) * F2 + 2 * jz);
+  V w;
+  if (ADD) {
+    w = *dst;
+    w.x += even;
+    w.y += odd;
+  } else {
+    w.x = even;
+    w.y = odd;
+  }
+  *dst = w;
 }
 
 template <typename R>
@@ -662,7 +1005,11 @@ static int launch_jacobi(const R* v, const R* f, R* out, const LevelGeom<R>& G, 
   dim3 block, grid;
   generic_shape(G.n, block, grid);
   timer_before(timer, s);
-  if (zero_in)
+  if (zero_in && tiled_ok(G.n) && fast)
+    launch_tiled<R, R, kOpJacobiZero, std::is_same<R, float>::value>(v, f, out, G, omega, keep, s);
+  else if (zero_in && tiled_ok(G.n))
+    launch_tiled<R, R, kOpJacobiZero, false>(v, f, out, G, omega, keep, s);
+  else if (zero_in)
     k_jacobi<R, true><<<grid, block, 0, s>>>(v, f, out, G, omega, keep);
   else if (tiled_ok(G.n) && fast)
     launch_tiled<R, R, kOpJacobi, std::is_same<R, float>::value>(v, f, out, G, omega, keep, s);
@@ -695,22 +1042,46 @@ static int launch_residual(const R* v, const R* f, O* r, const LevelGeom<R>& G, 
 
 template <typename R, typename RO>
 static int launch_restrict(const R* fine, RO* coarse, const size_t* nf, hipStream_t s) {
-  const size_t total = (nf[0] / 2) * (nf[1] / 2) * (nf[2] / 2);
-  k_restrict<R, RO><<<ceil_div(total, kThreads), kThreads, 0, s>>>(
-      fine, coarse, static_cast<int>(nf[0]), static_cast<int>(nf[1]), static_cast<int>(nf[2]));
+  if (nf[2] % 2 != 0 || reinterpret_cast<uintptr_t>(fine) % (2 * sizeof(R)) != 0) {
+    // rows that do not start on pair boundaries: one thread per coarse cell
+    const size_t total = (nf[0] / 2) * (nf[1] / 2) * (nf[2] / 2);
+    k_restrict<R, RO><<<ceil_div(total, kThreads), kThreads, 0, s>>>(
+        fine, coarse, static_cast<int>(nf[0]), static_cast<int>(nf[1]), static_cast<int>(nf[2]));
+    PRH_LAUNCHED();
+    return PRH_OK;
+  }
+  const int nc[3] = {static_cast<int>(nf[0] / 2), static_cast<int>(nf[1] / 2),
+                     static_cast<int>(nf[2] / 2)};
+  dim3 block, grid;
+  generic_shape(nc, block, grid);
+  k_restrict_rows<R, RO><<<grid, block, 0, s>>>(fine, coarse, static_cast<int>(nf[0]),
+                                                static_cast<int>(nf[1]), static_cast<int>(nf[2]));
   PRH_LAUNCHED();
   return PRH_OK;
 }
 
 template <typename RC, typename R>
 static int launch_prolong(const RC* coarse, R* fine, const size_t* nc, bool add, hipStream_t s) {
-  const size_t total = 8 * ncells(nc);
   const int c0 = static_cast<int>(nc[0]), c1 = static_cast<int>(nc[1]),
             c2 = static_cast<int>(nc[2]);
+  if (reinterpret_cast<uintptr_t>(fine) % (2 * sizeof(R)) != 0) {
+    // a fine mesh that does not start on a pair boundary: one thread per cell
+    const size_t total = 8 * ncells(nc);
+    if (add)
+      k_prolong<RC, R, true><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
+    else
+      k_prolong<RC, R, false><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
+    PRH_LAUNCHED();
+    return PRH_OK;
+  }
+  // threads over (coarse z, fine y, fine x)
+  const int shape[3] = {2 * c0, 2 * c1, c2};
+  dim3 block, grid;
+  generic_shape(shape, block, grid);
   if (add)
-    k_prolong<RC, R, true><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
+    k_prolong_rows<RC, R, true><<<grid, block, 0, s>>>(coarse, fine, c0, c1, c2);
   else
-    k_prolong<RC, R, false><<<ceil_div(total, kThreads), kThreads, 0, s>>>(coarse, fine, c0, c1, c2);
+    k_prolong_rows<RC, R, false><<<grid, block, 0, s>>>(coarse, fine, c0, c1, c2);
   PRH_LAUNCHED();
   return PRH_OK;
 }

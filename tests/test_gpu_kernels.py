@@ -12,8 +12,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 8, 8), (8, 12, 16), (16, 16, 16), (6, 10, 12), (32, 32, 32), (24, 40, 20),
-          # shapes taking the 2.5D-blocked kernel (n2 % 64 == 0, n1 % 16 == 0), incl. a partial x run
-          (6, 16, 64), (10, 32, 128), (64, 64, 64)]
+          # shapes taking the blocked kernels: 64-wide tiles (n2 % 64 == 0, n1 % 16 == 0, incl. a partial
+          # x run) and, in FP32, whole rows of 64, 128, 256 and 512 cells (n1 % 8 == 0)
+          (6, 16, 64), (10, 32, 128), (64, 64, 64), (6, 16, 256), (4, 8, 512), (9, 24, 512)]
 LOS = [None, [0., 0., 1.], [0.3, 0.4, np.sqrt(1 - 0.25)]]
 BOX = dict(boxsize=np.array([1000., 1200., 900.]), boxcenter=np.array([0., 100., 1500.]))
 
@@ -86,6 +87,19 @@ def test_fmg_64(gpu, oracle, dtype):
         va = oracle.fmg(f, 1000., [0., 0., 1500.], 0.4, los=los)
         vb = gpu.fmg(f, 1000., [0., 0., 1500.], 0.4, los=los)
         assert_bitexact(va, vb, 'fmg 64^3')
+
+
+@pytest.mark.parametrize('dtype', ['f4', 'f8'])
+def test_fmg_128(gpu, oracle, dtype):
+    # blocked kernels on two levels (128^3 and 64^3), zero-start sweeps on a blocked level, and the
+    # row-wise restriction and prolongation at full size
+    rng = np.random.RandomState(8)
+    shape = (128, 128, 128)
+    f = rng.normal(size=shape).astype(dtype)
+    for los in [None, [0., 0., 1.]]:
+        va = oracle.fmg(f, 1000., [0., 0., 1500.], 0.4, los=los)
+        vb = gpu.fmg(f, 1000., [0., 0., 1500.], 0.4, los=los)
+        assert_bitexact(va, vb, 'fmg 128^3')
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
