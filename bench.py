@@ -144,8 +144,8 @@ def run_serving(args):
                          '200 GB budget, 4 slots, grouped solves'.format(len(spec), args.serving_rate),
                 catalogs='generated on the device per job from its seed (uniform randoms; data Zel\'dovich-displaced '
                          'by 256 random Fourier modes of P(k)), inside the job latency',
-                mixed={k: (round(v, 5) if isinstance(v, float) else v) for k, v in mixed.items()},
-                plain={k: (round(v, 5) if isinstance(v, float) else v) for k, v in plain.items()},
+                mixed={k: (float('{:.5g}'.format(v)) if isinstance(v, float) else v) for k, v in mixed.items()},
+                plain={k: (float('{:.5g}'.format(v)) if isinstance(v, float) else v) for k, v in plain.items()},
                 error_against='plain FP64 serving run of the same jobs (subset)')
 
 

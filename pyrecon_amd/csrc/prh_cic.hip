@@ -890,6 +890,7 @@ template <typename R, typename P, typename W, int MODE>
 static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
                      size_t n, hipStream_t s) {
   PaintScratch& S = ws->paint;
+  S.order_pos = nullptr;  // `sorted` is rewritten below
   k_zero_misc<<<1, 64, 0, s>>>(S.misc);
   PRH_LAUNCHED();
   if (n == 0) return PRH_OK;
@@ -915,7 +916,11 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
   if (MODE == 0 || multi) return PRH_OK;
   uint32_t flags;
   PRH_TRY(read_flags(ws, s, &flags));
-  return flags_to_status(flags);
+  PRH_TRY(flags_to_status(flags));
+  S.order_pos = pos;
+  S.order_n = n;
+  S.order_pdt = Tag<P>::code;
+  return PRH_OK;
 }
 
 static int check_paint_ws(const prh_workspace* ws, const size_t* nmesh, int dtype) {
@@ -1041,6 +1046,18 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
     return PRH_OK;
   }
   PaintScratch& S = ws->paint;
+  if (S.order_pos == static_cast<const void*>(pos) && S.order_n == n && S.order_pdt == Tag<P>::code) {
+    // the catalog painted last: its cell order (a permutation of [0, n)) serves as the bins
+    timer_before(PRH_TIMER_READ_SHIFTS, s);
+    k_read_binned<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, pos, n, out, misc,
+                                                                           S.sorted);
+    timer_after(PRH_TIMER_READ_SHIFTS, s, bytes);
+    PRH_LAUNCHED();
+    return PRH_OK;
+  }
+  // the tile order goes to the memory of the point-order payload (dead after a paint), so that
+  // `sorted` keeps the last paint's cell order
+  uint32_t* order = static_cast<uint32_t*>(S.payload);
   const int t0 = (F.n[0] + kRT - 1) / kRT, t1 = (F.n[1] + kRT - 1) / kRT, t2 = (F.n[2] + kRT - 1) / kRT;
   const size_t ntile = static_cast<size_t>(t0) * t1 * t2;
   const unsigned nscan = ceil_div(ntile, kScanBlock);
@@ -1055,11 +1072,11 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
     PRH_LAUNCHED();
     k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.count, ntile, S.partial, S.start);
     PRH_LAUNCHED();
-    k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
+    k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, order);
     PRH_LAUNCHED();
     timer_before(PRH_TIMER_READ_SHIFTS, s);
     k_read_binned<R, P, O><<<grid, kCicThreads, 0, s>>>(phi, A, pos + 3 * o, m, out + ostride * o,
-                                                        misc, S.sorted);
+                                                        misc, order);
     timer_after(PRH_TIMER_READ_SHIFTS, s,
                 static_cast<double>(m) * (3 * sizeof(P) + ostride * sizeof(O)) +
                     static_cast<double>(F.n[0]) * F.n[1] * F.n[2] * sizeof(R));

@@ -20,11 +20,30 @@ namespace prh {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ void unflatten(size_t i, int n1, int n2, int& ix, int& iy, int& iz) {
-  iz = static_cast<int>(i % n2);
-  const size_t t = i / n2;
-  iy = static_cast<int>(t % n1);
-  ix = static_cast<int>(t / n1);
+// (ix, iy, iz) of the flat index i of an (n0, n1, n2) mesh.  A 32-bit index
+// (the single-workgroup levels) takes shifts for power-of-two extents and
+// 32-bit division otherwise.
+template <typename I>
+__device__ __forceinline__ void unflatten(I i, int n1, int n2, int& ix, int& iy, int& iz) {
+  if constexpr (sizeof(I) == 4) {
+    const unsigned u = static_cast<unsigned>(i), m1 = n1, m2 = n2;
+    if ((m1 & (m1 - 1)) == 0 && (m2 & (m2 - 1)) == 0) {
+      const int l1 = __builtin_ctz(m1), l2 = __builtin_ctz(m2);
+      iz = static_cast<int>(u & (m2 - 1));
+      iy = static_cast<int>((u >> l2) & (m1 - 1));
+      ix = static_cast<int>(u >> (l1 + l2));
+    } else {
+      const unsigned t = u / m2;
+      iz = static_cast<int>(u - t * m2);
+      iy = static_cast<int>(t % m1);
+      ix = static_cast<int>(t / m1);
+    }
+  } else {
+    iz = static_cast<int>(i % n2);
+    const size_t t = i / n2;
+    iy = static_cast<int>(t % n1);
+    ix = static_cast<int>(t / n1);
+  }
 }
 
 // Generic kernels: one thread per cell on a 3D grid (x = blockIdx.z), for any
@@ -607,9 +626,9 @@ __device__ __forceinline__ R restrict_value(const R* __restrict__ u, int f0, int
   return restrict_sum<R>([&](int a, int b, int c) { return u[xo[a] + yo[b] + zo[c]]; });
 }
 
-template <typename R, typename RO>
+template <typename R, typename RO, typename I>
 __device__ __forceinline__ void restrict_at(const R* __restrict__ u, RO* __restrict__ c, int f0,
-                                            int f1, int f2, size_t j) {
+                                            int f1, int f2, I j) {
   int jx, jy, jz;
   unflatten(j, f1 / 2, f2 / 2, jx, jy, jz);
   c[j] = static_cast<RO>(restrict_value(u, f0, f1, f2, jx, jy, jz));
@@ -678,9 +697,9 @@ __global__ __launch_bounds__(kThreads) void k_restrict_rows(const R* __restrict_
 // Trilinear prolongation from the coarse mesh (c0, c1, c2) to the fine mesh,
 // one thread per fine cell (src/multigrid.c:177-205), either overwriting the
 // fine value or adding to it (the V-cycle correction, src/multigrid.c:296).
-template <typename RC, typename R, bool ADD>
+template <typename RC, typename R, bool ADD, typename I>
 __device__ __forceinline__ void prolong_at(const RC* __restrict__ uc, R* __restrict__ fine,
-                                           int c0, int c1, int c2, size_t i) {
+                                           int c0, int c1, int c2, I i) {
   const int F1 = 2 * c1, F2 = 2 * c2;
   int X, Y, Z;
   unflatten(i, F1, F2, X, Y, Z);

@@ -22,7 +22,8 @@
 
 namespace prh {
 
-constexpr int kSmoothThreads = 256;
+constexpr int kSmoothThreads = 512;
+constexpr int kMaxFly = 2;  // butterflies per thread and pass (held in registers)
 constexpr size_t kSmoothLds = 81920;      // target LDS bytes of a block (2 blocks per CU)
 constexpr size_t kSmoothLdsMax = 163840;  // upper limit (one line per block)
 
@@ -70,18 +71,25 @@ __device__ __forceinline__ void dft4(const Cx<R> c[4], Cx<R> y[4], bool inverse)
   y[3] = {d02.re - rot.re, d02.im - rot.im};
 }
 
-// One Stockham pass of radix RAD over LP = 2^llp lines of length n (a -> b).
+// One Stockham pass of radix RAD over LP = 2^llp lines of length n, in place:
+// every thread reads and transforms its (at most kMaxFly) butterflies into
+// registers, the block synchronises, then the results are written over the
+// same buffer.  The host keeps LP * n / RAD <= kMaxFly * kSmoothThreads.
 template <typename R, int RAD>
-__device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>* __restrict__ b,
-                                              int n, int llp, const Pass& P,
+__device__ __forceinline__ void stockham_pass(Cx<R>* __restrict__ a, int n, int llp, const Pass& P,
                                               const Cx<R>* __restrict__ tw, bool inverse) {
   const int m = P.m, Ns = P.ns;
   const int rstep = n / RAD;  // index step of the RAD-th roots of unity
   const int total = 1 << (llp + P.lm);
-  for (int t = threadIdx.x; t < total; t += kSmoothThreads) {
+  Cx<R> o[kMaxFly][RAD];
+  int dst[kMaxFly];
+#pragma unroll
+  for (int fly = 0; fly < kMaxFly; fly++) {
+    const int t = threadIdx.x + fly * kSmoothThreads;
     const int l = t >> P.lm;
     const int j = t & ((1 << P.lm) - 1);
-    if (j >= m) continue;
+    dst[fly] = -1;
+    if (t >= total || j >= m) continue;
     int k, jq;
     if (P.ns_shift >= 0) {
       k = j & (Ns - 1);
@@ -101,10 +109,10 @@ __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>
       if (inverse) w.im = -w.im;
       v[r] = cmul(v[r], w);
     }
-    Cx<R> o[RAD];
+    Cx<R>* out = o[fly];
     if constexpr (RAD == 2) {
-      o[0] = {v[0].re + v[1].re, v[0].im + v[1].im};
-      o[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
+      out[0] = {v[0].re + v[1].re, v[0].im + v[1].im};
+      out[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
     } else if constexpr (RAD == 8) {
       // 2 x 4 split: a_k = v_k + v_k+4, b_k = (v_k - v_k+4) w8^k, then a 4-point DFT of
       // each; X_2m from a, X_2m+1 from b
@@ -123,11 +131,11 @@ __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>
       dft4(bb, yb, inverse);
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        o[2 * q] = ya[q];
-        o[2 * q + 1] = yb[q];
+        out[2 * q] = ya[q];
+        out[2 * q + 1] = yb[q];
       }
     } else if constexpr (RAD == 4) {
-      dft4(v, o, inverse);
+      dft4(v, out, inverse);
     } else {
       // direct DFT with the RAD-th roots of unity taken from the table
 #pragma unroll
@@ -141,38 +149,37 @@ __device__ __forceinline__ void stockham_pass(const Cx<R>* __restrict__ a, Cx<R>
           acc.re += p.re;
           acc.im += p.im;
         }
-        o[q] = acc;
+        out[q] = acc;
       }
     }
-    const int d = l * n + jq * Ns * RAD + k;
-#pragma unroll
-    for (int r = 0; r < RAD; r++) b[lds_pad(d + r * Ns)] = o[r];
+    dst[fly] = l * n + jq * Ns * RAD + k;
   }
+  __syncthreads();
+#pragma unroll
+  for (int fly = 0; fly < kMaxFly; fly++)
+    if (dst[fly] >= 0) {
+#pragma unroll
+      for (int r = 0; r < RAD; r++) a[lds_pad(dst[fly] + r * Ns)] = o[fly][r];
+    }
+  __syncthreads();
 }
 
-// Runs the passes of `P` on the 2^llp lines in b0; returns the buffer holding
-// the result (natural order).
+// Runs the passes of `P` on the 2^llp lines in `buf` (in place, natural order
+// in and out).
 template <typename R>
-__device__ Cx<R>* fft_lines(Cx<R>* b0, Cx<R>* b1, int llp, const AxisPlan& P,
-                            const Cx<R>* __restrict__ tw, bool inverse) {
-  Cx<R>* a = b0;
-  Cx<R>* b = b1;
+__device__ void fft_lines(Cx<R>* buf, int llp, const AxisPlan& P, const Cx<R>* __restrict__ tw,
+                          bool inverse) {
   for (int p = 0; p < P.npass; p++) {
     const Pass& S = P.pass[p];
     switch (S.rad) {
-      case 2: stockham_pass<R, 2>(a, b, P.n, llp, S, tw, inverse); break;
-      case 3: stockham_pass<R, 3>(a, b, P.n, llp, S, tw, inverse); break;
-      case 4: stockham_pass<R, 4>(a, b, P.n, llp, S, tw, inverse); break;
-      case 8: stockham_pass<R, 8>(a, b, P.n, llp, S, tw, inverse); break;
-      case 5: stockham_pass<R, 5>(a, b, P.n, llp, S, tw, inverse); break;
-      default: stockham_pass<R, 7>(a, b, P.n, llp, S, tw, inverse); break;
+      case 2: stockham_pass<R, 2>(buf, P.n, llp, S, tw, inverse); break;
+      case 3: stockham_pass<R, 3>(buf, P.n, llp, S, tw, inverse); break;
+      case 4: stockham_pass<R, 4>(buf, P.n, llp, S, tw, inverse); break;
+      case 8: stockham_pass<R, 8>(buf, P.n, llp, S, tw, inverse); break;
+      case 5: stockham_pass<R, 5>(buf, P.n, llp, S, tw, inverse); break;
+      default: stockham_pass<R, 7>(buf, P.n, llp, S, tw, inverse); break;
     }
-    __syncthreads();
-    Cx<R>* t = a;
-    a = b;
-    b = t;
   }
-  return a;
 }
 
 // Real lines of one axis: line q starts at (q / I) * ostride + (q % I) * istride
@@ -190,8 +197,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
   const int LP = 1 << llp;
   const size_t buf = padded_len(static_cast<size_t>(LP) * P.n);
   Cx<R>* b0 = reinterpret_cast<Cx<R>*>(smem);
-  Cx<R>* b1 = b0 + buf;
-  Cx<R>* stw = b1 + buf;  // twiddle table, staged once per block
+  Cx<R>* stw = b0 + buf;  // twiddle table, staged once per block
   const int n = P.n;
   for (int k = threadIdx.x; k < n; k += kSmoothThreads) stw[k] = tw_global[k];
   // XCD-aware order: the 8 XCDs take blocks round-robin; give consecutive
@@ -230,7 +236,8 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
       if (dst[u] >= 0) reinterpret_cast<R*>(b0)[dst[u]] = val[u];
   }
   __syncthreads();
-  Cx<R>* spec = fft_lines<R>(b0, b1, llp, P, stw, false);
+  fft_lines<R>(b0, llp, P, stw, false);
+  Cx<R>* spec = b0;
   for (int t = threadIdx.x; t < (LP << P.ln); t += kSmoothThreads) {
     const int e = t & ((1 << P.ln) - 1);
     if (e >= n) continue;
@@ -241,8 +248,8 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
     spec[q].im *= g;
   }
   __syncthreads();
-  Cx<R>* other = spec == b0 ? b1 : b0;
-  Cx<R>* res = fft_lines<R>(spec, other, llp, P, stw, true);
+  fft_lines<R>(b0, llp, P, stw, true);
+  const Cx<R>* res = b0;
   for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
     int lp, e;
     if (CONTIG) {
@@ -319,6 +326,16 @@ static bool factor_plan(int n, AxisPlan& P) {
   return true;
 }
 
+// log2 of the butterflies a block can hold in one pass
+static int fly_log2() { return ceil_log2(kMaxFly * kSmoothThreads); }
+
+// largest log2 butterfly range of the passes of a plan
+static int plan_max_lm(const AxisPlan& P) {
+  int lm = 0;
+  for (int i = 0; i < P.npass; i++) lm = P.pass[i].lm > lm ? P.pass[i].lm : lm;
+  return lm;
+}
+
 int smooth_setup(prh_workspace* ws) {
   SmoothState& S = ws->smooth;
   S.have = true;
@@ -326,7 +343,8 @@ int smooth_setup(prh_workspace* ws) {
   for (int d = 0; d < 3; d++) {
     const int n = static_cast<int>(ws->n[d]);
     AxisPlan P;
-    if (!factor_plan(n, P) || (2 * padded_len(n) + n) * 2 * w > kSmoothLdsMax) {
+    if (!factor_plan(n, P) || (padded_len(n) + n) * 2 * w > kSmoothLdsMax ||
+        plan_max_lm(P) > fly_log2()) {
       S.have = false;  // smoothing refused at call time for this shape
       continue;
     }
@@ -363,8 +381,10 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     return PRH_ERR_STATE;
   }
   if (!ws->smooth.have) {
-    set_error("smoothing supports mesh sizes whose prime factors are 2, 3, 5 and 7, up to %zu per "
-              "dimension", kSmoothLdsMax * 8 / (50 * sizeof(R)));
+    set_error("smoothing supports dimensions whose prime factors are 2, 3, 5 and 7, of at most %zu "
+              "cells (one line and its twiddles in LDS), and whose radix passes have at most %d "
+              "butterflies per line", (kSmoothLdsMax / (2 * sizeof(R)) - 1) * 16 / 33,
+              kMaxFly * kSmoothThreads);
     return PRH_ERR_ARG;
   }
   if (ws->dtype != Tag<R>::code) {
@@ -394,12 +414,14 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     AxisPlan P;
     factor_plan(static_cast<int>(nmesh[d]), P);
     const int n = P.n;
-    // lines per block: the largest power of two whose two buffers fit kSmoothLds
+    // lines per block: the largest power of two whose buffer fits kSmoothLds and whose
+    // butterflies fit the block's registers in every pass
     int llp = 0;
     auto bytes = [&](int l) {
-      return (2 * padded_len((size_t(1) << l) * static_cast<size_t>(n)) + n) * 2 * sizeof(R);
+      return (padded_len((size_t(1) << l) * static_cast<size_t>(n)) + n) * 2 * sizeof(R);
     };
-    while (llp < 4 && bytes(llp + 1) <= kSmoothLds) llp++;
+    const int lm = plan_max_lm(P);
+    while (llp < 5 && bytes(llp + 1) <= kSmoothLds && llp + 1 + lm <= fly_log2()) llp++;
     const int LP = 1 << llp;
     long long nreal, ostride, istride, es;
     int I;

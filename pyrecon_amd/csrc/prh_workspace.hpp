@@ -36,6 +36,14 @@ struct PaintScratch {
   uint32_t* misc = nullptr;     // [0] error flags, [1] number of long segments
   uint32_t* big = nullptr;      // long-segment cell list (ncell worst case is bounded by chunk/32)
   size_t nbig = 0;
+  // After a paint of one chunk, `sorted` is a permutation of that catalog's
+  // indices in cell order.  A later read of the same catalog (same positions
+  // pointer, count and dtype) visits its points in that order instead of
+  // binning them again.  Any permutation gives the same results (each point is
+  // evaluated on its own); the order only sets the cache locality.
+  const void* order_pos = nullptr;
+  size_t order_n = 0;
+  int order_pdt = 0;
 };
 
 // Tables of the separable Fourier-space smoothing: per axis, the twiddles
