@@ -316,7 +316,10 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
     }
   }
 }
-constexpr int kTileY = 16;
+#ifndef PRH_TILE_Y
+#define PRH_TILE_Y 16
+#endif
+constexpr int kTileY = PRH_TILE_Y;  // rows per 64-wide tile (4 per thread at 16)
 
 // Full-row blocked kernel for FP32 levels with n2 = 64 * WPR * CZ and
 // n1 % RY == 0.  A block owns RY complete rows (every z) of a run of x
@@ -921,16 +924,21 @@ struct CoarseArgs {
   const R* src0[kCoarseGroup];
 };
 
+// All single-workgroup levels live in LDS during the call: iterate, its
+// ping-pong partner, source and the per-cell g of each level (at most
+// 4096 (1 + 1/8 + 1/64 + 1/512) cells: 150 KB in FP64).
+constexpr int kCoarseLds = 4682;
+
 template <typename R>
 __device__ __forceinline__ void cw_jacobi(const LevelGeom<R>& G, const R* in, const R* src, R* out,
-                                          R omega, R keep, bool zero_in) {
+                                          const R* gtab, R omega, R keep, bool zero_in) {
   const int n1 = G.n[1], n2 = G.n[2];
   const int total = G.n[0] * n1 * n2;
   for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
     int ix, iy, iz;
     unflatten(i, n1, n2, ix, iy, iz);
-    const Coef<R> k = cell_coef(G, ix, iy, iz);
-    const Hood<R> h = zero_in ? zero_hood<R>() : gather_hood(in, G.n[0], n1, n2, ix, iy, iz);
+    const Coef<R> k = cell_coef_g(G, ix, iy, iz, gtab[i]);
+    const Hood<R> h = zero_in ? zero_hood<R>() : gather_hood<R, int>(in, G.n[0], n1, n2, ix, iy, iz);
     out[i] = jacobi_update(k, h, src[i], omega, keep, G.local);
   }
   __syncthreads();
@@ -938,7 +946,8 @@ __device__ __forceinline__ void cw_jacobi(const LevelGeom<R>& G, const R* in, co
 
 template <typename R>
 __device__ __forceinline__ void cw_smooth(const LevelGeom<R>& G, R*& cur, R*& alt, const R* src,
-                                          int niter, R omega, R keep, bool zero_start) {
+                                          const R* gtab, int niter, R omega, R keep,
+                                          bool zero_start) {
   if (niter <= 0) {
     if (zero_start) {
       const int total = G.n[0] * G.n[1] * G.n[2];
@@ -948,7 +957,7 @@ __device__ __forceinline__ void cw_smooth(const LevelGeom<R>& G, R*& cur, R*& al
     return;
   }
   for (int it = 0; it < niter; it++) {
-    cw_jacobi(G, cur, src, alt, omega, keep, zero_start && it == 0);
+    cw_jacobi(G, cur, src, alt, gtab, omega, keep, zero_start && it == 0);
     R* t = cur;
     cur = alt;
     alt = t;
@@ -956,41 +965,75 @@ __device__ __forceinline__ void cw_smooth(const LevelGeom<R>& G, R*& cur, R*& al
 }
 
 // src/multigrid.c:270-300 from relative level 0 to the coarsest, unrolled over
-// the (at most kMaxCoarse) levels so the level pointers stay in registers.
+// the (at most kMaxCoarse) levels.  Level 0's iterate and source are read from
+// global memory at the start and its iterate written back at the end; every
+// other step stays in LDS.  Per cell the arithmetic is the generic kernels'
+// (g computed once per call by cell_coef() itself), so the bits are the same.
 template <typename R>
 __global__ __launch_bounds__(kCoarseThreads) void k_vcycle_coarse(CoarseArgs<R> A) {
+  __shared__ R sv[kCoarseLds], st[kCoarseLds], sf[kCoarseLds], sg[kCoarseLds];
   const int job = blockIdx.x;
   const int nl = A.nlev;
-  R* cur[kMaxCoarse];
-  R* alt[kMaxCoarse];
-  const R* src[kMaxCoarse];
+  int off[kMaxCoarse];
+  {
+    int o = 0;
+#pragma unroll
+    for (int m = 0; m < kMaxCoarse; m++) {
+      off[m] = o;
+      if (m < nl) {
+        const LevelGeom<R>& G = A.L[job][m].G;
+        o += G.n[0] * G.n[1] * G.n[2];
+      }
+    }
+  }
+  {
+    const LevelGeom<R>& G = A.L[job][0].G;
+    const int total = G.n[0] * G.n[1] * G.n[2];
+    const R* c0 = A.cur0[job];
+    const R* f0 = A.src0[job];
+    for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
+      sf[i] = f0[i];
+      if (!A.zero_start) sv[i] = c0[i];
+    }
+  }
 #pragma unroll
   for (int m = 0; m < kMaxCoarse; m++) {
     if (m >= nl) break;
-    const CoarseLevel<R>& Lm = A.L[job][m];
-    if (m == 0) {
-      cur[0] = A.cur0[job];
-      alt[0] = A.alt0[job];
-      src[0] = A.src0[job];
-    } else {
-      cur[m] = Lm.v;
-      alt[m] = Lm.tmp;
-      src[m] = Lm.f;
+    const LevelGeom<R>& G = A.L[job][m].G;
+    const int total = G.n[0] * G.n[1] * G.n[2];
+    for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
+      int ix, iy, iz;
+      unflatten(i, G.n[1], G.n[2], ix, iy, iz);
+      sg[off[m] + i] = cell_coef(G, ix, iy, iz).g;
     }
-    cw_smooth(Lm.G, cur[m], alt[m], src[m], A.niter, A.omega, A.keep, m > 0 || A.zero_start);
+  }
+  __syncthreads();
+  R* cur[kMaxCoarse];
+  R* alt[kMaxCoarse];
+#pragma unroll
+  for (int m = 0; m < kMaxCoarse; m++) {
+    cur[m] = sv + off[m];
+    alt[m] = st + off[m];
+  }
+#pragma unroll
+  for (int m = 0; m < kMaxCoarse; m++) {
+    if (m >= nl) break;
+    const LevelGeom<R>& G = A.L[job][m].G;
+    const R* src = sf + off[m];
+    const R* gt = sg + off[m];
+    cw_smooth(G, cur[m], alt[m], src, gt, A.niter, A.omega, A.keep, m > 0 || A.zero_start);
     if (m + 1 < nl) {
-      const LevelGeom<R>& G = Lm.G;
       const int total = G.n[0] * G.n[1] * G.n[2];
       for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
         int ix, iy, iz;
         unflatten(i, G.n[1], G.n[2], ix, iy, iz);
-        const Coef<R> k = cell_coef(G, ix, iy, iz);
-        const Hood<R> h = gather_hood(cur[m], G.n[0], G.n[1], G.n[2], ix, iy, iz);
-        alt[m][i] = residual_value(k, h, src[m][i], G.local);
+        const Coef<R> k = cell_coef_g(G, ix, iy, iz, gt[i]);
+        const Hood<R> h = gather_hood<R, int>(cur[m], G.n[0], G.n[1], G.n[2], ix, iy, iz);
+        alt[m][i] = residual_value(k, h, src[i], G.local);
       }
       __syncthreads();
       const int ctotal = (G.n[0] / 2) * (G.n[1] / 2) * (G.n[2] / 2);
-      R* fc = A.L[job][m + 1].f;
+      R* fc = sf + off[m + 1];
       for (int j = threadIdx.x; j < ctotal; j += kCoarseThreads)
         restrict_at<R, R>(alt[m], fc, G.n[0], G.n[1], G.n[2], j);
       __syncthreads();
@@ -999,7 +1042,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vcycle_coarse(CoarseArgs<R> 
 #pragma unroll
   for (int m = kMaxCoarse - 1; m >= 0; m--) {
     if (m >= nl) continue;
-    const CoarseLevel<R>& Lm = A.L[job][m];
+    const LevelGeom<R>& G = A.L[job][m].G;
     if (m + 1 < nl) {
       const LevelGeom<R>& Gc = A.L[job][m + 1].G;
       const int ftotal = 8 * Gc.n[0] * Gc.n[1] * Gc.n[2];
@@ -1007,7 +1050,13 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vcycle_coarse(CoarseArgs<R> 
         prolong_at<R, R, true>(cur[m + 1], cur[m], Gc.n[0], Gc.n[1], Gc.n[2], i);
       __syncthreads();
     }
-    cw_smooth(Lm.G, cur[m], alt[m], src[m], A.niter, A.omega, A.keep, false);
+    cw_smooth(G, cur[m], alt[m], sf + off[m], sg + off[m], A.niter, A.omega, A.keep, false);
+  }
+  {
+    const LevelGeom<R>& G = A.L[job][0].G;
+    const int total = G.n[0] * G.n[1] * G.n[2];
+    R* c0 = A.cur0[job];
+    for (int i = threadIdx.x; i < total; i += kCoarseThreads) c0[i] = cur[0][i];
   }
 }
 

@@ -27,6 +27,26 @@ struct Coef {
   R px, py, pz, g, cx, cy, cz;
 };
 
+// cell_coef() with g given (the same expressions after g).
+template <typename R>
+__device__ __forceinline__ Coef<R> cell_coef_g(const LevelGeom<R>& G, int ix, int iy, int iz, R g) {
+  Coef<R> k;
+  if (G.local) {
+    k.px = static_cast<R>(ix) + G.org[0];
+    k.py = static_cast<R>(iy) + G.org[1];
+    k.pz = static_cast<R>(iz) + G.org[2];
+  } else {
+    k.px = G.losc[0];
+    k.py = G.losc[1];
+    k.pz = G.losc[2];
+  }
+  k.g = g;
+  k.cx = G.ih2[0] + k.g * k.px * k.px;
+  k.cy = G.ih2[1] + k.g * k.py * k.py;
+  k.cz = G.ih2[2] + k.g * k.pz * k.pz;
+  return k;
+}
+
 template <typename R>
 __device__ __forceinline__ Coef<R> cell_coef(const LevelGeom<R>& G, int ix, int iy, int iz) {
   Coef<R> k;
@@ -187,15 +207,16 @@ __device__ __forceinline__ float residual_fast(const Hood<float>& h, float f, fl
 }
 
 // Gather the neighbourhood of (ix, iy, iz) from a periodic C-order mesh.
-template <typename R>
+// I: index type (size_t; int for meshes below 2^31 cells, e.g. levels in LDS).
+template <typename R, typename I = size_t>
 __device__ __forceinline__ Hood<R> gather_hood(const R* __restrict__ v, int n0, int n1, int n2,
                                                int ix, int iy, int iz) {
-  const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
-  const size_t x0 = ix * s0, xp = (ix + 1 == n0 ? 0 : ix + 1) * s0,
-               xm = (ix == 0 ? n0 - 1 : ix - 1) * s0;
-  const size_t y0 = iy * s1, yp = (iy + 1 == n1 ? 0 : iy + 1) * s1,
-               ym = (iy == 0 ? n1 - 1 : iy - 1) * s1;
-  const size_t z0 = iz, zp = (iz + 1 == n2 ? 0 : iz + 1), zm = (iz == 0 ? n2 - 1 : iz - 1);
+  const I s0 = static_cast<I>(n1) * n2, s1 = static_cast<I>(n2);
+  const I x0 = ix * s0, xp = (ix + 1 == n0 ? 0 : ix + 1) * s0,
+          xm = (ix == 0 ? n0 - 1 : ix - 1) * s0;
+  const I y0 = iy * s1, yp = (iy + 1 == n1 ? 0 : iy + 1) * s1,
+          ym = (iy == 0 ? n1 - 1 : iy - 1) * s1;
+  const I z0 = iz, zp = (iz + 1 == n2 ? 0 : iz + 1), zm = (iz == 0 ? n2 - 1 : iz - 1);
   Hood<R> h;
   h.c = v[x0 + y0 + z0];
   h.xp = v[xp + y0 + z0];
