@@ -128,6 +128,10 @@ def serving_bench(args, precision, stream_spec, keep=None, reference=None):
         report['shift_max_mpc_h'] = float(dx.max())
         report['shift_rms_mpc_h'] = float(np.sqrt(np.mean(dx**2)))
         report['error_jobs'] = len(kept)
+        per = {i: float(np.abs(kept[i] - reference[i]).max()) for i in sorted(kept)}
+        worst = max(per, key=per.get)
+        report['worst_job'] = dict(index=worst, nmesh=stream_spec[worst]['nmesh'],
+                                   ndata=stream_spec[worst]['ndata'], shift_max_mpc_h=per[worst])
     return report, kept
 
 

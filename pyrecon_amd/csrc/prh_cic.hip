@@ -21,6 +21,7 @@
 // runs give identical bits.  Chunks are applied in order, which preserves
 // the point order across chunk boundaries.
 #include <type_traits>
+#include <utility>
 
 #include "prh_reduce.hpp"
 
@@ -941,7 +942,10 @@ template <typename R, typename P, typename W, int MODE>
 static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
                      size_t n, hipStream_t s) {
   PaintScratch& S = ws->paint;
-  S.order_pos = nullptr;  // `sorted` is rewritten below
+  // this paint's order goes to the older of the two order arrays
+  std::swap(S.sorted, S.sorted_prev);
+  S.order_prev = S.order;
+  S.order = PaintScratch::Order();
   k_zero_misc<<<1, 64, 0, s>>>(S.misc);
   PRH_LAUNCHED();
   if (n == 0) return PRH_OK;
@@ -968,9 +972,9 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
   uint32_t flags;
   PRH_TRY(read_flags(ws, s, &flags));
   PRH_TRY(flags_to_status(flags));
-  S.order_pos = pos;
-  S.order_n = n;
-  S.order_pdt = Tag<P>::code;
+  S.order.pos = pos;
+  S.order.n = n;
+  S.order.pdt = Tag<P>::code;
   return PRH_OK;
 }
 
@@ -1097,17 +1101,22 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
     return PRH_OK;
   }
   PaintScratch& S = ws->paint;
-  if (S.order_pos == static_cast<const void*>(pos) && S.order_n == n && S.order_pdt == Tag<P>::code) {
-    // the catalog painted last: its cell order (a permutation of [0, n)) serves as the bins
+  auto painted = [&](const PaintScratch::Order& o) {
+    return o.pos == static_cast<const void*>(pos) && o.n == n && o.pdt == Tag<P>::code;
+  };
+  const uint32_t* porder = painted(S.order) ? S.sorted : painted(S.order_prev) ? S.sorted_prev : nullptr;
+  if (porder) {
+    // one of the last two catalogs painted: its cell order (a permutation of [0, n)) serves as
+    // the bins
     timer_before(PRH_TIMER_READ_SHIFTS, s);
     k_read_binned<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, pos, n, out, misc,
-                                                                           S.sorted);
+                                                                           porder);
     timer_after(PRH_TIMER_READ_SHIFTS, s, bytes);
     PRH_LAUNCHED();
     return PRH_OK;
   }
   // the tile order goes to the memory of the point-order payload (dead after a paint), so that
-  // `sorted` keeps the last paint's cell order
+  // both paint orders are kept
   uint32_t* order = static_cast<uint32_t*>(S.payload);
   const int t0 = (F.n[0] + kRT - 1) / kRT, t1 = (F.n[1] + kRT - 1) / kRT, t2 = (F.n[2] + kRT - 1) / kRT;
   const size_t ntile = static_cast<size_t>(t0) * t1 * t2;

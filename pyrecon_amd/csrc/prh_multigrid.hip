@@ -317,9 +317,9 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
   }
 }
 #ifndef PRH_TILE_Y
-#define PRH_TILE_Y 16
+#define PRH_TILE_Y 8
 #endif
-constexpr int kTileY = PRH_TILE_Y;  // rows per 64-wide tile (4 per thread at 16)
+constexpr int kTileY = PRH_TILE_Y;  // rows per 64-wide tile (2 per thread at 8)
 
 // Full-row blocked kernel for FP32 levels with n2 = 64 * WPR * CZ and
 // n1 % RY == 0.  A block owns RY complete rows (every z) of a run of x

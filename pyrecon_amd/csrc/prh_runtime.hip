@@ -106,7 +106,7 @@ static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags)
   if (flags & PRH_WS_PAINT) {
     const size_t nc = cells(n);
     const size_t npart = (nc + 8191) / 8192;
-    p.paint = nc * 4 + (nc + 1) * 4 + (npart + 1) * 4 + chunk * 4 * 3 + 2 * chunk * 4 * w + 4 * 4 +
+    p.paint = nc * 4 + (nc + 1) * 4 + (npart + 1) * 4 + chunk * 4 * 4 + 2 * chunk * 4 * w + 4 * 4 +
               (chunk / 32 + 2) * 4;
   }
   if (flags & PRH_WS_FFT) p.fft = smooth_bytes(n, dtype);
@@ -140,6 +140,7 @@ static int paint_alloc(prh_workspace* ws, size_t chunk) {
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.key = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.rank = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.sorted = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.sorted_prev = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &P.payload, chunk * 4 * static_cast<size_t>(ws->dtype)));
   PRH_TRY(ws_alloc(ws, &P.spay, chunk * 4 * static_cast<size_t>(ws->dtype)));
   PRH_TRY(ws_alloc(ws, &p, 4 * 4)); P.misc = static_cast<uint32_t*>(p);
@@ -330,7 +331,8 @@ int prh_workspace_destroy(prh_workspace* ws) {
     }
   PaintScratch& P = ws->paint;
   for (void* p : {(void*)P.count, (void*)P.start, (void*)P.partial, (void*)P.key, (void*)P.rank,
-                  (void*)P.sorted, P.payload, P.spay, (void*)P.misc, (void*)P.big})
+                  (void*)P.sorted, (void*)P.sorted_prev, P.payload, P.spay, (void*)P.misc,
+                  (void*)P.big})
     release(p);
   for (int d = 0; d < 3; d++) {
     release(ws->smooth.tw[d]);

@@ -40,10 +40,16 @@ struct PaintScratch {
   // indices in cell order.  A later read of the same catalog (same positions
   // pointer, count and dtype) visits its points in that order instead of
   // binning them again.  Any permutation gives the same results (each point is
-  // evaluated on its own); the order only sets the cache locality.
-  const void* order_pos = nullptr;
-  size_t order_n = 0;
-  int order_pdt = 0;
+  // evaluated on its own); the order only sets the cache locality.  The two
+  // arrays alternate between paints, so the orders of the last two catalogs
+  // painted (data and randoms) are both kept; `sorted` is the one of the last.
+  uint32_t* sorted_prev = nullptr;  // chunk      the previous paint's order
+  struct Order {
+    const void* pos = nullptr;
+    size_t n = 0;
+    int pdt = 0;
+  };
+  Order order, order_prev;  // tags of `sorted` and `sorted_prev`
 };
 
 // Tables of the separable Fourier-space smoothing: per axis, the twiddles

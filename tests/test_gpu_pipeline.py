@@ -84,14 +84,14 @@ def test_pipeline_f8_stages(c1):
 
 
 def test_read_reuses_paint_order(c1):
-    # a read of the catalog painted last visits its points in the paint's cell order (no binning);
+    # a read of one of the last two catalogs painted visits its points in the paint's cell order;
     # the results are the oracle's bit for bit, also after the positions change in place (any
     # permutation of the points gives the same values) and around reads of another catalog
     from pyrecon_amd import DeviceArray
     gpu, cpu = make_pair(c1)
     rand = c1['randoms'][:300000]
-    drand = DeviceArray.from_numpy(rand)
-    for r, d, rr in ((gpu, DeviceArray.from_numpy(c1['data']), drand), (cpu, c1['data'], rand)):
+    drand, ddata = DeviceArray.from_numpy(rand), DeviceArray.from_numpy(c1['data'])
+    for r, d, rr in ((gpu, ddata, drand), (cpu, c1['data'], rand)):
         r.assign_data(d, c1['data_weights'])
         r.assign_randoms(rr, c1['randoms_weights'][:300000])
         r.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
@@ -99,8 +99,11 @@ def test_read_reuses_paint_order(c1):
     gpu.mesh_phi.value = cpu.mesh_phi
     expect = cpu.read_shifted_positions(rand)
     assert np.array_equal(bits(gpu.read_shifted_positions(drand).numpy()), bits(expect))
-    # another catalog in between (binned) leaves the paint order in place
+    # another catalog in between (binned) leaves the paint orders in place; the data, painted
+    # before the randoms, is read in its own paint order
     assert np.array_equal(bits(gpu.read_shifted_positions(c1['data'])), bits(cpu.read_shifted_positions(c1['data'])))
+    assert np.array_equal(bits(gpu.read_shifted_positions(ddata).numpy()),
+                          bits(cpu.read_shifted_positions(c1['data'])))
     assert np.array_equal(bits(gpu.read_shifts(drand, field='rsd').numpy()), bits(cpu.read_shifts(rand, field='rsd')))
     # same buffer, new contents: the stale order is still a permutation of the points
     moved = np.ascontiguousarray(rand[np.random.RandomState(3).permutation(len(rand))])
