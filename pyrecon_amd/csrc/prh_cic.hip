@@ -256,31 +256,42 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_scatter(
   if (i < m) sorted[start[key[i]] + rank[i]] = static_cast<uint32_t>(i);
 }
 
-// Paint step 4a: short groups sorted by one thread; long ones queued.
+// Paint step 4a: short groups sorted by one thread; long ones queued.  A
+// thread takes kOrderCells consecutive cells (most groups have 0 or 1
+// point: the pass is one read of `start`, and fewer, fuller workgroups).
+constexpr int kOrderCells = 8;
 __global__ __launch_bounds__(kCicThreads) void k_paint_order(const uint32_t* __restrict__ start,
                                                              size_t ncell,
                                                              uint32_t* __restrict__ sorted,
                                                              uint32_t* __restrict__ misc,
                                                              uint32_t* __restrict__ big,
                                                              uint32_t nbig) {
-  const size_t c = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
-  if (c >= ncell) return;
-  const uint32_t s = start[c], e = start[c + 1];
-  const uint32_t len = e - s;
-  if (len < 2) return;
-  if (len > kShortSegment) {
-    const uint32_t slot = atomicAdd(&misc[1], 1u);
-    if (slot < nbig) big[slot] = static_cast<uint32_t>(c);
-    return;
-  }
-  for (uint32_t a = s + 1; a < e; a++) {
-    const uint32_t v = sorted[a];
-    uint32_t b = a;
-    while (b > s && sorted[b - 1] > v) {
-      sorted[b] = sorted[b - 1];
-      b--;
+  const size_t c0 = (blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x) * kOrderCells;
+  if (c0 >= ncell) return;
+  const int nc = ncell - c0 < static_cast<size_t>(kOrderCells) ? static_cast<int>(ncell - c0) : kOrderCells;
+  uint32_t st[kOrderCells + 1];
+#pragma unroll
+  for (int q = 0; q <= kOrderCells; q++) st[q] = q <= nc ? start[c0 + q] : 0u;
+#pragma unroll
+  for (int q = 0; q < kOrderCells; q++) {
+    if (q >= nc) break;
+    const uint32_t s = st[q], e = st[q + 1];
+    const uint32_t len = e - s;
+    if (len < 2) continue;
+    if (len > kShortSegment) {
+      const uint32_t slot = atomicAdd(&misc[1], 1u);
+      if (slot < nbig) big[slot] = static_cast<uint32_t>(c0 + q);
+      continue;
     }
-    sorted[b] = v;
+    for (uint32_t a = s + 1; a < e; a++) {
+      const uint32_t v = sorted[a];
+      uint32_t b = a;
+      while (b > s && sorted[b - 1] > v) {
+        sorted[b] = sorted[b - 1];
+        b--;
+      }
+      sorted[b] = v;
+    }
   }
 }
 
@@ -885,7 +896,7 @@ static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* p
   PRH_LAUNCHED();
   k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
   PRH_LAUNCHED();
-  const unsigned gcell = ceil_div(ncell, kCicThreads);
+  const unsigned gcell = ceil_div(ceil_div(ncell, kOrderCells), kCicThreads);
   k_paint_order<<<gcell, kCicThreads, 0, s>>>(S.start, ncell, S.sorted, S.misc, S.big,
                                               static_cast<uint32_t>(S.nbig));
   PRH_LAUNCHED();

@@ -51,11 +51,22 @@ struct AxisPlan {
   Pass pass[24];
 };
 
-// LDS position of element `idx` of the line buffers: one pad slot every 16
-// elements, so that the strided writes of the early Stockham passes (stride
-// radix elements) spread over the banks instead of hitting a few of them.
+// LDS position of element `idx` of the line buffers.  The early Stockham
+// passes write with a stride of `radix` elements; without a remap those
+// writes fall on a few banks.  Default: the low 3 bits of the index are
+// XOR-ed with bits 3-5 (a permutation within each group of 8 elements, no
+// extra space; the buffers are rounded up to whole groups).  Otherwise: one
+// pad slot every 16 elements.
+#ifndef PRH_SMOOTH_SWIZZLE
+#define PRH_SMOOTH_SWIZZLE 1
+#endif
+#if PRH_SMOOTH_SWIZZLE
+__device__ __forceinline__ int lds_pad(int idx) { return idx ^ ((idx >> 3) & 7); }
+__host__ __device__ constexpr size_t padded_len(size_t n) { return (n + 7) & ~size_t(7); }
+#else
 __device__ __forceinline__ int lds_pad(int idx) { return idx + (idx >> 4); }
 __host__ __device__ constexpr size_t padded_len(size_t n) { return n + n / 16 + 1; }
+#endif
 
 // 4-point DFT (forward: exp(-2 pi i / 4) = -i; inverse: +i).
 template <typename R>
