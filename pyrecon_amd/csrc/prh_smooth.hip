@@ -23,6 +23,9 @@
 namespace prh {
 
 constexpr int kSmoothThreads = 512;
+#ifndef PRH_SMOOTH_REG512
+#define PRH_SMOOTH_REG512 1
+#endif
 constexpr int kMaxFly = 2;  // butterflies per thread and pass (held in registers)
 constexpr size_t kSmoothLds = 81920;      // target LDS bytes of a block (2 blocks per CU)
 constexpr size_t kSmoothLdsMax = 163840;  // upper limit (one line per block)
@@ -279,6 +282,129 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_axis(
   }
 }
 
+// Lines of 512 cells along z (contiguous), with the transforms in registers:
+// one wavefront per complex line (two real rows), lane L holding elements
+// L + 64 r, r = 0..7.  A 512-point DFT is 8 x 64 and the 64 is 8 x 8:
+//   1. DFT-8 over r in each lane, times W512^(L k1);
+//   2. transpose through LDS to lanes (j1, k1) = j1 + 8 k1 holding the
+//      values of lanes j1 + 8 j2, j2 = 0..7; DFT-8 over j2, times W64^(j1 m1);
+//   3. transpose to lanes (k1, m1) = k1 + 8 m1; DFT-8 over j1.
+// Lane k1 + 8 m1 then holds X[k1 + 8 m1 + 64 m2], m2 = 0..7: the same layout
+// as the input, so the gain is applied in place and the inverse transform is
+// the same three steps with conjugate roots.  Each transpose is one LDS write
+// and one read of 8 values per lane; nothing else leaves registers.
+template <typename R>
+__device__ __forceinline__ void dft8(Cx<R> v[8], bool inverse) {
+  constexpr R h = R(0.70710678118654752440);
+  Cx<R> aa[4], bb[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    aa[q] = {v[q].re + v[q + 4].re, v[q].im + v[q + 4].im};
+    bb[q] = {v[q].re - v[q + 4].re, v[q].im - v[q + 4].im};
+  }
+  bb[1] = cmul(bb[1], Cx<R>{h, inverse ? h : -h});
+  bb[2] = inverse ? Cx<R>{-bb[2].im, bb[2].re} : Cx<R>{bb[2].im, -bb[2].re};
+  bb[3] = cmul(bb[3], Cx<R>{-h, inverse ? h : -h});
+  Cx<R> ya[4], yb[4];
+  dft4(aa, ya, inverse);
+  dft4(bb, yb, inverse);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    v[2 * q] = ya[q];
+    v[2 * q + 1] = yb[q];
+  }
+}
+
+constexpr int kReg512Waves = 4;  // lines in flight per workgroup
+
+// exchange slot of value k of lane L (XOR-swizzled within each lane's 8)
+__device__ __forceinline__ int xslot(int lane, int k) { return lane * 8 + (k ^ (lane & 7)); }
+
+template <typename R>
+__device__ __forceinline__ void fft512_lane(Cx<R> v[8], Cx<R>* __restrict__ xb,
+                                            const Cx<R>* __restrict__ tw, int lane, bool inverse) {
+  auto root = [&](int k) {  // W512^k (conjugate for the inverse)
+    Cx<R> w = tw[k & 511];
+    if (inverse) w.im = -w.im;
+    return w;
+  };
+  // 1. over r (elements lane + 64 r)
+  dft8(v, inverse);
+#pragma unroll
+  for (int k = 1; k < 8; k++) v[k] = cmul(v[k], root(lane * k));
+  // 2. lane (j1, k1) gathers the values k1 of lanes j1 + 8 j2
+#pragma unroll
+  for (int k = 0; k < 8; k++) xb[xslot(lane, k)] = v[k];
+  __syncthreads();
+  const int j1 = lane & 7, k1 = lane >> 3;
+#pragma unroll
+  for (int j2 = 0; j2 < 8; j2++) v[j2] = xb[xslot(j1 + 8 * j2, k1)];
+  dft8(v, inverse);
+#pragma unroll
+  for (int m1 = 1; m1 < 8; m1++) v[m1] = cmul(v[m1], root(8 * j1 * m1));
+  __syncthreads();
+  // 3. lane (k1, m1) gathers the values m1 of lanes j1 + 8 k1
+#pragma unroll
+  for (int m1 = 0; m1 < 8; m1++) xb[xslot(lane, m1)] = v[m1];
+  __syncthreads();
+  const int kk1 = lane & 7, m1 = lane >> 3;
+#pragma unroll
+  for (int jj = 0; jj < 8; jj++) v[jj] = xb[xslot(jj + 8 * kk1, m1)];
+  dft8(v, inverse);
+  __syncthreads();
+}
+
+// `nrows` rows of 512 contiguous cells (row q at q * 512); wave w of block b
+// filters complex line p = b * kReg512Waves + w (rows 2p, 2p + 1), then moves
+// on by the grid's number of lines.
+template <typename R>
+__global__ __launch_bounds__(64 * kReg512Waves) void k_smooth_reg512(
+    R* __restrict__ mesh, long long nrows, const Cx<R>* __restrict__ tw_global,
+    const R* __restrict__ gain) {
+  __shared__ Cx<R> stw[512];
+  __shared__ Cx<R> xbuf[kReg512Waves][512];
+  for (int k = threadIdx.x; k < 512; k += 64 * kReg512Waves) stw[k] = tw_global[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  Cx<R>* xb = xbuf[w];
+  const long long ncx = (nrows + 1) / 2;
+  const long long step = static_cast<long long>(gridDim.x) * kReg512Waves;
+  // every wave of the block runs the same number of iterations (barriers inside)
+  for (long long p0 = static_cast<long long>(blockIdx.x) * kReg512Waves; p0 < ncx; p0 += step) {
+    const long long p = p0 + w;
+    const bool live = p < ncx;
+    const long long qa = 2 * p, qb = 2 * p + 1;
+    const R* ra = mesh + (live ? qa : 0) * 512;
+    const bool hasb = live && qb < nrows;
+    const R* rb = mesh + (hasb ? qb : 0) * 512;
+    Cx<R> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int e = lane + 64 * r;
+      v[r] = {live ? ra[e] : R(0), hasb ? rb[e] : R(0)};
+    }
+    fft512_lane(v, xb, stw, lane, false);
+    // lane holds X[lane + 64 m2]
+#pragma unroll
+    for (int m2 = 0; m2 < 8; m2++) {
+      const R g = gain[lane + 64 * m2];
+      v[m2].re *= g;
+      v[m2].im *= g;
+    }
+    fft512_lane(v, xb, stw, lane, true);
+    if (live) {
+      R* wa = mesh + qa * 512;
+#pragma unroll
+      for (int r = 0; r < 8; r++) wa[lane + 64 * r] = v[r].re;
+      if (hasb) {
+        R* wb = mesh + qb * 512;
+#pragma unroll
+        for (int r = 0; r < 8; r++) wb[lane + 64 * r] = v[r].im;
+      }
+    }
+  }
+}
+
 // gain[k] = exp(-(r k_phys)^2 / 2) / n with k_phys = 2 pi fftfreq(n, L / n)[k]
 // (numpy's ordering: k < (n - 1) / 2 + 1 -> k, else k - n), in double.
 template <typename R>
@@ -454,7 +580,12 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
       PRH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_smooth_axis<R, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     }
-    if (d == 2)
+    if (d == 2 && n == 512 && PRH_SMOOTH_REG512) {
+      const long long ncx = (nreal + 1) / 2;
+      const long long nb = (ncx + kReg512Waves - 1) / kReg512Waves;
+      const unsigned g512 = static_cast<unsigned>(nb < 8192 ? nb : 8192);
+      k_smooth_reg512<R><<<g512, 64 * kReg512Waves, 0, s>>>(mesh, nreal, tw, gain);
+    } else if (d == 2)
       k_smooth_axis<R, true><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride, istride,
                                                                es, llp, tw, gain);
     else

@@ -149,7 +149,9 @@ def test_reads(gpu, oracle, dtype, shape):
                     gpu.read_finite_difference_cic(mesh, boxsize, pos), 'read_finite_difference_cic')
 
 
-SMOOTH_SHAPES = [(8, 8, 8), (8, 12, 20), (6, 10, 14), (15, 9, 25), (10, 32, 128), (64, 64, 64), (96, 80, 112)]
+SMOOTH_SHAPES = [(8, 8, 8), (8, 12, 20), (6, 10, 14), (15, 9, 25), (10, 32, 128), (64, 64, 64), (96, 80, 112),
+                 # 512-cell z lines: the register-resident transform (an even and an odd number of rows)
+                 (4, 6, 512), (3, 5, 512)]
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
