@@ -16,6 +16,8 @@
 // lines held in LDS, ping-ponging between two buffers; twiddles come from a
 // per-axis table exp(-2 pi i k / n) built on the host in double precision.
 #include <cmath>
+#include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "prh_workspace.hpp"
@@ -405,6 +407,70 @@ __global__ __launch_bounds__(64 * kReg512Waves) void k_smooth_reg512(
   }
 }
 
+// Lines of 512 cells along x or y (stride `es`), transforms as in
+// k_smooth_reg512.  A block takes 16 adjacent real lines (consecutive z, one
+// 128-byte row of FP64 per element index): coalesced loads into an LDS tile
+// [element][line] (row stride 17 against bank conflicts), each of the 8
+// wavefronts reads its complex line (real lines 2w, 2w + 1) into registers,
+// the tile memory then serves as the transposition buffers, and the results
+// go back through the tile to coalesced stores.  Requires I % 16 == 0 and
+// consecutive lines adjacent (istride 1).
+constexpr int kReg512SWaves = 8;
+constexpr int kTileRow = 17;  // LDS row stride of the [element][16 lines] tile
+
+template <typename R>
+__global__ __launch_bounds__(64 * kReg512SWaves) void k_smooth_reg512s(
+    R* __restrict__ mesh, long long nreal, int I, long long ostride, long long es,
+    const Cx<R>* __restrict__ tw_global, const R* __restrict__ gain) {
+  constexpr int NT = 64 * kReg512SWaves;
+  __shared__ Cx<R> stw[512];
+  __shared__ R tile[512 * kTileRow];  // also 8 transposition buffers of 512 Cx<R>
+  static_assert(sizeof(R) * 512 * kTileRow >= sizeof(Cx<R>) * 512 * kReg512SWaves, "tile");
+  for (int k = threadIdx.x; k < 512; k += NT) stw[k] = tw_global[k];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  Cx<R>* xb = reinterpret_cast<Cx<R>*>(tile) + 512 * w;
+  const int c = threadIdx.x & 15, e0 = threadIdx.x >> 4;  // load/store: line c, elements e0 + 32 k
+  const long long ngroups = nreal / 16;
+  for (long long g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const long long q0 = 16 * g;
+    R* base = mesh + (q0 / I) * ostride + (q0 % I);
+    __syncthreads();  // the tile is free (previous group stored, twiddles staged)
+#pragma unroll 4
+    for (int k = 0; k < 16; k++) {
+      const int e = e0 + 32 * k;
+      tile[e * kTileRow + c] = base[static_cast<long long>(e) * es + c];
+    }
+    __syncthreads();
+    Cx<R> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const R* t = tile + (lane + 64 * r) * kTileRow + 2 * w;
+      v[r] = {t[0], t[1]};
+    }
+    __syncthreads();  // the tile becomes the transposition buffers
+    fft512_lane(v, xb, stw, lane, false);
+#pragma unroll
+    for (int m2 = 0; m2 < 8; m2++) {
+      const R gg = gain[lane + 64 * m2];
+      v[m2].re *= gg;
+      v[m2].im *= gg;
+    }
+    fft512_lane(v, xb, stw, lane, true);  // ends with a barrier
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      R* t = tile + (lane + 64 * r) * kTileRow + 2 * w;
+      t[0] = v[r].re;
+      t[1] = v[r].im;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 16; k++) {
+      const int e = e0 + 32 * k;
+      base[static_cast<long long>(e) * es + c] = tile[e * kTileRow + c];
+    }
+  }
+}
+
 // gain[k] = exp(-(r k_phys)^2 / 2) / n with k_phys = 2 pi fftfreq(n, L / n)[k]
 // (numpy's ordering: k < (n - 1) / 2 + 1 -> k, else k - n), in double.
 template <typename R>
@@ -585,9 +651,14 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
       const long long nb = (ncx + kReg512Waves - 1) / kReg512Waves;
       const unsigned g512 = static_cast<unsigned>(nb < 8192 ? nb : 8192);
       k_smooth_reg512<R><<<g512, 64 * kReg512Waves, 0, s>>>(mesh, nreal, tw, gain);
-    } else if (d == 2)
+    } else if (d == 2) {
       k_smooth_axis<R, true><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride, istride,
                                                                es, llp, tw, gain);
+    } else if (n == 512 && PRH_SMOOTH_REG512 && I % 16 == 0 && istride == 1) {
+      const long long ngroups = nreal / 16;
+      const unsigned g512 = static_cast<unsigned>(ngroups < 8192 ? ngroups : 8192);
+      k_smooth_reg512s<R><<<g512, 64 * kReg512SWaves, 0, s>>>(mesh, nreal, I, ostride, es, tw, gain);
+    }
     else
       k_smooth_axis<R, false><<<grid, kSmoothThreads, lds, s>>>(mesh, P, nreal, I, ostride,
                                                                 istride, es, llp, tw, gain);

@@ -150,8 +150,10 @@ def test_reads(gpu, oracle, dtype, shape):
 
 
 SMOOTH_SHAPES = [(8, 8, 8), (8, 12, 20), (6, 10, 14), (15, 9, 25), (10, 32, 128), (64, 64, 64), (96, 80, 112),
-                 # 512-cell z lines: the register-resident transform (an even and an odd number of rows)
-                 (4, 6, 512), (3, 5, 512)]
+                 # 512-cell lines: the register-resident transforms along z (an even and an odd number
+                 # of rows), x and y (16 adjacent rows per tile), and x with a z extent that is not a
+                 # multiple of 16 (the LDS transform)
+                 (4, 6, 512), (3, 5, 512), (512, 2, 16), (2, 512, 32), (512, 3, 5)]
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
