@@ -7,19 +7,21 @@
 //   1. key:     base cell b(i) and fractional offsets of every point, with
 //               the Cartesian -> mesh-unit transform, the wrap or range check
 //               and the finite check of pyrecon/mesh.py:544-556 fused in;
-//               a per-cell counter gives each point an arrival rank;
-//   2. scan:    exclusive prefix of the per-cell counts (cell start offsets);
-//   3. scatter: point indices grouped by base cell;
-//   4. order:   each cell's group sorted by point index (groups are short;
-//               long groups get one workgroup each), which makes the result
-//               independent of the arrival order of step 1;
+//   2. sort:    a stable LSD radix sort of the (base cell, point index)
+//               pairs (digits of up to 9 bits, one histogram, scan and
+//               ranked scatter per digit), so that point indices come out
+//               grouped by base cell and increasing within each group;
+//   3. starts:  each cell's first position in the sorted order, from the
+//               sorted keys;
+//   4. permute: the payload (offsets, weight) into the sorted order;
 //   5. gather:  each cell merges the 8 groups of the base cells that touch
 //               it by point index and accumulates the corner weights into its
 //               current value in that order.
 // The arithmetic per contribution is the reference's ((wx*wy)*wz)*w, so the
 // mesh is bit-identical to the reference's serial accumulation, and repeated
-// runs give identical bits.  Chunks are applied in order, which preserves
-// the point order across chunk boundaries.
+// runs give identical bits (no step depends on the timing of atomics).
+// Chunks are applied in order, which preserves the point order across chunk
+// boundaries.
 #include <type_traits>
 #include <utility>
 
@@ -29,8 +31,6 @@ namespace prh {
 
 constexpr int kCicThreads = 256;
 constexpr uint32_t kErrRange = 1u, kErrNonFinite = 2u;
-constexpr uint32_t kShortSegment = 32;
-constexpr int kLongSortCapacity = 8192;  // u32 keys sorted in LDS by one workgroup
 
 template <typename A, typename B>
 using promote_t = typename std::conditional<(sizeof(A) >= sizeof(B)), A, B>::type;
@@ -118,8 +118,8 @@ __device__ __forceinline__ int base_index(int ip, int n) {
 template <typename R, typename P, typename W, int MODE>
 __global__ __launch_bounds__(kCicThreads) void k_paint_key(
     const P* __restrict__ pos, const W* __restrict__ w, size_t m, Frame<R> F,
-    uint32_t* __restrict__ key, uint32_t* __restrict__ rank, Payload<R>* __restrict__ payload,
-    uint32_t* __restrict__ count, uint32_t* __restrict__ misc, int check_only) {
+    uint32_t* __restrict__ key, Payload<R>* __restrict__ payload, uint32_t* __restrict__ misc,
+    int check_only) {
   const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   uint32_t err = 0;
   if (i < m) {
@@ -142,7 +142,6 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_key(
       const uint32_t k = (static_cast<uint32_t>(b[0]) * F.n[1] + b[1]) * F.n[2] + b[2];
       key[i] = k;
       payload[i] = q;
-      rank[i] = atomicAdd(&count[k], 1u);
     }
   }
   // one atomic per wavefront with a non-zero error
@@ -154,7 +153,9 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_key(
   }
 }
 
-// Paint step 2: exclusive scan of count[0, n) into start[0, n], zeroing count.
+// Exclusive scan of count[0, n) into start[0, n] (start[n] = total), zeroing
+// count: the digit offsets of the radix sort and the tile starts of the read
+// binning.
 constexpr int kScanItems = 32;
 constexpr int kScanBlock = kCicThreads * kScanItems;  // 8192 cells per block
 
@@ -248,104 +249,254 @@ __global__ __launch_bounds__(kCicThreads) void k_scan_final(uint32_t* __restrict
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = partial[gridDim.x];
 }
 
-// Paint step 3
-__global__ __launch_bounds__(kCicThreads) void k_paint_scatter(
+// Counting-sort scatter of the read binning: index i to start[key[i]] + rank[i].
+__global__ __launch_bounds__(kCicThreads) void k_group_scatter(
     const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank, size_t m,
     const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted) {
   const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   if (i < m) sorted[start[key[i]] + rank[i]] = static_cast<uint32_t>(i);
 }
 
-// Paint step 4a: short groups sorted by one thread; long ones queued.  A
-// thread takes kOrderCells consecutive cells (most groups have 0 or 1
-// point: the pass is one read of `start`, and fewer, fuller workgroups).
-constexpr int kOrderCells = 8;
-__global__ __launch_bounds__(kCicThreads) void k_paint_order(const uint32_t* __restrict__ start,
-                                                             size_t ncell,
-                                                             uint32_t* __restrict__ sorted,
-                                                             uint32_t* __restrict__ misc,
-                                                             uint32_t* __restrict__ big,
-                                                             uint32_t nbig) {
-  const size_t c0 = (blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x) * kOrderCells;
-  if (c0 >= ncell) return;
-  const int nc = ncell - c0 < static_cast<size_t>(kOrderCells) ? static_cast<int>(ncell - c0) : kOrderCells;
-  uint32_t st[kOrderCells + 1];
+// Paint step 2: stable LSD radix sort of (key, index) pairs, one digit of
+// `bits` (<= 9) bits per pass.  A pass splits the pairs into tiles of
+// kRadixTile; k_radix_count writes each tile's digit counts digit-major
+// (hist[d * ntiles + tile]), their exclusive scan gives every (digit, tile)
+// its first output position, and k_radix_scatter ranks each tile's pairs
+// within their digit in input order and writes them there.  Tiles are dealt to
+// the 8 XCDs in contiguous ranges, so that the counts and the output runs of
+// neighbouring tiles meet in one L2.
+constexpr int kRadixThreads = 256;
+constexpr int kRadixWaves = kRadixThreads / 64;
+constexpr int kRadixItems = 16;                             // per thread
+constexpr int kRadixTile = kRadixThreads * kRadixItems;     // 4096 pairs
+constexpr int kRadixWaveItems = 64 * kRadixItems;           // 1024 pairs per wave
+constexpr int kRadixMaxBits = 9;
+constexpr int kRadixMaxBins = 1 << kRadixMaxBits;
+constexpr int kXcds = 8;
+
+// Tile of a radix block under the XCD-contiguous dealing: the grid has
+// kXcds * per blocks, block b runs on XCD b % kXcds and takes tile
+// (b % kXcds) * per + b / kXcds; blocks past the last tile return.
+__device__ __forceinline__ uint32_t radix_tile(uint32_t per) {
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+__global__ __launch_bounds__(kRadixThreads) void k_radix_count(const uint32_t* __restrict__ key,
+                                                               size_t m, int shift, int bits,
+                                                               uint32_t ntiles, uint32_t per,
+                                                               uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kRadixMaxBins];
+  const uint32_t tile = radix_tile(per);
+  if (tile >= ntiles) return;
+  const int nb = 1 << bits;
+  const uint32_t mask = static_cast<uint32_t>(nb - 1);
+  for (int d = threadIdx.x; d < nb; d += kRadixThreads) h[d] = 0u;
+  __syncthreads();
+  const size_t base = static_cast<size_t>(tile) * kRadixTile;
+  uint32_t k[kRadixItems];
 #pragma unroll
-  for (int q = 0; q <= kOrderCells; q++) st[q] = q <= nc ? start[c0 + q] : 0u;
+  for (int j = 0; j < kRadixItems; j++) {
+    const size_t i = base + j * kRadixThreads + threadIdx.x;
+    k[j] = i < m ? key[i] : 0u;
+  }
 #pragma unroll
-  for (int q = 0; q < kOrderCells; q++) {
-    if (q >= nc) break;
-    const uint32_t s = st[q], e = st[q + 1];
-    const uint32_t len = e - s;
-    if (len < 2) continue;
-    if (len > kShortSegment) {
-      const uint32_t slot = atomicAdd(&misc[1], 1u);
-      if (slot < nbig) big[slot] = static_cast<uint32_t>(c0 + q);
-      continue;
+  for (int j = 0; j < kRadixItems; j++) {
+    const size_t i = base + j * kRadixThreads + threadIdx.x;
+    if (i < m) atomicAdd(&h[(k[j] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < nb; d += kRadixThreads)
+    hist[static_cast<size_t>(d) * ntiles + tile] = h[d];
+}
+
+// Wave-wide inclusive prefix sum (64 lanes).
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// One pass of the sort.  Wave w of a tile holds the tile's pairs
+// [w * 1024, (w + 1) * 1024) in 16 rounds of 64 consecutive pairs, so
+// (wave, round, lane) is the input order.  In each round the lanes with equal
+// digits find each other by `bits` ballots; a pair's rank is the number of
+// equal-digit pairs before it in the wave (the wave's running count in LDS
+// plus the equal lanes below it).  The tile then stages its pairs in LDS in
+// (digit, rank) order and writes each digit's run to its scanned offset.
+// `vin` null: the values are the input positions (first pass).
+__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, size_t m, int shift,
+    int bits, uint32_t ntiles, uint32_t per, const uint32_t* __restrict__ goff,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+  __shared__ uint32_t s_wcnt[kRadixWaves][kRadixMaxBins];  // per-wave counts, then prefixes
+  __shared__ uint32_t s_lbeg[kRadixMaxBins];               // digit's first slot in the tile
+  __shared__ uint32_t s_gbeg[kRadixMaxBins];               // digit's first output position
+  __shared__ uint32_t s_wsum[kRadixWaves];
+  __shared__ uint32_t s_key[kRadixTile];
+  __shared__ uint32_t s_val[kRadixTile];
+  const uint32_t tile = radix_tile(per);
+  if (tile >= ntiles) return;
+  const int nb = 1 << bits;
+  const uint32_t mask = static_cast<uint32_t>(nb - 1);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int e = threadIdx.x; e < kRadixWaves * kRadixMaxBins; e += kRadixThreads)
+    (&s_wcnt[0][0])[e] = 0u;
+  const size_t tbase = static_cast<size_t>(tile) * kRadixTile;
+  const size_t wbase = tbase + static_cast<size_t>(w) * kRadixWaveItems;
+  uint32_t k[kRadixItems], v[kRadixItems], r[kRadixItems];
+#pragma unroll
+  for (int j = 0; j < kRadixItems; j++) {
+    const size_t i = wbase + j * 64 + lane;
+    k[j] = i < m ? kin[i] : 0u;
+    v[j] = i < m ? (vin ? vin[i] : static_cast<uint32_t>(i)) : 0u;
+  }
+  __syncthreads();
+  const uint64_t below = (uint64_t(1) << lane) - 1u;
+#pragma unroll
+  for (int j = 0; j < kRadixItems; j++) {
+    const bool ok = wbase + j * 64 + lane < m;
+    const uint32_t d = (k[j] >> shift) & mask;
+    uint64_t peers = __ballot(ok);
+    for (int b = 0; b < bits; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t set = __ballot(bit);
+      peers &= bit ? set : ~set;
     }
-    for (uint32_t a = s + 1; a < e; a++) {
-      const uint32_t v = sorted[a];
-      uint32_t b = a;
-      while (b > s && sorted[b - 1] > v) {
-        sorted[b] = sorted[b - 1];
-        b--;
+    const uint32_t before = static_cast<uint32_t>(__popcll(peers & below));
+    const uint32_t prior = ok ? s_wcnt[w][d] : 0u;
+    r[j] = prior + before;
+    // the lowest lane of each digit advances the wave's count
+    if (ok && before == 0u) s_wcnt[w][d] = prior + static_cast<uint32_t>(__popcll(peers));
+  }
+  __syncthreads();
+  // per digit: exclusive prefix over the waves, and the tile's count
+  uint32_t tc[2] = {0u, 0u};
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int d = 2 * threadIdx.x + q;
+    uint32_t run = 0;
+#pragma unroll
+    for (int u = 0; u < kRadixWaves; u++) {
+      const uint32_t c = s_wcnt[u][d];
+      s_wcnt[u][d] = run;
+      run += c;
+    }
+    tc[q] = run;
+  }
+  // exclusive scan of the tile counts over the digits (two digits per thread)
+  const uint32_t pair = tc[0] + tc[1];
+  const uint32_t incl = wave_inclusive_sum(pair);
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t wpre = 0;
+  for (int u = 0; u < w; u++) wpre += s_wsum[u];
+  const uint32_t ex = wpre + incl - pair;
+  {
+    const int d0 = 2 * threadIdx.x;
+    s_lbeg[d0] = ex;
+    s_lbeg[d0 + 1] = ex + tc[0];
+    if (d0 < nb) s_gbeg[d0] = goff[static_cast<size_t>(d0) * ntiles + tile];
+    if (d0 + 1 < nb) s_gbeg[d0 + 1] = goff[static_cast<size_t>(d0 + 1) * ntiles + tile];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kRadixItems; j++) {
+    if (wbase + j * 64 + lane < m) {
+      const uint32_t d = (k[j] >> shift) & mask;
+      const uint32_t slot = s_lbeg[d] + s_wcnt[w][d] + r[j];
+      if (slot < static_cast<uint32_t>(kRadixTile)) {
+        s_key[slot] = k[j];
+        s_val[slot] = v[j];
       }
-      sorted[b] = v;
+    }
+  }
+  __syncthreads();
+  const size_t rem = m - tbase;
+  const int cnt = rem < static_cast<size_t>(kRadixTile) ? static_cast<int>(rem) : kRadixTile;
+  for (int e = threadIdx.x; e < cnt; e += kRadixThreads) {
+    const uint32_t kk = s_key[e];
+    const uint32_t d = (kk >> shift) & mask;
+    const size_t g = static_cast<size_t>(s_gbeg[d]) + (static_cast<uint32_t>(e) - s_lbeg[d]);
+    if (g < m) {
+      kout[g] = kk;
+      vout[g] = s_val[e];
     }
   }
 }
 
-// Paint step 4b: one workgroup per long group.  Groups up to
-// kLongSortCapacity are bitonic-sorted in LDS; longer ones are placed by
-// rank (count of smaller indices; indices are distinct) through `scratch`.
-__global__ __launch_bounds__(kCicThreads) void k_paint_order_long(
-    const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted,
-    const uint32_t* __restrict__ misc, const uint32_t* __restrict__ big, uint32_t nbig,
-    uint32_t* __restrict__ scratch) {
-  __shared__ uint32_t lds[kLongSortCapacity];
-  const uint32_t nlong = misc[1] < nbig ? misc[1] : nbig;
-  for (uint32_t q = blockIdx.x; q < nlong; q += gridDim.x) {
-    const uint32_t c = big[q];
-    const uint32_t s = start[c], len = start[c + 1] - s;
-    if (len <= static_cast<uint32_t>(kLongSortCapacity)) {
-      uint32_t p2 = 1;
-      while (p2 < len) p2 <<= 1;
-      for (uint32_t i = threadIdx.x; i < p2; i += kCicThreads)
-        lds[i] = i < len ? sorted[s + i] : 0xffffffffu;
-      __syncthreads();
-      for (uint32_t k = 2; k <= p2; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-          for (uint32_t i = threadIdx.x; i < p2; i += kCicThreads) {
-            const uint32_t l = i ^ j;
-            if (l > i) {
-              const bool up = (i & k) == 0;
-              const uint32_t a = lds[i], b = lds[l];
-              if ((a > b) == up) {
-                lds[i] = b;
-                lds[l] = a;
-              }
-            }
-          }
-          __syncthreads();
-        }
-      }
-      for (uint32_t i = threadIdx.x; i < len; i += kCicThreads) sorted[s + i] = lds[i];
-      __syncthreads();
-    } else {
-      for (uint32_t i = threadIdx.x; i < len; i += kCicThreads) {
-        const uint32_t v = sorted[s + i];
-        uint32_t r = 0;
-        for (uint32_t j = 0; j < len; j++) r += sorted[s + j] < v;
-        scratch[s + r] = v;
-      }
-      __syncthreads();
-      for (uint32_t i = threadIdx.x; i < len; i += kCicThreads) sorted[s + i] = scratch[s + i];
-      __syncthreads();
-    }
-  }
+// Paint step 3: cell starts from the sorted keys, kStartCells cells per block.
+// bnd[b] is the first sorted position whose key is >= b * kStartCells; a block
+// marks the first position of each of its non-empty cells and fills the empty
+// ones with the next non-empty cell's start by a suffix minimum.
+constexpr int kStartItems = 32;
+constexpr int kStartCells = kCicThreads * kStartItems;  // 8192
+
+__global__ __launch_bounds__(kCicThreads) void k_start_bounds(const uint32_t* __restrict__ skey,
+                                                              size_t m, uint32_t nsb,
+                                                              uint32_t* __restrict__ bnd) {
+  const size_t p = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (p > m) return;
+  // blocks b with key[p - 1] < b * kStartCells <= key[p] start at p (key[m] = infinity)
+  const int64_t lo = p > 0 ? static_cast<int64_t>(skey[p - 1] / kStartCells) : -1;
+  const int64_t hi = p < m ? static_cast<int64_t>(skey[p] / kStartCells) : static_cast<int64_t>(nsb);
+  for (int64_t b = lo + 1; b <= hi && b <= static_cast<int64_t>(nsb); b++)
+    bnd[b] = static_cast<uint32_t>(p);
 }
 
-// Paint step 4c: payload into cell-grouped order, so that step 5 reads each
+__global__ __launch_bounds__(kCicThreads) void k_cell_start(const uint32_t* __restrict__ skey,
+                                                            const uint32_t* __restrict__ bnd,
+                                                            size_t ncell, size_t m,
+                                                            uint32_t* __restrict__ start) {
+  __shared__ uint32_t s[kStartCells + kStartCells / 32];
+  __shared__ uint32_t s_wmin[kCicThreads / 64];
+  auto pad = [](int e) { return e + (e >> 5); };
+  const size_t c0 = blockIdx.x * static_cast<size_t>(kStartCells);
+  const size_t rem = ncell - c0;
+  const int cnt = rem < static_cast<size_t>(kStartCells) ? static_cast<int>(rem) : kStartCells;
+  // (clamped into [0, m]: every position read below stays inside the sorted keys)
+  const uint32_t hi = min(bnd[blockIdx.x + 1], static_cast<uint32_t>(m));
+  const uint32_t lo = min(bnd[blockIdx.x], hi);
+  for (int e = threadIdx.x; e < kStartCells; e += kCicThreads) s[pad(e)] = hi;
+  __syncthreads();
+  for (uint32_t p = lo + threadIdx.x; p < hi; p += kCicThreads) {
+    const uint32_t k = skey[p];
+    if (p == 0u || skey[p - 1] != k) {
+      const size_t e = static_cast<size_t>(k) - c0;
+      if (e < static_cast<size_t>(cnt)) s[pad(static_cast<int>(e))] = p;
+    }
+  }
+  __syncthreads();
+  // suffix minimum: thread t owns cells [32 t, 32 t + 32)
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int e0 = threadIdx.x * kStartItems;
+  uint32_t seg = hi;
+  for (int q = 0; q < kStartItems; q++) seg = min(seg, s[pad(e0 + q)]);
+  uint32_t x = seg;  // minimum over this lane and the lanes above it
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_down(x, off);
+    if (lane + off < 64) x = min(x, y);
+  }
+  uint32_t after = __shfl_down(x, 1);
+  if (lane == 63) after = hi;
+  if (lane == 0) s_wmin[w] = x;
+  __syncthreads();
+  for (int u = w + 1; u < kCicThreads / 64; u++) after = min(after, s_wmin[u]);
+  uint32_t run = after;
+  for (int q = kStartItems - 1; q >= 0; q--) {
+    run = min(run, s[pad(e0 + q)]);
+    s[pad(e0 + q)] = run;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < cnt; e += kCicThreads) start[c0 + e] = s[pad(e)];
+  if (c0 + cnt == ncell && threadIdx.x == 0) start[ncell] = static_cast<uint32_t>(m);
+}
+
+// Paint step 4: payload into cell-grouped order, so that step 5 reads each
 // group contiguously.
 template <typename R>
 __global__ __launch_bounds__(kCicThreads) void k_paint_permute(const uint32_t* __restrict__ sorted,
@@ -876,32 +1027,69 @@ __global__ __launch_bounds__(kCicThreads) void k_bin_key(const P* __restrict__ p
 // Host drivers
 // ---------------------------------------------------------------------------
 
+// Digit widths of the radix sort of keys in [0, ncell): the fewest passes of
+// at most kRadixMaxBits bits, as even as possible.
+struct RadixPlan {
+  int npass = 0;
+  int bits[4] = {};
+  int shift[4] = {};
+};
+static RadixPlan radix_plan(size_t ncell) {
+  int nbits = 1;
+  while (nbits < 32 && (size_t(1) << nbits) < ncell) nbits++;
+  RadixPlan P;
+  P.npass = (nbits + kRadixMaxBits - 1) / kRadixMaxBits;
+  int sh = 0;
+  for (int p = 0; p < P.npass; p++) {
+    P.bits[p] = nbits / P.npass + (p < nbits % P.npass ? 1 : 0);
+    P.shift[p] = sh;
+    sh += P.bits[p];
+  }
+  return P;
+}
+
 template <typename R, typename P, typename W, int MODE>
 static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
                        size_t m, bool check_only, hipStream_t s) {
   PaintScratch& S = ws->paint;
   const unsigned grid = ceil_div(m, kCicThreads);
   k_paint_key<R, P, W, MODE><<<grid, kCicThreads, 0, s>>>(
-      pos, w, m, F, S.key, S.rank, static_cast<Payload<R>*>(S.payload), S.count, S.misc,
-      check_only ? 1 : 0);
+      pos, w, m, F, S.key, static_cast<Payload<R>*>(S.payload), S.misc, check_only ? 1 : 0);
   PRH_LAUNCHED();
   if (check_only) return PRH_OK;
   const size_t ncell = S.ncell;
-  const unsigned nscan = ceil_div(ncell, kScanBlock);
-  k_scan_reduce<<<nscan, kCicThreads, 0, s>>>(S.count, ncell, S.partial);
+  // sort: keys alternate between key and rank, values between vtmp and sorted (the last
+  // pass writes sorted); the first pass takes the input positions as values
+  const RadixPlan RP = radix_plan(ncell);
+  const uint32_t ntiles = static_cast<uint32_t>(ceil_div(m, kRadixTile));
+  const uint32_t per = static_cast<uint32_t>(ceil_div(ntiles, kXcds));
+  const unsigned rgrid = per * kXcds;
+  uint32_t* kbuf[2] = {S.key, S.rank};
+  uint32_t* vbuf[2] = {S.sorted, S.vtmp};
+  const uint32_t* vin = nullptr;
+  for (int p = 0; p < RP.npass; p++) {
+    const size_t nh = (size_t(1) << RP.bits[p]) * ntiles;
+    const unsigned nscan = ceil_div(nh, kScanBlock);
+    k_radix_count<<<rgrid, kRadixThreads, 0, s>>>(kbuf[p & 1], m, RP.shift[p], RP.bits[p], ntiles,
+                                                  per, S.hist);
+    PRH_LAUNCHED();
+    k_scan_reduce<<<nscan, kCicThreads, 0, s>>>(S.hist, nh, S.partial);
+    PRH_LAUNCHED();
+    k_scan_partials<<<1, kCicThreads, 0, s>>>(S.partial, static_cast<int>(nscan));
+    PRH_LAUNCHED();
+    k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.hist, nh, S.partial, S.goff);
+    PRH_LAUNCHED();
+    uint32_t* vout = vbuf[(RP.npass - 1 - p) & 1];
+    k_radix_scatter<<<rgrid, kRadixThreads, 0, s>>>(kbuf[p & 1], vin, m, RP.shift[p], RP.bits[p],
+                                                    ntiles, per, S.goff, kbuf[(p + 1) & 1], vout);
+    PRH_LAUNCHED();
+    vin = vout;
+  }
+  const uint32_t* skey = kbuf[RP.npass & 1];
+  const uint32_t nsb = static_cast<uint32_t>(ceil_div(ncell, kStartCells));
+  k_start_bounds<<<ceil_div(m + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, nsb, S.bnd);
   PRH_LAUNCHED();
-  k_scan_partials<<<1, kCicThreads, 0, s>>>(S.partial, static_cast<int>(nscan));
-  PRH_LAUNCHED();
-  k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.count, ncell, S.partial, S.start);
-  PRH_LAUNCHED();
-  k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, S.sorted);
-  PRH_LAUNCHED();
-  const unsigned gcell = ceil_div(ceil_div(ncell, kOrderCells), kCicThreads);
-  k_paint_order<<<gcell, kCicThreads, 0, s>>>(S.start, ncell, S.sorted, S.misc, S.big,
-                                              static_cast<uint32_t>(S.nbig));
-  PRH_LAUNCHED();
-  k_paint_order_long<<<256, kCicThreads, 0, s>>>(S.start, S.sorted, S.misc, S.big,
-                                                 static_cast<uint32_t>(S.nbig), S.rank);
+  k_cell_start<<<nsb, kCicThreads, 0, s>>>(skey, S.bnd, ncell, m, S.start);
   PRH_LAUNCHED();
   k_paint_permute<R><<<grid, kCicThreads, 0, s>>>(S.sorted,
                                                   static_cast<const Payload<R>*>(S.payload),
@@ -1143,7 +1331,7 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
     PRH_LAUNCHED();
     k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.count, ntile, S.partial, S.start);
     PRH_LAUNCHED();
-    k_paint_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, order);
+    k_group_scatter<<<grid, kCicThreads, 0, s>>>(S.key, S.rank, m, S.start, order);
     PRH_LAUNCHED();
     timer_before(PRH_TIMER_READ_SHIFTS, s);
     k_read_binned<R, P, O><<<grid, kCicThreads, 0, s>>>(phi, A, pos + 3 * o, m, out + ostride * o,

@@ -92,6 +92,24 @@ struct Plan {
 
 static size_t cells(const size_t* n) { return n[0] * n[1] * n[2]; }
 
+// Element counts of the paint scratch arrays.  The radix sort's histograms
+// hold 2^9 digits per tile of 4096 points; the scan partials serve both them
+// and the read binning's tile counts (8^3-cell tiles); the block bounds of the
+// cell-start pass cover 8192 cells each.
+struct PaintSizes {
+  size_t ncell, ntile, nhist, npartial, nbnd;
+};
+static PaintSizes paint_sizes(const size_t* n, size_t chunk) {
+  PaintSizes z;
+  z.ncell = n[0] * n[1] * n[2];
+  z.ntile = ((n[0] + 7) / 8) * ((n[1] + 7) / 8) * ((n[2] + 7) / 8);
+  z.nhist = size_t(512) * ((chunk + 4095) / 4096);
+  const size_t a = (z.ntile + 8191) / 8192, b = (z.nhist + 8191) / 8192;
+  z.npartial = a > b ? a : b;
+  z.nbnd = (z.ncell + 8191) / 8192 + 1;
+  return z;
+}
+
 static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags) {
   Plan p;
   size_t shapes[kMaxLevels][3];
@@ -104,10 +122,9 @@ static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags)
     for (int l = 0; l < nl; l++) p.mixed += cells(shapes[l]) * sizeof(float) * 3;
   }
   if (flags & PRH_WS_PAINT) {
-    const size_t nc = cells(n);
-    const size_t npart = (nc + 8191) / 8192;
-    p.paint = nc * 4 + (nc + 1) * 4 + (npart + 1) * 4 + chunk * 4 * 4 + 2 * chunk * 4 * w + 4 * 4 +
-              (chunk / 32 + 2) * 4;
+    const PaintSizes z = paint_sizes(n, chunk);
+    p.paint = z.ntile * 4 + (z.ncell + 1) * 4 + (z.npartial + 1) * 4 + chunk * 4 * 5 +
+              (2 * z.nhist + 1) * 4 + (z.nbnd + 1) * 4 + 2 * chunk * 4 * w + 4 * 4;
   }
   if (flags & PRH_WS_FFT) p.fft = smooth_bytes(n, dtype);
   p.red = (kReduceBlocks * 2 + 16) * sizeof(double) + 16 * sizeof(uint32_t);
@@ -129,23 +146,27 @@ static int hierarchy_alloc(prh_workspace* ws, Hierarchy& H, size_t elem, bool fu
 
 static int paint_alloc(prh_workspace* ws, size_t chunk) {
   PaintScratch& P = ws->paint;
+  const PaintSizes z = paint_sizes(ws->n, chunk);
   P.chunk = chunk;
-  P.ncell = cells(ws->n);
-  P.npartial = (P.ncell + 8191) / 8192;
-  P.nbig = chunk / 32 + 2;
+  P.ncell = z.ncell;
+  P.npartial = z.npartial;
+  P.nhist = z.nhist;
   void* p;
-  PRH_TRY(ws_alloc(ws, &p, P.ncell * 4)); P.count = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, z.ntile * 4)); P.count = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, (P.ncell + 1) * 4)); P.start = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, (P.npartial + 1) * 4)); P.partial = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.key = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.rank = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.vtmp = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, P.nhist * 4)); P.hist = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, (P.nhist + 1) * 4)); P.goff = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, (z.nbnd + 1) * 4)); P.bnd = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.sorted = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.sorted_prev = static_cast<uint32_t*>(p);
   PRH_TRY(ws_alloc(ws, &P.payload, chunk * 4 * static_cast<size_t>(ws->dtype)));
   PRH_TRY(ws_alloc(ws, &P.spay, chunk * 4 * static_cast<size_t>(ws->dtype)));
   PRH_TRY(ws_alloc(ws, &p, 4 * 4)); P.misc = static_cast<uint32_t*>(p);
-  PRH_TRY(ws_alloc(ws, &p, P.nbig * 4)); P.big = static_cast<uint32_t*>(p);
-  PRH_HIP(hipMemset(P.count, 0, P.ncell * 4));
+  PRH_HIP(hipMemset(P.count, 0, z.ntile * 4));
   PRH_HIP(hipMemset(P.misc, 0, 16));
   return PRH_OK;
 }
@@ -331,8 +352,8 @@ int prh_workspace_destroy(prh_workspace* ws) {
     }
   PaintScratch& P = ws->paint;
   for (void* p : {(void*)P.count, (void*)P.start, (void*)P.partial, (void*)P.key, (void*)P.rank,
-                  (void*)P.sorted, (void*)P.sorted_prev, P.payload, P.spay, (void*)P.misc,
-                  (void*)P.big})
+                  (void*)P.vtmp, (void*)P.hist, (void*)P.goff, (void*)P.bnd, (void*)P.sorted,
+                  (void*)P.sorted_prev, P.payload, P.spay, (void*)P.misc})
     release(p);
   for (int d = 0; d < 3; d++) {
     release(ws->smooth.tw[d]);

@@ -24,18 +24,21 @@ struct Hierarchy {
 struct PaintScratch {
   size_t chunk = 0;
   size_t ncell = 0;
-  uint32_t* count = nullptr;    // ncell      per-cell point counts (kept zero between chunks)
-  uint32_t* start = nullptr;    // ncell + 1  exclusive prefix of count
+  uint32_t* count = nullptr;    // ntile      per-tile point counts of the read binning (kept zero)
+  uint32_t* start = nullptr;    // ncell + 1  first sorted position of each cell (read binning: tile)
   uint32_t* partial = nullptr;  // scan block partials
   size_t npartial = 0;
-  uint32_t* key = nullptr;      // chunk      base cell of each point
-  uint32_t* rank = nullptr;     // chunk      arrival rank within its cell; reused as sort scratch
+  uint32_t* key = nullptr;      // chunk      base cell of each point; radix sort keys
+  uint32_t* rank = nullptr;     // chunk      radix sort keys (other buffer); read binning ranks
+  uint32_t* vtmp = nullptr;     // chunk      radix sort values between passes
+  uint32_t* hist = nullptr;     // nhist      per-tile digit counts of one radix pass
+  uint32_t* goff = nullptr;     // nhist + 1  their exclusive scan
+  size_t nhist = 0;
+  uint32_t* bnd = nullptr;      // ncell / 8192 + 2  first sorted position of each 8192-cell block
   uint32_t* sorted = nullptr;   // chunk      point indices grouped by cell, ascending within a cell
   void* payload = nullptr;      // chunk      (t_x, t_y, t_z, w) in the mesh precision, point order
   void* spay = nullptr;         // chunk      the same payload in cell-grouped order
-  uint32_t* misc = nullptr;     // [0] error flags, [1] number of long segments
-  uint32_t* big = nullptr;      // long-segment cell list (ncell worst case is bounded by chunk/32)
-  size_t nbig = 0;
+  uint32_t* misc = nullptr;     // [0] error flags
   // After a paint of one chunk, `sorted` is a permutation of that catalog's
   // indices in cell order.  A later read of the same catalog (same positions
   // pointer, count and dtype) visits its points in that order instead of
