@@ -211,6 +211,108 @@ def kernel_only(args):
         dtype, launches, ms / launches, nbytes / (ms * 1e-3) / 1e9))
 
 
+def recon_step(P, ws, geo, catalogs, outs, precision, nrefine=0, stages=None):
+    """One reconstruction: paint data and randoms, smoothing and density contrast, the solve in
+    ``precision``, and the shifted positions of data and randoms (RecSym) into ``outs``.  With
+    ``stages`` (a dict), each stage's wall time is added to it after a synchronisation."""
+    from catalogs import RECON_KWARGS, SMOOTHING_RADIUS
+    d, dw, r, rw = catalogs
+    out_d, out_r = outs
+
+    def mark(name):
+        if stages is not None:
+            P.synchronize()
+            stages[name] = stages.get(name, 0.) + time.perf_counter() - mark.t
+            mark.t = time.perf_counter()
+    mark.t = time.perf_counter()
+    recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws)
+    recon.assign_data(d, dw)
+    recon.assign_randoms(r, rw)
+    mark('paint')
+    recon.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    mark('smoothing+density')
+    recon.run(precision=precision, nrefine=nrefine)
+    mark('multigrid')
+    recon.read_shifted_positions(d, out=out_d)
+    recon.read_shifted_positions(r, out=out_r)
+    mark('shifts')
+    return recon
+
+
+def run_large(args):
+    """Configuration 4: nmesh=1024, boxsize 4000 Mpc/h, 5e7 data and 2.5e8 randoms in the shell.
+
+    Three paths on the same device-resident catalogs: plain FP64 (FP64 meshes, the reference's
+    algorithm; the oracle of the other two), mixed (FP64 meshes, FP32 V-cycles inside FP64 defect
+    correction) and FP32 storage (FP32 meshes, about 4.3 GB each: the reference's FLOAT32
+    algorithm).  For each: recon/s, per-stage times, the finest-level Jacobi sweep against HBM
+    peak, and the errors of the potential and of the shifted positions against plain FP64."""
+    import pyrecon_amd as P
+    from pyrecon_amd import _lib
+    from pyrecon_amd.device import device_name
+    from catalogs import make_catalogs
+    log('device', device_name())
+    cat = load_catalogs(args, make_catalogs)
+    cfg = cat['config']
+    nd, nr = len(cat['data']), len(cat['randoms'])
+    log('catalogs: {:d} data, {:d} randoms'.format(nd, nr))
+    dev = tuple(P.DeviceArray.from_numpy(cat[k]) for k in ('data', 'data_weights', 'randoms', 'randoms_weights'))
+    del cat['data'], cat['data_weights'], cat['randoms'], cat['randoms_weights']
+    out_d, out_r = P.DeviceArray((nd, 3), 'f8'), P.DeviceArray((nr, 3), 'f8')
+    shape = (cfg['nmesh'],) * 3
+    # one paint chunk holds the 2.5e8 randoms (no separate validation pass)
+    chunk = 1 << (max(nd, nr) - 1).bit_length()
+    paths = [('fp64', 'f8', 'plain'), ('mixed', 'f8', 'mixed'), ('fp32_storage', 'f4', 'plain')]
+    report, ref = {}, None
+    for name, dtype, precision in paths:
+        ws = P.Workspace(shape, dtype, paint_chunk=chunk)
+        log('{}: workspace {:.2f} GB'.format(name, ws.nbytes / 1e9))
+        geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype=dtype)
+        for _ in range(args.warmup):
+            recon_step(P, ws, geo, dev, (out_d, out_r), precision, args.nrefine)
+        P.synchronize()
+        with _lib.KernelTimer(_lib.TIMER_JACOBI_FINEST) as kt:
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                recon_step(P, ws, geo, dev, (out_d, out_r), precision, args.nrefine)
+            P.synchronize()
+            elapsed = time.perf_counter() - t0
+            ms_j, nj, bytes_j = kt.read()
+        stages = {}
+        recon = recon_step(P, ws, geo, dev, (out_d, out_r), precision, args.nrefine, stages=stages)
+        phi = recon.mesh_phi.value.astype('f8')
+        phi -= phi.mean()
+        xd, xr = out_d.numpy(), out_r.numpy()
+        del recon
+        entry = dict(value=round(args.steps / elapsed, 4), unit='recon/s',
+                     ms_per_step=round(1e3 * elapsed / args.steps, 2), mesh_dtype=dtype, solve=precision,
+                     stages_ms={k: round(1e3 * v, 2) for k, v in stages.items()},
+                     jacobi_finest=dict(launches=nj, avg_launch_ms=round(ms_j / max(nj, 1), 5),
+                                        achieved_gbs=round(bytes_j / (ms_j * 1e-3) / 1e9, 1) if ms_j else None,
+                                        frac_of_hbm_peak=round(bytes_j / (ms_j * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                        if ms_j else None))
+        if ref is None:
+            ref = (phi, xd, xr)
+        else:
+            dx = np.concatenate([np.abs(xd - ref[1]).ravel(), np.abs(xr - ref[2]).ravel()])
+            entry['accuracy'] = dict(phi_rel_l2=float(np.sqrt(np.sum((phi - ref[0])**2) / np.sum(ref[0]**2))),
+                                     shift_max_mpc_h=float(dx.max()), shift_rms_mpc_h=float(np.sqrt(np.mean(dx**2))),
+                                     against='plain FP64 GPU path (potentials compared after removing their means)')
+            del dx
+        del phi, xd, xr
+        log(name, entry)
+        report[name] = entry
+        ws.close()
+        P.empty_cache()
+    line = dict(metric='recon/s at nmesh=1024 (configuration 4), FP32 storage and mixed vs FP64, 1 GPU',
+                config=dict(workload='configuration 4: nmesh=1024, boxsize=4000 Mpc/h, boxcenter=(0,0,3000), '
+                                     '{:d} data + {:d} randoms in the 1000<r<3000 shell, FKP weights, local LOS, '
+                                     'f=0.8, bias=2, R=15, 5 sweeps x 6 V-cycles'.format(nd, nr),
+                            nmesh=cfg['nmesh'], ndata=nd, nrandoms=nr, inputs='device-resident'),
+                n_gpus=1, steps=args.steps, warmup=args.warmup, data='synthetic', paths=report)
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(cat, threads):
     """Reference C kernels (oracle/_ref, built from the reference's own src/*.c) plus the
     reference's numpy glue, timed on bounded samples of each stage of one reconstruction
@@ -307,10 +409,12 @@ def main():
     if args.serving_only:
         print(json.dumps(dict(metric=METRIC, serving=run_serving(args))), flush=True)
         return
+    if args.config == 'c4':
+        return run_large(args)
     import pyrecon_amd as P
     from pyrecon_amd import _lib
     from pyrecon_amd.device import device_name
-    from catalogs import make_catalogs, RECON_KWARGS, SMOOTHING_RADIUS
+    from catalogs import make_catalogs
     log('device', device_name())
     cat = load_catalogs(args, make_catalogs)
     cfg = cat['config']
@@ -325,24 +429,7 @@ def main():
     geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype='f8')
 
     def step(precision, stages=None):
-        def mark(name):
-            if stages is not None:
-                P.synchronize()
-                stages[name] = stages.get(name, 0.) + time.perf_counter() - mark.t
-                mark.t = time.perf_counter()
-        mark.t = time.perf_counter()
-        recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws)
-        recon.assign_data(d, dw)
-        recon.assign_randoms(r, rw)
-        mark('paint')
-        recon.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
-        mark('smoothing+density')
-        recon.run(precision=precision, nrefine=args.nrefine)
-        mark('multigrid')
-        recon.read_shifted_positions(d, out=out_d)
-        recon.read_shifted_positions(r, out=out_r)
-        mark('shifts')
-        return recon
+        return recon_step(P, ws, geo, (d, dw, r, rw), (out_d, out_r), precision, args.nrefine, stages)
 
     def timed(precision, steps, warmup):
         for _ in range(warmup): step(precision)

@@ -33,7 +33,7 @@ extern "C" {
 #define PRH_ERR_NONFINITE (-3) /* NaN/inf positions or weights (reference: MeshError, mesh.py:552-553) */
 #define PRH_ERR_HIP (-4)       /* HIP runtime error */
 #define PRH_ERR_NOMEM (-5)     /* device allocation failed */
-#define PRH_ERR_FFT (-6)       /* hipFFT error */
+#define PRH_ERR_FFT (-6)       /* unused (kept for numbering) */
 #define PRH_ERR_CAPACITY (-7)  /* request larger than the workspace was built for */
 #define PRH_ERR_STATE (-8)     /* call out of order */
 
@@ -77,14 +77,14 @@ int prh_stream_wait_event(void *stream, void *event); /* later work on stream wa
  * Owns everything a reconstruction of one mesh shape needs besides the
  * caller's meshes: the multigrid hierarchy (replacing the malloc/calloc per
  * level per call of src/multigrid.c:280-297, 314-324), the paint scratch
- * (points are painted in chunks of at most paint_chunk points), the hipFFT
- * plans and spectrum buffer, and the FP32 hierarchy of the mixed-precision
+ * (points are painted in chunks of at most paint_chunk points), the Fourier
+ * tables (twiddles, gains), and the FP32 hierarchy of the mixed-precision
  * solve.  Nothing is allocated by the hot calls.  flags: PRH_WS_* below.
  * A workspace may be used by one stream at a time.                          */
 typedef struct prh_workspace prh_workspace;
 #define PRH_WS_MULTIGRID 1u /* multigrid hierarchy in the workspace precision */
 #define PRH_WS_PAINT 2u     /* CIC paint scratch */
-#define PRH_WS_FFT 4u       /* hipFFT plans + spectrum buffer */
+#define PRH_WS_FFT 4u       /* Fourier tables (twiddles, gains) of the smoothing and transforms */
 #define PRH_WS_MIXED 8u     /* FP32 hierarchy for prh_fmg_mixed (FP64 workspaces) */
 #define PRH_WS_ALL 15u
 int prh_workspace_create(prh_workspace **ws, const size_t *nmesh, int dtype, size_t paint_chunk,
@@ -176,11 +176,53 @@ int prh_paint_f64(prh_workspace *ws, double *mesh, const size_t *nmesh, const do
                   void *stream);
 
 /* replaces RealMesh.smooth_gaussian(method='fft') (mesh.py:655-680): in-place
- * multiplication of the spectrum by exp(-(R k)^2 / 2), R2C/C2R hipFFT.     */
+ * multiplication of the spectrum by exp(-(R k)^2 / 2), done as a 1-D
+ * forward transform, gain and inverse transform along each axis in turn.   */
 int prh_smooth_gaussian_f32(prh_workspace *ws, float *mesh, const size_t *nmesh,
                             const float *boxsize, const double *radius, void *stream);
 int prh_smooth_gaussian_f64(prh_workspace *ws, double *mesh, const size_t *nmesh,
                             const double *boxsize, const double *radius, void *stream);
+
+/* ---- FFT reconstructions (plane_parallel_fft.py, iterative_fft.py) -------
+ * 3-D c2c transform of an (n0, n1, n2) mesh, as numpy.fft.fftn / ifftn
+ * without the 1 / N of the inverse (RealMesh.to_complex and
+ * ComplexMesh.to_real, mesh.py:697-705, 827-844, 974-988).  Complex meshes are
+ * interleaved (re, im) in the mesh precision.  Forward (inverse = 0): src real
+ * (io = PRH_FFT_REAL_IN) or complex -> dst complex.  Inverse: src complex,
+ * transformed in place along x and y -> dst complex, or its real part
+ * (io = PRH_FFT_REAL_OUT).  dst may equal src for complex -> complex.       */
+#define PRH_FFT_COMPLEX 0
+#define PRH_FFT_REAL_IN 1
+#define PRH_FFT_REAL_OUT 2
+int prh_fft3d_f32(prh_workspace *ws, const void *src, void *dst, const size_t *nmesh, int inverse,
+                  int io, void *stream);
+int prh_fft3d_f64(prh_workspace *ws, const void *src, void *dst, const size_t *nmesh, int inverse,
+                  int io, void *stream);
+
+/* Fourier-space operators on a complex spectrum, k_i = 2 pi fftfreq(n_i,
+ * L_i / n_i) (ComplexMesh.coords, mesh.py:812-820); dst may equal src; norm
+ * multiplies the output (1 / N before an inverse transform), except for
+ * PRH_SPEC_INV_K2.  los: double[3] (PLANE_PARALLEL, LOS_SQ), else NULL.     */
+#define PRH_SPEC_PLANE_PARALLEL 0 /* i k_a (src / k^2) / (1 + beta mu^2)    plane_parallel_fft.py:45-53 */
+#define PRH_SPEC_INV_K2 1         /* src / k^2, 0 at k = 0 (prod_sum, exp=-1) iterative_fft.py:37-39 */
+#define PRH_SPEC_LOS_SQ 2         /* (k . los)^2 src                        iterative_fft.py:44 */
+#define PRH_SPEC_KA_KB 3          /* k_a k_b src                            iterative_fft.py:58 */
+#define PRH_SPEC_GRAD 4           /* i k_a (src / k^2), 0 at k = 0          iterative_fft.py:75-81 */
+int prh_spectral_f32(prh_workspace *ws, int op, const void *src, void *dst, const size_t *nmesh,
+                     const float *boxsize, const double *los, double beta, double norm, int a, int b,
+                     void *stream);
+int prh_spectral_f64(prh_workspace *ws, int op, const void *src, void *dst, const size_t *nmesh,
+                     const double *boxsize, const double *los, double beta, double norm, int a, int b,
+                     void *stream);
+
+/* RSD removal of one IterativeFFTReconstruction step (iterative_fft.py:41-67):
+ * a < 0 (global line of sight): delta -= factor * t; else (local line of
+ * sight) delta -= factor * x_a x_b t / |x|^2 at the mesh coordinates
+ * x_i = offset_i + cellsize_i j_i (RealMesh.coords, mesh.py:522-527).       */
+int prh_rsd_update_f32(float *delta, const float *t, const size_t *nmesh, const float *boxsize,
+                       const float *boxcenter, double factor, int a, int b, void *stream);
+int prh_rsd_update_f64(double *delta, const double *t, const size_t *nmesh, const double *boxsize,
+                       const double *boxcenter, double factor, int a, int b, void *stream);
 
 /* Deterministic FP64 sums of one or two meshes (recon.py:227 np.sum calls).
  * sums: host double[2]; synchronizes stream.                               */

@@ -13,7 +13,10 @@ A restatement of the reference's Python layer with numpy, calling the C oracle
 - Martin White's density contrast of OriginalMultiGridReconstruction:
   pyrecon/multigrid.py:69-97;
 - the solve: pyrecon/multigrid.py:99-135 (``beta = f / bias``, recon.py:122-125);
-- the shifts: pyrecon/multigrid.py:137-158 and pyrecon/recon.py:304-325.
+- the shifts: pyrecon/multigrid.py:137-158 and pyrecon/recon.py:304-325;
+- the FFT reconstructions: pyrecon/plane_parallel_fft.py:41-55 and
+  pyrecon/iterative_fft.py:12-82, with the base-class shifts of
+  pyrecon/recon.py:255-325.
 
 The numpy operations (and therefore the dtypes of every intermediate) are the
 reference's, so with the same native kernels this reproduces the reference's
@@ -232,3 +235,140 @@ class OracleOriginalMultiGridReconstruction(OracleMultiGridReconstruction):
         self.mesh_delta = np.ascontiguousarray(self.smooth_gaussian(delta, smoothing_radius))
         self.mesh_data = self.mesh_randoms = None
 
+
+
+class _OracleFFTReconstruction(OracleMultiGridReconstruction):
+    """Shared parts of the FFT reconstructions: numpy's c2c engine (pyrecon/mesh.py:974-988, through
+    RealMesh.to_complex and ComplexMesh.to_real, 697-705 and 827-844), the wavenumbers and mesh
+    coordinates (ComplexMesh.coords 812-820, RealMesh.coords 522-527), and the base-class shifts,
+    which read each of the three displacement meshes by CIC (pyrecon/recon.py:255-325)."""
+
+    @property
+    def ctype(self):
+        return np.dtype('c{:d}'.format(2 * self.dtype.itemsize))
+
+    def to_complex(self, value):
+        return np.fft.fftn(value).astype(self.ctype, copy=False)
+
+    def to_real(self, valuek):
+        return np.fft.ifftn(valuek).astype(self.ctype, copy=False).real
+
+    def kcoords(self):
+        k = [2 * np.pi * np.fft.fftfreq(n, d=d) for n, d in zip(self.nmesh, self.boxsize / self.nmesh)]
+        return _broadcast(k)
+
+    def xcoords(self):
+        x = [o + d * np.arange(n) for n, o, d in zip(self.nmesh, self.offset, self.boxsize / self.nmesh)]
+        return _broadcast(x)
+
+    def read_cic_nowrap(self, mesh, positions):
+        """RealMesh.read_cic(positions, wrap=False) (mesh.py:570-610)."""
+        dtype = positions.dtype
+        self._check_range(positions)
+        positions = self.mesh_units(positions)
+        return self.kernels.read_cic(np.ascontiguousarray(mesh, dtype=self.dtype), positions).astype(dtype, copy=False)
+
+    def read_shifts(self, positions, field='disp+rsd'):
+        field = field.lower()
+        if field not in ['disp', 'rsd', 'disp+rsd']:
+            raise OracleError('Unknown field {}'.format(field))
+        shifts = np.empty_like(positions)
+        if self.wrap: positions = self.wrap_positions(positions)
+        for iaxis, psi in enumerate(self.mesh_psi):
+            shifts[:, iaxis] = self.read_cic_nowrap(psi, positions)
+        if field == 'disp':
+            return shifts
+        if self.los is None:
+            los = positions / distance(positions)[:, None]
+        else:
+            los = self.los
+        rsd = self.f * np.sum(shifts * los, axis=-1)[:, None] * los
+        if field == 'rsd':
+            return rsd
+        shifts += rsd
+        return shifts
+
+
+def _broadcast(arrays):
+    """pyrecon/utils.py:125-143: 1D arrays to broadcastable 3D arrays."""
+    out = []
+    for iaxis, a in enumerate(arrays):
+        sl = [None] * len(arrays)
+        sl[iaxis] = slice(None)
+        out.append(a[tuple(sl)])
+    return out
+
+
+class OraclePlaneParallelFFTReconstruction(_OracleFFTReconstruction):
+    """Eisenstein et al. 2007 with a global line of sight (pyrecon/plane_parallel_fft.py:41-55)."""
+
+    def run(self):
+        if self.los is None:
+            raise OracleError('A (global) line-of-sight must be provided')
+        delta_k = self.to_complex(self.mesh_delta)
+        del self.mesh_delta
+        k = self.kcoords()
+        k2 = sum(kk**2 for kk in k)
+        k2[0, 0, 0] = 1.
+        delta_k /= k2
+        mu2 = sum(kk * ll for ll, kk in zip(k, self.los))**2 / k2
+        psis = []
+        for iaxis in range(3):
+            tmp = 1j * k[iaxis] * delta_k / (1. + self.beta * mu2)
+            psis.append(self.to_real(tmp))
+        self.mesh_psi = psis
+
+
+class OracleIterativeFFTReconstruction(_OracleFFTReconstruction):
+    """Burden et al. 2015, field-level iterations (pyrecon/iterative_fft.py:12-82)."""
+
+    def run(self, niterations=3):
+        self._iter = 0
+        self.mesh_delta_real = np.array(self.mesh_delta, copy=True)
+        for _ in range(niterations):
+            self._iterate()
+        del self.mesh_delta
+        self.mesh_psi = self._compute_psi()
+        del self.mesh_delta_real
+
+    def _iterate(self):
+        delta_k = self.to_complex(self.mesh_delta_real)
+        k = self.kcoords()
+        # ComplexMesh.prod_sum(exp=-1) (src/mesh.c:327-365): squares cast to the mesh dtype and
+        # summed as (y + x) + z in it; real and imaginary parts divided separately
+        k2 = [np.asarray(kk**2, dtype=self.dtype) for kk in k]
+        den = (k2[1] + k2[0]) + k2[2]
+        with np.errstate(divide='ignore', invalid='ignore'):
+            re = delta_k.real / den
+            im = delta_k.imag / den
+        delta_k = (re + 1j * im).astype(self.ctype, copy=False)
+        delta_k[0, 0, 0] = 0.
+        self.mesh_delta_real = np.array(self.mesh_delta, copy=True)
+        if self.los is not None:
+            disp_deriv_k = sum(kk * ll for ll, kk in zip(k, self.los))**2 * delta_k
+            delta_rsd = self.to_real(disp_deriv_k)
+            factor = self.beta
+            if self._iter == 0:
+                factor /= (1. + self.beta)
+            self.mesh_delta_real -= factor * delta_rsd
+        else:
+            x = self.xcoords()
+            x2 = sum(x_**2 for x_ in x)
+            for i in range(3):
+                for j in range(i, 3):
+                    disp_deriv_k = k[i] * k[j] * delta_k
+                    delta_rsd = x[i] * x[j] * self.to_real(disp_deriv_k) / x2
+                    factor = (1. + (i != j)) * self.beta
+                    if self._iter == 0:
+                        factor /= (1. + self.beta)
+                    self.mesh_delta_real -= factor * delta_rsd
+        self._iter += 1
+
+    def _compute_psi(self):
+        delta_k = self.to_complex(self.mesh_delta_real)
+        k = self.kcoords()
+        k2 = sum(kk**2 for kk in k)
+        k2[0, 0, 0] = 1.
+        delta_k /= k2
+        delta_k[0, 0, 0] = 0.
+        return [self.to_real(1j * k[iaxis] * delta_k) for iaxis in range(3)]

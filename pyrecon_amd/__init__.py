@@ -1,4 +1,5 @@
-"""pyrecon_amd: MI355X-native MultiGridReconstruction (drop-in for pyrecon's hot path).
+"""pyrecon_amd: MI355X-native MultiGridReconstruction (drop-in for pyrecon's hot path), and the
+FFT reconstructions PlaneParallelFFTReconstruction and IterativeFFTReconstruction on the same kernels.
 
 Usage is the reference's::
 
@@ -20,8 +21,11 @@ from .device import DeviceArray, Stream, Event, synchronize, empty_cache
 from .mesh import MeshInfo, RealMesh, Workspace
 from .recon import BaseReconstruction, ReconstructionError
 from .multigrid import MultiGridReconstruction, OriginalMultiGridReconstruction, run_group
+from .plane_parallel_fft import PlaneParallelFFTReconstruction
+from .iterative_fft import IterativeFFTReconstruction
 from .serving import ReconServer, ReconJob, AdmissionError
 
-__all__ = ['MultiGridReconstruction', 'OriginalMultiGridReconstruction', 'BaseReconstruction', 'ReconstructionError',
+__all__ = ['MultiGridReconstruction', 'OriginalMultiGridReconstruction', 'PlaneParallelFFTReconstruction',
+           'IterativeFFTReconstruction', 'BaseReconstruction', 'ReconstructionError',
            'MeshError', 'DeviceError', 'MeshInfo', 'RealMesh', 'Workspace', 'DeviceArray', 'Stream', 'Event',
            'synchronize', 'empty_cache', 'run_group', 'ReconServer', 'ReconJob', 'AdmissionError']

@@ -29,6 +29,8 @@ ERR_STATE = -8
 F32, F64 = 4, 8
 FIELD_DISP, FIELD_RSD, FIELD_DISP_RSD, FIELD_VALUE = 0, 1, 2, 3
 TIMER_JACOBI_FINEST, TIMER_PAINT_GATHER, TIMER_READ_SHIFTS = 0, 1, 2
+FFT_COMPLEX, FFT_REAL_IN, FFT_REAL_OUT = 0, 1, 2
+SPEC_PLANE_PARALLEL, SPEC_INV_K2, SPEC_LOS_SQ, SPEC_KA_KB, SPEC_GRAD = 0, 1, 2, 3, 4
 
 WS_MULTIGRID, WS_PAINT, WS_FFT, WS_MIXED = 1, 2, 4, 8
 WS_ALL = 15
@@ -39,7 +41,7 @@ class MeshError(Exception):
 
 
 class DeviceError(RuntimeError):
-    """HIP / hipFFT / allocation failure reported by the library."""
+    """HIP or allocation failure reported by the library."""
 
 
 def dtype_code(dtype):
@@ -114,6 +116,9 @@ def _signatures():
             'prh_martin_sums_' + suffix: (_i, [_vp, _vp, _vp, _sz, real, _vp, _vp]),
             'prh_martin_contrast_' + suffix: (_i, [_vp, _vp, _vp, _vp, _sz, real, real, real, _vp, _vp]),
             'prh_subtract_nonzero_mean_' + suffix: (_i, [_vp, _sz, real, _vp]),
+            'prh_fft3d_' + suffix: (_i, [_vp, _vp, _vp, _szp, _i, _i, _vp]),
+            'prh_spectral_' + suffix: (_i, [_vp, _i, _vp, _vp, _szp, _vp, _vp, _d, _d, _i, _i, _vp]),
+            'prh_rsd_update_' + suffix: (_i, [_vp, _vp, _szp, _vp, _vp, _d, _i, _i, _vp]),
             'prh_read_shifts_' + suffix: (_i, [_vp, _vp, _szp, _vp, _vp, _vp, _i, _sz, _d, _vp, _i, _i, _i,
                                                 _vp, _i, ctypes.POINTER(_i), _vp]),
         })

@@ -604,13 +604,23 @@ __global__ __launch_bounds__(kGThreads) void k_paint_gather_tiled(
   const int bx = tile / (tiles2 * tiles1);
   const int x0 = bx * kGT, y0 = by * kGT, z0 = bz * kGT;
   auto wrap = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+  // staged cell e = (ex, ey, ez) is the base cell at (x0 - 1 + ex, ...) and serves the target
+  // planes ex - 1 and ex; it is staged only if the first of them that exists lies inside the
+  // mesh (x0 + max(ex - 1, 0) < n0, ...).  Its coordinate then lies in [-1, n), so one period of
+  // wrap maps it into the mesh.  Other cells (past the end of a partial tile, where a mesh
+  // extent is below the tile edge) stage nothing.
+  auto needed = [](int o, int ei, int n) { return o + (ei > 0 ? ei - 1 : 0) < n; };
   for (int e = tid; e < kGCells; e += kGThreads) {
     const int ex = e / (kGE * kGE), ey = (e / kGE) % kGE, ez = e % kGE;
-    const size_t bc = (static_cast<size_t>(wrap(x0 - 1 + ex, n0)) * n1 + wrap(y0 - 1 + ey, n1)) * n2 +
-                      wrap(z0 - 1 + ez, n2);
-    const uint32_t g0 = start[bc];
+    uint32_t g0 = 0u, cnt = 0u;
+    if (needed(x0, ex, n0) && needed(y0, ey, n1) && needed(z0, ez, n2)) {
+      const size_t bc = (static_cast<size_t>(wrap(x0 - 1 + ex, n0)) * n1 + wrap(y0 - 1 + ey, n1)) * n2 +
+                        wrap(z0 - 1 + ez, n2);
+      g0 = start[bc];
+      cnt = start[bc + 1] - g0;
+    }
     s_gbeg[e] = g0;
-    s_cnt[e] = start[bc + 1] - g0;
+    s_cnt[e] = cnt;
   }
   __syncthreads();
   // offsets: row sums, an exclusive scan over the 81 rows (first wave, by

@@ -667,6 +667,366 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
   return PRH_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// 3-D complex transforms and Fourier-space operators of the FFT
+// reconstructions (pyrecon/plane_parallel_fft.py, pyrecon/iterative_fft.py).
+//
+// The reference transforms with numpy's c2c fftn / ifftn (RealMesh.to_complex
+// and ComplexMesh.to_real, pyrecon/mesh.py:697-705, 827-844, 974-988).  Here a
+// 3-D transform is three passes of 1-D transforms, one per axis, over an
+// interleaved complex mesh of the same (n0, n1, n2) shape, with the Stockham
+// machinery of the smoothing.  The forward pass along z may read a real mesh
+// (imaginary part 0), and the last inverse pass may write the real part only,
+// so no separate real <-> complex copies are made.  No pass normalises: the
+// 1 / N of the inverse is folded into the Fourier-space operator that precedes
+// it.
+// ---------------------------------------------------------------------------
+enum FftIO { kFftComplex = 0, kFftRealIn = 1, kFftRealOut = 2 };
+
+// The lines of one axis (same addressing as k_smooth_axis, in elements of the
+// mesh); block b takes complex lines [b LP, (b + 1) LP), remapped so that
+// consecutive bundles run on one XCD.  With kFftComplex, src and dst may be the
+// same buffer: a block reads all its lines before it writes any, and blocks
+// own disjoint lines.
+template <typename R, bool CONTIG, int IO>
+__global__ __launch_bounds__(kSmoothThreads) void k_fft_axis(const void* src, void* dst, AxisPlan P,
+                                                             long long nlines, int I, long long ostride,
+                                                             long long istride, long long es, int llp,
+                                                             const Cx<R>* __restrict__ tw_global,
+                                                             int inverse) {
+  extern __shared__ double smem[];
+  const int LP = 1 << llp;
+  const int n = P.n;
+  Cx<R>* b0 = reinterpret_cast<Cx<R>*>(smem);
+  Cx<R>* stw = b0 + padded_len(static_cast<size_t>(LP) * n);
+  for (int k = threadIdx.x; k < n; k += kSmoothThreads) stw[k] = tw_global[k];
+  const unsigned nb = gridDim.x, bid = blockIdx.x, per = nb / 8;
+  const unsigned blk = bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+  const long long l0 = static_cast<long long>(blk) * LP;
+  auto base = [&](long long q) { return (q / I) * ostride + (q % I) * istride; };
+  const int nel = CONTIG ? LP << P.ln : LP * n;
+  constexpr int kBatch = 8;
+  for (int t0 = threadIdx.x; t0 < nel; t0 += kBatch * kSmoothThreads) {
+    Cx<R> val[kBatch];
+    int slot[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int t = t0 + u * kSmoothThreads;
+      int lp, e;
+      if (CONTIG) {
+        lp = t >> P.ln;
+        e = t & ((1 << P.ln) - 1);
+      } else {
+        e = t >> llp;
+        lp = t & (LP - 1);
+      }
+      const long long q = l0 + lp;
+      const bool ok = t < nel && e < n;
+      val[u] = Cx<R>{R(0), R(0)};
+      if (ok && q < nlines) {
+        const long long at = base(q) + e * es;
+        if (IO == kFftRealIn)
+          val[u].re = static_cast<const R*>(src)[at];
+        else
+          val[u] = static_cast<const Cx<R>*>(src)[at];
+      }
+      slot[u] = ok ? lds_pad(lp * n + e) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++)
+      if (slot[u] >= 0) b0[slot[u]] = val[u];
+  }
+  __syncthreads();
+  fft_lines<R>(b0, llp, P, stw, inverse != 0);
+  for (int t = threadIdx.x; t < nel; t += kSmoothThreads) {
+    int lp, e;
+    if (CONTIG) {
+      lp = t >> P.ln;
+      e = t & ((1 << P.ln) - 1);
+      if (e >= n) continue;
+    } else {
+      e = t >> llp;
+      lp = t & (LP - 1);
+    }
+    const long long q = l0 + lp;
+    if (q >= nlines) continue;
+    const long long at = base(q) + e * es;
+    const Cx<R> v = b0[lds_pad(lp * n + e)];
+    if (IO == kFftRealOut)
+      static_cast<R*>(dst)[at] = v.re;
+    else
+      static_cast<Cx<R>*>(dst)[at] = v;
+  }
+}
+
+template <typename R, bool CONTIG, int IO>
+static int launch_fft_axis(const AxisPlan& P, const void* src, void* dst, long long nlines, int I,
+                           long long ostride, long long istride, long long es, int llp, size_t lds,
+                           const Cx<R>* tw, int inverse, hipStream_t s) {
+  const void* fn = reinterpret_cast<const void*>(&k_fft_axis<R, CONTIG, IO>);
+  if (lds > 65536)
+    PRH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  const long long LP = 1LL << llp;
+  const unsigned grid = static_cast<unsigned>((nlines + LP - 1) / LP);
+  k_fft_axis<R, CONTIG, IO><<<grid, kSmoothThreads, lds, s>>>(src, dst, P, nlines, I, ostride, istride,
+                                                              es, llp, tw, inverse);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+// One pass of 1-D transforms along axis d of an (n0, n1, n2) mesh.
+template <typename R>
+static int fft_pass(prh_workspace* ws, int d, const void* src, void* dst, int io, int inverse,
+                    hipStream_t s) {
+  AxisPlan P;
+  factor_plan(static_cast<int>(ws->n[d]), P);
+  const int n = P.n;
+  const long long n0 = static_cast<long long>(ws->n[0]), n1 = static_cast<long long>(ws->n[1]),
+                  n2 = static_cast<long long>(ws->n[2]);
+  auto bytes = [&](int l) {
+    return (padded_len((size_t(1) << l) * static_cast<size_t>(n)) + n) * 2 * sizeof(R);
+  };
+  // lines per block: as many as fit kSmoothLds and the block's butterflies (at most 32)
+  const int lm = plan_max_lm(P);
+  int llp = 0;
+  while (llp < 5 && bytes(llp + 1) <= kSmoothLds && llp + 1 + lm <= fly_log2()) llp++;
+  long long nlines, ostride, istride, es;
+  int I;
+  if (d == 2) {
+    nlines = n0 * n1; I = static_cast<int>(n1); ostride = n1 * n2; istride = n2; es = 1;
+  } else if (d == 1) {
+    nlines = n0 * n2; I = static_cast<int>(n2); ostride = n1 * n2; istride = 1; es = n2;
+  } else {
+    nlines = n1 * n2; I = static_cast<int>(n2); ostride = n2; istride = 1; es = n1 * n2;
+  }
+  const Cx<R>* tw = static_cast<const Cx<R>*>(ws->smooth.tw[d]);
+  const size_t lds = bytes(llp);
+  if (d == 2) {
+    if (io == kFftRealIn)
+      return launch_fft_axis<R, true, kFftRealIn>(P, src, dst, nlines, I, ostride, istride, es, llp, lds,
+                                                  tw, inverse, s);
+    if (io == kFftRealOut)
+      return launch_fft_axis<R, true, kFftRealOut>(P, src, dst, nlines, I, ostride, istride, es, llp, lds,
+                                                   tw, inverse, s);
+    return launch_fft_axis<R, true, kFftComplex>(P, src, dst, nlines, I, ostride, istride, es, llp, lds,
+                                                 tw, inverse, s);
+  }
+  if (io == kFftRealIn)
+    return launch_fft_axis<R, false, kFftRealIn>(P, src, dst, nlines, I, ostride, istride, es, llp, lds,
+                                                 tw, inverse, s);
+  if (io == kFftRealOut)
+    return launch_fft_axis<R, false, kFftRealOut>(P, src, dst, nlines, I, ostride, istride, es, llp, lds,
+                                                  tw, inverse, s);
+  return launch_fft_axis<R, false, kFftComplex>(P, src, dst, nlines, I, ostride, istride, es, llp, lds,
+                                                tw, inverse, s);
+}
+
+static int check_transform_ws(prh_workspace* ws, const size_t* nmesh, int code) {
+  if (!(ws->flags & PRH_WS_FFT)) {
+    set_error("workspace was built without Fourier tables (PRH_WS_FFT)");
+    return PRH_ERR_STATE;
+  }
+  if (!ws->smooth.have) {
+    set_error("Fourier transforms support dimensions whose prime factors are 2, 3, 5 and 7");
+    return PRH_ERR_ARG;
+  }
+  if (ws->dtype != code) {
+    set_error("workspace precision does not match the mesh");
+    return PRH_ERR_ARG;
+  }
+  for (int d = 0; d < 3; d++)
+    if (ws->n[d] != nmesh[d]) {
+      set_error("workspace mesh does not match nmesh");
+      return PRH_ERR_CAPACITY;
+    }
+  return PRH_OK;
+}
+
+// Forward: src real (PRH_FFT_REAL_IN) or complex -> dst complex, axes z, y, x.
+// Inverse: src complex (transformed in place along x and y) -> dst complex, or
+// its real part (PRH_FFT_REAL_OUT), along z last.
+template <typename R>
+static int do_fft3d(prh_workspace* ws, const void* src, void* dst, const size_t* nmesh, int inverse,
+                    int io, void* stream) {
+  if (!ws || !src || !dst || !nmesh || io < 0 || io > 2) return PRH_ERR_ARG;
+  if ((io == kFftRealIn && inverse) || (io == kFftRealOut && !inverse)) {
+    set_error("a real input goes with the forward transform, a real output with the inverse");
+    return PRH_ERR_ARG;
+  }
+  PRH_TRY(check_transform_ws(ws, nmesh, Tag<R>::code));
+  hipStream_t s = as_stream(stream);
+  if (!inverse) {
+    PRH_TRY(fft_pass<R>(ws, 2, src, dst, io, 0, s));
+    PRH_TRY(fft_pass<R>(ws, 1, dst, dst, kFftComplex, 0, s));
+    PRH_TRY(fft_pass<R>(ws, 0, dst, dst, kFftComplex, 0, s));
+  } else {
+    void* work = const_cast<void*>(src);
+    PRH_TRY(fft_pass<R>(ws, 0, work, work, kFftComplex, 1, s));
+    PRH_TRY(fft_pass<R>(ws, 1, work, work, kFftComplex, 1, s));
+    PRH_TRY(fft_pass<R>(ws, 2, work, dst, io, 1, s));
+  }
+  return PRH_OK;
+}
+
+// Fourier-space operators.  k_i = 2 pi fftfreq(n_i, L_i / n_i) (ComplexMesh.
+// coords, pyrecon/mesh.py:812-820) is formed per cell from its index exactly
+// as numpy forms it: (2 pi) * (f * val) with val = 1 / (n_i (L_i / n_i)).
+// The arithmetic is double (the reference's wavenumbers are float64 arrays);
+// values the reference stores in the mesh precision are rounded to R where it
+// stores them.
+struct SpecArgs {
+  double val[3];
+  double los[3];
+  double beta;
+  double norm;  // factor on the output (1 / N before an inverse transform)
+  int n[3];
+  int a, b;     // axes of the operator
+};
+
+__device__ __forceinline__ double wavenumber(int i, int n, double val) {
+  const int f = i < (n - 1) / 2 + 1 ? i : i - n;
+  constexpr double twopi = 2 * 3.141592653589793;
+  return twopi * (static_cast<double>(f) * val);
+}
+
+enum SpecOp {
+  kSpecPlaneParallel = 0,  // dst = i k_a (src / k^2) / (1 + beta mu^2) (plane_parallel_fft.py:45-53)
+  kSpecInvK2 = 1,          // dst = src / (k_y^2 + k_x^2 + k_z^2) in R, 0 at k = 0 (iterative_fft.py:37-39)
+  kSpecLosSq = 2,          // dst = (k . los)^2 src (iterative_fft.py:44)
+  kSpecKaKb = 3,           // dst = k_a k_b src (iterative_fft.py:58)
+  kSpecGrad = 4,           // dst = i k_a (src / k^2), 0 at k = 0 (iterative_fft.py:75-81)
+};
+
+template <typename R, int OP>
+__global__ __launch_bounds__(256) void k_spectral(const Cx<R>* src, Cx<R>* dst, SpecArgs A) {
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y;
+  const int ix = blockIdx.z;
+  if (iz >= A.n[2]) return;
+  const size_t i = (static_cast<size_t>(ix) * A.n[1] + iy) * A.n[2] + iz;
+  const double k[3] = {wavenumber(ix, A.n[0], A.val[0]), wavenumber(iy, A.n[1], A.val[1]),
+                       wavenumber(iz, A.n[2], A.val[2])};
+  const Cx<R> c = src[i];
+  const bool zero = (ix | iy | iz) == 0;
+  Cx<R> out;
+  if constexpr (OP == kSpecInvK2) {
+    // RealMesh.prod_sum(exp=-1) (src/mesh.c:327-365): the squared wavenumbers are cast to R and
+    // summed in R as (y + x) + z; complex parts are divided separately
+    const R k2 = (static_cast<R>(k[1] * k[1]) + static_cast<R>(k[0] * k[0])) + static_cast<R>(k[2] * k[2]);
+    out = zero ? Cx<R>{R(0), R(0)} : Cx<R>{c.re / k2, c.im / k2};
+  } else if constexpr (OP == kSpecLosSq) {
+    const double kl = ((0. + k[0] * A.los[0]) + k[1] * A.los[1]) + k[2] * A.los[2];
+    const double m = kl * kl;
+    out = {static_cast<R>(m * static_cast<double>(c.re) * A.norm),
+           static_cast<R>(m * static_cast<double>(c.im) * A.norm)};
+  } else if constexpr (OP == kSpecKaKb) {
+    const double m = k[A.a] * k[A.b];
+    out = {static_cast<R>(m * static_cast<double>(c.re) * A.norm),
+           static_cast<R>(m * static_cast<double>(c.im) * A.norm)};
+  } else {
+    // k^2 as the Python sum of the broadcast squares, 1 at k = 0; the spectrum divided in place
+    // (rounded to the mesh precision)
+    const double k2 = zero ? 1. : ((0. + k[0] * k[0]) + k[1] * k[1]) + k[2] * k[2];
+    const double dre = static_cast<double>(static_cast<R>(static_cast<double>(c.re) / k2));
+    const double dim = static_cast<double>(static_cast<R>(static_cast<double>(c.im) / k2));
+    const double ka = k[A.a];
+    // (i k_a) d = (-k_a d.im, k_a d.re)
+    double re = -(ka * dim), im = ka * dre;
+    if constexpr (OP == kSpecPlaneParallel) {
+      const double kl = ((0. + k[0] * A.los[0]) + k[1] * A.los[1]) + k[2] * A.los[2];
+      const double den = 1. + A.beta * (kl * kl / k2);
+      re /= den;
+      im /= den;
+      out = {static_cast<R>(re * A.norm), static_cast<R>(im * A.norm)};
+    } else {
+      out = zero ? Cx<R>{R(0), R(0)} : Cx<R>{static_cast<R>(re * A.norm), static_cast<R>(im * A.norm)};
+    }
+  }
+  dst[i] = out;
+}
+
+template <typename R>
+static int do_spectral(prh_workspace* ws, int op, const void* src, void* dst, const size_t* nmesh,
+                       const R* boxsize, const double* los, double beta, double norm, int a, int b,
+                       void* stream) {
+  if (!ws || !src || !dst || !nmesh || !boxsize || op < 0 || op > 4 || a < 0 || a > 2 || b < 0 ||
+      b > 2)
+    return PRH_ERR_ARG;
+  if ((op == kSpecPlaneParallel || op == kSpecLosSq) && !los) return PRH_ERR_ARG;
+  PRH_TRY(check_transform_ws(ws, nmesh, Tag<R>::code));
+  SpecArgs A;
+  for (int d = 0; d < 3; d++) {
+    A.n[d] = static_cast<int>(nmesh[d]);
+    // d = boxsize / nmesh (float64, as numpy divides the mesh-dtype boxsize by the int nmesh)
+    const double cell = static_cast<double>(boxsize[d]) / static_cast<double>(A.n[d]);
+    A.val[d] = 1.0 / (static_cast<double>(A.n[d]) * cell);
+    A.los[d] = los ? los[d] : 0.;
+  }
+  A.beta = beta;
+  A.norm = norm;
+  A.a = a;
+  A.b = b;
+  const dim3 grid(ceil_div(nmesh[2], 256), A.n[1], A.n[0]);
+  const Cx<R>* in = static_cast<const Cx<R>*>(src);
+  Cx<R>* out = static_cast<Cx<R>*>(dst);
+  hipStream_t s = as_stream(stream);
+  switch (op) {
+    case kSpecPlaneParallel: k_spectral<R, kSpecPlaneParallel><<<grid, 256, 0, s>>>(in, out, A); break;
+    case kSpecInvK2: k_spectral<R, kSpecInvK2><<<grid, 256, 0, s>>>(in, out, A); break;
+    case kSpecLosSq: k_spectral<R, kSpecLosSq><<<grid, 256, 0, s>>>(in, out, A); break;
+    case kSpecKaKb: k_spectral<R, kSpecKaKb><<<grid, 256, 0, s>>>(in, out, A); break;
+    default: k_spectral<R, kSpecGrad><<<grid, 256, 0, s>>>(in, out, A); break;
+  }
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+// Real-space RSD removal of one iteration (iterative_fft.py:41-67): with a
+// global line of sight (a < 0) delta_real = delta_real - factor * t; with a
+// local one delta_real = delta_real - factor * ((x_a x_b) t / |x|^2), where
+// x_i = offset_i + (L_i / n_i) j_i are the mesh coordinates (RealMesh.coords,
+// pyrecon/mesh.py:522-527) in float64.
+template <typename R>
+__global__ __launch_bounds__(256) void k_rsd_update(R* __restrict__ delta, const R* __restrict__ t,
+                                                    int n0, int n1, int n2, double o0, double o1,
+                                                    double o2, double c0, double c1, double c2,
+                                                    double factor, int a, int b) {
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y;
+  const int ix = blockIdx.z;
+  if (iz >= n2) return;
+  const size_t i = (static_cast<size_t>(ix) * n1 + iy) * n2 + iz;
+  double rsd = static_cast<double>(t[i]);
+  if (a >= 0) {
+    const double x[3] = {o0 + c0 * static_cast<double>(ix), o1 + c1 * static_cast<double>(iy),
+                         o2 + c2 * static_cast<double>(iz)};
+    const double x2 = ((0. + x[0] * x[0]) + x[1] * x[1]) + x[2] * x[2];
+    rsd = ((x[a] * x[b]) * rsd) / x2;
+  }
+  delta[i] = static_cast<R>(static_cast<double>(delta[i]) - factor * rsd);
+}
+
+template <typename R>
+static int do_rsd_update(R* delta, const R* t, const size_t* nmesh, const R* boxsize,
+                         const R* boxcenter, double factor, int a, int b, void* stream) {
+  if (!delta || !t || !nmesh || !boxsize || !boxcenter || a > 2 || b > 2 || (a >= 0 && b < 0))
+    return PRH_ERR_ARG;
+  double o[3], c[3];
+  for (int d = 0; d < 3; d++) {
+    // MeshInfo.offset in the mesh dtype (mesh.py:462-465); cellsize in float64
+    o[d] = static_cast<double>(static_cast<R>(boxcenter[d] - boxsize[d] / R(2)));
+    c[d] = static_cast<double>(boxsize[d]) / static_cast<double>(static_cast<int>(nmesh[d]));
+  }
+  const int n0 = static_cast<int>(nmesh[0]), n1 = static_cast<int>(nmesh[1]),
+            n2 = static_cast<int>(nmesh[2]);
+  const dim3 grid(ceil_div(nmesh[2], 256), n1, n0);
+  k_rsd_update<R><<<grid, 256, 0, as_stream(stream)>>>(delta, t, n0, n1, n2, o[0], o[1], o[2], c[0],
+                                                       c[1], c[2], factor, a, b);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
 }  // namespace prh
 
 using namespace prh;
@@ -680,6 +1040,35 @@ int prh_smooth_gaussian_f32(prh_workspace* ws, float* mesh, const size_t* nmesh,
 int prh_smooth_gaussian_f64(prh_workspace* ws, double* mesh, const size_t* nmesh,
                             const double* boxsize, const double* radius, void* stream) {
   return do_smooth<double>(ws, mesh, nmesh, boxsize, radius, stream);
+}
+
+int prh_fft3d_f32(prh_workspace* ws, const void* src, void* dst, const size_t* nmesh, int inverse,
+                  int io, void* stream) {
+  return do_fft3d<float>(ws, src, dst, nmesh, inverse, io, stream);
+}
+int prh_fft3d_f64(prh_workspace* ws, const void* src, void* dst, const size_t* nmesh, int inverse,
+                  int io, void* stream) {
+  return do_fft3d<double>(ws, src, dst, nmesh, inverse, io, stream);
+}
+
+int prh_spectral_f32(prh_workspace* ws, int op, const void* src, void* dst, const size_t* nmesh,
+                     const float* boxsize, const double* los, double beta, double norm, int a, int b,
+                     void* stream) {
+  return do_spectral<float>(ws, op, src, dst, nmesh, boxsize, los, beta, norm, a, b, stream);
+}
+int prh_spectral_f64(prh_workspace* ws, int op, const void* src, void* dst, const size_t* nmesh,
+                     const double* boxsize, const double* los, double beta, double norm, int a, int b,
+                     void* stream) {
+  return do_spectral<double>(ws, op, src, dst, nmesh, boxsize, los, beta, norm, a, b, stream);
+}
+
+int prh_rsd_update_f32(float* delta, const float* t, const size_t* nmesh, const float* boxsize,
+                       const float* boxcenter, double factor, int a, int b, void* stream) {
+  return do_rsd_update<float>(delta, t, nmesh, boxsize, boxcenter, factor, a, b, stream);
+}
+int prh_rsd_update_f64(double* delta, const double* t, const size_t* nmesh, const double* boxsize,
+                       const double* boxcenter, double factor, int a, int b, void* stream) {
+  return do_rsd_update<double>(delta, t, nmesh, boxsize, boxcenter, factor, a, b, stream);
 }
 
 }  // extern "C"

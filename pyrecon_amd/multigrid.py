@@ -1,7 +1,7 @@
 """Drop-in ``MultiGridReconstruction`` on MI355X (pyrecon/multigrid.py:161-167).
 
 The constructor and methods are the reference's; the work runs in the HIP
-library: deterministic CIC paint, hipFFT smoothing and density contrast, the
+library: deterministic CIC paint, Fourier smoothing and density contrast, the
 full-multigrid solve (bit-identical to the reference's C in ``precision='plain'``)
 and the fused finite-difference read of the shifts.
 

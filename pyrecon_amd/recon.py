@@ -43,7 +43,7 @@ class BaseReconstruction(object):
     f, bias : float
     los : None, 'local', 'x', 'y', 'z', int or 3-vector
     fft_engine, fft_wisdom, save_fft_wisdom, fft_plan : accepted for compatibility; the device
-        always uses hipFFT.
+        always uses its own transforms.
     wrap : bool
     kwargs : MeshInfo arguments (nmesh, boxsize, boxcenter, cellsize, positions, boxpad, dtype);
         ``nthreads`` is accepted and ignored.  Device-specific: ``stream``, ``paint_chunk``,
@@ -239,3 +239,82 @@ def _make_property(name):
 
 for _name in ['boxsize', 'boxcenter', 'nmesh', 'offset', 'cellsize']:
     setattr(BaseReconstruction, _name, _make_property(_name))
+
+
+class SpectralReconstruction(BaseReconstruction):
+    """Device parts shared by the FFT reconstructions: 3-D transforms of the mesh, Fourier-space
+    operators, and the base-class shifts, read by CIC from the three displacement meshes
+    :attr:`mesh_psi` (pyrecon/recon.py:255-325)."""
+
+    def _complex(self):
+        """An interleaved complex mesh (n0, n1, n2, 2) in the mesh precision."""
+        return DeviceArray(tuple(self.info.shape) + (2,), self.info.dtype, stream=self.stream)
+
+    def _fft(self, src, dst, inverse, io):
+        nm, nm_ptr = _lib.size3(self.info.shape)
+        fn = getattr(lib, 'prh_fft3d_' + _suffix(self.info.dtype))
+        check(fn(self.workspace.handle, src.ptr, dst.ptr, nm_ptr, int(inverse), int(io), stream_handle(self.stream)),
+              'fft3d')
+
+    def _spectrum(self, mesh):
+        """numpy.fft.fftn of a real mesh (RealMesh.to_complex, pyrecon/mesh.py:697-705)."""
+        out = self._complex()
+        self._fft(mesh.buffer, out, False, _lib.FFT_REAL_IN)
+        return out
+
+    def _spectral(self, op, src, dst, a=0, b=0, norm=1., los=None, beta=0.):
+        nm, nm_ptr = _lib.size3(self.info.shape)
+        bs, bs_ptr = _lib.vec3(self.info.boxsize, self.info.dtype)
+        los_arr = None if los is None else (ctypes.c_double * 3)(*[float(x) for x in los])
+        fn = getattr(lib, 'prh_spectral_' + _suffix(self.info.dtype))
+        check(fn(self.workspace.handle, int(op), src.ptr, dst.ptr, nm_ptr, bs_ptr, los_arr, float(beta), float(norm),
+                 int(a), int(b), stream_handle(self.stream)), 'spectral operator')
+
+    def _new_mesh(self):
+        mesh = RealMesh(info=self.info, workspace=self.workspace, stream=self.stream)
+        mesh._allocate(zero=False)
+        return mesh
+
+    def _to_real(self, spectrum, out=None):
+        """Real part of numpy.fft.ifftn times N (ComplexMesh.to_real, pyrecon/mesh.py:827-844; the 1 / N is
+        applied by the Fourier-space operator that made ``spectrum``); ``spectrum`` is overwritten."""
+        if out is None: out = self._new_mesh()
+        self._fft(spectrum, out.buffer, True, _lib.FFT_REAL_OUT)
+        return out
+
+    def read_shifts(self, positions, field='disp+rsd'):
+        """Displacements at ``positions`` (pyrecon/recon.py:255-302): each component read by CIC from
+        :attr:`mesh_psi` on the device, positions wrapped first if :attr:`wrap`; the RSD algebra is the
+        reference's numpy.  Host arrays in and out (device inputs are copied to the host)."""
+        field = field.lower()
+        allowed_fields = ['disp', 'rsd', 'disp+rsd']
+        if field not in allowed_fields:
+            raise ReconstructionError('Unknown field {}. Choices are {}'.format(field, allowed_fields))
+        if getattr(self, 'mesh_psi', None) is None:
+            raise ReconstructionError('run the reconstruction before reading shifts')
+        if isinstance(positions, DeviceArray): positions = positions.numpy(stream=self.stream)
+        positions = np.asarray(positions)
+        shifts = np.empty_like(positions)
+        if self.wrap: positions = self.info.wrap(positions)
+        pos, _ = as_device(positions, stream=self.stream)
+        for iaxis, psi in enumerate(self.mesh_psi):
+            shifts[:, iaxis] = psi.read_cic(pos, wrap=False).numpy(stream=self.stream)
+        if field == 'disp':
+            return shifts
+        if self.los is None:
+            los = positions / distance(positions)[:, None]
+        else:
+            los = self.los
+        rsd = self.f * np.sum(shifts * los, axis=-1)[:, None] * los
+        if field == 'rsd':
+            return rsd
+        shifts += rsd
+        return shifts
+
+    def read_shifted_positions(self, positions, field='disp+rsd'):
+        """``positions - read_shifts(positions)``, wrapped if :attr:`wrap` (pyrecon/recon.py:304-325)."""
+        if isinstance(positions, DeviceArray): positions = positions.numpy(stream=self.stream)
+        shifts = self.read_shifts(positions, field=field)
+        positions = positions - shifts
+        if self.wrap: positions = self.info.wrap(positions)
+        return positions

@@ -128,6 +128,9 @@ CONFIGS = {
                geometry='box', weights='uniform'),
     'c3': dict(nmesh=512, boxsize=4000., boxcenter=(0., 0., 3000.), ndata=20000000, nrandoms=100000000,
                nlattice=512, geometry='shell', weights='fkp'),
+    # about 20% of the lattice lies in the shell after thinning: 672^3 gives 6.2e7 >= 5e7 points
+    'c4': dict(nmesh=1024, boxsize=4000., boxcenter=(0., 0., 3000.), ndata=50000000, nrandoms=250000000,
+               nlattice=672, geometry='shell', weights='fkp'),
 }
 RECON_KWARGS = dict(f=0.8, bias=2.0, los=None)
 SMOOTHING_RADIUS = 15.

@@ -14,7 +14,9 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(8, 8, 8), (8, 12, 16), (16, 16, 16), (6, 10, 12), (32, 32, 32), (24, 40, 20),
           # shapes taking the blocked kernels: 64-wide tiles (n2 % 64 == 0, n1 % 8 == 0, incl. a partial
           # x run) and, in FP32, whole rows of 64, 128, 256 and 512 cells (n1 % 8 == 0)
-          (6, 16, 64), (10, 32, 128), (64, 64, 64), (6, 16, 256), (4, 8, 512), (9, 24, 512)]
+          (6, 16, 64), (10, 32, 128), (64, 64, 64), (6, 16, 256), (4, 8, 512), (9, 24, 512),
+          # 1024-cell rows (the 1024^3 workload's z extent) take the 64-wide tiles
+          (4, 8, 1024)]
 LOS = [None, [0., 0., 1.], [0.3, 0.4, np.sqrt(1 - 0.25)]]
 BOX = dict(boxsize=np.array([1000., 1200., 900.]), boxcenter=np.array([0., 100., 1500.]))
 
@@ -103,7 +105,7 @@ def test_fmg_128(gpu, oracle, dtype):
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
-@pytest.mark.parametrize('shape', [(8, 8, 8), (16, 12, 10), (2, 2, 2), (64, 64, 64)])
+@pytest.mark.parametrize('shape', [(8, 8, 8), (16, 12, 10), (2, 2, 2), (64, 64, 64), (2, 4, 1024)])
 @pytest.mark.parametrize('chunk', [1 << 16, 1000])
 def test_assign_cic(gpu, oracle, dtype, shape, chunk):
     rng = np.random.RandomState(11)
@@ -136,7 +138,7 @@ def test_assign_cic_dense_cells(gpu, oracle, dtype):
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
-@pytest.mark.parametrize('shape', [(8, 8, 8), (16, 12, 10), (64, 64, 64)])
+@pytest.mark.parametrize('shape', [(8, 8, 8), (16, 12, 10), (64, 64, 64), (4, 2, 1024)])
 def test_reads(gpu, oracle, dtype, shape):
     rng = np.random.RandomState(13)
     n = 50000
@@ -153,7 +155,9 @@ SMOOTH_SHAPES = [(8, 8, 8), (8, 12, 20), (6, 10, 14), (15, 9, 25), (10, 32, 128)
                  # 512-cell lines: the register-resident transforms along z (an even and an odd number
                  # of rows), x and y (16 adjacent rows per tile), and x with a z extent that is not a
                  # multiple of 16 (the LDS transform)
-                 (4, 6, 512), (3, 5, 512), (512, 2, 16), (2, 512, 32), (512, 3, 5)]
+                 (4, 6, 512), (3, 5, 512), (512, 2, 16), (2, 512, 32), (512, 3, 5),
+                 # 1024-cell lines along each axis (LDS transforms)
+                 (2, 4, 1024), (1024, 2, 8), (4, 1024, 2)]
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
