@@ -183,7 +183,8 @@ int prh_smooth_gaussian_f32(prh_workspace *ws, float *mesh, const size_t *nmesh,
 int prh_smooth_gaussian_f64(prh_workspace *ws, double *mesh, const size_t *nmesh,
                             const double *boxsize, const double *radius, void *stream);
 
-/* ---- FFT reconstructions (plane_parallel_fft.py, iterative_fft.py) -------
+/* ---- FFT reconstructions (plane_parallel_fft.py, iterative_fft.py,
+ *      iterative_fft_particle.py) ---------------------------------------------
  * 3-D c2c transform of an (n0, n1, n2) mesh, as numpy.fft.fftn / ifftn
  * without the 1 / N of the inverse (RealMesh.to_complex and
  * ComplexMesh.to_real, mesh.py:697-705, 827-844, 974-988).  Complex meshes are
@@ -208,6 +209,7 @@ int prh_fft3d_f64(prh_workspace *ws, const void *src, void *dst, const size_t *n
 #define PRH_SPEC_LOS_SQ 2         /* (k . los)^2 src                        iterative_fft.py:44 */
 #define PRH_SPEC_KA_KB 3          /* k_a k_b src                            iterative_fft.py:58 */
 #define PRH_SPEC_GRAD 4           /* i k_a (src / k^2), 0 at k = 0          iterative_fft.py:75-81 */
+#define PRH_SPEC_I_KA 5           /* (src i) k_a                            iterative_fft_particle.py:112 */
 int prh_spectral_f32(prh_workspace *ws, int op, const void *src, void *dst, const size_t *nmesh,
                      const float *boxsize, const double *los, double beta, double norm, int a, int b,
                      void *stream);

@@ -16,7 +16,8 @@ A restatement of the reference's Python layer with numpy, calling the C oracle
 - the shifts: pyrecon/multigrid.py:137-158 and pyrecon/recon.py:304-325;
 - the FFT reconstructions: pyrecon/plane_parallel_fft.py:41-55 and
   pyrecon/iterative_fft.py:12-82, with the base-class shifts of
-  pyrecon/recon.py:255-325.
+  pyrecon/recon.py:255-325, and the particle-level iterations of
+  pyrecon/iterative_fft_particle.py:9-239.
 
 The numpy operations (and therefore the dtypes of every intermediate) are the
 reference's, so with the same native kernels this reproduces the reference's
@@ -372,3 +373,135 @@ class OracleIterativeFFTReconstruction(_OracleFFTReconstruction):
         delta_k /= k2
         delta_k[0, 0, 0] = 0.
         return [self.to_real(1j * k[iaxis] * delta_k) for iaxis in range(3)]
+
+
+class OracleOriginalIterativeFFTParticleReconstruction(_OracleFFTReconstruction):
+    """Bautista et al. 2018, particle-level iterations (pyrecon/iterative_fft_particle.py:9-239)."""
+
+    def assign_data(self, positions, weights=None):
+        if weights is None:
+            weights = np.ones_like(positions, shape=(len(positions),))
+        if self.wrap: positions = self.wrap_positions(positions)
+        if self.mesh_data is None:
+            self._positions_data = positions
+            self._weights_data = weights
+        else:
+            self._positions_data = np.concatenate([self._positions_data, positions], axis=0)
+            self._weights_data = np.concatenate([self._weights_data, weights], axis=0)
+        self.mesh_data = self._paint(self.mesh_data, positions, weights)
+
+    def set_density_contrast(self, ran_min=0.01, smoothing_radius=15.):
+        # the meshes are taken as already smoothed and are kept (iterative_fft_particle.py:33-59)
+        self.ran_min = ran_min
+        self.smoothing_radius = smoothing_radius
+        if self.has_randoms:
+            sum_data, sum_randoms = np.sum(self.mesh_data), np.sum(self.mesh_randoms)
+            alpha = sum_data * 1. / sum_randoms
+            delta = self.mesh_data - alpha * self.mesh_randoms
+            threshold = ran_min * sum_randoms / self._size_randoms
+            mask = self.mesh_randoms > threshold
+            delta[mask] /= (self.bias * alpha * self.mesh_randoms[mask])
+            delta[~mask] = 0.
+        else:
+            delta = self.mesh_data / np.mean(self.mesh_data) - 1.
+            delta /= self.bias
+        self.mesh_delta = delta
+
+    def read_cic_wrap(self, mesh, positions):
+        """RealMesh.read_cic(positions, wrap=True) (mesh.py:570-610)."""
+        dtype = positions.dtype
+        positions = self.mesh_units(self.wrap_positions(positions))
+        return self.kernels.read_cic(np.ascontiguousarray(mesh, dtype=self.dtype), positions).astype(dtype, copy=False)
+
+    def run(self, niterations=3):
+        self._iter = 0
+        self.mesh_data = self.smooth_gaussian(self.mesh_data, self.smoothing_radius)
+        if self.has_randoms: self.mesh_randoms = self.smooth_gaussian(self.mesh_randoms, self.smoothing_radius)
+        self._positions_rec_data = self._positions_data.copy()
+        for it in range(niterations):
+            self.mesh_psi = self._iterate(return_psi=it == niterations - 1)
+        self.mesh_randoms = None
+
+    def _iterate(self, return_psi=False):
+        if self._iter > 0:
+            wrap = self.wrap
+            self.wrap = True
+            self.mesh_data = self._paint(None, self._positions_rec_data, self._weights_data)
+            self.wrap = wrap
+            self.mesh_data = self.smooth_gaussian(self.mesh_data, self.smoothing_radius)
+        self.set_density_contrast(ran_min=self.ran_min, smoothing_radius=self.smoothing_radius)
+        self.mesh_data = None
+        delta_k = self.to_complex(self.mesh_delta)
+        del self.mesh_delta
+        k = self.kcoords()
+        # ComplexMesh.prod_sum(exp=-1), as in OracleIterativeFFTReconstruction._iterate
+        k2 = [np.asarray(kk**2, dtype=self.dtype) for kk in k]
+        den = (k2[1] + k2[0]) + k2[2]
+        with np.errstate(divide='ignore', invalid='ignore'):
+            delta_k = (delta_k.real / den + 1j * (delta_k.imag / den)).astype(self.ctype, copy=False)
+        delta_k[0, 0, 0] = 0.
+        shifts = np.empty_like(self._positions_rec_data)
+        psis = []
+        for iaxis in range(3):
+            if not return_psi and self.los is not None and self.los[iaxis] == 0:
+                shifts[:, iaxis] = 0.
+                continue
+            psi = self.to_real(delta_k * 1j * k[iaxis])
+            shifts[:, iaxis] = self.read_cic_wrap(psi, self._positions_rec_data)
+            if return_psi: psis.append(psi)
+        if self.los is None:
+            los = self._positions_data / distance(self._positions_data)[:, None]
+        else:
+            los = self.los
+        if self._iter == 0:
+            shifts -= self.beta / (1 + self.beta) * np.sum(shifts * los, axis=-1)[:, None] * los
+        self._positions_rec_data = self._positions_data - self.f * np.sum(shifts * los, axis=-1)[:, None] * los
+        self._iter += 1
+        if return_psi:
+            return psis
+
+    def read_shifts(self, positions, field='disp+rsd'):
+        field = field.lower()
+        if field not in ['disp', 'rsd', 'disp+rsd']:
+            raise OracleError('Unknown field {}'.format(field))
+
+        def read_cic(positions, wrap=False):
+            shifts = np.empty_like(positions)
+            for iaxis, psi in enumerate(self.mesh_psi):
+                shifts[:, iaxis] = self.read_cic_wrap(psi, positions) if wrap else self.read_cic_nowrap(psi, positions)
+            return shifts
+
+        if isinstance(positions, str) and positions == 'data':
+            shifts = read_cic(self._positions_rec_data, wrap=True)
+            if field == 'disp':
+                return shifts
+            rsd = self._positions_data - self._positions_rec_data
+            if field == 'rsd':
+                return rsd
+            shifts += rsd
+            return shifts
+        if self.wrap: positions = self.wrap_positions(positions)
+        shifts = read_cic(positions, wrap=False)
+        if field == 'disp':
+            return shifts
+        if self.los is None:
+            los = positions / distance(positions)[:, None]
+        else:
+            los = self.los
+        rsd = self.f * np.sum(shifts * los, axis=-1)[:, None] * los
+        if field == 'rsd':
+            return rsd
+        real_positions = positions - rsd
+        shifts = read_cic(real_positions, wrap=True)
+        return shifts + rsd
+
+    def read_shifted_positions(self, positions, field='disp+rsd'):
+        shifts = self.read_shifts(positions, field=field)
+        if isinstance(positions, str) and positions == 'data':
+            positions = self._positions_data
+        positions = positions - shifts
+        if self.wrap: positions = self.wrap_positions(positions)
+        return positions
+
+
+OracleIterativeFFTParticleReconstruction = OracleOriginalIterativeFFTParticleReconstruction

@@ -1,5 +1,5 @@
 """pyrecon_amd: MI355X-native MultiGridReconstruction (drop-in for pyrecon's hot path), and the
-FFT reconstructions PlaneParallelFFTReconstruction and IterativeFFTReconstruction on the same kernels.
+FFT reconstructions (plane-parallel, iterative, iterative particle) on the same kernels.
 
 Usage is the reference's::
 
@@ -23,9 +23,11 @@ from .recon import BaseReconstruction, ReconstructionError
 from .multigrid import MultiGridReconstruction, OriginalMultiGridReconstruction, run_group
 from .plane_parallel_fft import PlaneParallelFFTReconstruction
 from .iterative_fft import IterativeFFTReconstruction
+from .iterative_fft_particle import OriginalIterativeFFTParticleReconstruction, IterativeFFTParticleReconstruction
 from .serving import ReconServer, ReconJob, AdmissionError
 
 __all__ = ['MultiGridReconstruction', 'OriginalMultiGridReconstruction', 'PlaneParallelFFTReconstruction',
-           'IterativeFFTReconstruction', 'BaseReconstruction', 'ReconstructionError',
+           'IterativeFFTReconstruction', 'OriginalIterativeFFTParticleReconstruction',
+           'IterativeFFTParticleReconstruction', 'BaseReconstruction', 'ReconstructionError',
            'MeshError', 'DeviceError', 'MeshInfo', 'RealMesh', 'Workspace', 'DeviceArray', 'Stream', 'Event',
            'synchronize', 'empty_cache', 'run_group', 'ReconServer', 'ReconJob', 'AdmissionError']

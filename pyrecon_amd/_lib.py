@@ -30,7 +30,7 @@ F32, F64 = 4, 8
 FIELD_DISP, FIELD_RSD, FIELD_DISP_RSD, FIELD_VALUE = 0, 1, 2, 3
 TIMER_JACOBI_FINEST, TIMER_PAINT_GATHER, TIMER_READ_SHIFTS = 0, 1, 2
 FFT_COMPLEX, FFT_REAL_IN, FFT_REAL_OUT = 0, 1, 2
-SPEC_PLANE_PARALLEL, SPEC_INV_K2, SPEC_LOS_SQ, SPEC_KA_KB, SPEC_GRAD = 0, 1, 2, 3, 4
+SPEC_PLANE_PARALLEL, SPEC_INV_K2, SPEC_LOS_SQ, SPEC_KA_KB, SPEC_GRAD, SPEC_I_KA = 0, 1, 2, 3, 4, 5
 
 WS_MULTIGRID, WS_PAINT, WS_FFT, WS_MIXED = 1, 2, 4, 8
 WS_ALL = 15

@@ -896,6 +896,7 @@ enum SpecOp {
   kSpecLosSq = 2,          // dst = (k . los)^2 src (iterative_fft.py:44)
   kSpecKaKb = 3,           // dst = k_a k_b src (iterative_fft.py:58)
   kSpecGrad = 4,           // dst = i k_a (src / k^2), 0 at k = 0 (iterative_fft.py:75-81)
+  kSpecIKa = 5,            // dst = (src i) k_a (iterative_fft_particle.py:112)
 };
 
 template <typename R, int OP>
@@ -920,6 +921,11 @@ __global__ __launch_bounds__(256) void k_spectral(const Cx<R>* src, Cx<R>* dst, 
     const double m = kl * kl;
     out = {static_cast<R>(m * static_cast<double>(c.re) * A.norm),
            static_cast<R>(m * static_cast<double>(c.im) * A.norm)};
+  } else if constexpr (OP == kSpecIKa) {
+    // (src * 1j) = (-im, re), then times k_a
+    const double ka = k[A.a];
+    out = {static_cast<R>(-(static_cast<double>(c.im) * ka) * A.norm),
+           static_cast<R>(static_cast<double>(c.re) * ka * A.norm)};
   } else if constexpr (OP == kSpecKaKb) {
     const double m = k[A.a] * k[A.b];
     out = {static_cast<R>(m * static_cast<double>(c.re) * A.norm),
@@ -950,7 +956,7 @@ template <typename R>
 static int do_spectral(prh_workspace* ws, int op, const void* src, void* dst, const size_t* nmesh,
                        const R* boxsize, const double* los, double beta, double norm, int a, int b,
                        void* stream) {
-  if (!ws || !src || !dst || !nmesh || !boxsize || op < 0 || op > 4 || a < 0 || a > 2 || b < 0 ||
+  if (!ws || !src || !dst || !nmesh || !boxsize || op < 0 || op > 5 || a < 0 || a > 2 || b < 0 ||
       b > 2)
     return PRH_ERR_ARG;
   if ((op == kSpecPlaneParallel || op == kSpecLosSq) && !los) return PRH_ERR_ARG;
@@ -976,6 +982,7 @@ static int do_spectral(prh_workspace* ws, int op, const void* src, void* dst, co
     case kSpecInvK2: k_spectral<R, kSpecInvK2><<<grid, 256, 0, s>>>(in, out, A); break;
     case kSpecLosSq: k_spectral<R, kSpecLosSq><<<grid, 256, 0, s>>>(in, out, A); break;
     case kSpecKaKb: k_spectral<R, kSpecKaKb><<<grid, 256, 0, s>>>(in, out, A); break;
+    case kSpecIKa: k_spectral<R, kSpecIKa><<<grid, 256, 0, s>>>(in, out, A); break;
     default: k_spectral<R, kSpecGrad><<<grid, 256, 0, s>>>(in, out, A); break;
   }
   PRH_LAUNCHED();

@@ -4,10 +4,11 @@
   cover every radix and 512-cell lines along each axis: relative L2 <= 1e-13
   (FP64), 2e-6 (FP32).
 - Fourier-space operators (prh_spectral_*) against the oracle's numpy
-  expressions of plane_parallel_fft.py:45-53 and iterative_fft.py:37-81.
-- PlaneParallelFFTReconstruction and IterativeFFTReconstruction (global and
-  local line of sight) end to end on the configuration-1 catalogs against
-  OraclePlaneParallelFFTReconstruction / OracleIterativeFFTReconstruction:
+  expressions of plane_parallel_fft.py:45-53, iterative_fft.py:37-81 and
+  iterative_fft_particle.py:112.
+- PlaneParallelFFTReconstruction, IterativeFFTReconstruction and
+  IterativeFFTParticleReconstruction (global and local line of sight) end to end on the configuration-1 catalogs against
+  their oracle classes in oracle/recon_numpy.py:
   displacement meshes to relative L2 <= 1e-10 (FP64; the smoothing and the
   transforms round differently from numpy's) and shifted positions to
   1e-8 Mpc/h; FP32 meshes to the reference's own f4-vs-f8 tolerance
@@ -113,6 +114,7 @@ def test_spectral_operators(dtype):
         g = 1j * k[a] * d
         g[0, 0, 0] = 0.
         assert rel_l2(run(_lib.SPEC_GRAD, a=a), norm * g) < tol
+        assert rel_l2(run(_lib.SPEC_I_KA, a=a), norm * (spec * 1j) * k[a]) < tol
         for b in range(a, 3):
             assert rel_l2(run(_lib.SPEC_KA_KB, a=a, b=b), norm * k[a] * k[b] * spec) < tol
     assert rel_l2(run(_lib.SPEC_LOS_SQ, use_los=True), norm * sum(kk * ll for ll, kk in zip(k, los))**2 * spec) < tol
@@ -168,3 +170,35 @@ def test_plane_parallel_needs_global_los():
     import pyrecon_amd
     with pytest.raises(pyrecon_amd.ReconstructionError):
         pyrecon_amd.PlaneParallelFFTReconstruction(f=0.8, bias=2., los=None, nmesh=16, boxsize=500., boxcenter=0.)
+
+
+@pytest.mark.parametrize('dtype', ['f8', 'f4'])
+@pytest.mark.parametrize('los', [None, 'x'])
+def test_iterative_particle_vs_oracle(c1, dtype, los):
+    # OriginalIterativeFFTParticleReconstruction (iterative_fft_particle.py:9-239): the data are repainted
+    # at their reconstructed positions each iteration; positions, fields and shifts against the oracle
+    import pyrecon_amd
+    from oracle import recon_numpy
+    cfg = c1['config']
+    geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype=dtype)
+    rkw = dict(RECON_KWARGS, los=los)
+    gpu = pyrecon_amd.IterativeFFTParticleReconstruction(**rkw, **geo)
+    cpu = recon_numpy.OracleIterativeFFTParticleReconstruction(**rkw, **geo)
+    for r in (gpu, cpu):
+        r.assign_data(c1['data'], c1['data_weights'])
+        r.assign_randoms(c1['randoms'], c1['randoms_weights'])
+        r.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+        r.run(niterations=3)
+    tol_x = 1e-8 if dtype == 'f8' else 1e-2
+    assert np.max(np.abs(gpu._positions_rec_data - cpu._positions_rec_data)) < tol_x
+    for a in range(3):
+        err = rel_l2(gpu.mesh_psi[a].value.astype('f8'), np.asarray(cpu.mesh_psi[a], dtype='f8'))
+        assert err < (1e-10 if dtype == 'f8' else 1e-4), (a, err)
+    for positions in ['data', c1['randoms'][:20000]]:
+        for field in ['disp', 'rsd', 'disp+rsd']:
+            xg = gpu.read_shifted_positions(positions, field=field)
+            xc = cpu.read_shifted_positions(positions, field=field)
+            if dtype == 'f8':
+                assert np.max(np.abs(xg - xc)) < tol_x, field
+            else:
+                assert np.allclose(xg, xc, atol=1e-2, rtol=1e-2), field

@@ -13,6 +13,13 @@
   own local ~ global check for the multigrid, test_multigrid.py:124-145).
 - Base-class shifts (pyrecon/recon.py:255-325): constant displacement meshes
   read back as the constants; 'disp+rsd' = 'disp' + f (s . los) los.
+- OriginalIterativeFFTParticleReconstruction (iterative_fft_particle.py:9-239):
+  after one iteration the data move by -f / (1 + beta) (psi . rhat) rhat, psi
+  being the field-level oracle's displacement of the same density contrast read
+  at the data (pins the first-iteration factor, the sign and the read
+  positions); with f = 0 the repainted data stay put and every iteration gives
+  the same field; the 'data' shifts split as rsd = x - x_rec and
+  x - shifts = x_rec - disp.
 """
 import numpy as np
 import pytest
@@ -120,3 +127,76 @@ def test_base_class_shifts_of_constant_psi(los):
     assert np.allclose(rsd, r.f * np.sum(c * rhat, axis=-1)[:, None] * rhat, atol=1e-12)
     assert np.allclose(r.read_shifts(pos), disp + rsd, atol=1e-12)
     assert np.allclose(r.read_shifted_positions(pos), pos - disp - rsd, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# IterativeFFTParticleReconstruction (pyrecon/iterative_fft_particle.py:9-239)
+# ---------------------------------------------------------------------------
+def _small_catalogs():
+    from catalogs import zeldovich_catalog, uniform_box
+    boxsize, boxcenter = 500., (0., 0., 800.)
+    data = zeldovich_catalog(24, boxsize, boxcenter, f=0.8, workers=2)
+    randoms = uniform_box(200000, boxsize, boxcenter, seed=7)
+    return dict(boxsize=boxsize, boxcenter=boxcenter, data=data, randoms=randoms)
+
+
+@pytest.fixture(scope='module')
+def small():
+    return _small_catalogs()
+
+
+def _particle(small, los, niterations, f=0.8):
+    from oracle.recon_numpy import OracleOriginalIterativeFFTParticleReconstruction
+    r = OracleOriginalIterativeFFTParticleReconstruction(f=f, bias=2., los=los, nmesh=32, boxsize=small['boxsize'],
+                                                         boxcenter=small['boxcenter'], dtype='f8')
+    r.assign_data(small['data'])
+    r.assign_randoms(small['randoms'])
+    r.set_density_contrast(smoothing_radius=15.)
+    r.run(niterations=niterations)
+    return r
+
+
+def _field_psi(small, los):
+    """The displacement of the smoothed density contrast, from the field-level oracle (no iteration)."""
+    r = OracleIterativeFFTReconstruction(f=0.8, bias=2., los=los, nmesh=32, boxsize=small['boxsize'],
+                                         boxcenter=small['boxcenter'], dtype='f8')
+    r.assign_data(small['data'])
+    r.assign_randoms(small['randoms'])
+    r.set_density_contrast(smoothing_radius=15.)
+    r.run(niterations=0)
+    return r
+
+
+@pytest.mark.parametrize('los', ['z', None])
+def test_particle_first_iteration(small, los):
+    # one iteration: x_rec = x - f / (1 + beta) (psi(x) . rhat) rhat with psi the displacement of the
+    # smoothed density contrast read at the data, rhat the line of sight
+    r = _particle(small, los, 1)
+    field = _field_psi(small, los)
+    x = small['data']
+    psi = np.stack([field.read_cic_nowrap(p, x) for p in field.mesh_psi], axis=-1)
+    rhat = x / np.sqrt(np.sum(x**2, axis=-1))[:, None] if los is None else np.broadcast_to(r.los, x.shape)
+    expect = x - r.f / (1. + r.beta) * np.sum(psi * rhat, axis=-1)[:, None] * rhat
+    scale = np.max(np.abs(psi))
+    assert np.max(np.abs(r._positions_rec_data - expect)) < 1e-10 * scale
+    for a in range(3):
+        assert np.allclose(r.mesh_psi[a], field.mesh_psi[a], rtol=0, atol=1e-12 * scale)
+
+
+def test_particle_without_rsd_is_stationary(small):
+    # f = 0: the data are repainted where they were, so every iteration gives the same field
+    one, three = _particle(small, 'z', 1, f=0.), _particle(small, 'z', 3, f=0.)
+    assert np.array_equal(three._positions_rec_data, small['data'])
+    scale = np.max(np.abs(one.mesh_psi[0]))
+    for a in range(3):
+        assert np.allclose(three.mesh_psi[a], one.mesh_psi[a], rtol=0, atol=1e-12 * scale)
+
+
+def test_particle_data_shifts(small):
+    r = _particle(small, None, 2)
+    x = small['data']
+    rsd = r.read_shifts('data', field='rsd')
+    assert np.array_equal(rsd, x - r._positions_rec_data)
+    disp = r.read_shifts('data', field='disp')
+    # the reconstructed data are the real-space estimate minus the displacement read there
+    assert np.allclose(r.read_shifted_positions('data'), r._positions_rec_data - disp, rtol=0, atol=1e-9)
