@@ -7,7 +7,8 @@ run on identical inputs (oracle/recon_numpy.py, whose C kernels are the referenc
 - FP64: potential (mean removed) relative L2 <= 1e-10, shifted positions <= 1e-8 Mpc/h;
 - mixed: shifted positions <= 1e-3 Mpc/h;
 - FP32: shifted positions within the reference's own f4-vs-f8 tolerance
-  (pyrecon/tests/test_multigrid.py:94, atol = rtol = 1e-2).
+  (pyrecon/tests/test_multigrid.py:94, atol = rtol = 1e-2) of the FP64 run, CIC meshes bit-exact
+  to the oracle's FP32 paint, and shifted positions close to the oracle's FP32 run.
 """
 import numpy as np
 import pytest
@@ -98,6 +99,22 @@ def test_config2_fp32(c2):
     err_phi = rel_l2(gpu.mesh_phi.value, ref['phi'])
     xd, xr = gpu.read_shifted_positions(cat['data']), gpu.read_shifted_positions(sample)
     dx = max(np.max(np.abs(xd - ref['data'])), np.max(np.abs(xr - ref['randoms'])))
-    print('configuration 2, FP32: phi rel L2 {:.3e}, max |dx| {:.3e} Mpc/h'.format(err_phi, dx))
+    rms = np.sqrt(np.mean(np.concatenate([(xd - ref['data']).ravel(), (xr - ref['randoms']).ravel()])**2))
+    print('configuration 2, FP32 vs FP64: phi rel L2 {:.3e}, max |dx| {:.3e} Mpc/h, rms {:.3e}'.format(err_phi, dx, rms))
     assert np.allclose(xd, ref['data'], atol=1e-2, rtol=1e-2)
     assert np.allclose(xr, ref['randoms'], atol=1e-2, rtol=1e-2)
+    # against the oracle's own FP32 run: the same algorithm (the paint and the multigrid are the
+    # reference's bit for bit, including the upper-edge weights of the two randoms whose FP32 mesh
+    # coordinate rounds to n); only the Fourier smoothing rounds differently
+    from oracle.recon_numpy import OracleMultiGridReconstruction
+    cpu = OracleMultiGridReconstruction(**RECON_KWARGS, **geo, dtype='f4')
+    cpu.assign_data(cat['data'], cat['data_weights'])
+    cpu.assign_randoms(cat['randoms'], cat['randoms_weights'])
+    assert np.array_equal(bits(meshes[0]), bits(cpu.mesh_data)) and np.array_equal(bits(meshes[1]), bits(cpu.mesh_randoms))
+    cpu.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    cpu.run()
+    d32 = np.concatenate([(xd - cpu.read_shifted_positions(cat['data'])).ravel(),
+                          (xr - cpu.read_shifted_positions(sample)).ravel()])
+    dx32, rms32 = np.max(np.abs(d32)), np.sqrt(np.mean(d32**2))
+    print('configuration 2, FP32 vs the FP32 oracle: max |dx| {:.3e} Mpc/h, rms {:.3e}'.format(dx32, rms32))
+    assert rms32 < 5e-4 and dx32 < 1e-2
