@@ -1,0 +1,64 @@
+"""The command-line reconstruction on the device (pyrecon_amd/cli.py, after bin/pyrecon:70-423):
+catalogs written to .npz files go through the configuration's conventions and come back equal,
+bit for bit, to the same calls made directly on the classes."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def cats(tmp_path_factory):
+    path = tmp_path_factory.mktemp('cli')
+    rng = np.random.RandomState(7)
+    boxsize, boxcenter = 600., np.array([0., 0., 1200.])
+    data = rng.uniform(-250., 250., (40000, 3)) + boxcenter
+    randoms = rng.uniform(-290., 290., (200000, 3)) + boxcenter
+    wdata = rng.uniform(0.5, 1.5, len(data))
+    np.savez(path / 'data.npz', POSITION=data, WEIGHT=wdata, ID=np.arange(len(data)))
+    np.savez(path / 'randoms.npz', X=randoms[:, 0], Y=randoms[:, 1], Z=randoms[:, 2])
+    return path, data, wdata, randoms, boxsize, boxcenter
+
+
+def _config(path, boxsize, boxcenter, name, convention, los='local', extra=None):
+    cfg = {'input': {'dir': str(path), 'data_fn': 'data.npz', 'randoms_fn': 'randoms.npz', 'xyz_data': 'POSITION',
+                     'xyz_randoms': ['X', 'Y', 'Z'], 'weights_data': '${WEIGHT}'},
+           'output': {'dir': str(path), 'data_fn': 'data_rec.npz', 'randoms_fn': 'randoms_rec.npz',
+                      'xyz_rec': 'POSITION_REC', 'columns_data': ['ID'], 'columns_randoms': ''},
+           'algorithm': dict({'name': name, 'convention': convention, 'los': los}, **(extra or {})),
+           'delta': {'smoothing_radius': 15.},
+           'cosmology': {'f': 0.8, 'bias': 2.},
+           'mesh': {'nmesh': 64, 'boxsize': boxsize, 'boxcenter': boxcenter.tolist(), 'dtype': 'f8'}}
+    return cfg
+
+
+@pytest.mark.parametrize('name, convention, los', [('MultiGridReconstruction', 'RecSym', 'local'),
+                                                   ('MultiGridReconstruction', 'RecIso', 'local'),
+                                                   ('MultiGridReconstruction', 'RSD', 'local'),
+                                                   ('PlaneParallelFFTReconstruction', 'RecSym', 'z'),
+                                                   ('IterativeFFTParticleReconstruction', 'RecIso', 'local')])
+def test_cli_matches_direct_calls(cats, name, convention, los):
+    import pyrecon_amd
+    from pyrecon_amd import cli
+    path, data, wdata, randoms, boxsize, boxcenter = cats
+    cli.main([], **_config(path, boxsize, boxcenter, name, convention, los))
+    out_d = np.load(path / 'data_rec.npz')
+    out_r = np.load(path / 'randoms_rec.npz')
+    assert np.array_equal(out_d['ID'], np.arange(len(data)))
+    recon = getattr(pyrecon_amd, name)(f=0.8, bias=2., los=los, nmesh=64, boxsize=boxsize, boxcenter=boxcenter,
+                                       dtype='f8')
+    recon.assign_data(data, wdata)
+    recon.assign_randoms(randoms)
+    recon.set_density_contrast(smoothing_radius=15.)
+    recon.run()
+    field = 'rsd' if convention == 'RSD' else 'disp+rsd'
+    if name == 'IterativeFFTParticleReconstruction':
+        expect_d = recon.read_shifted_positions('data', field=field)
+    else:
+        expect_d = recon.read_shifted_positions(data, field=field)
+    if convention == 'RSD':
+        expect_r = randoms
+    else:
+        expect_r = recon.read_shifted_positions(randoms, field='disp+rsd' if convention == 'RecSym' else 'disp')
+    assert np.array_equal(out_d['POSITION_REC'], expect_d)
+    assert np.array_equal(out_r['POSITION_REC'], expect_r)
