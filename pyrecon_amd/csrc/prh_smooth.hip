@@ -322,11 +322,12 @@ constexpr int kReg512Waves = 4;  // lines in flight per workgroup
 // exchange slot of value k of lane L (XOR-swizzled within each lane's 8)
 __device__ __forceinline__ int xslot(int lane, int k) { return lane * 8 + (k ^ (lane & 7)); }
 
-template <typename R>
+// tw holds W_{512 S}^k: S = 1 for 512-cell lines, 2 for the halves of 1024-cell lines.
+template <typename R, int S = 1>
 __device__ __forceinline__ void fft512_lane(Cx<R> v[8], Cx<R>* __restrict__ xb,
                                             const Cx<R>* __restrict__ tw, int lane, bool inverse) {
   auto root = [&](int k) {  // W512^k (conjugate for the inverse)
-    Cx<R> w = tw[k & 511];
+    Cx<R> w = tw[(k & 511) * S];
     if (inverse) w.im = -w.im;
     return w;
   };
@@ -465,6 +466,136 @@ __global__ __launch_bounds__(64 * kReg512SWaves) void k_smooth_reg512s(
     __syncthreads();
 #pragma unroll 4
     for (int k = 0; k < 16; k++) {
+      const int e = e0 + 32 * k;
+      base[static_cast<long long>(e) * es + c] = tile[e * kTileRow + c];
+    }
+  }
+}
+
+// Lines of 1024 cells in registers: lane L holds elements L + 64 r, r = 0..15.
+// Forward: one radix-2 step in the lane (a = x_n + x_{n+512}, b = (x_n -
+// x_{n+512}) W1024^n for n = L + 64 r, r < 8), then the 512-point transforms of
+// a and b (fft512_lane with every second root of the 1024-point table): lane
+// L then holds X[2 (L + 64 m)] in a[m] and X[2 (L + 64 m) + 1] in b[m].  The
+// inverse runs the two 512-point inverse transforms and combines x_n = a'_n +
+// W1024^-n b'_n, x_{n+512} = a'_n - W1024^-n b'_n.
+template <typename R>
+__device__ __forceinline__ void filter1024_lane(Cx<R> v[16], Cx<R>* __restrict__ xb,
+                                                const Cx<R>* __restrict__ tw,
+                                                const R* __restrict__ gain, int lane) {
+  Cx<R> a[8], b[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    a[r] = {v[r].re + v[r + 8].re, v[r].im + v[r + 8].im};
+    b[r] = cmul(Cx<R>{v[r].re - v[r + 8].re, v[r].im - v[r + 8].im}, tw[lane + 64 * r]);
+  }
+  fft512_lane<R, 2>(a, xb, tw, lane, false);
+  fft512_lane<R, 2>(b, xb, tw, lane, false);
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    const R ga = gain[2 * (lane + 64 * m)], gb = gain[2 * (lane + 64 * m) + 1];
+    a[m].re *= ga;
+    a[m].im *= ga;
+    b[m].re *= gb;
+    b[m].im *= gb;
+  }
+  fft512_lane<R, 2>(a, xb, tw, lane, true);
+  fft512_lane<R, 2>(b, xb, tw, lane, true);  // ends with a barrier
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    Cx<R> w = tw[lane + 64 * r];
+    w.im = -w.im;
+    const Cx<R> t = cmul(b[r], w);
+    v[r] = {a[r].re + t.re, a[r].im + t.im};
+    v[r + 8] = {a[r].re - t.re, a[r].im - t.im};
+  }
+}
+
+// `nrows` rows of 1024 contiguous cells; as k_smooth_reg512 with one complex
+// line (two real rows) per wavefront.
+template <typename R>
+__global__ __launch_bounds__(64 * kReg512Waves) void k_smooth_reg1024(
+    R* __restrict__ mesh, long long nrows, const Cx<R>* __restrict__ tw_global,
+    const R* __restrict__ gain) {
+  __shared__ Cx<R> stw[1024];
+  __shared__ Cx<R> xbuf[kReg512Waves][512];
+  for (int k = threadIdx.x; k < 1024; k += 64 * kReg512Waves) stw[k] = tw_global[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  Cx<R>* xb = xbuf[w];
+  const long long ncx = (nrows + 1) / 2;
+  const long long step = static_cast<long long>(gridDim.x) * kReg512Waves;
+  // every wave of the block runs the same number of iterations (barriers inside)
+  for (long long p0 = static_cast<long long>(blockIdx.x) * kReg512Waves; p0 < ncx; p0 += step) {
+    const long long p = p0 + w;
+    const bool live = p < ncx;
+    const long long qa = 2 * p, qb = 2 * p + 1;
+    const R* ra = mesh + (live ? qa : 0) * 1024;
+    const bool hasb = live && qb < nrows;
+    const R* rb = mesh + (hasb ? qb : 0) * 1024;
+    Cx<R> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int e = lane + 64 * r;
+      v[r] = {live ? ra[e] : R(0), hasb ? rb[e] : R(0)};
+    }
+    filter1024_lane<R>(v, xb, stw, gain, lane);
+    if (live) {
+      R* wa = mesh + qa * 1024;
+#pragma unroll
+      for (int r = 0; r < 16; r++) wa[lane + 64 * r] = v[r].re;
+      if (hasb) {
+        R* wb = mesh + qb * 1024;
+#pragma unroll
+        for (int r = 0; r < 16; r++) wb[lane + 64 * r] = v[r].im;
+      }
+    }
+  }
+}
+
+// Lines of 1024 cells along x or y, as k_smooth_reg512s: 16 adjacent real
+// lines staged through an LDS tile [element][line] (139 KB in FP64), which
+// then serves as the 8 wavefronts' transposition buffers.
+template <typename R>
+__global__ __launch_bounds__(64 * kReg512SWaves) void k_smooth_reg1024s(
+    R* __restrict__ mesh, long long nreal, int I, long long ostride, long long es,
+    const Cx<R>* __restrict__ tw_global, const R* __restrict__ gain) {
+  constexpr int NT = 64 * kReg512SWaves;
+  __shared__ Cx<R> stw[1024];
+  __shared__ R tile[1024 * kTileRow];
+  static_assert(sizeof(R) * 1024 * kTileRow >= sizeof(Cx<R>) * 512 * kReg512SWaves, "tile");
+  for (int k = threadIdx.x; k < 1024; k += NT) stw[k] = tw_global[k];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  Cx<R>* xb = reinterpret_cast<Cx<R>*>(tile) + 512 * w;
+  const int c = threadIdx.x & 15, e0 = threadIdx.x >> 4;  // load/store: line c, elements e0 + 32 k
+  const long long ngroups = nreal / 16;
+  for (long long g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const long long q0 = 16 * g;
+    R* base = mesh + (q0 / I) * ostride + (q0 % I);
+    __syncthreads();  // the tile is free (previous group stored, twiddles staged)
+#pragma unroll 4
+    for (int k = 0; k < 32; k++) {
+      const int e = e0 + 32 * k;
+      tile[e * kTileRow + c] = base[static_cast<long long>(e) * es + c];
+    }
+    __syncthreads();
+    Cx<R> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const R* t = tile + (lane + 64 * r) * kTileRow + 2 * w;
+      v[r] = {t[0], t[1]};
+    }
+    __syncthreads();  // the tile becomes the transposition buffers
+    filter1024_lane<R>(v, xb, stw, gain, lane);  // ends with a barrier
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      R* t = tile + (lane + 64 * r) * kTileRow + 2 * w;
+      t[0] = v[r].re;
+      t[1] = v[r].im;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 32; k++) {
       const int e = e0 + 32 * k;
       base[static_cast<long long>(e) * es + c] = tile[e * kTileRow + c];
     }
@@ -646,7 +777,16 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
       PRH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_smooth_axis<R, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     }
-    if (d == 2 && n == 512 && PRH_SMOOTH_REG512) {
+    if (d == 2 && n == 1024 && PRH_SMOOTH_REG512) {
+      const long long ncx = (nreal + 1) / 2;
+      const long long nb = (ncx + kReg512Waves - 1) / kReg512Waves;
+      const unsigned g1024 = static_cast<unsigned>(nb < 8192 ? nb : 8192);
+      k_smooth_reg1024<R><<<g1024, 64 * kReg512Waves, 0, s>>>(mesh, nreal, tw, gain);
+    } else if (n == 1024 && PRH_SMOOTH_REG512 && I % 16 == 0 && istride == 1) {
+      const long long ngroups = nreal / 16;
+      const unsigned g1024 = static_cast<unsigned>(ngroups < 8192 ? ngroups : 8192);
+      k_smooth_reg1024s<R><<<g1024, 64 * kReg512SWaves, 0, s>>>(mesh, nreal, I, ostride, es, tw, gain);
+    } else if (d == 2 && n == 512 && PRH_SMOOTH_REG512) {
       const long long ncx = (nreal + 1) / 2;
       const long long nb = (ncx + kReg512Waves - 1) / kReg512Waves;
       const unsigned g512 = static_cast<unsigned>(nb < 8192 ? nb : 8192);

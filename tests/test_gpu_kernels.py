@@ -156,8 +156,10 @@ SMOOTH_SHAPES = [(8, 8, 8), (8, 12, 20), (6, 10, 14), (15, 9, 25), (10, 32, 128)
                  # of rows), x and y (16 adjacent rows per tile), and x with a z extent that is not a
                  # multiple of 16 (the LDS transform)
                  (4, 6, 512), (3, 5, 512), (512, 2, 16), (2, 512, 32), (512, 3, 5),
-                 # 1024-cell lines along each axis (LDS transforms)
-                 (2, 4, 1024), (1024, 2, 8), (4, 1024, 2)]
+                 # 1024-cell lines: in registers along z (an even and an odd number of rows) and
+                 # along x and y (16 adjacent rows per tile), and the LDS transform along x and y
+                 # with a z extent that is not a multiple of 16
+                 (2, 4, 1024), (3, 5, 1024), (1024, 2, 16), (2, 1024, 32), (1024, 2, 8), (4, 1024, 2)]
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
