@@ -2,7 +2,7 @@
 
 Stage by stage on the configuration-1 catalogs (nmesh=64, 1e5 data, 1e6 randoms):
 - CIC meshes after painting Cartesian positions: bit-exact;
-- density contrast: relative L2 <= 1e-12 (hipFFT R2C vs numpy's c2c FFT);
+- density contrast: relative L2 <= 1e-12 (the library's separable Fourier smoothing vs numpy's c2c FFT);
 - potential (mean removed): relative L2 <= 1e-10;
 - shifted positions: max |diff| <= 1e-8 Mpc/h; and bit-exact when the GPU reads
   the oracle's own potential.
