@@ -652,6 +652,10 @@ __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
   restrict_body<R, RO>(u, c, f0, f1, f2);
 }
 
+// FP32 transfers with two coarse cells per thread (16-byte fine accesses)
+#ifndef PRH_TRANSFER_QUADS
+#define PRH_TRANSFER_QUADS 1
+#endif
 template <typename R> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
@@ -695,6 +699,55 @@ __global__ __launch_bounds__(kThreads) void k_restrict_rows(const R* __restrict_
   if (in)
     c[(static_cast<size_t>(jx) * c1 + jy) * c2 + jz] =
         static_cast<RO>(restrict_sum<R>([&](int a, int b, int q) { return w[a][b][q]; }));
+}
+
+// FP32 restriction with two coarse z per thread (coarse z = 2 jq, 2 jq + 1):
+// each of the 9 fine rows is read as one float4 (fine z = 4 jq .. 4 jq + 3),
+// fine z = 4 jq - 1 comes from the previous lane's last value (read directly
+// by the first thread of each block row), and the two coarse values are
+// stored as one float2.  The sums are restrict_sum()'s, so the bits are the
+// pair kernel's.  Requires f2 % 4 == 0, the fine mesh on a 16-byte and the
+// coarse mesh on an 8-byte boundary.
+template <typename RO>
+__global__ __launch_bounds__(kThreads) void k_restrict_quads(const float* __restrict__ u,
+                                                             RO* __restrict__ c, int f0, int f1,
+                                                             int f2) {
+  using VO = typename Pair<RO>::type;
+  const int c1 = f1 / 2, c2 = f2 / 2, q2 = c2 / 2;
+  const int jq0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jy0 = blockIdx.y * blockDim.y + threadIdx.y;
+  const int jx = blockIdx.z;
+  const bool in = jq0 < q2 && jy0 < c1;
+  // lanes past the edge compute a clamped cell (every lane takes part in the shuffles)
+  const int jq = jq0 < q2 ? jq0 : q2 - 1, jy = jy0 < c1 ? jy0 : c1 - 1;
+  const size_t s0 = static_cast<size_t>(f1) * f2, s1 = static_cast<size_t>(f2);
+  const int fx = 2 * jx, fy = 2 * jy, fz = 4 * jq;
+  const size_t xo[3] = {(fx == 0 ? f0 - 1 : fx - 1) * s0, fx * s0, (fx + 1 == f0 ? 0 : fx + 1) * s0};
+  const size_t yo[3] = {(fy == 0 ? f1 - 1 : fy - 1) * s1, fy * s1, (fy + 1 == f1 ? 0 : fy + 1) * s1};
+  const size_t zm = static_cast<size_t>(fz == 0 ? f2 - 1 : fz - 1);
+  const bool first = threadIdx.x == 0;
+  float w0[3][3][3], w1[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const float* row = u + xo[a] + yo[b];
+      const float4 q = *reinterpret_cast<const float4*>(row + fz);
+      const float left = __shfl_up(q.w, 1);
+      w0[a][b][0] = first ? row[zm] : left;
+      w0[a][b][1] = q.x;
+      w0[a][b][2] = q.y;
+      w1[a][b][0] = q.y;
+      w1[a][b][1] = q.z;
+      w1[a][b][2] = q.w;
+    }
+  // fine z = 4 jq + 3 is the last of the quad and never wraps (f2 % 4 == 0)
+  if (in) {
+    VO out;
+    out.x = static_cast<RO>(restrict_sum<float>([&](int a, int b, int k) { return w0[a][b][k]; }));
+    out.y = static_cast<RO>(restrict_sum<float>([&](int a, int b, int k) { return w1[a][b][k]; }));
+    *reinterpret_cast<VO*>(c + (static_cast<size_t>(jx) * c1 + jy) * c2 + 2 * jq) = out;
+  }
 }
 
 // Trilinear prolongation from the coarse mesh (c0, c1, c2) to the fine mesh,
@@ -800,6 +853,66 @@ __global__ __launch_bounds__(kThreads) void k_prolong_rows(const RC* __restrict_
   } else {
     w.x = even;
     w.y = odd;
+  }
+  *dst = w;
+}
+
+// FP32 prolongation with two coarse z per thread: fine z = 4 jq .. 4 jq + 3
+// (the even and odd cells of coarse z = 2 jq and 2 jq + 1) written as one
+// float4; each value is formed with the operands and order of prolong_at().
+// Requires c2 even and the fine mesh on a 16-byte boundary.
+template <typename RC, bool ADD>
+__global__ __launch_bounds__(kThreads) void k_prolong_quads(const RC* __restrict__ uc,
+                                                            float* __restrict__ fine, int c0,
+                                                            int c1, int c2) {
+  using R = float;
+  const int F1 = 2 * c1, F2 = 2 * c2;
+  const int jq = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int X = blockIdx.z;
+  if (2 * jq >= c2 || Y >= F1) return;
+  const int jx = X >> 1, jy = Y >> 1;
+  const size_t s0 = static_cast<size_t>(c1) * c2, s1 = static_cast<size_t>(c2);
+  const size_t x0 = jx * s0, xp = (jx + 1 == c0 ? 0 : jx + 1) * s0;
+  const size_t y0 = jy * s1, yp = (jy + 1 == c1 ? 0 : jy + 1) * s1;
+  const size_t z0 = 2 * jq, z1 = z0 + 1, z2 = (z1 + 1 == static_cast<size_t>(c2) ? 0 : z1 + 1);
+  auto u = [uc](size_t idx) { return static_cast<R>(uc[idx]); };
+  // even and odd fine values of coarse z = zc, whose next coarse z is zn
+  auto cell = [&](size_t zc, size_t zn, R& even, R& odd) {
+    const R a = u(x0 + y0 + zc), az = u(x0 + y0 + zn);
+    if (X & 1) {
+      const R ax = u(xp + y0 + zc), axz = u(xp + y0 + zn);
+      if (Y & 1) {
+        const R ay = u(x0 + yp + zc), ayz = u(x0 + yp + zn);
+        const R axy = u(xp + yp + zc), axyz = u(xp + yp + zn);
+        even = (a + ax + ay + axy) / R(4);
+        odd = (a + ax + ay + az + axy + axz + ayz + axyz) / R(8);
+      } else {
+        even = (a + ax) / R(2);
+        odd = (a + ax + az + axz) / R(4);
+      }
+    } else if (Y & 1) {
+      const R ay = u(x0 + yp + zc), ayz = u(x0 + yp + zn);
+      even = (a + ay) / R(2);
+      odd = (a + ay + az + ayz) / R(4);
+    } else {
+      even = a;
+      odd = (a + az) / R(2);
+    }
+  };
+  R e0, o0, e1, o1;
+  cell(z0, z1, e0, o0);
+  cell(z1, z2, e1, o1);
+  float4* dst = reinterpret_cast<float4*>(fine + (static_cast<size_t>(X) * F1 + Y) * F2 + 4 * jq);
+  float4 w;
+  if (ADD) {
+    w = *dst;
+    w.x += e0;
+    w.y += o0;
+    w.z += e1;
+    w.w += o1;
+  } else {
+    w = make_float4(e0, o0, e1, o1);
   }
   *dst = w;
 }
@@ -1121,6 +1234,17 @@ static int launch_restrict(const R* fine, RO* coarse, const size_t* nf, hipStrea
   const int nc[3] = {static_cast<int>(nf[0] / 2), static_cast<int>(nf[1] / 2),
                      static_cast<int>(nf[2] / 2)};
   dim3 block, grid;
+  if constexpr (std::is_same<R, float>::value) {
+    if (PRH_TRANSFER_QUADS && nf[2] % 4 == 0 && reinterpret_cast<uintptr_t>(fine) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(coarse) % (2 * sizeof(RO)) == 0) {
+      const int nq[3] = {nc[0], nc[1], nc[2] / 2};
+      generic_shape(nq, block, grid);
+      k_restrict_quads<RO><<<grid, block, 0, s>>>(fine, coarse, static_cast<int>(nf[0]),
+                                                  static_cast<int>(nf[1]), static_cast<int>(nf[2]));
+      PRH_LAUNCHED();
+      return PRH_OK;
+    }
+  }
   generic_shape(nc, block, grid);
   k_restrict_rows<R, RO><<<grid, block, 0, s>>>(fine, coarse, static_cast<int>(nf[0]),
                                                 static_cast<int>(nf[1]), static_cast<int>(nf[2]));
@@ -1142,9 +1266,22 @@ static int launch_prolong(const RC* coarse, R* fine, const size_t* nc, bool add,
     PRH_LAUNCHED();
     return PRH_OK;
   }
+  dim3 block, grid;
+  if constexpr (std::is_same<R, float>::value) {
+    if (PRH_TRANSFER_QUADS && c2 % 2 == 0 && reinterpret_cast<uintptr_t>(fine) % 16 == 0) {
+      // threads over (coarse z pair, fine y, fine x)
+      const int shape[3] = {2 * c0, 2 * c1, c2 / 2};
+      generic_shape(shape, block, grid);
+      if (add)
+        k_prolong_quads<RC, true><<<grid, block, 0, s>>>(coarse, fine, c0, c1, c2);
+      else
+        k_prolong_quads<RC, false><<<grid, block, 0, s>>>(coarse, fine, c0, c1, c2);
+      PRH_LAUNCHED();
+      return PRH_OK;
+    }
+  }
   // threads over (coarse z, fine y, fine x)
   const int shape[3] = {2 * c0, 2 * c1, c2};
-  dim3 block, grid;
   generic_shape(shape, block, grid);
   if (add)
     k_prolong_rows<RC, R, true><<<grid, block, 0, s>>>(coarse, fine, c0, c1, c2);
