@@ -13,6 +13,9 @@ The configuration file has the reference's sections and keys (``input``, ``outpu
 - outputs keep the requested input columns and add the input and reconstructed positions
   under the ``xyz``/``rdz`` and ``xyz_rec``/``rdz_rec`` names.
 
+The mask and weight expressions are evaluated with Python's ``eval`` over numpy and the catalog
+columns, as the reference does: a configuration file is trusted input.
+
 Catalog files are numpy ``.npz`` (one array per column) or ``.npy`` (a structured array); FITS and
 HDF5 files are read and written when fitsio or h5py is importable.  Redshift <-> distance uses a
 flat LCDM cosmology with ``cosmology.Omega_m`` (the reference uses astropy's FlatLambdaCDM).

@@ -7,9 +7,6 @@ of the displacement (reduced by beta / (1 + beta) in the first iteration) is the
 original positions.  Paint, smoothing, density contrast, transforms, Fourier-space operators and
 CIC reads run in the HIP library; the per-point RSD algebra is the reference's numpy.
 """
-import ctypes
-import logging
-
 import numpy as np
 
 from . import _lib
