@@ -62,3 +62,35 @@ def test_cli_matches_direct_calls(cats, name, convention, los):
         expect_r = recon.read_shifted_positions(randoms, field='disp+rsd' if convention == 'RecSym' else 'disp')
     assert np.array_equal(out_d['POSITION_REC'], expect_d)
     assert np.array_equal(out_r['POSITION_REC'], expect_r)
+
+
+def test_cli_rdz_round_trip(cats):
+    # RA, Dec, redshift in and out: the reconstructed positions written as rdz_rec convert back to the
+    # direct calls' Cartesian positions (flat LCDM distance, spline inversion of the redshift)
+    import pyrecon_amd
+    from pyrecon_amd import cli, utils
+    path, data, wdata, randoms, boxsize, boxcenter = cats
+    dist_of_z = utils.FlatLCDMDistance(0.3)
+    z_of_dist = utils.DistanceToRedshift(dist_of_z)
+    dist, ra, dec = utils.cartesian_to_sky(data)
+    np.savez(path / 'data_rdz.npz', RA=ra, DEC=dec, Z=z_of_dist(dist), WEIGHT=wdata)
+    cfg = _config(path, boxsize, boxcenter, 'MultiGridReconstruction', 'RecSym')
+    cfg['input'].update({'data_fn': 'data_rdz.npz', 'rdz_data': ['RA', 'DEC', 'Z']})
+    cfg['input'].pop('xyz_data')
+    cfg['output'].update({'rdz_rec_data': ['RA_REC', 'DEC_REC', 'Z_REC'], 'xyz_data': 'POSITION',
+                          'columns_data': ''})
+    cfg['cosmology']['Omega_m'] = 0.3
+    cli.main([], **cfg)
+    out = np.load(path / 'data_rec.npz')
+    # the input positions as the CLI reconstructed them from RA, Dec, z
+    pos_in = out['POSITION']
+    assert np.max(np.abs(pos_in - data)) < 1e-6
+    recon = pyrecon_amd.MultiGridReconstruction(f=0.8, bias=2., los='local', nmesh=64, boxsize=boxsize,
+                                                boxcenter=boxcenter, dtype='f8')
+    recon.assign_data(pos_in, wdata)
+    recon.assign_randoms(randoms)
+    recon.set_density_contrast(smoothing_radius=15.)
+    recon.run()
+    expect = recon.read_shifted_positions(pos_in)
+    back = utils.sky_to_cartesian(dist_of_z(out['Z_REC']), out['RA_REC'], out['DEC_REC'])
+    assert np.max(np.abs(back - expect)) < 1e-6
