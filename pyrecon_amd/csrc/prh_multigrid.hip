@@ -333,6 +333,9 @@ constexpr int kRowsY = 8;
 #ifndef PRH_ROWS_BLOCKS
 #define PRH_ROWS_BLOCKS 512
 #endif
+#ifndef PRH_ROWS_MIN_XLEN
+#define PRH_ROWS_MIN_XLEN 4  // shortest x run of a whole-row block
+#endif
 #ifndef PRH_ROWS_MIN_WAVES
 #define PRH_ROWS_MIN_WAVES 1
 #endif
@@ -501,7 +504,7 @@ inline int rows_xlen(const int* n) {
   long chunks = (PRH_ROWS_BLOCKS + tiles - 1) / tiles;
   if (chunks > n[0]) chunks = n[0];
   int xlen = static_cast<int>((n[0] + chunks - 1) / chunks);
-  return xlen < 4 ? (n[0] < 4 ? n[0] : 4) : xlen;
+  return xlen < PRH_ROWS_MIN_XLEN ? (n[0] < PRH_ROWS_MIN_XLEN ? n[0] : PRH_ROWS_MIN_XLEN) : xlen;
 }
 
 inline bool tiled_ok(const int* n) { return n[2] % kTileZ == 0 && n[1] % kTileY == 0 && n[0] >= 1; }
