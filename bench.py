@@ -55,6 +55,8 @@ def parse_args():
     p.add_argument('--serving-jobs', type=int, default=256, help='jobs of the serving stream (0: skip it)')
     p.add_argument('--serving-rate', type=float, default=4.0, help='Poisson arrival rate of the serving stream (jobs/s)')
     p.add_argument('--serving-only', action='store_true', help='run only the serving stream')
+    p.add_argument('--host-io-steps', type=int, default=2,
+                   help='timed mixed steps with numpy inputs and outputs (PCIe-inclusive rate; 0: skip)')
     p.add_argument('--catalog-cache', default=None,
                    help='directory where the generated catalogs are saved and, on later runs, loaded from')
     return p.parse_args()
@@ -456,6 +458,31 @@ def main():
     stages_plain = {}
     step('plain', stages=stages_plain)
 
+    host_io = None
+    if args.host_io_steps > 0:
+        # end to end from numpy: catalogs uploaded and shifted positions returned inside each step
+        from catalogs import RECON_KWARGS, SMOOTHING_RADIUS
+
+        def host_step():
+            recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws)
+            recon.assign_data(cat['data'], cat['data_weights'])
+            recon.assign_randoms(cat['randoms'], cat['randoms_weights'])
+            recon.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+            recon.run(precision='mixed', nrefine=args.nrefine)
+            return recon.read_shifted_positions(cat['data']), recon.read_shifted_positions(cat['randoms'])
+
+        host_step()
+        P.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.host_io_steps): host_step()
+        P.synchronize()
+        th = (time.perf_counter() - t0) / args.host_io_steps
+        host_io = dict(value=round(1. / th, 4), unit='recon/s', ms_per_step=round(1e3 * th, 2),
+                       steps=args.host_io_steps, precision='mixed',
+                       note='catalogs uploaded from numpy (pageable host memory) and shifted positions of data '
+                            'and randoms returned to numpy inside each step: PCIe-inclusive, not the headline value')
+        log('host io', host_io)
+
     accuracy = None
     if not args.no_accuracy:
         # mixed vs plain FP64 (the plain FP64 solve is bit-identical to the reference's C multigrid)
@@ -509,7 +536,7 @@ def main():
                 fp64_plain=dict(value=round(value64, 4), unit='recon/s', ms_per_step=round(1e3 * elapsed64 / nfp64, 3),
                                 jacobi_avg_launch_ms=round(ms_j64 / max(nj64, 1), 5),
                                 jacobi_achieved_gbs=round(bytes_j64 / (ms_j64 * 1e-3) / 1e9, 1) if ms_j64 else None),
-                speedup_mixed_vs_fp64=round(value / value64, 3), accuracy=accuracy,
+                speedup_mixed_vs_fp64=round(value / value64, 3), accuracy=accuracy, host_io=host_io,
                 stages_ms=dict(mixed={k: round(1e3 * v, 2) for k, v in stages_mixed.items()},
                                plain={k: round(1e3 * v, 2) for k, v in stages_plain.items()}),
                 serving=serving)
