@@ -62,44 +62,16 @@ def parse_args():
     return p.parse_args()
 
 
-def serving_stream(njobs, rate, seed=2024):
-    """Configuration 5: Poisson arrivals; nmesh in {128, 256, 512} with probabilities
-    {0.5, 0.35, 0.15}; boxsize = 7.8 nmesh Mpc/h centred at (0, 0, boxsize); data count
-    lognormal with median 1e6 and sigma(ln N) = 1, clipped to [1e5, 2e7]; randoms 10x."""
-    rng = np.random.RandomState(seed)
-    arrivals = np.cumsum(rng.exponential(1. / rate, njobs))
-    nmesh = rng.choice([128, 256, 512], size=njobs, p=[0.5, 0.35, 0.15])
-    ndata = np.clip(np.exp(np.log(1e6) + rng.normal(size=njobs)), 1e5, 2e7).astype(np.int64)
-    return [dict(arrival=float(a), nmesh=int(n), ndata=int(d), seed=int(seed * 1000 + i))
-            for i, (a, n, d) in enumerate(zip(arrivals, nmesh, ndata))]
-
-
 def serving_bench(args, precision, stream_spec, keep=None, reference=None):
     """Run the serving stream through one ReconServer; returns the report and, for the jobs in
     ``keep``, the shifted data positions (host); with ``reference``, errors against it."""
-    from pyrecon_amd.serving import ReconServer, ReconJob, synthetic_catalog, zeldovich_modes
-    from catalogs import RECON_KWARGS, SMOOTHING_RADIUS
+    from pyrecon_amd.serving import ReconServer
+    from catalogs import serving_job
     keep = set(keep or [])
     kept, errors = {}, []
 
     def make_job(i, spec):
-        n, nd = spec['nmesh'], spec['ndata']
-        boxsize = 7.8 * n
-        boxcenter = (0., 0., boxsize)
-        modes = zeldovich_modes([boxsize] * 3, [n] * 3, spec['seed'])
-
-        def data(stream):
-            return synthetic_catalog(nd, boxsize, boxcenter, 2 * spec['seed'], modes=modes, f=RECON_KWARGS['f'],
-                                     weights=(0.5, 1.5), stream=stream)
-
-        def randoms(stream):
-            return synthetic_catalog(10 * nd, boxsize, boxcenter, 2 * spec['seed'] + 1, weights=(1., 1.),
-                                     stream=stream)
-
-        return ReconJob(data, randoms, ndata=nd, nrandoms=10 * nd, nmesh=n, boxsize=boxsize, boxcenter=boxcenter,
-                        f=RECON_KWARGS['f'], bias=RECON_KWARGS['bias'], los=RECON_KWARGS['los'], dtype='f8',
-                        smoothing_radius=SMOOTHING_RADIUS, precision=precision, shifted=('data', 'randoms'),
-                        field='disp+rsd', randoms_field='disp+rsd', to_host=False, name=str(i))
+        return serving_job(spec, precision, name=str(i))
 
     def done(i, fut):
         res = fut.result()
@@ -138,6 +110,7 @@ def serving_bench(args, precision, stream_spec, keep=None, reference=None):
 
 
 def run_serving(args):
+    from catalogs import serving_stream
     spec = serving_stream(args.serving_jobs, args.serving_rate)
     keep = [i for i, s in enumerate(spec) if s['ndata'] <= 4 * 10**6][::8]
     log('serving stream: {:d} jobs at {:.1f}/s'.format(len(spec), args.serving_rate))
@@ -227,7 +200,7 @@ def recon_step(P, ws, geo, catalogs, outs, precision, nrefine=0, stages=None):
             stages[name] = stages.get(name, 0.) + time.perf_counter() - mark.t
             mark.t = time.perf_counter()
     mark.t = time.perf_counter()
-    recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws)
+    recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws, precision=precision)
     recon.assign_data(d, dw)
     recon.assign_randoms(r, rw)
     mark('paint')
@@ -464,7 +437,7 @@ def main():
         from catalogs import RECON_KWARGS, SMOOTHING_RADIUS
 
         def host_step():
-            recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws)
+            recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws, precision='mixed')
             recon.assign_data(cat['data'], cat['data_weights'])
             recon.assign_randoms(cat['randoms'], cat['randoms_weights'])
             recon.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)

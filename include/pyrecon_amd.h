@@ -175,6 +175,17 @@ int prh_paint_f64(prh_workspace *ws, double *mesh, const size_t *nmesh, const do
                   const void *weights, int weight_dtype, size_t npositions, int wrap,
                   void *stream);
 
+/* the mixed-precision variant of prh_paint_f64 (same arguments and checks;
+ * replaces the same assign_cic, mesh.c:26, with FP32 corner weights): points
+ * are sorted by 8^3-cell tile, each tile sums its FP32 contributions exactly
+ * as 64-bit integers at the scale 2^(62 - e) of its own weight bound 2^e, and
+ * tiles are combined in FP64 in a fixed order, so repeated calls give identical
+ * bits (not the reference's).  Needs PRH_WS_PAINT | PRH_WS_MIXED.          */
+int prh_paint_mixed_f64(prh_workspace *ws, double *mesh, const size_t *nmesh,
+                        const double *boxsize, const double *boxcenter, const void *positions,
+                        int pos_dtype, const void *weights, int weight_dtype, size_t npositions,
+                        int wrap, void *stream);
+
 /* replaces RealMesh.smooth_gaussian(method='fft') (mesh.py:655-680): in-place
  * multiplication of the spectrum by exp(-(R k)^2 / 2), done as a 1-D
  * forward transform, gain and inverse transform along each axis in turn.   */

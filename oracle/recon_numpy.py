@@ -48,8 +48,11 @@ class OracleMultiGridReconstruction(object):
     ``nmesh``, ``boxsize`` and ``boxcenter``."""
 
     def __init__(self, f=0., bias=1., los=None, nmesh=None, boxsize=None, boxcenter=None, dtype='f8', wrap=False,
-                 kernels=None):
+                 kernels=None, fft_workers=None):
         self.f = f
+        # fft_workers: None -> numpy.fft (the reference's engine); an int -> scipy.fft with that many
+        # threads, the same pocketfft algorithm, for meshes where numpy's single thread is too slow
+        self.fft_workers = fft_workers
         self.bias = bias
         self.wrap = wrap
         self.dtype = np.dtype(dtype)
@@ -130,7 +133,7 @@ class OracleMultiGridReconstruction(object):
         radius_ = np.empty_like(self.boxsize, order='C')
         radius_[:] = radius
         ctype = np.dtype('c{:d}'.format(2 * self.dtype.itemsize))
-        valuek = np.fft.fftn(value).astype(ctype, copy=False)
+        valuek = self._fftn(value).astype(ctype, copy=False)
         coords = []
         for n, d in zip(self.nmesh, self.boxsize / self.nmesh):
             coords.append(2 * np.pi * np.fft.fftfreq(n, d=d))
@@ -141,8 +144,15 @@ class OracleMultiGridReconstruction(object):
             grids.append(k[tuple(sl)])
         k2 = sum(-0.5 * (r * k)**2 for r, k in zip(radius_, grids))
         valuek *= np.exp(k2)
-        real = np.fft.ifftn(valuek).astype(ctype, copy=False).real
+        real = self._fftn(valuek, inverse=True).astype(ctype, copy=False).real
         return real.astype(self.dtype, copy=False)
+
+    def _fftn(self, value, inverse=False):
+        workers = getattr(self, 'fft_workers', None)
+        if workers is None:
+            return (np.fft.ifftn if inverse else np.fft.fftn)(value)
+        import scipy.fft
+        return (scipy.fft.ifftn if inverse else scipy.fft.fftn)(value, workers=int(workers))
 
     def set_density_contrast(self, ran_min=0.01, smoothing_radius=15.):
         data = self.smooth_gaussian(self.mesh_data, smoothing_radius)

@@ -1,359 +1,371 @@
-"""Command-line reconstruction of catalogs (pyrecon's bin/pyrecon:70-423).
+"""Command-line reconstruction of catalog files.
 
-    python -m pyrecon_amd.cli config.yaml [--data-fn F] [--randoms-fn F] [--output-data-fn F]
-                                          [--output-randoms-fn F] [--growth-rate f] [--galaxy-bias b]
+    python -m pyrecon_amd.cli [config.yaml] [--data-fn F] [--randoms-fn F] [--output-data-fn F]
+                              [--output-randoms-fn F] [--growth-rate f] [--galaxy-bias b]
 
-The configuration file has the reference's sections and keys (``input``, ``output``,
-``algorithm``, ``delta``, ``cosmology``, ``mesh``; bin/config_example.yaml):
-- positions come from ``xyz`` (1 or 3 columns) or ``rdz`` (RA, Dec, redshift) columns, with
-  ``_data`` / ``_randoms`` variants;
-- ``weights`` and ``mask`` are expressions over ``${column}`` names, evaluated with numpy;
-- the ``convention`` is RecSym (data and randoms shifted by 'disp+rsd'), RecIso (randoms by
-  'disp') or RSD (data by 'rsd', randoms unchanged);
-- outputs keep the requested input columns and add the input and reconstructed positions
-  under the ``xyz``/``rdz`` and ``xyz_rec``/``rdz_rec`` names.
+The configuration file has the sections and keys of pyrecon's command line (bin/pyrecon:70-423,
+bin/config_example.yaml), so its configuration files run unchanged:
 
-The mask and weight expressions are evaluated with Python's ``eval`` over numpy and the catalog
-columns, as the reference does: a configuration file is trusted input.
+- ``input``: ``data_fn``, ``randoms_fn`` and ``dir``; positions as ``xyz`` (one (N, 3) column or
+  three columns) or ``rdz`` (RA, Dec, redshift) columns, each key optionally suffixed ``_data`` /
+  ``_randoms``; ``weights`` and ``mask`` expressions over ``${column}`` names (suffixable too);
+  ``hdf5_prefix``.
+- ``output``: ``data_fn``, ``randoms_fn``, ``dir``; ``columns`` (suffixable) to carry over from the
+  input; ``xyz`` / ``rdz`` columns for the input positions and ``xyz_rec`` / ``rdz_rec`` for the
+  reconstructed ones (suffixable); ``hdf5_prefix``.
+- ``algorithm``: ``name`` (a reconstruction class of this package), ``convention`` (RecSym: data
+  and randoms shifted by 'disp+rsd'; RecIso: randoms by 'disp'; RSD: data by 'rsd', randoms
+  unchanged), ``los``, ``nthreads``; any other key is passed to ``run``.
+- ``delta``: ``selection_function`` ('randoms' or 'uniform'; by default 'randoms' when a randoms
+  file is given), the rest is passed to ``set_density_contrast``.
+- ``cosmology``: ``f``, ``bias``, and ``Omega_m`` for redshift <-> distance.
+- ``mesh``: constructor keywords (``nmesh``, ``boxsize``, ``boxcenter``, ``dtype``, ``wrap``, ...).
 
-Catalog files are numpy ``.npz`` (one array per column) or ``.npy`` (a structured array); FITS and
-HDF5 files are read and written when fitsio or h5py is importable.  Redshift <-> distance uses a
-flat LCDM cosmology with ``cosmology.Omega_m`` (the reference uses astropy's FlatLambdaCDM).
+Settings are layered: built-in defaults, then keyword arguments of :func:`main`, then the file,
+then the command-line options.  An empty value (empty string, list or mapping) means "not set".
+Where a suffixed key is present it replaces the plain one, even when it is empty (an empty
+``weights_randoms`` gives unit weights to the randoms whatever ``weights`` says).
+
+Mask and weight expressions are Python expressions over numpy (``np``) and the catalog columns,
+evaluated with ``eval``: a configuration file is trusted input, as it is for the reference.
+Catalog files are read and written by :mod:`pyrecon_amd.catalog_io`.
 """
 import argparse
+import copy
 import logging
 import os
 import re
-from numbers import Number
 
 import numpy as np
 
-from . import utils
+from . import catalog_io, utils
 
 logger = logging.getLogger('pyrecon_amd.cli')
 
-EXTENSIONS = {'npz': ['.npz'], 'npy': ['.npy'], 'fits': ['.fits'],
-              'hdf5': ['.hdf', '.h4', '.hdf4', '.he2', '.h5', '.hdf5', '.he5', '.h5py']}
-ALLOWED_FORMATS = ['rdz', 'xyz']
-ALLOWED_CONVENTIONS = ['recsym', 'reciso', 'rsd']
-ALLOWED_SELECTION_FUNCTIONS = ['uniform', 'randoms', '']
-
-
-class ConfigError(Exception):
-    """Raised for an invalid configuration (bin/pyrecon:60-61)."""
-
-
-def _yaml_loader():
-    """yaml.SafeLoader that reads 1e-5-style floats and 'None' (bin/pyrecon:28-50)."""
-    import yaml
-
-    class Loader(yaml.SafeLoader):
-        pass
-
-    Loader.add_implicit_resolver(u'tag:yaml.org,2002:float', re.compile(u'''^(?:
-        [-+]?(?:[0-9][0-9_]*)\\.[0-9_]*(?:[eE][-+]?[0-9]+)?
-        |[-+]?(?:[0-9][0-9_]*)(?:[eE][-+]?[0-9]+)
-        |\\.[0-9_]+(?:[eE][-+][0-9]+)?
-        |[-+]?[0-9][0-9_]*(?::[0-5]?[0-9])+\\.[0-9_]*
-        |[-+]?\\.(?:inf|Inf|INF)
-        |\\.(?:nan|NaN|NAN))$''', re.X), list(u'-+0123456789.'))
-    Loader.add_implicit_resolver('!none', re.compile('None$'), first='None')
-    Loader.add_constructor('!none', lambda loader, node: None)
-    return Loader
-
-
-def default_config():
-    return {'input': {}, 'output': {}, 'algorithm': {'name': 'MultiGridReconstruction', 'convention': 'RecSym'},
+ROLES = ('data', 'randoms')
+POSITION_KINDS = ('rdz', 'xyz')
+# convention -> (field of the data shifts, field of the randoms shifts; None: randoms not moved)
+CONVENTIONS = {'recsym': ('disp+rsd', 'disp+rsd'), 'reciso': ('disp+rsd', 'disp'), 'rsd': ('rsd', None)}
+SELECTION_FUNCTIONS = ('randoms', 'uniform')
+DEFAULTS = {'input': {}, 'output': {},
+            'algorithm': {'name': 'MultiGridReconstruction', 'convention': 'RecSym'},
             'delta': {'smoothing_radius': 15.}, 'cosmology': {}, 'mesh': {'nmesh': 512, 'dtype': 'f4'}}
 
 
-def update_config(c1, c2):
-    """Update c1 in place with c2, section by section; empty values become None."""
-    for section, value in c2.items():
-        c1.setdefault(section, {})
-        c1[section].update(value or {})
-    for section in c1:
-        for name, value in c1[section].items():
-            if not isinstance(value, Number) and not isinstance(value, np.ndarray) and not value:
-                c1[section][name] = None
+class ConfigError(Exception):
+    """The configuration cannot be run (the reference raises ``ConfigError`` for the same cases)."""
 
 
-def make_list(cols):
-    if cols is None: return []
-    if isinstance(cols, str): cols = [cols]
-    return list(cols)
+_FLOAT_TEXT = re.compile(r'^[-+]?(\d[\d_]*\.?[\d_]*|\.\d[\d_]*)([eE][-+]?\d+)?$')
 
 
-def remove_duplicates(cols):
-    toret = []
-    for col in cols:
-        if col not in toret: toret.append(col)
-    return toret
+def normalize(value):
+    """A configuration value as the run uses it: empty -> None; text spelling a number (YAML 1.1
+    reads '1e-5' as text) -> float; 'None' / 'none' -> None."""
+    if isinstance(value, str):
+        text = value.strip()
+        if text in ('', 'None', 'none'):
+            return None
+        if _FLOAT_TEXT.match(text) and ('.' in text or 'e' in text.lower()):
+            return float(text.replace('_', ''))
+        return value
+    if isinstance(value, (list, tuple, dict)) and len(value) == 0:
+        return None
+    return value
 
 
-def sepjoin(*paths):
-    """Join HDF5 dataset paths with single '/' separators."""
-    sep = '/'
-    toret = ''
-    for path in paths:
-        if path is None: continue
-        if toret.endswith(sep) and path.startswith(sep):
-            toret = path
-        elif (not toret.endswith(sep)) and (not path.startswith(sep)):
-            toret += sep + path
+class Settings(object):
+    """Configuration sections ``{section: {key: value}}`` merged from successive layers."""
+
+    def __init__(self, *layers):
+        self.sections = copy.deepcopy(DEFAULTS)
+        for layer in layers:
+            self.merge(layer)
+
+    def merge(self, layer):
+        for section, entries in (layer or {}).items():
+            target = self.sections.setdefault(section, {})
+            for key, value in (entries or {}).items():
+                target[key] = normalize(value)
+        return self
+
+    def section(self, name):
+        return self.sections.setdefault(name, {})
+
+    def get(self, section, key, default=None):
+        """The value of ``key`` if set (not None), else ``default``."""
+        value = self.section(section).get(key)
+        return default if value is None else value
+
+    def suffixed(self, section, key, role):
+        """``key_role`` if that key is present (even empty), else ``key``."""
+        entries = self.section(section)
+        specific = '{}_{}'.format(key, role)
+        return entries[specific] if specific in entries else entries.get(key)
+
+    def pop(self, section, key, default=None):
+        value = self.section(section).pop(key, None)
+        return default if value is None else value
+
+
+def as_columns(value):
+    """Column names as a list: None -> [], a name -> [name]."""
+    if value is None:
+        return []
+    if isinstance(value, str):
+        return [value]
+    return [str(v) for v in value]
+
+
+def unique(names):
+    return list(dict.fromkeys(names))
+
+
+class ColumnExpression(object):
+    """A mask or weight expression over ``${column}`` names.
+
+    Each ``${...}`` is replaced by a private variable bound to that column when evaluated, so any
+    column name works, including ones that are not Python identifiers.
+    """
+
+    _REF = re.compile(r'\$\{([^}]*)\}')
+
+    def __init__(self, text):
+        self.text = '' if text is None else str(text)
+        self.columns = unique(m.group(1) for m in self._REF.finditer(self.text))
+        slot = {name: '_col{:d}'.format(i) for i, name in enumerate(self.columns)}
+        self.code = self._REF.sub(lambda m: slot[m.group(1)], self.text)
+        self._slots = slot
+
+    def __bool__(self):
+        return bool(self.text.strip())
+
+    def evaluate(self, catalog):
+        namespace = {'np': np}
+        namespace.update({slot: catalog[name] for name, slot in self._slots.items()})
+        return eval(self.code, namespace, {})
+
+
+class Cosmology(object):
+    """Redshift <-> comoving distance (Mpc/h) from ``cosmology.Omega_m``, built when first needed."""
+
+    def __init__(self, Omega_m):
+        self.Omega_m = Omega_m
+        self._distance = self._redshift = None
+
+    def distance(self, z):
+        if self._distance is None:
+            if self.Omega_m is None:
+                raise ConfigError('rdz positions need cosmology.Omega_m (redshift <-> distance)')
+            self._distance = utils.FlatLCDMDistance(self.Omega_m)
+        return self._distance(z)
+
+    def redshift(self, dist):
+        if self._redshift is None:
+            self.distance(0.)
+            self._redshift = utils.DistanceToRedshift(self._distance)
+        return self._redshift(dist)
+
+
+class PositionColumns(object):
+    """Where a catalog keeps its positions: ``kind`` 'xyz' (one (N, 3) column or three columns) or
+    'rdz' (RA and Dec in degrees, redshift)."""
+
+    def __init__(self, kind, columns):
+        self.kind, self.columns = kind, as_columns(columns)
+        if kind == 'rdz' and len(self.columns) != 3:
+            raise ConfigError('rdz positions take 3 columns (RA, Dec, redshift), got {}'.format(self.columns))
+        if kind == 'xyz' and len(self.columns) not in (1, 3):
+            raise ConfigError('xyz positions take 1 column of shape (N, 3) or 3 columns, got {}'.format(self.columns))
+
+    def to_cartesian(self, catalog, cosmology):
+        cols = [catalog[c] for c in self.columns]
+        if self.kind == 'rdz':
+            return utils.sky_to_cartesian(cosmology.distance(cols[2]), cols[0], cols[1])
+        if len(cols) == 1:
+            return np.asarray(cols[0])
+        return np.stack(cols, axis=-1)
+
+    def from_cartesian(self, positions, cosmology):
+        if self.kind == 'rdz':
+            dist, ra, dec = utils.cartesian_to_sky(positions)
+            return dict(zip(self.columns, (ra, dec, cosmology.redshift(dist))))
+        if len(self.columns) == 1:
+            return {self.columns[0]: positions}
+        return {c: positions[:, i] for i, c in enumerate(self.columns)}
+
+
+def input_positions(settings, role):
+    """The input :class:`PositionColumns` of ``role``: a role-suffixed key wins over the plain ones."""
+    for keys in (['{}_{}'.format(kind, role) for kind in POSITION_KINDS], list(POSITION_KINDS)):
+        found = [(key.split('_')[0], settings.get('input', key)) for key in keys]
+        found = [(kind, cols) for kind, cols in found if cols is not None]
+        if len(found) > 1:
+            raise ConfigError('{} positions are given in two formats ({})'.format(role, ', '.join(k for k, _ in found)))
+        if found:
+            return PositionColumns(*found[0])
+    raise ConfigError('no position columns for the {}: give one of {}'.format(
+        role, ', '.join('{0} or {0}_{1}'.format(k, role) for k in POSITION_KINDS)))
+
+
+def output_positions(settings, role, reconstructed):
+    """The :class:`PositionColumns` under which ``role``'s input (or reconstructed) positions are written."""
+    tag = '_rec' if reconstructed else ''
+    out = []
+    for kind in POSITION_KINDS:
+        cols = settings.get('output', '{}{}_{}'.format(kind, tag, role))
+        if cols is None:
+            cols = settings.get('output', kind + tag)
+        if cols is not None:
+            out.append(PositionColumns(kind, cols))
+    return out
+
+
+class CatalogPlan(object):
+    """How one catalog (data or randoms) is read: file, positions, mask, weights, kept columns."""
+
+    def __init__(self, settings, role, path):
+        self.role, self.path = role, path
+        self.positions = input_positions(settings, role)
+        self.mask = ColumnExpression(settings.suffixed('input', 'mask', role))
+        self.weights = ColumnExpression(settings.suffixed('input', 'weights', role))
+        self.keep = as_columns(settings.suffixed('output', 'columns', role))
+        self.prefix = settings.get('input', 'hdf5_prefix', '')
+
+    def columns(self):
+        return unique(self.positions.columns + self.weights.columns + self.mask.columns + self.keep)
+
+    def load(self, cosmology):
+        """``(kept columns, positions (N, 3), weights or None)`` after the mask."""
+        names = self.columns()
+        try:
+            catalog = catalog_io.read_catalog(self.path, names, prefix=self.prefix)
+        except catalog_io.CatalogFormatError as exc:
+            raise ConfigError(str(exc)) from exc
+        logger.info('%s catalog %s: %d rows', self.role, self.path, len(catalog[names[0]]))
+        if self.mask:
+            selected = np.asarray(self.mask.evaluate(catalog))
+            catalog = {name: value[selected] for name, value in catalog.items()}
+            logger.info('%s catalog after the mask: %d rows', self.role, len(catalog[names[0]]))
+        positions = self.positions.to_cartesian(catalog, cosmology)
+        weights = self.weights.evaluate(catalog) if self.weights else None
+        kept = {name: catalog[name] for name in self.keep}
+        return kept, positions, weights
+
+
+def settings_from(args=None, **input_config):
+    """Parse the command line and layer defaults, ``input_config``, the file and the options."""
+    parser = argparse.ArgumentParser(prog='python -m pyrecon_amd.cli',
+                                     description='Reconstruct catalog files on the GPU.',
+                                     formatter_class=argparse.ArgumentDefaultsHelpFormatter)
+    parser.add_argument('config_fn', nargs='?', default=None, help='YAML configuration file')
+    parser.add_argument('--data-fn', help='input data file (overrides the configuration)')
+    parser.add_argument('--randoms-fn', help='input randoms file (overrides the configuration)')
+    parser.add_argument('--output-data-fn', help='output data file (overrides the configuration)')
+    parser.add_argument('--output-randoms-fn', help='output randoms file (overrides the configuration)')
+    parser.add_argument('--growth-rate', type=float, help='logarithmic growth rate f (overrides the configuration)')
+    parser.add_argument('--galaxy-bias', type=float, help='linear galaxy bias (overrides the configuration)')
+    opt = parser.parse_args(args=args)
+    settings = Settings(input_config)
+    if opt.config_fn is not None:
+        import yaml
+        logger.info('Reading configuration %s', opt.config_fn)
+        with open(opt.config_fn, 'r') as fh:
+            settings.merge(yaml.safe_load(fh))
+    overrides = [('input', 'data_fn', opt.data_fn), ('input', 'randoms_fn', opt.randoms_fn),
+                 ('output', 'data_fn', opt.output_data_fn), ('output', 'randoms_fn', opt.output_randoms_fn),
+                 ('cosmology', 'f', opt.growth_rate), ('cosmology', 'bias', opt.galaxy_bias)]
+    for section, key, value in overrides:
+        if value is not None:
+            settings.section(section)[key] = value
+    return settings
+
+
+def plan(settings):
+    """Resolve the catalogs of a run: ``(selection, {role: CatalogPlan}, {role: output path})``."""
+    selection = str(settings.pop('delta', 'selection_function', '')).lower()
+    if selection and selection not in SELECTION_FUNCTIONS:
+        raise ConfigError('selection_function must be one of {}, not "{}"'.format(SELECTION_FUNCTIONS, selection))
+    paths = {role: settings.get('input', role + '_fn') for role in ROLES}
+    if paths['data'] is None:
+        raise ConfigError('no data catalog: set input.data_fn')
+    if paths['randoms'] is None:
+        if selection == 'randoms':
+            raise ConfigError('selection_function "randoms" needs a randoms catalog (input.randoms_fn)')
+        selection = 'uniform'
+    elif not selection:
+        selection = 'randoms'
+    outputs = {}
+    for role in ROLES:
+        path = settings.get('output', role + '_fn')
+        if path is None:
+            continue
+        if paths[role] is None:
+            raise ConfigError('cannot write a {0} catalog without an input {0} catalog'.format(role))
+        outputs[role] = os.path.join(settings.get('output', 'dir', ''), path)
+    in_dir = settings.get('input', 'dir', '')
+    plans = {role: CatalogPlan(settings, role, os.path.join(in_dir, path)) for role, path in paths.items()
+             if path is not None}
+    return selection, plans, outputs
+
+
+def reconstruct(settings, selection, inputs):
+    """Run the configured algorithm on the loaded ``inputs`` ``{role: (kept, positions, weights)}``;
+    returns ``{role: reconstructed positions}`` of the roles the convention moves or keeps."""
+    import pyrecon_amd
+    name = settings.pop('algorithm', 'name')
+    cls = getattr(pyrecon_amd, str(name), None)
+    if not (isinstance(cls, type) and hasattr(cls, 'assign_data') and hasattr(cls, 'run')):
+        raise ConfigError('unknown reconstruction algorithm "{}"'.format(name))
+    convention = str(settings.pop('algorithm', 'convention', '')).lower()
+    if convention not in CONVENTIONS:
+        raise ConfigError('convention must be one of RecSym, RecIso, RSD, not "{}"'.format(convention))
+    los = settings.pop('algorithm', 'los')
+    nthreads = settings.pop('algorithm', 'nthreads')
+    cosmo = {k: settings.get('cosmology', k) for k in ('f', 'bias') if settings.get('cosmology', k) is not None}
+    mesh = {k: v for k, v in settings.section('mesh').items() if v is not None}
+    use_randoms = selection == 'randoms'
+    box_from = inputs['randoms'][1] if use_randoms else None
+    recon = cls(los=los, positions=box_from, nthreads=nthreads, **cosmo, **mesh)
+    recon.assign_data(inputs['data'][1], inputs['data'][2])
+    if use_randoms:
+        recon.assign_randoms(inputs['randoms'][1], inputs['randoms'][2])
+    recon.set_density_contrast(**{k: v for k, v in settings.section('delta').items() if v is not None})
+    recon.run(**{k: v for k, v in settings.section('algorithm').items() if v is not None})
+    data_field, randoms_field = CONVENTIONS[convention]
+    # the particle-level iterative algorithm returns its own iterated data positions for 'data'
+    particle = type(recon) is getattr(pyrecon_amd, 'IterativeFFTParticleReconstruction', None)
+    out = {'data': recon.read_shifted_positions('data' if particle else inputs['data'][1], field=data_field)}
+    if 'randoms' in inputs:
+        if randoms_field is None:
+            out['randoms'] = inputs['randoms'][1]
         else:
-            toret += path
-    return toret
-
-
-def decode_eval_str(s):
-    """'${A} > 0' -> ('A > 0', ['A'])."""
-    if s is None:
-        return '', []
-    toret = str(s)
-    columns = []
-    for replace in re.finditer(r'(\${.*?})', toret):
-        value = replace.group(1)
-        col = value[2:-1]
-        toret = toret.replace(value, col)
-        if col not in columns: columns.append(col)
-    return toret, columns
-
-
-def _file_format(fn):
-    for file_format, exts in EXTENSIONS.items():
-        if any(fn.endswith(ext) for ext in exts):
-            return file_format
-    allowed = ', '.join(['{} ({})'.format(f, ', '.join(e)) for f, e in EXTENSIONS.items()])
-    raise ConfigError('File format of {} is not recognized. Formats {} only are handled.'.format(fn, allowed))
-
-
-def read_catalog(fn, columns, hdf5_prefix=''):
-    file_format = _file_format(fn)
-    catalog = {}
-    if file_format == 'npz':
-        with np.load(fn, allow_pickle=False) as file:
-            for col in columns: catalog[col] = file[col]
-    elif file_format == 'npy':
-        array = np.load(fn, allow_pickle=False)
-        for col in columns: catalog[col] = array[col]
-    elif file_format == 'fits':
-        try:
-            import fitsio
-        except ImportError as exc:
-            raise ConfigError('reading {} needs fitsio, which is not installed'.format(fn)) from exc
-        tmp = fitsio.read(fn, columns=columns)
-        for col in columns: catalog[col] = tmp[col]
-    else:
-        try:
-            import h5py
-        except ImportError as exc:
-            raise ConfigError('reading {} needs h5py, which is not installed'.format(fn)) from exc
-        with h5py.File(fn, 'r') as file:
-            for col in columns: catalog[col] = file[sepjoin(hdf5_prefix, col)][:]
-    return catalog
-
-
-def write_catalog(fn, catalog, columns, hdf5_prefix=''):
-    file_format = _file_format(fn)
-    if file_format == 'npz':
-        np.savez(fn, **{col: catalog[col] for col in columns})
-    elif file_format == 'npy':
-        array = np.empty(len(catalog[columns[0]]), dtype=[(col, catalog[col].dtype, catalog[col].shape[1:])
-                                                           for col in columns])
-        for col in columns: array[col] = catalog[col]
-        np.save(fn, array)
-    elif file_format == 'fits':
-        try:
-            import fitsio
-        except ImportError as exc:
-            raise ConfigError('writing {} needs fitsio, which is not installed'.format(fn)) from exc
-        array = np.empty(len(catalog[columns[0]]), dtype=[(col, catalog[col].dtype, catalog[col].shape[1:])
-                                                           for col in columns])
-        for col in columns: array[col] = catalog[col]
-        fitsio.write(fn, array, clobber=True)
-    else:
-        try:
-            import h5py
-        except ImportError as exc:
-            raise ConfigError('writing {} needs h5py, which is not installed'.format(fn)) from exc
-        with h5py.File(fn, 'w') as file:
-            for col in columns: file[sepjoin(hdf5_prefix, col)] = catalog[col]
-
-
-def comoving_distance(config):
-    Omega_m = config['cosmology'].get('Omega_m', None)
-    if Omega_m is None:
-        raise ConfigError('Provide Omega_m for redshift -> distance conversion.')
-    return utils.FlatLCDMDistance(Omega_m)
-
-
-def load_inputs(config):
-    """Read the input catalogs of ``config`` (bin/pyrecon:166-337).
-
-    Returns (selection_function, input_fns, output_fns, catalogs, positions, weights,
-    keep_input_columns)."""
-    selection_function = (config['delta'].pop('selection_function', '') or '').lower()
-    if selection_function not in ALLOWED_SELECTION_FUNCTIONS:
-        raise ConfigError('Unknown input selection function {}. Choices are {}'.format(
-            selection_function, ALLOWED_SELECTION_FUNCTIONS))
-    input_fns, output_fns = {}, {}
-    for name in ['data', 'randoms']:
-        tmp_fn = config['input'].get('{}_fn'.format(name), None)
-        if tmp_fn is None:
-            if name == 'randoms':
-                if selection_function == 'randoms':
-                    raise ConfigError('Please provide randoms catalog.')
-                if not selection_function:
-                    logger.info('No randoms provided.')
-                    selection_function = 'uniform'
-            else:
-                raise ConfigError('Please provide data catalog.')
-        else:
-            input_fns[name] = tmp_fn
-        tmp_fn = config['output'].get('{}_fn'.format(name), None)
-        if tmp_fn is not None:
-            if name not in input_fns:
-                raise ConfigError('Cannot output {} catalog if not provided as input.'.format(name))
-            output_fns[name] = tmp_fn
-    if not selection_function:
-        selection_function = 'randoms'
-    logger.info('Using {} selection function.'.format(selection_function))
-    catalogs, positions, weights, keep_input_columns = {}, {}, {}, {}
-    for name in input_fns:
-        fmt, position_columns = None, None
-        for key in ['{}_' + name, '{}']:
-            for format in ALLOWED_FORMATS:
-                cols = config['input'].get(key.format(format), None)
-                if cols is not None:
-                    if fmt is not None:
-                        raise ConfigError('Cannot use two different input formats')
-                    fmt, position_columns = format, cols
-            if fmt is not None: break
-        if fmt is None:
-            raise ConfigError('Unknown input format. Choices are {}'.format(ALLOWED_FORMATS))
-        position_columns = make_list(position_columns)
-        fn = os.path.join(config['input'].get('dir', None) or '', input_fns[name])
-        mask_str, mask_columns = decode_eval_str(config['input'].get('mask_{}'.format(name),
-                                                                     config['input'].get('mask', None)))
-        weight_str, weight_columns = decode_eval_str(config['input'].get('weights_{}'.format(name),
-                                                                         config['input'].get('weights', None)))
-        keep_input_columns[name] = make_list(config['output'].get('columns_{}'.format(name),
-                                                                  config['output'].get('columns', None)))
-        columns = remove_duplicates(position_columns + weight_columns + mask_columns + keep_input_columns[name])
-        logger.info('Loading {} catalog {}.'.format(name, fn))
-        catalog = read_catalog(fn, columns, hdf5_prefix=config['input'].get('hdf5_prefix', None) or '')
-        if mask_str:
-            mask = eval(mask_str, dict(np=np, **catalog), {})
-            for col in columns: catalog[col] = catalog[col][mask]
-            logger.info('Size of catalog {} after masking is {:d}.'.format(name, len(catalog[columns[0]])))
-        if fmt == 'rdz':
-            if len(position_columns) != 3:
-                raise ConfigError('Format rdz requires 3 position columns')
-            dist = comoving_distance(config)(catalog[position_columns[2]])
-            positions[name] = utils.sky_to_cartesian(dist, catalog[position_columns[0]], catalog[position_columns[1]])
-        else:
-            if len(position_columns) == 3:
-                positions[name] = np.array([catalog[col] for col in position_columns]).T
-            elif len(position_columns) == 1:
-                positions[name] = catalog[position_columns[0]]
-            else:
-                raise ConfigError('Format xyz requires 1 or 3 position columns')
-        weights[name] = eval(weight_str, dict(np=np, **catalog), {}) if weight_str else None
-        for col in columns:
-            if col not in keep_input_columns[name]:
-                del catalog[col]
-        catalogs[name] = catalog
-    return selection_function, input_fns, output_fns, catalogs, positions, weights, keep_input_columns
+            out['randoms'] = recon.read_shifted_positions(inputs['randoms'][1], field=randoms_field)
+    return {role: np.asarray(x) for role, x in out.items()}
 
 
 def main(args=None, **input_config):
-    """Run a reconstruction as configured; returns the reconstructed positions by catalog name."""
-    import pyrecon_amd
-    parser = argparse.ArgumentParser(formatter_class=argparse.ArgumentDefaultsHelpFormatter)
-    parser.add_argument('config_fn', action='store', type=str, nargs='?', default=None, help='Name of configuration file')
-    parser.add_argument('--data-fn', nargs='?', help='Path to input data file (overrides configuration file)')
-    parser.add_argument('--randoms-fn', nargs='?', help='Path to input randoms file (overrides configuration file)')
-    parser.add_argument('--output-data-fn', nargs='?', help='Path to output data file (overrides configuration file)')
-    parser.add_argument('--output-randoms-fn', nargs='?', help='Path to output randoms file (overrides configuration file)')
-    parser.add_argument('--growth-rate', nargs='?', type=float, help='Logarithmic growth rate (overrides configuration file)')
-    parser.add_argument('--galaxy-bias', nargs='?', type=float, help='Linear galaxy bias (overrides configuration file)')
-    opt = parser.parse_args(args=args)
-    logging.basicConfig(level=logging.INFO, format='[%(asctime)s] %(name)s %(message)s')
-    config = default_config()
-    update_config(config, input_config)
-    if opt.config_fn is not None:
-        logger.info('Loading config file {}.'.format(opt.config_fn))
-        with open(opt.config_fn, 'r') as file:
-            import yaml
-            update_config(config, yaml.load(file, Loader=_yaml_loader()))
-    if opt.data_fn: config['input']['data_fn'] = opt.data_fn
-    if opt.randoms_fn: config['input']['randoms_fn'] = opt.randoms_fn
-    if opt.output_data_fn: config['output']['data_fn'] = opt.output_data_fn
-    if opt.output_randoms_fn: config['output']['randoms_fn'] = opt.output_randoms_fn
-    if opt.growth_rate: config['cosmology']['f'] = opt.growth_rate
-    if opt.galaxy_bias: config['cosmology']['bias'] = opt.galaxy_bias
-    algorithm = getattr(pyrecon_amd, config['algorithm'].pop('name'))
-    config_cosmo = {name: value for name, value in config['cosmology'].items() if name in ['f', 'bias']}
-    config_cosmo['los'] = config['algorithm'].pop('los', None)
-    convention = (config['algorithm'].pop('convention') or '').lower()
-    config['algorithm'].pop('nthreads', None)
-    if convention not in ALLOWED_CONVENTIONS:
-        raise ConfigError('Unknown convention {}. Choices are {}'.format(convention, ALLOWED_CONVENTIONS))
-    logger.info('Convention is {}.'.format(convention))
-    (selection_function, input_fns, output_fns, catalogs, positions, weights,
-     keep_input_columns) = load_inputs(config)
-    mesh = {name: value for name, value in config['mesh'].items() if value is not None}
-    recon = algorithm(**config_cosmo, **mesh,
-                      positions=positions['randoms'] if selection_function == 'randoms' else None)
-    recon.assign_data(positions['data'], weights['data'])
-    if selection_function == 'randoms': recon.assign_randoms(positions['randoms'], weights['randoms'])
-    recon.set_density_contrast(**{k: v for k, v in config['delta'].items() if v is not None})
-    recon.run(**{k: v for k, v in config['algorithm'].items() if v is not None})
-    positions_rec = {}
-    field = 'rsd' if convention == 'rsd' else 'disp+rsd'
-    if isinstance(recon, pyrecon_amd.OriginalIterativeFFTParticleReconstruction):
-        positions_rec['data'] = recon.read_shifted_positions('data', field=field)
-    else:
-        positions_rec['data'] = recon.read_shifted_positions(positions['data'], field=field)
-    if 'randoms' in output_fns:
-        if convention == 'rsd':
-            positions_rec['randoms'] = positions['randoms']
-        else:
-            field = 'disp+rsd' if convention == 'recsym' else 'disp'
-            positions_rec['randoms'] = recon.read_shifted_positions(positions['randoms'], field=field)
-    for name in output_fns:
-        catalog = catalogs[name]
-        columns = list(keep_input_columns[name])
-        for rec, positions_ in zip(['', '_rec'], [positions[name], positions_rec[name]]):
-            for format in ALLOWED_FORMATS:
-                position_columns = config['output'].get('{}{}_{}'.format(format, rec, name), None)
-                if position_columns is None:
-                    position_columns = config['output'].get('{}{}'.format(format, rec), None)
-                if position_columns is None: continue
-                position_columns = make_list(position_columns)
-                if format == 'rdz':
-                    if len(position_columns) != 3:
-                        raise ConfigError('Format rdz requires 3 position columns')
-                    dist, ra, dec = utils.cartesian_to_sky(positions_)
-                    z = utils.DistanceToRedshift(comoving_distance(config))(dist)
-                    for col, value in zip(position_columns, [ra, dec, z]):
-                        catalog[col] = value
-                else:
-                    if len(position_columns) == 3:
-                        for icol, col in enumerate(position_columns):
-                            catalog[col] = positions_[:, icol]
-                    elif len(position_columns) == 1:
-                        catalog[position_columns[0]] = positions_
-                    else:
-                        raise ConfigError('Format xyz requires 1 or 3 position columns')
-                columns += position_columns
-        columns = remove_duplicates(columns)
-        fn = os.path.join(config['output'].get('dir', None) or '', output_fns[name])
-        logger.info('Saving {} catalog to {}.'.format(name, fn))
-        write_catalog(fn, catalog, columns, hdf5_prefix=config['output'].get('hdf5_prefix', None) or '')
-    return positions_rec
+    """Run the command line; ``input_config`` (``{section: {key: value}}``) is layered under the file."""
+    settings = settings_from(args, **input_config)
+    selection, plans, outputs = plan(settings)
+    logger.info('Selection function: %s', selection)
+    cosmology = Cosmology(settings.get('cosmology', 'Omega_m'))
+    inputs = {role: p.load(cosmology) for role, p in plans.items()}
+    rec = reconstruct(settings, selection, inputs)
+    prefix = settings.get('output', 'hdf5_prefix', '')
+    for role, path in outputs.items():
+        catalog = dict(inputs[role][0])
+        for reconstructed, positions in ((False, inputs[role][1]), (True, rec[role])):
+            for spec in output_positions(settings, role, reconstructed):
+                catalog.update(spec.from_cartesian(positions, cosmology))
+        logger.info('Writing %s catalog %s', role, path)
+        try:
+            catalog_io.write_catalog(path, catalog, prefix=prefix)
+        except catalog_io.CatalogFormatError as exc:
+            raise ConfigError(str(exc)) from exc
+    return rec
 
 
 if __name__ == '__main__':
+    logging.basicConfig(level=logging.INFO, format='%(asctime)s %(name)s %(levelname)s %(message)s')
     main()

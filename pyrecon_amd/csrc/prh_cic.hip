@@ -22,6 +22,7 @@
 // runs give identical bits (no step depends on the timing of atomics).
 // Chunks are applied in order, which preserves the point order across chunk
 // boundaries.
+#include <cstring>
 #include <type_traits>
 #include <utility>
 
@@ -127,6 +128,87 @@ __device__ __forceinline__ int base_index(int ip, int n) {
   return static_cast<int>(u % static_cast<unsigned long long>(n));
 }
 
+// 64-bit hash of a catalog's position bytes: the wrapping sum over points of a
+// mix of (index, x, y, z) bits (splitmix64's finaliser).  Integer sums do not
+// depend on the summation order, so block partials may be combined in any
+// order; the index in the mix makes a permutation of the points hash
+// differently.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z ^= z >> 30;
+  z *= 0xbf58476d1ce4e5b9ull;
+  z ^= z >> 27;
+  z *= 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return z;
+}
+__device__ __forceinline__ uint64_t word_bits(double x) { return __double_as_longlong(x); }
+__device__ __forceinline__ uint64_t word_bits(float x) { return static_cast<uint32_t>(__float_as_int(x)); }
+template <typename P>
+__device__ __forceinline__ uint64_t point_hash(size_t i, const P x[3]) {
+  uint64_t h = mix64(static_cast<uint64_t>(i) + 0x9e3779b97f4a7c15ull);
+  for (int d = 0; d < 3; d++) h = mix64(h ^ word_bits(x[d]));
+  return h;
+}
+
+// Sum of one value per thread over a block of kCicThreads threads, written to
+// partial[blockIdx.x] (wrapping 64-bit integer sum).
+__device__ __forceinline__ void block_hash_partial(uint64_t h, uint64_t* __restrict__ partial) {
+  __shared__ uint64_t s_h[kCicThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off);
+  if ((threadIdx.x & 63) == 0) s_h[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < kCicThreads / 64; w++) t += s_h[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// Wave-level OR of a paint kernel's error bits into misc[0], one atomic per wave.
+__device__ __forceinline__ void flag_paint_errors(uint32_t err, uint32_t* misc) {
+  const uint64_t bad = __ballot(err != 0);
+  if (bad) {
+    uint32_t e = err;
+    for (int off = 32; off > 0; off >>= 1) e |= __shfl_xor(e, off);
+    if ((threadIdx.x & 63) == 0) atomicOr(&misc[0], e);
+  }
+}
+
+// out[0] = sum of partial[0, n) (one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_hash_final(const uint64_t* __restrict__ partial, size_t n,
+                                                     uint64_t* __restrict__ out) {
+  __shared__ uint64_t s_h[16];
+  uint64_t h = 0;
+  for (size_t i = threadIdx.x; i < n; i += 1024) h += partial[i];
+  for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off);
+  if ((threadIdx.x & 63) == 0) s_h[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < 16; w++) t += s_h[w];
+    out[0] = t;
+  }
+}
+
+// Partial hashes of a catalog (the read's side of the comparison): one
+// kCicThreads block per kCicThreads * kHashItems points.
+constexpr int kHashItems = 16;
+template <typename P>
+__global__ __launch_bounds__(kCicThreads) void k_pos_hash(const P* __restrict__ pos, size_t m,
+                                                          uint64_t* __restrict__ partial) {
+  const size_t base = blockIdx.x * static_cast<size_t>(kCicThreads) * kHashItems + threadIdx.x;
+  uint64_t h = 0;
+#pragma unroll
+  for (int j = 0; j < kHashItems; j++) {
+    const size_t i = base + static_cast<size_t>(j) * kCicThreads;
+    if (i < m) {
+      const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+      h += point_hash<P>(i, x);
+    }
+  }
+  block_hash_partial(h, partial);
+}
+
 // ---------------------------------------------------------------------------
 // Paint step 1 (MODE 0: positions already in mesh units, no checks;
 // MODE 1: Cartesian positions, range check; MODE 2: Cartesian, wrapped)
@@ -135,11 +217,13 @@ template <typename R, typename P, typename W, int MODE>
 __global__ __launch_bounds__(kCicThreads) void k_paint_key(
     const P* __restrict__ pos, const W* __restrict__ w, size_t m, Frame<R> F,
     uint32_t* __restrict__ key, Payload<R>* __restrict__ payload, uint32_t* __restrict__ misc,
-    int check_only) {
+    int check_only, uint64_t* __restrict__ hpart) {
   const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   uint32_t err = 0;
+  uint64_t hash = 0;
   if (i < m) {
     const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    if (hpart) hash = point_hash<P>(i, x);
     R p[3];
     if (MODE == 0) {
       for (int d = 0; d < 3; d++) p[d] = static_cast<R>(x[d]);
@@ -167,6 +251,7 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_key(
     for (int off = 32; off > 0; off >>= 1) e |= __shfl_xor(e, off);
     if ((threadIdx.x & 63) == 0) atomicOr(&misc[0], e);
   }
+  if (hpart) block_hash_partial(hash, hpart);
 }
 
 // Exclusive scan of count[0, n) into start[0, n] (start[n] = total), zeroing
@@ -443,14 +528,15 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
 constexpr int kStartItems = 32;
 constexpr int kStartCells = kCicThreads * kStartItems;  // 8192
 
+// The keys' cell is key >> shift (the tile paint sorts by tile id, shift 9).
 __global__ __launch_bounds__(kCicThreads) void k_start_bounds(const uint32_t* __restrict__ skey,
                                                               size_t m, uint32_t nsb,
-                                                              uint32_t* __restrict__ bnd) {
+                                                              uint32_t* __restrict__ bnd, int shift) {
   const size_t p = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   if (p > m) return;
   // blocks b with key[p - 1] < b * kStartCells <= key[p] start at p (key[m] = infinity)
-  const int64_t lo = p > 0 ? static_cast<int64_t>(skey[p - 1] / kStartCells) : -1;
-  const int64_t hi = p < m ? static_cast<int64_t>(skey[p] / kStartCells) : static_cast<int64_t>(nsb);
+  const int64_t lo = p > 0 ? static_cast<int64_t>((skey[p - 1] >> shift) / kStartCells) : -1;
+  const int64_t hi = p < m ? static_cast<int64_t>((skey[p] >> shift) / kStartCells) : static_cast<int64_t>(nsb);
   for (int64_t b = lo + 1; b <= hi && b <= static_cast<int64_t>(nsb); b++)
     bnd[b] = static_cast<uint32_t>(p);
 }
@@ -458,7 +544,7 @@ __global__ __launch_bounds__(kCicThreads) void k_start_bounds(const uint32_t* __
 __global__ __launch_bounds__(kCicThreads) void k_cell_start(const uint32_t* __restrict__ skey,
                                                             const uint32_t* __restrict__ bnd,
                                                             size_t ncell, size_t m,
-                                                            uint32_t* __restrict__ start) {
+                                                            uint32_t* __restrict__ start, int shift) {
   __shared__ uint32_t s[kStartCells + kStartCells / 32];
   __shared__ uint32_t s_wmin[kCicThreads / 64];
   auto pad = [](int e) { return e + (e >> 5); };
@@ -471,8 +557,8 @@ __global__ __launch_bounds__(kCicThreads) void k_cell_start(const uint32_t* __re
   for (int e = threadIdx.x; e < kStartCells; e += kCicThreads) s[pad(e)] = hi;
   __syncthreads();
   for (uint32_t p = lo + threadIdx.x; p < hi; p += kCicThreads) {
-    const uint32_t k = skey[p];
-    if (p == 0u || skey[p - 1] != k) {
+    const uint32_t k = skey[p] >> shift;
+    if (p == 0u || (skey[p - 1] >> shift) != k) {
       const size_t e = static_cast<size_t>(k) - c0;
       if (e < static_cast<size_t>(cnt)) s[pad(static_cast<int>(e))] = p;
     }
@@ -506,12 +592,24 @@ __global__ __launch_bounds__(kCicThreads) void k_cell_start(const uint32_t* __re
 
 // Paint step 4: payload into cell-grouped order, so that step 5 reads each
 // group contiguously.
-template <typename R>
+// With `spos`, the positions are laid out in the same order (kept for reads).
+template <typename R, typename P>
 __global__ __launch_bounds__(kCicThreads) void k_paint_permute(const uint32_t* __restrict__ sorted,
                                                                const Payload<R>* __restrict__ payload,
-                                                               Payload<R>* __restrict__ spay, size_t m) {
+                                                               Payload<R>* __restrict__ spay, size_t m,
+                                                               const P* __restrict__ pos,
+                                                               P* __restrict__ spos) {
   const size_t s = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
-  if (s < m) spay[s] = payload[sorted[s]];
+  if (s >= m) return;
+  const uint32_t i = sorted[s];
+  spay[s] = payload[i];
+  if (spos) {
+    const P x0 = pos[3 * static_cast<size_t>(i)], x1 = pos[3 * static_cast<size_t>(i) + 1],
+            x2 = pos[3 * static_cast<size_t>(i) + 2];
+    spos[3 * s] = x0;
+    spos[3 * s + 1] = x1;
+    spos[3 * s + 2] = x2;
+  }
 }
 
 // CIC weight of a point at corner k = (a, b, c) of its base cell, in the
@@ -778,6 +876,225 @@ __global__ void k_zero_misc(uint32_t* misc) {
 }
 
 // ---------------------------------------------------------------------------
+// FP32 paint with exact accumulation per tile (the mixed-precision path)
+// ---------------------------------------------------------------------------
+// The paint above reproduces the reference's serial sum bit for bit.  The
+// mixed-precision reconstruction instead paints with FP32 corner weights, and
+// only needs every repeated run to give the same bits.  Points are sorted by
+// 8^3-cell tile alone (the key's low 9 bits carry the cell inside the tile and
+// ride along), so a 2- or 3-digit radix sort replaces the cell sort and the
+// point-order merge.  One workgroup per tile:
+//   - loads its points (positions in tile order, their FP32 weights),
+//   - bounds B = sum |w| prod_d max(|1 - d_d|, |d_d|) over them (an FP64
+//     reduction in a fixed order) and takes the scale 2^(62 - e), B < 2^e,
+//   - rounds each FP32 contribution ((wx*wy)*wz)*w times the scale to an
+//     integer and adds it into a 9^3 array of 64-bit integers in LDS: integer
+//     sums do not depend on their order, and the scaled sum stays below 2^62,
+//   - converts to FP64 (the scale is a power of two) and adds the cells that
+//     only this tile reaches (local indices 1 .. extent - 1 on every axis)
+//     into the mesh; the others go to a per-tile boundary array.
+// A second pass adds, for each cell with a local index 0, its own tile's value
+// and the upper halos of the (up to 7) lower neighbour tiles in a fixed order.
+// The contributions are rounded to 2^-62 B (relative to the tile's own
+// weight bound), far below the FP32 weights' rounding.
+constexpr int kPT = 8;                           // tile edge (cells)
+constexpr int kPTH = kPT + 1;                    // with the upper halo
+constexpr int kPTCells = kPTH * kPTH * kPTH;     // 729
+constexpr int kPTShift = 9;                      // key = tile << 9 | ((cx * 8 + cy) * 8 + cz)
+
+struct TileGrid {
+  int n[3];  // mesh shape
+  int t[3];  // tiles per axis
+};
+
+template <typename P, typename W, int MODE>
+__global__ __launch_bounds__(kCicThreads) void k_tile_key(const P* __restrict__ pos,
+                                                          const W* __restrict__ w, size_t m,
+                                                          Frame<double> F, TileGrid TG,
+                                                          uint32_t* __restrict__ key,
+                                                          uint32_t* __restrict__ misc,
+                                                          uint64_t* __restrict__ hpart) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  uint32_t err = 0;
+  uint64_t hash = 0;
+  if (i < m) {
+    const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    if (hpart) hash = point_hash<P>(i, x);
+    double p[3];
+    to_mesh_units<double, P>(x, F, MODE == 2, p, err);
+    if (w && !isfinite(static_cast<double>(w[i]))) err |= kErrNonFinite;
+    int b[3];
+    for (int d = 0; d < 3; d++) b[d] = base_index(static_cast<int>(p[d]), F.n[d]);
+    const uint32_t tile = (static_cast<uint32_t>(b[0] >> 3) * TG.t[1] + (b[1] >> 3)) * TG.t[2] + (b[2] >> 3);
+    key[i] = (tile << kPTShift) | (((b[0] & 7) * 8 + (b[1] & 7)) * 8 + (b[2] & 7));
+  }
+  flag_paint_errors(err, misc);
+  if (hpart) block_hash_partial(hash, hpart);
+}
+
+// Positions and FP32 weights in tile order.
+template <typename P, typename W>
+__global__ __launch_bounds__(kCicThreads) void k_tile_permute(const uint32_t* __restrict__ sorted,
+                                                              const P* __restrict__ pos,
+                                                              const W* __restrict__ w, size_t m,
+                                                              P* __restrict__ spos,
+                                                              float* __restrict__ sw) {
+  const size_t s = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (s >= m) return;
+  const size_t i = sorted[s];
+  const P x0 = pos[3 * i], x1 = pos[3 * i + 1], x2 = pos[3 * i + 2];
+  sw[s] = w ? static_cast<float>(w[i]) : 1.f;
+  spos[3 * s] = x0;
+  spos[3 * s + 1] = x1;
+  spos[3 * s + 2] = x2;
+}
+
+// FP32 offsets of a point from its base cell (recomputed from its position as
+// k_tile_key computed the cell).
+template <typename P>
+__device__ __forceinline__ void tile_offsets(const P* __restrict__ spos, size_t s,
+                                             const Frame<double>& F, int wrap, float d[3]) {
+  const P x[3] = {spos[3 * s], spos[3 * s + 1], spos[3 * s + 2]};
+  double p[3];
+  uint32_t err = 0;
+  to_mesh_units<double, P>(x, F, wrap, p, err);
+  for (int k = 0; k < 3; k++) {
+    const int b = base_index(static_cast<int>(p[k]), F.n[k]);
+    d[k] = static_cast<float>(p[k] - static_cast<double>(b));
+  }
+}
+
+struct TilePos {
+  int o[3], e[3];  // origin and extent (cells) on each axis
+};
+__device__ __forceinline__ TilePos tile_pos(uint32_t t, const TileGrid& TG) {
+  TilePos T;
+  const int c[3] = {static_cast<int>(t / (static_cast<uint32_t>(TG.t[1]) * TG.t[2])),
+                    static_cast<int>((t / TG.t[2]) % TG.t[1]), static_cast<int>(t % TG.t[2])};
+  for (int d = 0; d < 3; d++) {
+    T.o[d] = kPT * c[d];
+    T.e[d] = TG.n[d] - T.o[d] < kPT ? TG.n[d] - T.o[d] : kPT;
+  }
+  return T;
+}
+
+template <typename P>
+__global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
+    double* __restrict__ mesh, Frame<double> F, TileGrid TG, int wrap, uint32_t ntiles,
+    uint32_t per, const uint32_t* __restrict__ start, const uint32_t* __restrict__ skey,
+    const P* __restrict__ spos, const float* __restrict__ sw, double* __restrict__ bnd,
+    const uint32_t* __restrict__ misc) {
+  __shared__ unsigned long long acc[kPTCells];
+  __shared__ double s_red[kCicThreads / 64];
+  if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
+  const uint32_t t = xcd_block(per);
+  if (t >= ntiles) return;
+  const uint32_t beg = start[t], end = start[t + 1];
+  if (beg >= end) return;  // empty: the boundary pass skips it
+  const TilePos T = tile_pos(t, TG);
+  for (int c = threadIdx.x; c < kPTCells; c += kCicThreads) acc[c] = 0ull;
+  // weight bound of the tile (fixed order: strided per thread, then a fixed tree)
+  double bsum = 0.0;
+  for (uint32_t q = beg + threadIdx.x; q < end; q += kCicThreads) {
+    float d[3];
+    tile_offsets<P>(spos, q, F, wrap, d);
+    double pw = fabs(static_cast<double>(sw[q]));
+    for (int k = 0; k < 3; k++)
+      pw *= fmax(fabs(1.0 - static_cast<double>(d[k])), fabs(static_cast<double>(d[k])));
+    bsum += pw;
+  }
+  for (int off = 32; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = bsum;
+  __syncthreads();
+  double B = 0.0;
+  for (int u = 0; u < kCicThreads / 64; u++) B += s_red[u];
+  int ex = 0;
+  if (B > 0.0 && isfinite(B)) frexp(B, &ex);  // B < 2^ex
+  const double scale = ldexp(1.0, 62 - ex), inv = ldexp(1.0, ex - 62);
+  for (uint32_t q = beg + threadIdx.x; q < end; q += kCicThreads) {
+    float d[3];
+    tile_offsets<P>(spos, q, F, wrap, d);
+    const float wq = sw[q];
+    const uint32_t cell = skey[q] & ((1u << kPTShift) - 1u);
+    const int cx = static_cast<int>(cell >> 6), cy = static_cast<int>((cell >> 3) & 7u),
+              cz = static_cast<int>(cell & 7u);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int a = (k >> 2) & 1, b = (k >> 1) & 1, c = k & 1;
+      const float wx = a ? d[0] : 1.f - d[0];
+      const float wy = b ? d[1] : 1.f - d[1];
+      const float wz = c ? d[2] : 1.f - d[2];
+      const float contrib = ((wx * wy) * wz) * wq;
+      const long long fixed = __double2ll_rn(static_cast<double>(contrib) * scale);
+      atomicAdd(&acc[((cx + a) * kPTH + (cy + b)) * kPTH + (cz + c)], static_cast<unsigned long long>(fixed));
+    }
+  }
+  __syncthreads();
+  const size_t n1 = static_cast<size_t>(TG.n[1]), n2 = static_cast<size_t>(TG.n[2]);
+  for (int c = threadIdx.x; c < kPTCells; c += kCicThreads) {
+    const int i = c / (kPTH * kPTH), j = (c / kPTH) % kPTH, k = c % kPTH;
+    if (i > T.e[0] || j > T.e[1] || k > T.e[2]) continue;
+    const double v = static_cast<double>(static_cast<long long>(acc[c])) * inv;
+    const bool own = i >= 1 && i < T.e[0] && j >= 1 && j < T.e[1] && k >= 1 && k < T.e[2];
+    if (own)
+      mesh[(static_cast<size_t>(T.o[0] + i) * n1 + (T.o[1] + j)) * n2 + (T.o[2] + k)] += v;
+    else
+      bnd[static_cast<size_t>(t) * kPTCells + c] = v;
+  }
+}
+
+// Cells with a local index 0: own value plus the lower neighbours' halos.
+__global__ __launch_bounds__(kCicThreads) void k_tile_boundary(double* __restrict__ mesh,
+                                                               TileGrid TG, uint32_t ntiles,
+                                                               uint32_t per,
+                                                               const uint32_t* __restrict__ start,
+                                                               const double* __restrict__ bnd,
+                                                               const uint32_t* __restrict__ misc) {
+  if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
+  const uint32_t t = xcd_block(per);
+  if (t >= ntiles) return;
+  const TilePos T = tile_pos(t, TG);
+  const int c[3] = {T.o[0] / kPT, T.o[1] / kPT, T.o[2] / kPT};
+  // lower neighbours: subset s of the axes (bit 2: x, bit 1: y, bit 0: z), s = 0 the tile itself
+  uint32_t nb[8];
+  bool full[8];
+  int em[3];  // extent of the lower neighbour along each axis (its halo index)
+  for (int d = 0; d < 3; d++) {
+    const int cm = c[d] == 0 ? TG.t[d] - 1 : c[d] - 1;
+    em[d] = TG.n[d] - kPT * cm < kPT ? TG.n[d] - kPT * cm : kPT;
+  }
+  for (int sub = 0; sub < 8; sub++) {
+    int cc[3];
+    for (int d = 0; d < 3; d++) {
+      const bool lo = (sub >> (2 - d)) & 1;
+      cc[d] = lo ? (c[d] == 0 ? TG.t[d] - 1 : c[d] - 1) : c[d];
+    }
+    nb[sub] = (static_cast<uint32_t>(cc[0]) * TG.t[1] + cc[1]) * TG.t[2] + cc[2];
+    full[sub] = start[nb[sub]] < start[nb[sub] + 1];
+  }
+  const size_t n1 = static_cast<size_t>(TG.n[1]), n2 = static_cast<size_t>(TG.n[2]);
+  for (int q = threadIdx.x; q < kPT * kPT * kPT; q += kCicThreads) {
+    const int i = q >> 6, j = (q >> 3) & 7, k = q & 7;
+    if (i >= T.e[0] || j >= T.e[1] || k >= T.e[2]) continue;
+    const int zero = (i == 0 ? 4 : 0) | (j == 0 ? 2 : 0) | (k == 0 ? 1 : 0);
+    if (zero == 0) continue;
+    bool any = false;
+    double sum = 0.0;
+    if (full[0]) {
+      sum = bnd[static_cast<size_t>(nb[0]) * kPTCells + (i * kPTH + j) * kPTH + k];
+      any = true;
+    }
+    for (int sub = 1; sub < 8; sub++) {
+      if ((sub & zero) != sub || !full[sub]) continue;
+      const int ii = (sub & 4) ? em[0] : i, jj = (sub & 2) ? em[1] : j, kk = (sub & 1) ? em[2] : k;
+      sum += bnd[static_cast<size_t>(nb[sub]) * kPTCells + (ii * kPTH + jj) * kPTH + kk];
+      any = true;
+    }
+    if (any) mesh[(static_cast<size_t>(T.o[0] + i) * n1 + (T.o[1] + j)) * n2 + (T.o[2] + k)] += sum;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Reads
 // ---------------------------------------------------------------------------
 
@@ -916,13 +1233,11 @@ constexpr int kRT = 8;  // read binning tile edge (cells)
 // the unwrapped input; with a global line of sight (mesh dtype R) the RSD term
 // is in Q = promote(P, R).  O is the output dtype.
 template <typename R, typename P, typename O>
-__device__ __forceinline__ uint32_t read_point(size_t i, const R* __restrict__ phi,
-                                               const ReadArgs<R>& A, const P* __restrict__ pos,
-                                               O* __restrict__ out) {
+__device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const R* __restrict__ phi,
+                                               const ReadArgs<R>& A, O* __restrict__ out) {
   using Q = promote_t<P, R>;
   const Frame<R>& F = A.F;
   uint32_t err = 0;
-  const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
   R p[3];
   to_mesh_units<R, P>(x, F, A.wrap, p, err);
   if (A.field == PRH_FIELD_VALUE) {
@@ -980,6 +1295,15 @@ __device__ __forceinline__ uint32_t read_point(size_t i, const R* __restrict__ p
   return err;
 }
 
+// The point i of `pos`, written at out[i].
+template <typename R, typename P, typename O>
+__device__ __forceinline__ uint32_t read_point(size_t i, const R* __restrict__ phi,
+                                               const ReadArgs<R>& A, const P* __restrict__ pos,
+                                               O* __restrict__ out) {
+  const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  return read_value<R, P, O>(i, x, phi, A, out);
+}
+
 // Wave-level OR of the status flags, one atomic per wave.
 __device__ __forceinline__ void flag_errors(uint32_t err, uint32_t* misc) {
   const uint64_t bad = __ballot(err != 0);
@@ -1017,6 +1341,25 @@ __global__ __launch_bounds__(kCicThreads) void k_read_binned(const R* __restrict
   const size_t t = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   uint32_t err = 0;
   if (t < m) err = read_point<R, P, O>(order[t], phi, A, pos, out);
+  flag_errors(err, misc);
+}
+
+// Points of the catalog painted last in its cell order, streaming the paint's
+// copy of the positions (spos[t] is the position of point order[t]); each
+// result is written at its point's own index.
+template <typename R, typename P, typename O>
+__global__ __launch_bounds__(kCicThreads) void k_read_sorted(const R* __restrict__ phi,
+                                                             ReadArgs<R> A,
+                                                             const P* __restrict__ spos, size_t m,
+                                                             O* __restrict__ out,
+                                                             uint32_t* __restrict__ misc,
+                                                             const uint32_t* __restrict__ order) {
+  const size_t t = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  uint32_t err = 0;
+  if (t < m) {
+    const P x[3] = {spos[3 * t], spos[3 * t + 1], spos[3 * t + 2]};
+    err = read_value<R, P, O>(order[t], x, phi, A, out);
+  }
   flag_errors(err, misc);
 }
 
@@ -1072,15 +1415,23 @@ static RadixPlan radix_plan(size_t ncell) {
   return P;
 }
 
+// `keep`: also hash the positions into S.hash[0] and copy them, in cell order,
+// into S.spos (a catalog painted in one chunk, for later reads).
 template <typename R, typename P, typename W, int MODE>
 static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
-                       size_t m, bool check_only, hipStream_t s) {
+                       size_t m, bool check_only, hipStream_t s, bool keep = false) {
   PaintScratch& S = ws->paint;
   const unsigned grid = ceil_div(m, kCicThreads);
+  keep = keep && !check_only && grid <= S.nhpart;
   k_paint_key<R, P, W, MODE><<<grid, kCicThreads, 0, s>>>(
-      pos, w, m, F, S.key, static_cast<Payload<R>*>(S.payload), S.misc, check_only ? 1 : 0);
+      pos, w, m, F, S.key, static_cast<Payload<R>*>(S.payload), S.misc, check_only ? 1 : 0,
+      keep ? S.hpart : nullptr);
   PRH_LAUNCHED();
   if (check_only) return PRH_OK;
+  if (keep) {
+    k_hash_final<<<1, 1024, 0, s>>>(S.hpart, grid, S.hash);
+    PRH_LAUNCHED();
+  }
   const size_t ncell = S.ncell;
   // sort: keys alternate between key and rank, values between vtmp and sorted (the last
   // pass writes sorted); the first pass takes the input positions as values
@@ -1111,13 +1462,14 @@ static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* p
   }
   const uint32_t* skey = kbuf[RP.npass & 1];
   const uint32_t nsb = static_cast<uint32_t>(ceil_div(ncell, kStartCells));
-  k_start_bounds<<<ceil_div(m + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, nsb, S.bnd);
+  k_start_bounds<<<ceil_div(m + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, nsb, S.bnd, 0);
   PRH_LAUNCHED();
-  k_cell_start<<<nsb, kCicThreads, 0, s>>>(skey, S.bnd, ncell, m, S.start);
+  k_cell_start<<<nsb, kCicThreads, 0, s>>>(skey, S.bnd, ncell, m, S.start, 0);
   PRH_LAUNCHED();
-  k_paint_permute<R><<<grid, kCicThreads, 0, s>>>(S.sorted,
-                                                  static_cast<const Payload<R>*>(S.payload),
-                                                  static_cast<Payload<R>*>(S.spay), m);
+  k_paint_permute<R, P><<<grid, kCicThreads, 0, s>>>(S.sorted,
+                                                     static_cast<const Payload<R>*>(S.payload),
+                                                     static_cast<Payload<R>*>(S.spay), m, pos,
+                                                     keep ? static_cast<P*>(S.spos) : nullptr);
   PRH_LAUNCHED();
   timer_before(PRH_TIMER_PAINT_GATHER, s);
   {
@@ -1142,11 +1494,15 @@ static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* p
   return PRH_OK;
 }
 
-static int read_flags(prh_workspace* ws, hipStream_t s, uint32_t* flags) {
+// Status flags of the last paint; with `hash`, also the hash of its positions.
+static int read_flags(prh_workspace* ws, hipStream_t s, uint32_t* flags, uint64_t* hash = nullptr) {
   PRH_HIP(hipMemcpyAsync(ws->flag_host, ws->paint.misc, 2 * sizeof(uint32_t),
                          hipMemcpyDeviceToHost, s));
+  if (hash)
+    PRH_HIP(hipMemcpyAsync(ws->red_host, ws->paint.hash, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   PRH_HIP(hipStreamSynchronize(s));
   *flags = ws->flag_host[0];
+  if (hash) std::memcpy(hash, ws->red_host, sizeof(uint64_t));
   return PRH_OK;
 }
 
@@ -1168,6 +1524,7 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
   PaintScratch& S = ws->paint;
   // this paint's order goes to the older of the two order arrays
   std::swap(S.sorted, S.sorted_prev);
+  std::swap(S.spos, S.spos_prev);
   S.order_prev = S.order;
   S.order = PaintScratch::Order();
   k_zero_misc<<<1, 64, 0, s>>>(S.misc);
@@ -1190,16 +1547,155 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
       k_zero_misc<<<1, 64, 0, s>>>(S.misc);
       PRH_LAUNCHED();
     }
-    PRH_TRY((paint_chunk<R, P, W, MODE>(ws, mesh, F, pos + 3 * o, w ? w + o : nullptr, m, false, s)));
+    PRH_TRY((paint_chunk<R, P, W, MODE>(ws, mesh, F, pos + 3 * o, w ? w + o : nullptr, m, false, s,
+                                        MODE != 0 && !multi)));
   }
   if (MODE == 0 || multi) return PRH_OK;
   uint32_t flags;
-  PRH_TRY(read_flags(ws, s, &flags));
+  uint64_t hash = 0;
+  PRH_TRY(read_flags(ws, s, &flags, &hash));
   PRH_TRY(flags_to_status(flags));
   S.order.pos = pos;
   S.order.n = n;
   S.order.pdt = Tag<P>::code;
+  S.order.copy = ceil_div(n, kCicThreads) <= S.nhpart;
+  S.order.hash = hash;
   return PRH_OK;
+}
+
+// The FP32 tile paint of one chunk (see k_tile_accumulate); `keep` as paint_chunk.
+template <typename P, typename W, int MODE>
+static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>& F,
+                            const TileGrid& TG, const P* pos, const W* w, size_t m, bool check_only,
+                            hipStream_t s, bool keep) {
+  PaintScratch& S = ws->paint;
+  const unsigned grid = ceil_div(m, kCicThreads);
+  keep = keep && !check_only && grid <= S.nhpart;
+  k_tile_key<P, W, MODE><<<grid, kCicThreads, 0, s>>>(pos, w, m, F, TG, S.key, S.misc,
+                                                      keep ? S.hpart : nullptr);
+  PRH_LAUNCHED();
+  if (check_only) return PRH_OK;
+  if (keep) {
+    k_hash_final<<<1, 1024, 0, s>>>(S.hpart, grid, S.hash);
+    PRH_LAUNCHED();
+  }
+  const size_t ntiles = static_cast<size_t>(TG.t[0]) * TG.t[1] * TG.t[2];
+  // sort the (key, index) pairs by tile: digits above the 9 cell bits
+  RadixPlan RP = radix_plan(ntiles);
+  for (int p = 0; p < RP.npass; p++) RP.shift[p] += kPTShift;
+  const uint32_t rtiles = static_cast<uint32_t>(ceil_div(m, kRadixTile));
+  const uint32_t rper = static_cast<uint32_t>(ceil_div(rtiles, kXcds));
+  uint32_t* kbuf[2] = {S.key, S.rank};
+  uint32_t* vbuf[2] = {S.sorted, S.vtmp};
+  const uint32_t* vin = nullptr;
+  for (int p = 0; p < RP.npass; p++) {
+    const size_t nh = (size_t(1) << RP.bits[p]) * rtiles;
+    const unsigned nscan = ceil_div(nh, kScanBlock);
+    k_radix_count<<<rper * kXcds, kRadixThreads, 0, s>>>(kbuf[p & 1], m, RP.shift[p], RP.bits[p], rtiles,
+                                                         rper, S.hist);
+    PRH_LAUNCHED();
+    k_scan_reduce<<<nscan, kCicThreads, 0, s>>>(S.hist, nh, S.partial);
+    PRH_LAUNCHED();
+    k_scan_partials<<<1, kCicThreads, 0, s>>>(S.partial, static_cast<int>(nscan));
+    PRH_LAUNCHED();
+    k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.hist, nh, S.partial, S.goff);
+    PRH_LAUNCHED();
+    uint32_t* vout = vbuf[(RP.npass - 1 - p) & 1];
+    k_radix_scatter<<<rper * kXcds, kRadixThreads, 0, s>>>(kbuf[p & 1], vin, m, RP.shift[p], RP.bits[p],
+                                                           rtiles, rper, S.goff, kbuf[(p + 1) & 1], vout);
+    PRH_LAUNCHED();
+    vin = vout;
+  }
+  const uint32_t* skey = kbuf[RP.npass & 1];
+  const uint32_t nsb = static_cast<uint32_t>(ceil_div(ntiles, kStartCells));
+  k_start_bounds<<<ceil_div(m + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, nsb, S.bnd, kPTShift);
+  PRH_LAUNCHED();
+  k_cell_start<<<nsb, kCicThreads, 0, s>>>(skey, S.bnd, ntiles, m, S.start, kPTShift);
+  PRH_LAUNCHED();
+  float* sw = static_cast<float*>(S.payload);
+  k_tile_permute<P, W><<<grid, kCicThreads, 0, s>>>(S.sorted, pos, w, m, static_cast<P*>(S.spos), sw);
+  PRH_LAUNCHED();
+  const uint32_t per = xcd_per(ntiles);
+  timer_before(PRH_TIMER_PAINT_GATHER, s);
+  k_tile_accumulate<P><<<per * kXcds, kCicThreads, 0, s>>>(mesh, F, TG, MODE == 2, static_cast<uint32_t>(ntiles),
+                                                           per, S.start, skey, static_cast<const P*>(S.spos), sw,
+                                                           S.tbnd, S.misc);
+  // algorithmic: tile starts, key, position and weight per point, the mesh read and written
+  timer_after(PRH_TIMER_PAINT_GATHER, s,
+              static_cast<double>(ntiles) * 4 + static_cast<double>(m) * (4 + 3 * sizeof(P) + 4) +
+                  static_cast<double>(S.ncell) * 16);
+  PRH_LAUNCHED();
+  k_tile_boundary<<<per * kXcds, kCicThreads, 0, s>>>(mesh, TG, static_cast<uint32_t>(ntiles), per, S.start,
+                                                      S.tbnd, S.misc);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+template <typename P, typename W, int MODE>
+static int tile_paint_all(prh_workspace* ws, double* mesh, const Frame<double>& F, const P* pos,
+                          const W* w, size_t n, hipStream_t s) {
+  PaintScratch& S = ws->paint;
+  TileGrid TG;
+  for (int d = 0; d < 3; d++) {
+    TG.n[d] = F.n[d];
+    TG.t[d] = (F.n[d] + kPT - 1) / kPT;
+  }
+  std::swap(S.sorted, S.sorted_prev);
+  std::swap(S.spos, S.spos_prev);
+  S.order_prev = S.order;
+  S.order = PaintScratch::Order();
+  k_zero_misc<<<1, 64, 0, s>>>(S.misc);
+  PRH_LAUNCHED();
+  if (n == 0) return PRH_OK;
+  const bool multi = n > S.chunk;
+  if (multi) {
+    // validate every point before the first chunk touches the mesh
+    for (size_t o = 0; o < n; o += S.chunk) {
+      const size_t m = n - o < S.chunk ? n - o : S.chunk;
+      PRH_TRY((tile_paint_chunk<P, W, MODE>(ws, mesh, F, TG, pos + 3 * o, w ? w + o : nullptr, m, true, s,
+                                            false)));
+    }
+    uint32_t flags;
+    PRH_TRY(read_flags(ws, s, &flags));
+    PRH_TRY(flags_to_status(flags));
+  }
+  for (size_t o = 0; o < n; o += S.chunk) {
+    const size_t m = n - o < S.chunk ? n - o : S.chunk;
+    if (o > 0) {
+      k_zero_misc<<<1, 64, 0, s>>>(S.misc);
+      PRH_LAUNCHED();
+    }
+    PRH_TRY((tile_paint_chunk<P, W, MODE>(ws, mesh, F, TG, pos + 3 * o, w ? w + o : nullptr, m, false, s,
+                                          !multi)));
+  }
+  if (multi) return PRH_OK;
+  uint32_t flags;
+  uint64_t hash = 0;
+  PRH_TRY(read_flags(ws, s, &flags, &hash));
+  PRH_TRY(flags_to_status(flags));
+  S.order.pos = pos;
+  S.order.n = n;
+  S.order.pdt = Tag<P>::code;
+  S.order.copy = ceil_div(n, kCicThreads) <= S.nhpart;
+  S.order.hash = hash;
+  return PRH_OK;
+}
+
+template <typename P>
+static int tile_paint_dispatch(prh_workspace* ws, double* mesh, const Frame<double>& F, const P* pos,
+                               const void* w, int wdt, size_t n, int wrap, hipStream_t s) {
+  if (!w || wdt == PRH_F64) {
+    const double* wd = static_cast<const double*>(w);
+    return wrap ? tile_paint_all<P, double, 2>(ws, mesh, F, pos, wd, n, s)
+                : tile_paint_all<P, double, 1>(ws, mesh, F, pos, wd, n, s);
+  }
+  if (wdt == PRH_F32) {
+    const float* wf = static_cast<const float*>(w);
+    return wrap ? tile_paint_all<P, float, 2>(ws, mesh, F, pos, wf, n, s)
+                : tile_paint_all<P, float, 1>(ws, mesh, F, pos, wf, n, s);
+  }
+  set_error("weight dtype must be PRH_F32 or PRH_F64");
+  return PRH_ERR_ARG;
 }
 
 static int check_paint_ws(const prh_workspace* ws, const size_t* nmesh, int dtype) {
@@ -1271,6 +1767,31 @@ static int do_paint(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* bo
   return PRH_ERR_ARG;
 }
 
+static int do_paint_mixed(prh_workspace* ws, double* mesh, const size_t* nmesh, const double* boxsize,
+                          const double* boxcenter, const void* pos, int pdt, const void* w, int wdt,
+                          size_t n, int wrap, void* stream) {
+  PRH_TRY(check_paint_ws(ws, nmesh, PRH_F64));
+  if (!boxsize || !boxcenter || (n && (!mesh || !pos))) return PRH_ERR_ARG;
+  if (!ws->paint.tbnd) {
+    set_error("the FP32 tile paint needs a workspace with PRH_WS_PAINT | PRH_WS_MIXED");
+    return PRH_ERR_STATE;
+  }
+  size_t ntiles = 1;
+  for (int d = 0; d < 3; d++) ntiles *= (nmesh[d] + kPT - 1) / kPT;
+  if (ntiles > (size_t(1) << (32 - kPTShift))) {
+    set_error("mesh has too many 8^3 tiles for 32-bit tile keys");
+    return PRH_ERR_ARG;
+  }
+  const Frame<double> F = make_frame<double>(nmesh, boxsize, boxcenter);
+  hipStream_t s = as_stream(stream);
+  if (pdt == PRH_F64)
+    return tile_paint_dispatch<double>(ws, mesh, F, static_cast<const double*>(pos), w, wdt, n, wrap, s);
+  if (pdt == PRH_F32)
+    return tile_paint_dispatch<float>(ws, mesh, F, static_cast<const float*>(pos), w, wdt, n, wrap, s);
+  set_error("positions dtype must be PRH_F32 or PRH_F64");
+  return PRH_ERR_ARG;
+}
+
 template <typename R>
 static int do_read_cic(const R* mesh, const size_t* nmesh, const R* pos, R* out, size_t n,
                        void* stream) {
@@ -1328,7 +1849,33 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
   auto painted = [&](const PaintScratch::Order& o) {
     return o.pos == static_cast<const void*>(pos) && o.n == n && o.pdt == Tag<P>::code;
   };
-  const uint32_t* porder = painted(S.order) ? S.sorted : painted(S.order_prev) ? S.sorted_prev : nullptr;
+  const bool last = painted(S.order);
+  const uint32_t* porder = last ? S.sorted : painted(S.order_prev) ? S.sorted_prev : nullptr;
+  const PaintScratch::Order* po = last ? &S.order : &S.order_prev;
+  if (porder && po->copy) {
+    // the paint kept these positions in cell order: if they are unchanged (same hash), stream
+    // the copy; the hash pass reads the positions once, in order
+    const unsigned hgrid = ceil_div(n, static_cast<size_t>(kCicThreads) * kHashItems);
+    if (hgrid <= S.nhpart) {
+      k_pos_hash<P><<<hgrid, kCicThreads, 0, s>>>(pos, n, S.hpart);
+      PRH_LAUNCHED();
+      k_hash_final<<<1, 1024, 0, s>>>(S.hpart, hgrid, S.hash + 1);
+      PRH_LAUNCHED();
+      PRH_HIP(hipMemcpyAsync(ws->red_host, S.hash + 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+      PRH_HIP(hipStreamSynchronize(s));
+      uint64_t h;
+      std::memcpy(&h, ws->red_host, sizeof(uint64_t));
+      if (h == po->hash) {
+        const P* spos = static_cast<const P*>(last ? S.spos : S.spos_prev);
+        timer_before(PRH_TIMER_READ_SHIFTS, s);
+        k_read_sorted<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, spos, n, out,
+                                                                               misc, porder);
+        timer_after(PRH_TIMER_READ_SHIFTS, s, bytes);
+        PRH_LAUNCHED();
+        return PRH_OK;
+      }
+    }
+  }
   if (porder) {
     // one of the last two catalogs painted: its cell order (a permutation of [0, n)) serves as
     // the bins
@@ -1484,6 +2031,13 @@ int prh_paint_f64(prh_workspace* ws, double* mesh, const size_t* nmesh, const do
                   void* stream) {
   return do_paint<double>(ws, mesh, nmesh, boxsize, boxcenter, positions, pos_dtype, weights,
                           weight_dtype, npositions, wrap, stream);
+}
+
+int prh_paint_mixed_f64(prh_workspace* ws, double* mesh, const size_t* nmesh, const double* boxsize,
+                        const double* boxcenter, const void* positions, int pdtype, const void* weights,
+                        int wdtype, size_t npositions, int wrap, void* stream) {
+  return do_paint_mixed(ws, mesh, nmesh, boxsize, boxcenter, positions, pdtype, weights, wdtype, npositions,
+                        wrap, stream);
 }
 
 int prh_read_shifts_f32(prh_workspace* ws, const float* phi, const size_t* nmesh, const float* boxsize,

@@ -944,6 +944,22 @@ __global__ __launch_bounds__(kThreads) void k_add_f32_f64(double* __restrict__ v
   if (i < n) v[i] += static_cast<double>(e[i]);
 }
 
+// Block partial sums of an FP64 mesh (deterministic, see prh_reduce.hpp).
+__global__ __launch_bounds__(kReduceThreads) void k_sum_f64(const double* __restrict__ a,
+                                                             size_t n, double* __restrict__ partial) {
+  block_reduce_range<1>(n, [&](size_t i, double* out) { out[0] = a[i]; }, partial, 0);
+}
+
+// out = a - sum[0] / n, the mesh with its mean removed (sum from k_reduce_final).
+__global__ __launch_bounds__(kThreads) void k_subtract_mean(const double* __restrict__ a,
+                                                             double* __restrict__ out,
+                                                             const double* __restrict__ sum,
+                                                             size_t n) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
+  const double mean = sum[0] / static_cast<double>(n);
+  if (i < n) out[i] = a[i] - mean;
+}
+
 // ---------------------------------------------------------------------------
 // Grouped launches: one launch applies the same operation at the same level
 // shape to up to kMaxGroup independent problems (serving several
@@ -1712,6 +1728,13 @@ static int residual_rms(prh_workspace* ws, const double* v, const double* f, flo
 // may follow.  The iterate tracks the FP64 multigrid's to FP32 rounding of
 // the corrections.  `count` problems of one shape advance in lock step (the
 // FP32 levels are grouped as in Solver); norms only for a single problem.
+// True when the operator's diagonal D is the same in every cell of every level:
+// a global line of sight, or a local one with equal cell sizes on the three axes
+// (then sum_i c_i = (3 + beta) / h^2).
+static bool centrable(const LevelGeom<double>& G) {
+  return !G.local || (G.h2[0] == G.h2[1] && G.h2[1] == G.h2[2]);
+}
+
 static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double* const* f,
                               double* const* v, const size_t* nmesh, const double* boxsize,
                               const double* boxcenter, const double* beta, double damping,
@@ -1741,10 +1764,33 @@ static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double*
   S.params(static_cast<float>(damping), niter, s);
   S.fast = true;
   const Ptrs<float> r32 = S.f(0), e32 = S.v(0);
+  // 0. the centred source.  The mean of f is a null mode of the operator: the
+  // iterate drifts by a constant per sweep, and in an FP32 correction that
+  // constant grows with the coarse cell sizes until its rounding swamps the
+  // correction's gradients.  Where every level's diagonal D is the same in
+  // every cell (global line of sight, or equal cell sizes), a sweep, residual,
+  // restriction or prolongation maps a constant to a constant, so each cycle
+  // maps f - m to its result on f minus a constant: solving with f - m (m the
+  // mean of f) changes the potential by a constant only, and its gradient not
+  // at all.  f - m is kept in the workspace's FP64 level-0 scratch.
+  const double* src[kMaxGroup];
+  for (int j = 0; j < count; j++) {
+    src[j] = f[j];
+    if (!centrable(G64[j]) || !ws[j]->mg.tmp[0]) continue;
+    double* fc = static_cast<double*>(ws[j]->mg.tmp[0]);
+    double* sum = ws[j]->red + 2 * kReduceBlocks + 2;
+    k_sum_f64<<<kReduceBlocks, kReduceThreads, 0, s>>>(f[j], total, ws[j]->red);
+    PRH_LAUNCHED();
+    k_reduce_final<1><<<1, 1024, 0, s>>>(ws[j]->red, kReduceBlocks, 0, sum);
+    PRH_LAUNCHED();
+    k_subtract_mean<<<grid, kThreads, 0, s>>>(f[j], fc, sum, total);
+    PRH_LAUNCHED();
+    src[j] = fc;
+  }
   // 1. coarse FMG stages in FP32, prolonged into the FP64 iterate
   if (S.nlev() > 1) {
     const Ptrs<float> f1 = S.f(1), v1 = S.v(1);
-    for (int j = 0; j < count; j++) PRH_TRY((launch_restrict<double, float>(f[j], f1[j], nmesh, s)));
+    for (int j = 0; j < count; j++) PRH_TRY((launch_restrict<double, float>(src[j], f1[j], nmesh, s)));
     PRH_TRY(S.fmg(f1, v1, ncycle, 1));
     for (int j = 0; j < count; j++)
       PRH_TRY((launch_prolong<float, double>(v1[j], v[j], S.n(1), false, s)));
@@ -1754,9 +1800,11 @@ static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double*
   for (int it = 0; it < ncorr; it++) {
     for (int j = 0; j < count; j++) {
       if (norms) {
-        PRH_TRY(residual_rms(ws[j], v[j], f[j], r32[j], G64[j], &norms[it], s));
+        // the norm is of the residual of the true source, f - A v
+        PRH_TRY(residual_rms(ws[j], v[j], f[j], src[j] == f[j] ? r32[j] : nullptr, G64[j], &norms[it], s));
+        if (src[j] != f[j]) PRH_TRY((launch_residual<double, float>(v[j], src[j], r32[j], G64[j], s)));
       } else {
-        PRH_TRY((launch_residual<double, float>(v[j], f[j], r32[j], G64[j], s)));
+        PRH_TRY((launch_residual<double, float>(v[j], src[j], r32[j], G64[j], s)));
       }
     }
     Ptrs<float> cur = e32, alt = S.tmp(0);

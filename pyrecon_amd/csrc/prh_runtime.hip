@@ -97,7 +97,7 @@ static size_t cells(const size_t* n) { return n[0] * n[1] * n[2]; }
 // and the read binning's tile counts (8^3-cell tiles); the block bounds of the
 // cell-start pass cover 8192 cells each.
 struct PaintSizes {
-  size_t ncell, ntile, nhist, npartial, nbnd;
+  size_t ncell, ntile, nhist, npartial, nbnd, nhpart;
 };
 static PaintSizes paint_sizes(const size_t* n, size_t chunk) {
   PaintSizes z;
@@ -107,6 +107,7 @@ static PaintSizes paint_sizes(const size_t* n, size_t chunk) {
   const size_t a = (z.ntile + 8191) / 8192, b = (z.nhist + 8191) / 8192;
   z.npartial = a > b ? a : b;
   z.nbnd = (z.ncell + 8191) / 8192 + 1;
+  z.nhpart = (chunk + 255) / 256 + 1;  // one partial hash per 256-thread block of the key step
   return z;
 }
 
@@ -124,7 +125,9 @@ static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags)
   if (flags & PRH_WS_PAINT) {
     const PaintSizes z = paint_sizes(n, chunk);
     p.paint = z.ntile * 4 + (z.ncell + 1) * 4 + (z.npartial + 1) * 4 + chunk * 4 * 5 +
-              (2 * z.nhist + 1) * 4 + (z.nbnd + 1) * 4 + 2 * chunk * 4 * w + 4 * 4;
+              (2 * z.nhist + 1) * 4 + (z.nbnd + 1) * 4 + 2 * chunk * 4 * w + 4 * 4 +
+              2 * chunk * 3 * sizeof(double) + (z.nhpart + 2) * sizeof(uint64_t);
+    if ((flags & PRH_WS_MIXED) && dtype == PRH_F64) p.paint += z.ntile * 729 * sizeof(double);
   }
   if (flags & PRH_WS_FFT) p.fft = smooth_bytes(n, dtype);
   p.red = (kReduceBlocks * 2 + 16) * sizeof(double) + 16 * sizeof(uint32_t);
@@ -166,6 +169,15 @@ static int paint_alloc(prh_workspace* ws, size_t chunk) {
   PRH_TRY(ws_alloc(ws, &P.payload, chunk * 4 * static_cast<size_t>(ws->dtype)));
   PRH_TRY(ws_alloc(ws, &P.spay, chunk * 4 * static_cast<size_t>(ws->dtype)));
   PRH_TRY(ws_alloc(ws, &p, 4 * 4)); P.misc = static_cast<uint32_t*>(p);
+  PRH_TRY(ws_alloc(ws, &P.spos, chunk * 3 * sizeof(double)));
+  PRH_TRY(ws_alloc(ws, &P.spos_prev, chunk * 3 * sizeof(double)));
+  P.nhpart = z.nhpart;
+  PRH_TRY(ws_alloc(ws, &p, z.nhpart * sizeof(uint64_t))); P.hpart = static_cast<uint64_t*>(p);
+  PRH_TRY(ws_alloc(ws, &p, 2 * sizeof(uint64_t))); P.hash = static_cast<uint64_t*>(p);
+  if ((ws->flags & PRH_WS_MIXED) && ws->dtype == PRH_F64) {
+    PRH_TRY(ws_alloc(ws, &p, z.ntile * 729 * sizeof(double)));
+    P.tbnd = static_cast<double*>(p);
+  }
   PRH_HIP(hipMemset(P.count, 0, z.ntile * 4));
   PRH_HIP(hipMemset(P.misc, 0, 16));
   return PRH_OK;
@@ -353,7 +365,8 @@ int prh_workspace_destroy(prh_workspace* ws) {
   PaintScratch& P = ws->paint;
   for (void* p : {(void*)P.count, (void*)P.start, (void*)P.partial, (void*)P.key, (void*)P.rank,
                   (void*)P.vtmp, (void*)P.hist, (void*)P.goff, (void*)P.bnd, (void*)P.sorted,
-                  (void*)P.sorted_prev, P.payload, P.spay, (void*)P.misc})
+                  (void*)P.sorted_prev, P.payload, P.spay, (void*)P.misc, P.spos, P.spos_prev,
+                  (void*)P.hpart, (void*)P.hash, (void*)P.tbnd})
     release(p);
   for (int d = 0; d < 3; d++) {
     release(ws->smooth.tw[d]);

@@ -47,10 +47,23 @@ struct PaintScratch {
   // arrays alternate between paints, so the orders of the last two catalogs
   // painted (data and randoms) are both kept; `sorted` is the one of the last.
   uint32_t* sorted_prev = nullptr;  // chunk      the previous paint's order
+  // With each order, a paint of one chunk of Cartesian positions keeps a copy
+  // of the positions in that order (`spos`, 3 values of the positions dtype per
+  // point) and a 64-bit hash of the positions' bytes in input order.  A read of
+  // positions with the same pointer, count, dtype and hash streams the copy
+  // instead of gathering the positions at random.
+  void* spos = nullptr;         // chunk x 3 doubles
+  void* spos_prev = nullptr;    // chunk x 3 doubles
+  uint64_t* hpart = nullptr;    // nhpart   per-block partial hashes
+  size_t nhpart = 0;
+  uint64_t* hash = nullptr;     // [2]      hash of the last paint's positions; of a read's
+  double* tbnd = nullptr;       // ntile x 729  boundary cells of the FP32 tile paint (PRH_WS_MIXED)
   struct Order {
     const void* pos = nullptr;
     size_t n = 0;
     int pdt = 0;
+    bool copy = false;  // spos holds the positions in this order
+    uint64_t hash = 0;
   };
   Order order, order_prev;  // tags of `sorted` and `sorted_prev`
 };

@@ -86,7 +86,7 @@ class OriginalMultiGridReconstruction(BaseReconstruction):
         self.mesh_randoms = None
         self.mesh_delta.smooth_gaussian(smoothing_radius, **kwargs)
 
-    def run(self, jacobi_damping_factor=0.4, jacobi_niterations=5, vcycle_niterations=6, precision='plain',
+    def run(self, jacobi_damping_factor=0.4, jacobi_niterations=5, vcycle_niterations=6, precision=None,
             nrefine=0, vcycles_per_refine=1, residual_norms=False):
         """Solve for the displacement potential :attr:`mesh_phi` (pyrecon/multigrid.py:99-135).
 
@@ -97,10 +97,12 @@ class OriginalMultiGridReconstruction(BaseReconstruction):
             (FP64 meshes: the coarse full-multigrid stages in FP32, then the
             ``vcycle_niterations`` finest-level V-cycles as FP64 defect corrections with FP32
             V-cycles, tracking the plain FP64 iterate; ``nrefine`` extra corrections may follow).
+            None (default): the constructor's ``precision``.
         residual_norms : if True (mixed only), store the FP64 RMS residual before each finest-level
             correction and at the end in :attr:`residual_norms` (one synchronisation each).
         """
         dtype = self.info.dtype
+        if precision is None: precision = self.precision
         delta = self._solve_source()
         phi = self._new_phi()
         nm, nm_ptr = _lib.size3(self.info.shape)

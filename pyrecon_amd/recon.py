@@ -45,6 +45,9 @@ class BaseReconstruction(object):
     fft_engine, fft_wisdom, save_fft_wisdom, fft_plan : accepted for compatibility; the device
         always uses its own transforms.
     wrap : bool
+    precision : 'plain' (default: the reference's algorithm in the mesh dtype, bit for bit where the
+        reference is deterministic C) or 'mixed' (FP64 meshes: FP32 CIC weights summed exactly per tile
+        and in FP64 across tiles, and the mixed-precision solve of ``run``).
     kwargs : MeshInfo arguments (nmesh, boxsize, boxcenter, cellsize, positions, boxpad, dtype);
         ``nthreads`` is accepted and ignored.  Device-specific: ``stream``, ``paint_chunk``,
         ``workspace`` (a :class:`~pyrecon_amd.mesh.Workspace` to share).
@@ -56,13 +59,18 @@ class BaseReconstruction(object):
 
     def __init__(self, f=0., bias=1., los=None, fft_engine='numpy', fft_wisdom=None, save_fft_wisdom=None,
                  fft_plan='measure', wrap=False, stream=None, paint_chunk=DEFAULT_PAINT_CHUNK, workspace=None,
-                 **kwargs):
+                 precision='plain', **kwargs):
         self.logger = logging.getLogger(self.__class__.__name__)
         self.set_cosmo(f=f, bias=bias)
         self.wrap = wrap
         self.stream = stream
         kwargs.pop('nthreads', None)
         self.info = MeshInfo(select_nmesh=self._select_nmesh, **kwargs)
+        if precision not in ('plain', 'mixed'):
+            raise ValueError('precision must be "plain" or "mixed"')
+        if precision == 'mixed' and self.info.dtype != np.float64:
+            raise ValueError('precision="mixed" needs an FP64 mesh (dtype="f8")')
+        self.precision = precision
         if workspace is None:
             workspace = Workspace(self.info.shape, self.info.dtype, paint_chunk=paint_chunk)
         elif tuple(workspace.nmesh) != self.info.shape or workspace.dtype != self.info.dtype:
@@ -99,12 +107,13 @@ class BaseReconstruction(object):
             self.los = los / distance(los)
 
     def assign_data(self, positions, weights=None):
-        self.mesh_data.assign_cic(positions, weights=weights, wrap=self.wrap)
+        self.mesh_data.assign_cic(positions, weights=weights, wrap=self.wrap, fp32_weights=self.precision == 'mixed')
 
     def assign_randoms(self, positions, weights=None):
         if self.mesh_randoms.buffer is None:
             self._size_randoms = 0
-        self.mesh_randoms.assign_cic(positions, weights=weights, wrap=self.wrap)
+        self.mesh_randoms.assign_cic(positions, weights=weights, wrap=self.wrap,
+                                     fp32_weights=self.precision == 'mixed')
         self._size_randoms += len(positions)
 
     @property

@@ -341,7 +341,7 @@ class ReconServer(object):
             cls = OriginalMultiGridReconstruction if job.original else MultiGridReconstruction
             recon = cls(f=job.f, bias=job.bias, los=job.los, nmesh=job.nmesh, boxsize=job.boxsize,
                         boxcenter=job.boxcenter, dtype=job.dtype, wrap=job.wrap, workspace=act.workspace,
-                        stream=stream)
+                        stream=stream, precision=job.precision)
             data_pos, data_w = self._catalog(job.data, stream)
             recon.assign_data(data_pos, data_w)
             rand_pos = None

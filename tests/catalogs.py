@@ -183,3 +183,49 @@ def sha256(*arrays):
     for a in arrays:
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
+
+
+def serving_stream(njobs, rate, seed=2024):
+    """Configuration 5: Poisson arrivals at ``rate`` jobs/s; nmesh in {128, 256, 512} with
+    probabilities {0.5, 0.35, 0.15}; boxsize = 7.8 nmesh Mpc/h centred at (0, 0, boxsize); data
+    count lognormal with median 1e6 and sigma(ln N) = 1, clipped to [1e5, 2e7]; randoms 10x."""
+    rng = np.random.RandomState(seed)
+    arrivals = np.cumsum(rng.exponential(1. / rate, njobs))
+    nmesh = rng.choice([128, 256, 512], size=njobs, p=[0.5, 0.35, 0.15])
+    ndata = np.clip(np.exp(np.log(1e6) + rng.normal(size=njobs)), 1e5, 2e7).astype(np.int64)
+    return [dict(arrival=float(a), nmesh=int(n), ndata=int(d), seed=int(seed * 1000 + i))
+            for i, (a, n, d) in enumerate(zip(arrivals, nmesh, ndata))]
+
+
+def serving_catalogs(spec):
+    """Geometry and device catalog makers ``fn(stream) -> (positions, weights)`` of one serving job:
+    data Zel'dovich-displaced by 256 random Fourier modes of P(k) with the RSD shift and weights
+    uniform in [0.5, 1.5] (configuration 2's generator, on the device), randoms uniform with unit
+    weights, each from the job's seed."""
+    from pyrecon_amd.serving import synthetic_catalog, zeldovich_modes
+    n, nd = spec['nmesh'], spec['ndata']
+    boxsize = 7.8 * n
+    boxcenter = (0., 0., boxsize)
+    modes = zeldovich_modes([boxsize] * 3, [n] * 3, spec['seed'])
+
+    def data(stream):
+        return synthetic_catalog(nd, boxsize, boxcenter, 2 * spec['seed'], modes=modes, f=RECON_KWARGS['f'],
+                                 weights=(0.5, 1.5), stream=stream)
+
+    def randoms(stream):
+        return synthetic_catalog(10 * nd, boxsize, boxcenter, 2 * spec['seed'] + 1, weights=(1., 1.), stream=stream)
+
+    return boxsize, boxcenter, data, randoms
+
+
+def serving_job(spec, precision, name=None, **kwargs):
+    """The :class:`pyrecon_amd.serving.ReconJob` of one configuration-5 job (RecSym shifts of data and
+    randoms, device outputs)."""
+    from pyrecon_amd.serving import ReconJob
+    n, nd = spec['nmesh'], spec['ndata']
+    boxsize, boxcenter, data, randoms = serving_catalogs(spec)
+    kw = dict(shifted=('data', 'randoms'), field='disp+rsd', randoms_field='disp+rsd', to_host=False)
+    kw.update(kwargs)
+    return ReconJob(data, randoms, ndata=nd, nrandoms=10 * nd, nmesh=n, boxsize=boxsize, boxcenter=boxcenter,
+                    f=RECON_KWARGS['f'], bias=RECON_KWARGS['bias'], los=RECON_KWARGS['los'], dtype='f8',
+                    smoothing_radius=SMOOTHING_RADIUS, precision=precision, name=name, **kw)

@@ -1,19 +1,20 @@
-"""Host side of the command-line reconstruction (pyrecon_amd/cli.py, after bin/pyrecon:70-423) and
-the coordinate utilities (pyrecon_amd/utils.py, after pyrecon/utils.py:146-282).
+"""Host side of the command-line reconstruction (pyrecon_amd/cli.py, interface of bin/pyrecon:70-423),
+its catalog files (pyrecon_amd/catalog_io.py) and the coordinate utilities (pyrecon_amd/utils.py,
+interface of pyrecon/utils.py:146-282).
 
 - sky <-> Cartesian conversions are inverse to each other and give the textbook values on the axes;
 - the flat LCDM comoving distance reduces to the Einstein-de Sitter closed form
   D = 2 (c / 100) (1 - 1 / sqrt(1 + z)) Mpc/h at Omega_m = 1, and matches
   D(0.5) = 1322.04 Mpc/h, D(1) = 2312.68 Mpc/h at Omega_m = 0.3 (1888.63 and 3303.83 Mpc for h = 0.7);
 - DistanceToRedshift inverts it;
-- catalogs are read with the reference's column, mask, weight and format rules.
+- settings layer and resolve suffixed keys as the reference's configuration does;
+- catalogs are read with the reference's column, mask, weight and format rules, and its
+  configuration errors are raised.
 """
-import os
-
 import numpy as np
 import pytest
 
-from pyrecon_amd import utils, cli
+from pyrecon_amd import utils, cli, catalog_io
 
 
 def test_sky_cartesian_round_trip():
@@ -22,8 +23,12 @@ def test_sky_cartesian_round_trip():
     dist, ra, dec = utils.cartesian_to_sky(pos)
     assert np.all((ra >= 0.) & (ra < 360.)) and np.all(np.abs(dec) <= 90.)
     assert np.allclose(utils.sky_to_cartesian(dist, ra, dec), pos, rtol=0, atol=1e-9)
-    d, r, de = utils.cartesian_to_sky(np.array([[2., 0., 0.], [0., 3., 0.], [0., 0., 4.], [0., -1., 0.]]))
-    assert np.allclose(d, [2., 3., 4., 1.]) and np.allclose(r[[0, 1, 3]], [0., 90., 270.]) and np.allclose(de[2], 90.)
+    d, r, de = utils.cartesian_to_sky(np.array([[2., 0., 0.], [0., 3., 0.], [0., 0., 4.], [0., -1., 0.], [-1., -1e-300, 0.]]))
+    assert np.allclose(d, [2., 3., 4., 1., 1.]) and np.allclose(r[[0, 1, 3, 4]], [0., 90., 270., 180.])
+    assert np.allclose(de, [0., 0., 90., 0., 0.])
+    d, r, de = utils.cartesian_to_sky(np.array([[0., -1., 1.]]), wrap=False, degree=False)
+    assert np.allclose(r, -np.pi / 2.) and np.allclose(de, np.pi / 4.) and np.allclose(d, np.sqrt(2.))
+    assert utils.sky_to_cartesian([1.], [0.], [0.], dtype='f4').dtype == np.float32
 
 
 def test_flat_lcdm_distance():
@@ -33,23 +38,51 @@ def test_flat_lcdm_distance():
     lcdm = utils.FlatLCDMDistance(0.3)
     assert np.allclose(lcdm([0.5, 1.]), [1888.63 * 0.7, 3303.83 * 0.7], rtol=2e-5)
     inv = utils.DistanceToRedshift(lcdm)
-    zz = np.linspace(0.01, 3., 50)
-    assert np.allclose(inv(lcdm(zz)), zz, rtol=1e-6)
+    zz = np.linspace(0.001, 3., 50)
+    assert np.allclose(inv(lcdm(zz)), zz, rtol=1e-10)
+    with pytest.raises(ValueError):
+        inv(-1.)
+    # a plain callable (no derivative): interpolation only
+    inv2 = utils.DistanceToRedshift(lambda z: 3000. * z, zmax=10.)
+    assert np.allclose(inv2([0., 300., 1500.]), [0., 0.1, 0.5], rtol=1e-8)
 
 
-def test_eval_strings_and_paths():
-    assert cli.decode_eval_str('(${RA}>0.) & (${DEC}<${RA})') == ('(RA>0.) & (DEC<RA)', ['RA', 'DEC'])
-    assert cli.decode_eval_str(None) == ('', [])
-    assert cli.sepjoin('/galaxies', 'RA') == '/galaxies/RA' and cli.sepjoin('/a/', '/b') == '/b'
-    assert cli.make_list('A') == ['A'] and cli.remove_duplicates(['A', 'B', 'A']) == ['A', 'B']
+def test_normalize_and_layers():
+    assert cli.normalize('1e-5') == 1e-5 and cli.normalize('None') is None and cli.normalize('') is None
+    assert cli.normalize([]) is None and cli.normalize('f4') == 'f4' and cli.normalize(0) == 0
+    s = cli.Settings({'input': {'mask': '${A} > 0', 'mask_randoms': '', 'weights': '${W}'}},
+                     {'mesh': {'nmesh': 64}})
+    assert s.get('mesh', 'nmesh') == 64 and s.get('mesh', 'dtype') == 'f4'  # layered over the defaults
+    assert s.suffixed('input', 'mask', 'data') == '${A} > 0'
+    assert s.suffixed('input', 'mask', 'randoms') is None  # present and empty: no mask for the randoms
+    assert s.suffixed('input', 'weights', 'randoms') == '${W}'
+    assert s.pop('algorithm', 'name') == 'MultiGridReconstruction' and s.get('algorithm', 'name') is None
 
 
-def test_yaml_loader(tmp_path):
-    import yaml
-    fn = tmp_path / 'c.yaml'
-    fn.write_text('a: 1e-5\nb: None\nc: 3\n')
-    out = yaml.load(open(fn), Loader=cli._yaml_loader())
-    assert out == {'a': 1e-5, 'b': None, 'c': 3}
+def test_column_expressions():
+    e = cli.ColumnExpression('(${RA}>0.) & (${DEC}<${RA})')
+    assert e.columns == ['RA', 'DEC']
+    cat = {'RA': np.array([1., -1., 3.]), 'DEC': np.array([0., -5., 4.])}
+    assert np.array_equal(e.evaluate(cat), [True, False, False])
+    w = cli.ColumnExpression('np.sqrt(${WEIGHT FKP}) * 2')  # names that are not identifiers
+    assert w.columns == ['WEIGHT FKP'] and np.allclose(w.evaluate({'WEIGHT FKP': np.array([4.])}), [4.])
+    assert not cli.ColumnExpression(None) and not cli.ColumnExpression('  ')
+
+
+def test_catalog_files(tmp_path):
+    cat = {'A': np.arange(5.), 'P': np.arange(15.).reshape(5, 3), 'I': np.arange(5)}
+    for ext in ('npz', 'npy'):
+        fn = str(tmp_path / ('c.' + ext))
+        catalog_io.write_catalog(fn, cat)
+        back = catalog_io.read_catalog(fn, ['P', 'I'])
+        assert list(back) == ['P', 'I'] and np.array_equal(back['P'], cat['P']) and np.array_equal(back['I'], cat['I'])
+        with pytest.raises(KeyError):
+            catalog_io.read_catalog(fn, ['missing'])
+    with pytest.raises(catalog_io.CatalogFormatError, match='not recognized'):
+        catalog_io.format_of('c.txt')
+    assert catalog_io.format_of('x.HDF5').name == 'HDF5'
+    assert catalog_io.hdf5_path('/galaxies', 'RA') == '/galaxies/RA' and catalog_io.hdf5_path('', 'RA') == '/RA'
+    assert catalog_io.hdf5_path('/galaxies/', '/b/c') == '/b/c' and catalog_io.hdf5_path('g', 'x/y') == '/g/x/y'
 
 
 def _write(tmp_path, n=500, seed=0):
@@ -65,50 +98,87 @@ def _write(tmp_path, n=500, seed=0):
     return cat, pos
 
 
+def _load(layer):
+    settings = cli.Settings(layer)
+    selection, plans, outputs = cli.plan(settings)
+    cosmology = cli.Cosmology(settings.get('cosmology', 'Omega_m'))
+    return selection, plans, outputs, {role: p.load(cosmology) for role, p in plans.items()}
+
+
 def test_load_inputs(tmp_path):
     cat, pos = _write(tmp_path)
-    config = cli.default_config()
-    cli.update_config(config, {'input': {'dir': str(tmp_path), 'data_fn': 'data.npz', 'randoms_fn': 'randoms.npz',
-                                         'rdz': ['RA', 'DEC', 'Z'], 'xyz_randoms': 'POSITION',
-                                         'weights': '${WEIGHT} * 2', 'mask': '(${RA} > 10.) & (${DEC} < 30.)',
-                                         'weights_randoms': '', 'mask_randoms': '${X} > 0.'},
-                               'output': {'data_fn': 'out.npz', 'columns': ['NZ'], 'columns_randoms': ''},
-                               'cosmology': {'Omega_m': 0.3}})
-    sel, ifn, ofn, catalogs, positions, weights, keep = cli.load_inputs(config)
-    assert sel == 'randoms' and set(ifn) == {'data', 'randoms'} and set(ofn) == {'data'}
+    selection, plans, outputs, inputs = _load(
+        {'input': {'dir': str(tmp_path), 'data_fn': 'data.npz', 'randoms_fn': 'randoms.npz',
+                   'rdz': ['RA', 'DEC', 'Z'], 'xyz_randoms': 'POSITION', 'weights': '${WEIGHT} * 2',
+                   'mask': '(${RA} > 10.) & (${DEC} < 30.)', 'weights_randoms': '', 'mask_randoms': '${X} > 0.'},
+         'output': {'data_fn': 'out.npz', 'columns': ['NZ'], 'columns_randoms': ''}, 'cosmology': {'Omega_m': 0.3}})
+    assert selection == 'randoms' and set(plans) == {'data', 'randoms'} and set(outputs) == {'data'}
     mask = (cat['RA'] > 10.) & (cat['DEC'] < 30.)
+    kept, positions, weights = inputs['data']
     dist = utils.FlatLCDMDistance(0.3)(cat['Z'][mask])
-    assert np.allclose(positions['data'], utils.sky_to_cartesian(dist, cat['RA'][mask], cat['DEC'][mask]))
-    assert np.array_equal(weights['data'], cat['WEIGHT'][mask] * 2)
-    assert np.array_equal(catalogs['data']['NZ'], cat['NZ'][mask]) and list(catalogs['data']) == ['NZ']
-    assert np.array_equal(positions['randoms'], pos[pos[:, 0] > 0.]) and weights['randoms'] is None
+    assert np.allclose(positions, utils.sky_to_cartesian(dist, cat['RA'][mask], cat['DEC'][mask]))
+    assert np.array_equal(weights, cat['WEIGHT'][mask] * 2)
+    assert list(kept) == ['NZ'] and np.array_equal(kept['NZ'], cat['NZ'][mask])
+    kept, positions, weights = inputs['randoms']
+    assert np.array_equal(positions, pos[pos[:, 0] > 0.]) and weights is None and kept == {}
     # three xyz columns, structured .npy input, no randoms -> uniform selection
-    config = cli.default_config()
-    cli.update_config(config, {'input': {'dir': str(tmp_path), 'data_fn': 'data.npy', 'xyz': ['RA', 'DEC', 'Z']}})
-    sel, ifn, ofn, catalogs, positions, weights, keep = cli.load_inputs(config)
-    assert sel == 'uniform' and np.array_equal(positions['data'], np.array([cat['RA'], cat['DEC'], cat['Z']]).T)
+    selection, plans, outputs, inputs = _load({'input': {'dir': str(tmp_path), 'data_fn': 'data.npy',
+                                                         'xyz': ['RA', 'DEC', 'Z']}})
+    assert selection == 'uniform' and np.array_equal(inputs['data'][1], np.array([cat['RA'], cat['DEC'], cat['Z']]).T)
+
+
+def test_output_positions():
+    s = cli.Settings({'output': {'xyz': 'POSITION', 'rdz_rec': ['RA_REC', 'DEC_REC', 'Z_REC'],
+                                 'xyz_rec_randoms': ['X', 'Y', 'Z']}})
+    data_in = cli.output_positions(s, 'data', False)
+    assert [(p.kind, p.columns) for p in data_in] == [('xyz', ['POSITION'])]
+    data_rec = cli.output_positions(s, 'data', True)
+    assert [(p.kind, p.columns) for p in data_rec] == [('rdz', ['RA_REC', 'DEC_REC', 'Z_REC'])]
+    ran_rec = cli.output_positions(s, 'randoms', True)
+    assert [(p.kind, p.columns) for p in ran_rec] == [('rdz', ['RA_REC', 'DEC_REC', 'Z_REC']), ('xyz', ['X', 'Y', 'Z'])]
+    cosmo = cli.Cosmology(0.3)
+    pos = utils.sky_to_cartesian(cosmo.distance(np.array([0.5, 1.])), [10., 20.], [-5., 30.])
+    cols = ran_rec[0].from_cartesian(pos, cosmo)
+    assert np.allclose(cols['RA_REC'], [10., 20.]) and np.allclose(cols['DEC_REC'], [-5., 30.])
+    assert np.allclose(cols['Z_REC'], [0.5, 1.], rtol=1e-10)
 
 
 @pytest.mark.parametrize('bad, match', [
-    ({'input': {'data_fn': 'data.npz'}}, 'Unknown input format'),
-    ({'input': {'data_fn': 'data.npz', 'rdz': ['RA', 'DEC', 'Z'], 'xyz': 'P'}}, 'two different input formats'),
+    ({'input': {'data_fn': 'data.npz'}}, 'no position columns'),
+    ({'input': {'data_fn': 'data.npz', 'rdz': ['RA', 'DEC', 'Z'], 'xyz': 'P'}}, 'two formats'),
+    ({'input': {'data_fn': 'data.npz', 'rdz_data': ['RA', 'DEC', 'Z'], 'xyz_data': 'P'}}, 'two formats'),
+    ({'input': {'data_fn': 'data.npz', 'rdz': ['RA', 'DEC']}}, '3 columns'),
+    ({'input': {'data_fn': 'data.npz', 'xyz': ['RA', 'DEC']}}, '1 column'),
     ({'input': {'data_fn': 'data.npz', 'rdz': ['RA', 'DEC', 'Z']}}, 'Omega_m'),
     ({'input': {'data_fn': 'data.fits', 'xyz': 'P'}}, 'fitsio'),
     ({'input': {'data_fn': 'data.txt', 'xyz': 'P'}}, 'not recognized'),
     ({'input': {'data_fn': 'data.npz', 'xyz': 'P'}, 'delta': {'selection_function': 'randoms'}}, 'randoms catalog'),
-    ({'input': {'xyz': 'P'}}, 'data catalog'),
-    ({'input': {'data_fn': 'data.npz', 'xyz': 'P'}, 'output': {'randoms_fn': 'r.npz'}}, 'Cannot output randoms'),
+    ({'input': {'data_fn': 'data.npz', 'xyz': 'P'}, 'delta': {'selection_function': 'other'}}, 'selection_function'),
+    ({'input': {'xyz': 'P'}}, 'no data catalog'),
+    ({'input': {'data_fn': 'data.npz', 'xyz': 'P'}, 'output': {'randoms_fn': 'r.npz'}}, 'without an input randoms'),
 ])
 def test_config_errors(tmp_path, bad, match):
     _write(tmp_path)
-    config = cli.default_config()
     bad = {k: dict(v) for k, v in bad.items()}
     bad.setdefault('input', {})['dir'] = str(tmp_path)
-    cli.update_config(config, bad)
     with pytest.raises(cli.ConfigError, match=match):
-        cli.load_inputs(config)
+        _load(bad)
 
 
-def test_unknown_convention(tmp_path):
+def test_unknown_convention_and_algorithm(tmp_path):
+    _write(tmp_path)
+    inp = {'dir': str(tmp_path), 'data_fn': 'data.npz', 'xyz': ['RA', 'DEC', 'Z']}
     with pytest.raises(cli.ConfigError, match='convention'):
-        cli.main([], input={'data_fn': str(tmp_path / 'x.npz'), 'xyz': 'P'}, algorithm={'convention': 'nope'})
+        cli.main([], input=inp, algorithm={'convention': 'nope'})
+    with pytest.raises(cli.ConfigError, match='algorithm'):
+        cli.main([], input=inp, algorithm={'name': 'NoSuchReconstruction'})
+
+
+def test_command_line_overrides(tmp_path):
+    import yaml
+    fn = tmp_path / 'c.yaml'
+    fn.write_text(yaml.safe_dump({'input': {'data_fn': 'a.npz'}, 'cosmology': {'f': 0.5, 'bias': 1.5},
+                                  'delta': {'smoothing_radius': '1e1'}}))
+    s = cli.settings_from([str(fn), '--data-fn', 'b.npz', '--galaxy-bias', '2.'], cosmology={'Omega_m': 0.31})
+    assert s.get('input', 'data_fn') == 'b.npz' and s.get('cosmology', 'bias') == 2. and s.get('cosmology', 'f') == 0.5
+    assert s.get('cosmology', 'Omega_m') == 0.31 and s.get('delta', 'smoothing_radius') == 10.

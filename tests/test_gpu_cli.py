@@ -1,4 +1,4 @@
-"""The command-line reconstruction on the device (pyrecon_amd/cli.py, after bin/pyrecon:70-423):
+"""The command-line reconstruction on the device (pyrecon_amd/cli.py, interface of bin/pyrecon:70-423):
 catalogs written to .npz files go through the configuration's conventions and come back equal,
 bit for bit, to the same calls made directly on the classes."""
 import numpy as np
@@ -94,3 +94,29 @@ def test_cli_rdz_round_trip(cats):
     expect = recon.read_shifted_positions(pos_in)
     back = utils.sky_to_cartesian(dist_of_z(out['Z_REC']), out['RA_REC'], out['DEC_REC'])
     assert np.max(np.abs(back - expect)) < 1e-6
+
+
+@pytest.mark.parametrize('convention', ['RecSym', 'RecIso', 'RSD'])
+def test_cli_matches_oracle(cats, convention):
+    # the command line against the CPU oracle's FP64 reconstruction of the same catalogs
+    # (oracle/recon_numpy.py; its C kernels are the reference's bit for bit): shifted positions of
+    # data and randoms within 1e-8 Mpc/h, as the drop-in classes are at configuration 1
+    from pyrecon_amd import cli
+    from oracle.recon_numpy import OracleMultiGridReconstruction
+    path, data, wdata, randoms, boxsize, boxcenter = cats
+    cli.main([], **_config(path, boxsize, boxcenter, 'MultiGridReconstruction', convention))
+    out_d = np.load(path / 'data_rec.npz')['POSITION_REC']
+    out_r = np.load(path / 'randoms_rec.npz')['POSITION_REC']
+    cpu = OracleMultiGridReconstruction(f=0.8, bias=2., los=None, nmesh=64, boxsize=boxsize, boxcenter=boxcenter,
+                                        dtype='f8')
+    cpu.assign_data(data, wdata)
+    cpu.assign_randoms(randoms)
+    cpu.set_density_contrast(smoothing_radius=15.)
+    cpu.run()
+    expect_d = cpu.read_shifted_positions(data, field='rsd' if convention == 'RSD' else 'disp+rsd')
+    if convention == 'RSD':
+        assert np.array_equal(out_r, randoms)
+    else:
+        expect_r = cpu.read_shifted_positions(randoms, field='disp+rsd' if convention == 'RecSym' else 'disp')
+        assert np.max(np.abs(out_r - expect_r)) < 1e-8
+    assert np.max(np.abs(out_d - expect_d)) < 1e-8

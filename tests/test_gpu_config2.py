@@ -47,9 +47,9 @@ def c2():
     return cat, geo, sample, ref
 
 
-def _gpu(cat, geo, dtype):
+def _gpu(cat, geo, dtype, precision='plain'):
     import pyrecon_amd
-    gpu = pyrecon_amd.MultiGridReconstruction(**RECON_KWARGS, **geo, dtype=dtype)
+    gpu = pyrecon_amd.MultiGridReconstruction(**RECON_KWARGS, **geo, dtype=dtype, precision=precision)
     gpu.assign_data(cat['data'], cat['data_weights'])
     gpu.assign_randoms(cat['randoms'], cat['randoms_weights'])
     return gpu
@@ -76,9 +76,9 @@ def test_config2_fp64(c2):
 
 def test_config2_mixed(c2):
     cat, geo, sample, ref = c2
-    gpu = _gpu(cat, geo, 'f8')
+    gpu = _gpu(cat, geo, 'f8', precision='mixed')
     gpu.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
-    gpu.run(precision='mixed')
+    gpu.run()
     err_phi = rel_l2(gpu.mesh_phi.value, ref['phi'])
     dx = max(np.max(np.abs(gpu.read_shifted_positions(cat['data']) - ref['data'])),
              np.max(np.abs(gpu.read_shifted_positions(sample) - ref['randoms'])))

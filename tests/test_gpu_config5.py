@@ -1,0 +1,111 @@
+"""Configuration 5, the serving stream: 24 jobs drawn from its distribution (nmesh in {128, 256, 512}
+with probabilities {0.5, 0.35, 0.15}, boxsize 7.8 nmesh Mpc/h, data lognormal with median 1e6 clipped
+to [1e5, 2e7], randoms 10x), all submitted at once, so the server runs saturated: four jobs on the
+device at a time and solves of the same mesh shape grouped.
+
+- Every job's shifted positions (all data, and every 37th random) are bit-identical in the mix and
+  run alone through the drop-in class, for the mixed path.
+- Grouped solves form: at least one group holds more than one job.
+- Every job's mixed result is within 1e-3 Mpc/h of its plain FP64 result,
+  which is the reference's FP64 algorithm bit for bit.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from catalogs import serving_stream, serving_catalogs, serving_job, RECON_KWARGS, SMOOTHING_RADIUS
+
+pytestmark = pytest.mark.gpu
+
+NJOBS = 24
+STRIDE = 37
+PAINT_CHUNK = 1 << 23
+T0 = time.time()
+
+
+def say(msg):
+    print('[config5 {:6.1f}s] {}'.format(time.time() - T0, msg), flush=True)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view('u{:d}'.format(a.dtype.itemsize))
+
+
+def alone(spec, precision):
+    import pyrecon_amd
+    boxsize, boxcenter, data, randoms = serving_catalogs(spec)
+    recon = pyrecon_amd.MultiGridReconstruction(**RECON_KWARGS, nmesh=spec['nmesh'], boxsize=boxsize,
+                                                boxcenter=boxcenter, dtype='f8', paint_chunk=PAINT_CHUNK,
+                                                precision=precision)
+    dpos, dw = data(None)
+    rpos, rw = randoms(None)
+    recon.assign_data(dpos, dw)
+    recon.assign_randoms(rpos, rw)
+    recon.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    recon.run(precision=precision)
+    xd = recon.read_shifted_positions(dpos).numpy()
+    xr = recon.read_shifted_positions(rpos).numpy()[::STRIDE].copy()
+    return xd, xr
+
+
+@pytest.fixture(scope='module')
+def stream():
+    specs = serving_stream(NJOBS, rate=4.)
+    say('jobs: nmesh {}, ndata {}'.format([s['nmesh'] for s in specs], [s['ndata'] for s in specs]))
+    return specs
+
+
+@pytest.fixture(scope='module')
+def plain_alone(stream):
+    out = [alone(spec, 'plain') for spec in stream]
+    say('plain FP64, alone')
+    return out
+
+
+def test_config5_mix_equals_alone(stream):
+    from pyrecon_amd.serving import ReconServer
+    expect = [alone(spec, 'mixed') for spec in stream]
+    say('mixed, alone')
+    got = [None] * len(stream)
+    with ReconServer(budget_bytes=200 * 10**9, slots=4, paint_chunk=PAINT_CHUNK) as server:
+        futures = [server.submit(serving_job(spec, 'mixed', name=str(i))) for i, spec in enumerate(stream)]
+        for i, fut in enumerate(futures):
+            res = fut.result(timeout=600)
+            got[i] = (res.shifted_data.numpy(), res.shifted_randoms.numpy()[::STRIDE].copy(), res.group_size)
+            res.shifted_data = res.shifted_randoms = None
+        report = server.report()
+    say('mix: {}'.format(report))
+    for i, ((xd, xr), (gd, gr, _)) in enumerate(zip(expect, got)):
+        assert np.array_equal(bits(gd), bits(xd)), 'job {:d}: data differ between alone and mix'.format(i)
+        assert np.array_equal(bits(gr), bits(xr)), 'job {:d}: randoms differ between alone and mix'.format(i)
+    groups = [g for _, _, g in got]
+    say('group sizes: {}'.format(groups))
+    assert max(groups) > 1, 'no grouped solve formed under saturation'
+    assert report['mean_group'] > 1.
+    assert report['jobs'] == len(stream)
+
+
+def test_config5_mixed_vs_plain(stream, plain_alone):
+    worst = (0., None)
+    for i, (spec, (pd, pr)) in enumerate(zip(stream, plain_alone)):
+        md, mr = alone(spec, 'mixed')
+        dx = max(np.max(np.abs(md - pd)), np.max(np.abs(mr - pr)))
+        if dx >= worst[0]: worst = (dx, i)
+    i = worst[1]
+    say('mixed against plain FP64, worst job {:d} (nmesh {:d}, ndata {:d}): max |dx| {:.3e} Mpc/h'.format(
+        i, stream[i]['nmesh'], stream[i]['ndata'], worst[0]))
+    assert worst[0] <= 1e-3
+
+
+def test_sparse_job_mixed_vs_plain():
+    """The sparsest field of the 256-job stream (job 191: nmesh 512, 3.3e5 data, 0.025 randoms per
+    cell), where the mean of the density contrast dominates the residual: the mixed path stays within
+    1e-3 Mpc/h of plain FP64."""
+    spec = serving_stream(256, rate=4.)[191]
+    assert spec['nmesh'] == 512 and spec['ndata'] < 400000
+    pd, pr = alone(spec, 'plain')
+    md, mr = alone(spec, 'mixed')
+    dx = max(np.max(np.abs(md - pd)), np.max(np.abs(mr - pr)))
+    say('sparse job 191, mixed against plain FP64: max |dx| {:.3e} Mpc/h'.format(dx))
+    assert dx <= 1e-3

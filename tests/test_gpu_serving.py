@@ -53,7 +53,8 @@ def alone(spec, seed):
     nmesh, ndata, precision, original, los = spec
     boxsize, boxcenter, data, randoms = catalogs(spec, seed)
     cls = pyrecon_amd.OriginalMultiGridReconstruction if original else pyrecon_amd.MultiGridReconstruction
-    recon = cls(f=0.8, bias=2., los=los, nmesh=nmesh, boxsize=boxsize, boxcenter=boxcenter, dtype='f8')
+    recon = cls(f=0.8, bias=2., los=los, nmesh=nmesh, boxsize=boxsize, boxcenter=boxcenter, dtype='f8',
+                precision=precision)
     dpos, dw = data(None)
     rpos, rw = randoms(None)
     recon.assign_data(dpos, dw)

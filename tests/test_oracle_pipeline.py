@@ -142,6 +142,22 @@ def test_catalog_regeneration_is_frozen():
     from catalogs import make_catalogs, sha256
     import json, os
     fixture = json.load(open(os.path.join(os.path.dirname(__file__), 'golden', 'catalogs_sha256.json')))
-    c1 = make_catalogs('c1')
-    assert sha256(c1['data'], c1['data_weights']) == fixture['c1']['data']
-    assert sha256(c1['randoms'], c1['randoms_weights']) == fixture['c1']['randoms']
+    for name in ('c1', 'c2'):
+        cat = make_catalogs(name)
+        assert sha256(cat['data'], cat['data_weights']) == fixture[name]['data'], name
+        assert sha256(cat['randoms'], cat['randoms_weights']) == fixture[name]['randoms'], name
+
+
+def test_threaded_smoothing_matches_numpy():
+    # the oracle's scipy.fft route (several threads, for 512^3 meshes) against numpy's engine
+    r = make(nmesh=(24, 16, 20), boxsize=(800., 600., 1000.))
+    rng = np.random.RandomState(3)
+    value = rng.uniform(0., 1., r.shape)
+    ref = r.smooth_gaussian(value, 15.)
+    r.fft_workers = 4
+    out = r.smooth_gaussian(value, 15.)
+    assert np.max(np.abs(out - ref)) < 1e-14 * np.max(np.abs(ref))
+    grids = np.meshgrid(*[np.arange(n) for n in r.shape], indexing='ij')
+    field = np.cos(2 * np.pi * (2 * grids[0] / 24 + grids[2] / 20))
+    k2 = (2 * np.pi * 2 / 800.)**2 + (2 * np.pi / 1000.)**2
+    assert np.allclose(r.smooth_gaussian(field, 15.), np.exp(-0.5 * 15.**2 * k2) * field, atol=1e-12)
