@@ -51,7 +51,10 @@ def parse_args():
     p.add_argument('--no-accuracy', action='store_true')
     p.add_argument('--kernel-only', default=None, choices=['jacobi32', 'jacobi64', 'copy', 'fmg_mixed', 'fmg64'],
                    help='run only one kernel repeatedly (for rocprofv3 counter runs)')
-    p.add_argument('--workers', type=int, default=min(16, os.cpu_count() or 1))
+    p.add_argument('--workers', type=int, default=len(os.sched_getaffinity(0)),
+                   help='host threads (catalog generation, CPU baseline); default: the cores this process may use')
+    p.add_argument('--cpu-baseline-full', action='store_true',
+                   help='time one full-size CPU reconstruction (minutes) and print it instead of the GPU bench')
     p.add_argument('--serving-jobs', type=int, default=256, help='jobs of the serving stream (0: skip it)')
     p.add_argument('--serving-rate', type=float, default=4.0, help='Poisson arrival rate of the serving stream (jobs/s)')
     p.add_argument('--serving-only', action='store_true', help='run only the serving stream')
@@ -62,33 +65,36 @@ def parse_args():
     return p.parse_args()
 
 
-def serving_bench(args, precision, stream_spec, keep=None, reference=None):
-    """Run the serving stream through one ReconServer; returns the report and, for the jobs in
-    ``keep``, the shifted data positions (host); with ``reference``, errors against it."""
+def serving_bench(args, precision, stream_spec, saturate=False, group=True, keep=False, reference=None):
+    """Run the serving stream through one ReconServer.  ``saturate``: every job submitted at once
+    (instead of at its Poisson arrival time).  ``keep``: return every job's shifted data positions
+    (host); with ``reference`` (such a dict of another run), the per-job errors against it."""
     from pyrecon_amd.serving import ReconServer
     from catalogs import serving_job
-    keep = set(keep or [])
-    kept, errors = {}, []
-
-    def make_job(i, spec):
-        return serving_job(spec, precision, name=str(i))
+    kept = {}
 
     def done(i, fut):
         res = fut.result()
-        if i in keep:
-            kept[i] = res.shifted_data.numpy()
+        if keep or reference is not None:
+            x = res.shifted_data.numpy()
+            if reference is not None:
+                d = np.abs(x - reference[i])
+                kept[i] = (float(d.max()), float(np.sum(d * d)), d.size)
+            else:
+                kept[i] = x
         res.shifted_data = res.shifted_randoms = None
 
-    server = ReconServer(budget_bytes=200 * 10**9, slots=4, paint_chunk=1 << 23)
+    server = ReconServer(budget_bytes=200 * 10**9, slots=4, paint_chunk=1 << 23, group_solves=group)
     futures = []
     t0 = time.perf_counter()
     for i, spec in enumerate(stream_spec):
-        delay = t0 + spec['arrival'] - time.perf_counter()
-        if delay > 0: time.sleep(delay)
-        fut = server.submit(make_job(i, spec))
+        if not saturate:
+            delay = t0 + spec['arrival'] - time.perf_counter()
+            if delay > 0: time.sleep(delay)
+        fut = server.submit(serving_job(spec, precision, name=str(i)))
         fut.add_done_callback(lambda f, i=i: done(i, f))
         futures.append(fut)
-        if (i + 1) % 32 == 0:
+        if (i + 1) % 64 == 0:
             log('serving {}: submitted {:d} jobs'.format(precision, i + 1))
     for fut in futures: fut.result()
     report = server.report()
@@ -96,36 +102,50 @@ def serving_bench(args, precision, stream_spec, keep=None, reference=None):
     for res in server.results:
         per_mesh.setdefault(res.job.nmesh[0], []).append(res.latency)
     report['p50_s_by_nmesh'] = {str(k): round(float(np.percentile(v, 50)), 4) for k, v in sorted(per_mesh.items())}
+    report['max_group'] = int(max(r.group_size for r in server.results))
     server.close()
     if reference is not None:
-        dx = np.concatenate([np.abs(kept[i] - reference[i]).ravel() for i in sorted(kept)])
-        report['shift_max_mpc_h'] = float(dx.max())
-        report['shift_rms_mpc_h'] = float(np.sqrt(np.mean(dx**2)))
+        worst = max(kept, key=lambda i: kept[i][0])
+        report['shift_max_mpc_h'] = kept[worst][0]
+        report['shift_rms_mpc_h'] = float(np.sqrt(sum(v[1] for v in kept.values()) / sum(v[2] for v in kept.values())))
         report['error_jobs'] = len(kept)
-        per = {i: float(np.abs(kept[i] - reference[i]).max()) for i in sorted(kept)}
-        worst = max(per, key=per.get)
-        report['worst_job'] = dict(index=worst, nmesh=stream_spec[worst]['nmesh'],
-                                   ndata=stream_spec[worst]['ndata'], shift_max_mpc_h=per[worst])
+        report['worst_job'] = dict(index=worst, nmesh=stream_spec[worst]['nmesh'], ndata=stream_spec[worst]['ndata'],
+                                   shift_max_mpc_h=kept[worst][0])
+        return report, None
     return report, kept
 
 
 def run_serving(args):
+    """Configuration 5.  At its Poisson arrival rate (4 jobs/s): latency percentiles, plain and mixed.
+    Saturated (all 256 jobs submitted at once): throughput with grouped solves and without, and the
+    mixed path's shifted-position error against plain FP64 over every job."""
     from catalogs import serving_stream
     spec = serving_stream(args.serving_jobs, args.serving_rate)
-    keep = [i for i, s in enumerate(spec) if s['ndata'] <= 4 * 10**6][::8]
     log('serving stream: {:d} jobs at {:.1f}/s'.format(len(spec), args.serving_rate))
-    plain, ref = serving_bench(args, 'plain', spec, keep=keep)
+
+    def fmt(rep):
+        return {k: (float('{:.5g}'.format(v)) if isinstance(v, float) else v) for k, v in rep.items()}
+
+    plain, _ = serving_bench(args, 'plain', spec)
     log('serving plain', plain)
-    mixed, _ = serving_bench(args, 'mixed', spec, keep=keep, reference=ref)
+    mixed, _ = serving_bench(args, 'mixed', spec)
     log('serving mixed', mixed)
+    sat_plain, ref = serving_bench(args, 'plain', spec, saturate=True, keep=True)
+    log('saturated plain', sat_plain)
+    sat_mixed, _ = serving_bench(args, 'mixed', spec, saturate=True, reference=ref)
+    del ref
+    log('saturated mixed', sat_mixed)
+    sat_mixed_nogroup, _ = serving_bench(args, 'mixed', spec, saturate=True, group=False)
+    log('saturated mixed, ungrouped', sat_mixed_nogroup)
     return dict(workload='configuration 5: {:d} jobs, Poisson {:.1f}/s, nmesh {{128,256,512}} p={{.5,.35,.15}}, '
                          'boxsize 7.8 nmesh, data lognormal median 1e6 [1e5, 2e7], randoms 10x, '
-                         '200 GB budget, 4 slots, grouped solves'.format(len(spec), args.serving_rate),
+                         '200 GB budget, 4 slots'.format(len(spec), args.serving_rate),
                 catalogs='generated on the device per job from its seed (uniform randoms; data Zel\'dovich-displaced '
                          'by 256 random Fourier modes of P(k)), inside the job latency',
-                mixed={k: (float('{:.5g}'.format(v)) if isinstance(v, float) else v) for k, v in mixed.items()},
-                plain={k: (float('{:.5g}'.format(v)) if isinstance(v, float) else v) for k, v in plain.items()},
-                error_against='plain FP64 serving run of the same jobs (subset)')
+                mixed=fmt(mixed), plain=fmt(plain),
+                saturated=dict(note='all jobs submitted at once',
+                               plain=fmt(sat_plain), mixed=fmt(sat_mixed), mixed_ungrouped=fmt(sat_mixed_nogroup)),
+                error_against='plain FP64 run of the same jobs (saturated), every job\'s shifted data positions')
 
 
 def kernel_only(args):
@@ -288,68 +308,138 @@ def run_large(args):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(cat, threads):
-    """Reference C kernels (oracle/_ref, built from the reference's own src/*.c) plus the
-    reference's numpy glue, timed on bounded samples of each stage of one reconstruction
-    and scaled to the full workload."""
+def host_cores():
+    """(usable, physical): the cores this process may run on, and the physical cores lscpu reports."""
+    usable = len(os.sched_getaffinity(0))
+    physical = None
+    try:
+        import subprocess
+        out = subprocess.run(['lscpu', '-p=CORE,SOCKET'], capture_output=True, text=True, timeout=10).stdout
+        pairs = {tuple(line.split(',')) for line in out.splitlines() if line and not line.startswith('#')}
+        physical = len(pairs) or None
+    except Exception:
+        pass
+    return usable, physical
+
+
+def cpu_baseline(cat, threads, full=False):
+    """The reference's C kernels (oracle/_ref, built from the reference's own src/*.c) plus the
+    reference's numpy glue, timed stage by stage for one reconstruction of the workload.
+
+    Default: bounded samples of each stage scaled to the workload (paint and shifts of 4e6 randoms,
+    one c2c FFT x4, the density on a 1/8 slab x8, one finest-level V-cycle x 6 x 8/7), with
+    ``threads`` OpenMP threads, plus the multigrid and the read at one thread (a 1/8-size
+    multigrid problem x8).  ``full``: every stage at full size, once (minutes of CPU time)."""
     from oracle.cpu import RefKernels, OracleKernels
     from oracle.recon_numpy import OracleMultiGridReconstruction
     kind = 'reference'
     try:
         kern = RefKernels()
-        kern.set_num_threads(threads)
     except OSError:
         kind, kern = 'port', OracleKernels()
-    os.environ['OMP_NUM_THREADS'] = str(threads)
     cfg = cat['config']
     n = cfg['nmesh']
+
+    def use_threads(k):
+        os.environ['OMP_NUM_THREADS'] = str(k)
+        if kind == 'reference': kern.set_num_threads(k)
+
+    use_threads(threads)
     recon = OracleMultiGridReconstruction(f=0.8, bias=2., los=None, nmesh=n, boxsize=cfg['boxsize'],
                                           boxcenter=cfg['boxcenter'], dtype='f8', kernels=kern)
     ntot = len(cat['data']) + len(cat['randoms'])
-    stages = {}
-    # 1. paint (serial C loop + numpy transform/checks): a sample of the randoms
-    ns = min(4000000, len(cat['randoms']))
+    stages, one = {}, {}
+    ns = ntot if full else min(4000000, len(cat['randoms']))
+    # 1. paint: the reference's serial C loop (one thread whatever OMP_NUM_THREADS) + numpy glue
     t = time.perf_counter()
-    recon.assign_randoms(cat['randoms'][:ns], cat['randoms_weights'][:ns])
+    if full:
+        recon.assign_data(cat['data'], cat['data_weights'])
+        recon.assign_randoms(cat['randoms'], cat['randoms_weights'])
+    else:
+        recon.assign_randoms(cat['randoms'][:ns], cat['randoms_weights'][:ns])
     stages['paint'] = (time.perf_counter() - t) * ntot / ns
-    # 2. smoothing: numpy c2c transforms; one forward transform timed, 4 transforms per reconstruction
-    mesh = recon.mesh_randoms
+    log('cpu baseline: paint {:.2f} s'.format(stages['paint']))
+    # 2.-3. smoothing (numpy c2c, single-threaded) and the density contrast
+    if full:
+        t = time.perf_counter()
+        recon.set_density_contrast(smoothing_radius=15.)
+        stages['smoothing+density'] = time.perf_counter() - t
+        delta = recon.mesh_delta
+    else:
+        mesh = recon.mesh_randoms
+        t = time.perf_counter()
+        spec = np.fft.fftn(mesh)
+        t_fft = time.perf_counter() - t
+        del spec
+        slab = mesh[:n // 8]
+        t = time.perf_counter()
+        s_d, s_r = np.sum(slab), np.sum(slab)
+        alpha = s_d * 1. / s_r
+        dslab = slab - alpha * slab
+        mask = slab > 0.01 * s_r / ns
+        dslab[mask] /= (2. * alpha * slab[mask])
+        dslab[~mask] = 0.
+        stages['smoothing+density'] = 4 * t_fft + (time.perf_counter() - t) * 8
+        delta = np.random.default_rng(1).standard_normal((n, n, n))
+    log('cpu baseline: smoothing+density {:.2f} s'.format(stages['smoothing+density']))
+    # 4. multigrid
     t = time.perf_counter()
-    spec = np.fft.fftn(mesh)
-    t_fft = time.perf_counter() - t
-    del spec
-    stages['smoothing'] = 4 * t_fft
-    # 3. density contrast: full-mesh numpy passes on a 1/8 slab, scaled
-    slab = mesh[:n // 8]
+    if full:
+        phi = kern.fmg(delta, cfg['boxsize'], cfg['boxcenter'], 0.4, 0.4, 5, 6, None)
+        stages['multigrid'] = time.perf_counter() - t
+    else:
+        v = np.zeros_like(delta)
+        kern.vcycle(v, delta, cfg['boxsize'], cfg['boxcenter'], 0.4, 0.4, 5, None)
+        stages['multigrid'] = (time.perf_counter() - t) * 6 * 8. / 7.
+        phi = v
+        # one thread: the same V-cycle on a 1/8-size problem, x 8
+        use_threads(1)
+        h = n // 2
+        small = np.ascontiguousarray(delta[:h, :h, :h])
+        vs = np.zeros_like(small)
+        t = time.perf_counter()
+        kern.vcycle(vs, small, [b / 2. for b in np.broadcast_to(cfg['boxsize'], 3)], cfg['boxcenter'], 0.4, 0.4, 5,
+                    None)
+        one['multigrid'] = (time.perf_counter() - t) * 8 * 6 * 8. / 7.
+        del vs, small
+        use_threads(threads)
+    log('cpu baseline: multigrid {:.2f} s'.format(stages['multigrid']))
+    # 5. shifts: the finite-difference read (OpenMP C) + numpy RSD algebra
+    recon.mesh_phi = phi
     t = time.perf_counter()
-    s_d, s_r = np.sum(slab), np.sum(slab)
-    alpha = s_d * 1. / s_r
-    delta = slab - alpha * slab
-    mask = slab > 0.01 * s_r / ns
-    delta[mask] /= (2. * alpha * slab[mask])
-    delta[~mask] = 0.
-    stages['density'] = (time.perf_counter() - t) * 8
-    # 4. multigrid: one finest-level V-cycle timed; FMG = 6 V-cycles per level, the
-    #    coarser levels add 1/7 of the finest-level cost (cell updates scale as 8^-l)
-    rng = np.random.default_rng(1)
-    f = rng.standard_normal((n, n, n))
-    v = np.zeros_like(f)
-    t = time.perf_counter()
-    kern.vcycle(v, f, cfg['boxsize'], cfg['boxcenter'], 0.4, 0.4, 5, None)
-    stages['multigrid'] = (time.perf_counter() - t) * 6 * 8. / 7.
-    del f, v
-    # 5. shifts: finite-difference read + numpy RSD algebra on a sample
-    recon.mesh_phi = mesh
-    t = time.perf_counter()
-    recon.read_shifted_positions(cat['randoms'][:ns])
+    if full:
+        recon.read_shifted_positions(cat['data'])
+        recon.read_shifted_positions(cat['randoms'])
+    else:
+        recon.read_shifted_positions(cat['randoms'][:ns])
     stages['shifts'] = (time.perf_counter() - t) * ntot / ns
+    if not full:
+        use_threads(1)
+        t = time.perf_counter()
+        recon.read_shifted_positions(cat['randoms'][:ns // 4])
+        one['shifts'] = (time.perf_counter() - t) * 4 * ntot / ns
+        use_threads(threads)
+    log('cpu baseline: shifts {:.2f} s'.format(stages['shifts']))
     total = sum(stages.values())
-    return dict(value=1. / total, unit='recon/s', cores=threads, kind=kind,
-                sample=('one reconstruction of the same workload timed stage by stage on samples: paint and shifts '
-                        'of {:d} randoms scaled to {:d} points, one {n}^3 c2c FFT x4, density on a 1/8 slab x8, '
-                        'one {n}^3 V-cycle x6x8/7; reference src/*.c via oracle/_ref with {:d} OpenMP threads, '
-                        'numpy glue single-threaded').format(ns, ntot, threads, n=n),
-                stages_s={k: round(v, 3) for k, v in stages.items()})
+    usable, physical = host_cores()
+    out = dict(value=1. / total, unit='recon/s', cores=threads, kind=kind,
+               host_cores=dict(usable=usable, physical=physical),
+               stages_s={k: round(v, 3) for k, v in stages.items()})
+    if full:
+        out['sample'] = ('one full reconstruction of the workload ({:d} points, {n}^3): the reference\'s serial paint, '
+                         'numpy c2c smoothing and density, full FMG, reads of all data and randoms; reference '
+                         'src/*.c via oracle/_ref with {:d} OpenMP threads').format(ntot, threads, n=n)
+    else:
+        one_total = stages['paint'] + stages['smoothing+density'] + one['multigrid'] + one['shifts']
+        out['one_thread'] = dict(value=1. / one_total, unit='recon/s',
+                                 stages_s={k: round(v, 3) for k, v in dict(stages, **one).items()})
+        out['sample'] = ('one reconstruction of the same workload timed stage by stage on samples: paint and shifts '
+                         'of {:d} randoms scaled to {:d} points, one {n}^3 c2c FFT x4, density on a 1/8 slab x8, '
+                         'one {n}^3 V-cycle x6x8/7; reference src/*.c via oracle/_ref with {:d} OpenMP threads '
+                         '(the paint loop is serial in the reference), numpy glue single-threaded; one_thread: '
+                         'the multigrid and the read at one thread ({h}^3 V-cycle x8, read of {:d} randoms '
+                         'scaled)').format(ns, ntot, threads, ns // 4, n=n, h=n // 2)
+    return out
 
 
 def load_catalogs(args, make_catalogs):
@@ -383,6 +473,12 @@ def main():
         return kernel_only(args)
     if args.serving_only:
         print(json.dumps(dict(metric=METRIC, serving=run_serving(args))), flush=True)
+        return
+    if args.cpu_baseline_full:
+        from catalogs import make_catalogs
+        cat = load_catalogs(args, make_catalogs)
+        print(json.dumps(dict(metric=METRIC, config=args.config,
+                              cpu_baseline=cpu_baseline(cat, threads=args.workers, full=True))), flush=True)
         return
     if args.config == 'c4':
         return run_large(args)

@@ -281,6 +281,18 @@ int prh_martin_contrast_f64(prh_workspace *ws, const double *data, const double 
 int prh_subtract_nonzero_mean_f32(float *delta, size_t size, float mean, void *stream);
 int prh_subtract_nonzero_mean_f64(double *delta, size_t size, double mean, void *stream);
 
+/* replaces BaseMesh.__mul__ / __div__ / __rdiv__ and their in-place forms
+ * (pyrecon/mesh.py:156-183) for a Python scalar: out[i] = a[i] * s (op
+ * PRH_OP_MUL), a[i] / s (PRH_OP_DIV) or s / a[i] (PRH_OP_RDIV), with s cast to
+ * the mesh precision first (numpy's rule for a Python scalar).  out may be a.  */
+#define PRH_OP_MUL 0
+#define PRH_OP_DIV 1
+#define PRH_OP_RDIV 2
+int prh_mesh_scalar_op_f32(const float *a, float *out, size_t size, int op, double scalar,
+                           void *stream);
+int prh_mesh_scalar_op_f64(const double *a, double *out, size_t size, int op, double scalar,
+                           void *stream);
+
 /* Grouped full multigrid (no reference counterpart; for serving several
  * reconstructions at once): count (1..8) independent problems of the same
  * nmesh and precision, job j with its own workspace ws[j], source f1h[j],

@@ -33,6 +33,7 @@ FFT_COMPLEX, FFT_REAL_IN, FFT_REAL_OUT = 0, 1, 2
 SPEC_PLANE_PARALLEL, SPEC_INV_K2, SPEC_LOS_SQ, SPEC_KA_KB, SPEC_GRAD, SPEC_I_KA = 0, 1, 2, 3, 4, 5
 
 WS_MULTIGRID, WS_PAINT, WS_FFT, WS_MIXED = 1, 2, 4, 8
+OP_MUL, OP_DIV, OP_RDIV = 0, 1, 2
 WS_ALL = 15
 
 
@@ -117,6 +118,7 @@ def _signatures():
             'prh_martin_sums_' + suffix: (_i, [_vp, _vp, _vp, _sz, real, _vp, _vp]),
             'prh_martin_contrast_' + suffix: (_i, [_vp, _vp, _vp, _vp, _sz, real, real, real, _vp, _vp]),
             'prh_subtract_nonzero_mean_' + suffix: (_i, [_vp, _sz, real, _vp]),
+            'prh_mesh_scalar_op_' + suffix: (_i, [_vp, _vp, _sz, _i, _d, _vp]),
             'prh_fft3d_' + suffix: (_i, [_vp, _vp, _vp, _szp, _i, _i, _vp]),
             'prh_spectral_' + suffix: (_i, [_vp, _i, _vp, _vp, _szp, _vp, _vp, _d, _d, _i, _i, _vp]),
             'prh_rsd_update_' + suffix: (_i, [_vp, _vp, _szp, _vp, _vp, _d, _i, _i, _vp]),

@@ -217,13 +217,11 @@ template <typename R, typename P, typename W, int MODE>
 __global__ __launch_bounds__(kCicThreads) void k_paint_key(
     const P* __restrict__ pos, const W* __restrict__ w, size_t m, Frame<R> F,
     uint32_t* __restrict__ key, Payload<R>* __restrict__ payload, uint32_t* __restrict__ misc,
-    int check_only, uint64_t* __restrict__ hpart) {
+    int check_only) {
   const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   uint32_t err = 0;
-  uint64_t hash = 0;
   if (i < m) {
     const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-    if (hpart) hash = point_hash<P>(i, x);
     R p[3];
     if (MODE == 0) {
       for (int d = 0; d < 3; d++) p[d] = static_cast<R>(x[d]);
@@ -251,7 +249,6 @@ __global__ __launch_bounds__(kCicThreads) void k_paint_key(
     for (int off = 32; off > 0; off >>= 1) e |= __shfl_xor(e, off);
     if ((threadIdx.x & 63) == 0) atomicOr(&misc[0], e);
   }
-  if (hpart) block_hash_partial(hash, hpart);
 }
 
 // Exclusive scan of count[0, n) into start[0, n] (start[n] = total), zeroing
@@ -592,24 +589,12 @@ __global__ __launch_bounds__(kCicThreads) void k_cell_start(const uint32_t* __re
 
 // Paint step 4: payload into cell-grouped order, so that step 5 reads each
 // group contiguously.
-// With `spos`, the positions are laid out in the same order (kept for reads).
-template <typename R, typename P>
+template <typename R>
 __global__ __launch_bounds__(kCicThreads) void k_paint_permute(const uint32_t* __restrict__ sorted,
                                                                const Payload<R>* __restrict__ payload,
-                                                               Payload<R>* __restrict__ spay, size_t m,
-                                                               const P* __restrict__ pos,
-                                                               P* __restrict__ spos) {
+                                                               Payload<R>* __restrict__ spay, size_t m) {
   const size_t s = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
-  if (s >= m) return;
-  const uint32_t i = sorted[s];
-  spay[s] = payload[i];
-  if (spos) {
-    const P x0 = pos[3 * static_cast<size_t>(i)], x1 = pos[3 * static_cast<size_t>(i) + 1],
-            x2 = pos[3 * static_cast<size_t>(i) + 2];
-    spos[3 * s] = x0;
-    spos[3 * s + 1] = x1;
-    spos[3 * s + 2] = x2;
-  }
+  if (s < m) spay[s] = payload[sorted[s]];
 }
 
 // CIC weight of a point at corner k = (a, b, c) of its base cell, in the
@@ -907,11 +892,20 @@ struct TileGrid {
   int t[3];  // tiles per axis
 };
 
+// A point's position (in the positions dtype) and FP32 weight, packed by the key
+// step into one aligned record so that the permute gathers a single record per point.
+template <typename P>
+struct alignas(4 * sizeof(P) > 16 ? 32 : 16) TileRec {
+  P x[3];
+  float w;
+};
+
 template <typename P, typename W, int MODE>
 __global__ __launch_bounds__(kCicThreads) void k_tile_key(const P* __restrict__ pos,
                                                           const W* __restrict__ w, size_t m,
                                                           Frame<double> F, TileGrid TG,
                                                           uint32_t* __restrict__ key,
+                                                          TileRec<P>* __restrict__ rec,
                                                           uint32_t* __restrict__ misc,
                                                           uint64_t* __restrict__ hpart) {
   const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
@@ -927,26 +921,47 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_key(const P* __restrict__ 
     for (int d = 0; d < 3; d++) b[d] = base_index(static_cast<int>(p[d]), F.n[d]);
     const uint32_t tile = (static_cast<uint32_t>(b[0] >> 3) * TG.t[1] + (b[1] >> 3)) * TG.t[2] + (b[2] >> 3);
     key[i] = (tile << kPTShift) | (((b[0] & 7) * 8 + (b[1] & 7)) * 8 + (b[2] & 7));
+    TileRec<P> r;
+    r.x[0] = x[0];
+    r.x[1] = x[1];
+    r.x[2] = x[2];
+    r.w = w ? static_cast<float>(w[i]) : 1.f;
+    rec[i] = r;
   }
   flag_paint_errors(err, misc);
   if (hpart) block_hash_partial(hash, hpart);
 }
 
+// start[t] = first sorted position whose tile (key >> kPTShift) is >= t, for
+// t in [0, ntiles]; one thread per tile, by binary search over the sorted keys.
+__global__ __launch_bounds__(kCicThreads) void k_tile_starts(const uint32_t* __restrict__ skey, size_t m,
+                                                             uint32_t ntiles, uint32_t* __restrict__ start) {
+  const size_t t = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (t > ntiles) return;
+  size_t lo = 0, hi = m;  // first position in [lo, hi) with tile >= t
+  while (lo < hi) {
+    const size_t mid = lo + (hi - lo) / 2;
+    if ((skey[mid] >> kPTShift) < t)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  start[t] = static_cast<uint32_t>(lo);
+}
+
 // Positions and FP32 weights in tile order.
-template <typename P, typename W>
+template <typename P>
 __global__ __launch_bounds__(kCicThreads) void k_tile_permute(const uint32_t* __restrict__ sorted,
-                                                              const P* __restrict__ pos,
-                                                              const W* __restrict__ w, size_t m,
+                                                              const TileRec<P>* __restrict__ rec, size_t m,
                                                               P* __restrict__ spos,
                                                               float* __restrict__ sw) {
   const size_t s = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   if (s >= m) return;
-  const size_t i = sorted[s];
-  const P x0 = pos[3 * i], x1 = pos[3 * i + 1], x2 = pos[3 * i + 2];
-  sw[s] = w ? static_cast<float>(w[i]) : 1.f;
-  spos[3 * s] = x0;
-  spos[3 * s + 1] = x1;
-  spos[3 * s + 2] = x2;
+  const TileRec<P> r = rec[sorted[s]];
+  sw[s] = r.w;
+  spos[3 * s] = r.x[0];
+  spos[3 * s + 1] = r.x[1];
+  spos[3 * s + 2] = r.x[2];
 }
 
 // FP32 offsets of a point from its base cell (recomputed from its position as
@@ -1415,23 +1430,15 @@ static RadixPlan radix_plan(size_t ncell) {
   return P;
 }
 
-// `keep`: also hash the positions into S.hash[0] and copy them, in cell order,
-// into S.spos (a catalog painted in one chunk, for later reads).
 template <typename R, typename P, typename W, int MODE>
 static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
-                       size_t m, bool check_only, hipStream_t s, bool keep = false) {
+                       size_t m, bool check_only, hipStream_t s) {
   PaintScratch& S = ws->paint;
   const unsigned grid = ceil_div(m, kCicThreads);
-  keep = keep && !check_only && grid <= S.nhpart;
   k_paint_key<R, P, W, MODE><<<grid, kCicThreads, 0, s>>>(
-      pos, w, m, F, S.key, static_cast<Payload<R>*>(S.payload), S.misc, check_only ? 1 : 0,
-      keep ? S.hpart : nullptr);
+      pos, w, m, F, S.key, static_cast<Payload<R>*>(S.payload), S.misc, check_only ? 1 : 0);
   PRH_LAUNCHED();
   if (check_only) return PRH_OK;
-  if (keep) {
-    k_hash_final<<<1, 1024, 0, s>>>(S.hpart, grid, S.hash);
-    PRH_LAUNCHED();
-  }
   const size_t ncell = S.ncell;
   // sort: keys alternate between key and rank, values between vtmp and sorted (the last
   // pass writes sorted); the first pass takes the input positions as values
@@ -1466,10 +1473,9 @@ static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* p
   PRH_LAUNCHED();
   k_cell_start<<<nsb, kCicThreads, 0, s>>>(skey, S.bnd, ncell, m, S.start, 0);
   PRH_LAUNCHED();
-  k_paint_permute<R, P><<<grid, kCicThreads, 0, s>>>(S.sorted,
-                                                     static_cast<const Payload<R>*>(S.payload),
-                                                     static_cast<Payload<R>*>(S.spay), m, pos,
-                                                     keep ? static_cast<P*>(S.spos) : nullptr);
+  k_paint_permute<R><<<grid, kCicThreads, 0, s>>>(S.sorted,
+                                                  static_cast<const Payload<R>*>(S.payload),
+                                                  static_cast<Payload<R>*>(S.spay), m);
   PRH_LAUNCHED();
   timer_before(PRH_TIMER_PAINT_GATHER, s);
   {
@@ -1547,23 +1553,23 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
       k_zero_misc<<<1, 64, 0, s>>>(S.misc);
       PRH_LAUNCHED();
     }
-    PRH_TRY((paint_chunk<R, P, W, MODE>(ws, mesh, F, pos + 3 * o, w ? w + o : nullptr, m, false, s,
-                                        MODE != 0 && !multi)));
+    // no copy of the positions here: gathering them into cell order costs this paint more than
+    // streaming them saves a later read (its cell order is kept and reused instead)
+    PRH_TRY((paint_chunk<R, P, W, MODE>(ws, mesh, F, pos + 3 * o, w ? w + o : nullptr, m, false, s)));
   }
   if (MODE == 0 || multi) return PRH_OK;
   uint32_t flags;
-  uint64_t hash = 0;
-  PRH_TRY(read_flags(ws, s, &flags, &hash));
+  PRH_TRY(read_flags(ws, s, &flags));
   PRH_TRY(flags_to_status(flags));
   S.order.pos = pos;
   S.order.n = n;
   S.order.pdt = Tag<P>::code;
-  S.order.copy = ceil_div(n, kCicThreads) <= S.nhpart;
-  S.order.hash = hash;
   return PRH_OK;
 }
 
-// The FP32 tile paint of one chunk (see k_tile_accumulate); `keep` as paint_chunk.
+// The FP32 tile paint of one chunk (see k_tile_accumulate).  `keep`: also hash the
+// positions into S.hash[0], so that a later read of the same catalog may stream
+// the positions this paint lays out in tile order (S.spos).
 template <typename P, typename W, int MODE>
 static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>& F,
                             const TileGrid& TG, const P* pos, const W* w, size_t m, bool check_only,
@@ -1571,7 +1577,9 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
   PaintScratch& S = ws->paint;
   const unsigned grid = ceil_div(m, kCicThreads);
   keep = keep && !check_only && grid <= S.nhpart;
-  k_tile_key<P, W, MODE><<<grid, kCicThreads, 0, s>>>(pos, w, m, F, TG, S.key, S.misc,
+  // the packed records go to the paint payload memory (32 bytes per point at most; chunk x 4 x 8 held)
+  TileRec<P>* rec = static_cast<TileRec<P>*>(S.payload);
+  k_tile_key<P, W, MODE><<<grid, kCicThreads, 0, s>>>(pos, w, m, F, TG, S.key, rec, S.misc,
                                                       keep ? S.hpart : nullptr);
   PRH_LAUNCHED();
   if (check_only) return PRH_OK;
@@ -1607,13 +1615,12 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
     vin = vout;
   }
   const uint32_t* skey = kbuf[RP.npass & 1];
-  const uint32_t nsb = static_cast<uint32_t>(ceil_div(ntiles, kStartCells));
-  k_start_bounds<<<ceil_div(m + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, nsb, S.bnd, kPTShift);
+  k_tile_starts<<<ceil_div(ntiles + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, static_cast<uint32_t>(ntiles),
+                                                                         S.start);
   PRH_LAUNCHED();
-  k_cell_start<<<nsb, kCicThreads, 0, s>>>(skey, S.bnd, ntiles, m, S.start, kPTShift);
-  PRH_LAUNCHED();
-  float* sw = static_cast<float*>(S.payload);
-  k_tile_permute<P, W><<<grid, kCicThreads, 0, s>>>(S.sorted, pos, w, m, static_cast<P*>(S.spos), sw);
+  // FP32 weights in tile order go to the sorted-payload memory (free in this paint)
+  float* sw = static_cast<float*>(S.spay);
+  k_tile_permute<P><<<grid, kCicThreads, 0, s>>>(S.sorted, rec, m, static_cast<P*>(S.spos), sw);
   PRH_LAUNCHED();
   const uint32_t per = xcd_per(ntiles);
   timer_before(PRH_TIMER_PAINT_GATHER, s);

@@ -103,6 +103,19 @@ __global__ __launch_bounds__(kDenThreads) void k_subtract_nonzero(R* __restrict_
   if (x != R(0)) delta[i] = x - mean;
 }
 
+// Mesh-scalar arithmetic of RealMesh (pyrecon/mesh.py:156-183): out = a * s,
+// a / s or s / a, elementwise in the mesh precision (numpy's result for a
+// Python scalar, which takes the array's dtype).
+template <typename R>
+__global__ __launch_bounds__(kDenThreads) void k_scalar_op(const R* __restrict__ a,
+                                                           R* __restrict__ out, size_t n, int op,
+                                                           R s) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kDenThreads) + threadIdx.x;
+  if (i >= n) return;
+  const R x = a[i];
+  out[i] = op == PRH_OP_MUL ? x * s : (op == PRH_OP_DIV ? x / s : s / x);
+}
+
 // Sums the nv partial rows left in ws->red by a block_reduce_range kernel and
 // copies the nv results to sums (host); synchronizes the stream.
 static int finish_sums(prh_workspace* ws, int nv, double* sums, hipStream_t s) {
@@ -158,6 +171,20 @@ static int do_subtract_nonzero(R* delta, size_t n, R mean, void* stream) {
   if (n == 0) return PRH_OK;
   k_subtract_nonzero<R><<<ceil_div(n, kDenThreads), kDenThreads, 0, as_stream(stream)>>>(
       delta, n, mean);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
+template <typename R>
+static int do_scalar_op(const R* a, R* out, size_t n, int op, double s, void* stream) {
+  if (n && (!a || !out)) return PRH_ERR_ARG;
+  if (op != PRH_OP_MUL && op != PRH_OP_DIV && op != PRH_OP_RDIV) {
+    set_error("unknown mesh-scalar operation %d", op);
+    return PRH_ERR_ARG;
+  }
+  if (n == 0) return PRH_OK;
+  k_scalar_op<R><<<ceil_div(n, kDenThreads), kDenThreads, 0, as_stream(stream)>>>(
+      a, out, n, op, static_cast<R>(s));
   PRH_LAUNCHED();
   return PRH_OK;
 }
@@ -227,6 +254,15 @@ int prh_subtract_nonzero_mean_f32(float* delta, size_t size, float mean, void* s
 }
 int prh_subtract_nonzero_mean_f64(double* delta, size_t size, double mean, void* stream) {
   return do_subtract_nonzero<double>(delta, size, mean, stream);
+}
+
+int prh_mesh_scalar_op_f32(const float* a, float* out, size_t size, int op, double scalar,
+                           void* stream) {
+  return do_scalar_op<float>(a, out, size, op, scalar, stream);
+}
+int prh_mesh_scalar_op_f64(const double* a, double* out, size_t size, int op, double scalar,
+                           void* stream) {
+  return do_scalar_op<double>(a, out, size, op, scalar, stream);
 }
 
 }  // extern "C"

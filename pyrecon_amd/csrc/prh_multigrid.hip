@@ -85,7 +85,9 @@ __global__ __launch_bounds__(kThreads) void k_residual(const R* __restrict__ v,
 // HBM about once and every stencil read hits LDS.  256 threads: 64 along z,
 // 4 along y, TY/4 rows each.  The arithmetic per cell is the generic
 // kernels' (same bits).
-enum StencilOp { kOpJacobi = 0, kOpResidual = 1, kOpJacobiZero = 2 };
+// kOpJacobiAcc: a Jacobi sweep whose result is added (as double) into TileCoef::acc
+// instead of being stored (the last sweep of a mixed-precision correction)
+enum StencilOp { kOpJacobi = 0, kOpResidual = 1, kOpJacobiZero = 2, kOpJacobiAcc = 3 };
 constexpr int kTileZ = 64;
 
 // Coefficients of a level passed to the blocked kernels: the geometry for a
@@ -95,6 +97,10 @@ struct TileCoef {
   LevelGeom<R> G;
   GlobalCoef<R> K;
   R dinv;  // 1 / K.d (fast variant only)
+  // kOpJacobiAcc: each result is added (as double) to acc at its cell instead of
+  // being stored (the last sweep of a mixed-precision correction, whose FP32
+  // result is only needed in the FP64 update v += e)
+  double* acc = nullptr;
 };
 
 // One cell of a blocked sweep: the coefficients at (px, py, pz) (A, B, C are
@@ -283,7 +289,10 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
       const R fv = fx[rowoff[k]];
       const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, fv, px, A, py[k], B[k], pz, C, pypz[k],
                                                      omega, keep);
-      ox[rowoff[k]] = static_cast<O>(res);
+      if constexpr (OP == kOpJacobiAcc)
+        T.acc[static_cast<size_t>(x) * s0 + rowoff[k]] += static_cast<double>(res);
+      else
+        ox[rowoff[k]] = static_cast<O>(res);
     }
   };
   if (OP == kOpJacobiZero) {
@@ -458,7 +467,10 @@ __global__ __launch_bounds__(64 * kRowsY * WPR, PRH_ROWS_MIN_WAVES) void k_stenc
       }
       const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, fv[c], px, A, py, B, pz[c], C[c],
                                                      pypz[c], omega, keep);
-      ox[zc] = static_cast<O>(res);
+      if constexpr (OP == kOpJacobiAcc)
+        T.acc[static_cast<size_t>(x) * s0 + rowoff + zc] += static_cast<double>(res);
+      else
+        ox[zc] = static_cast<O>(res);
     }
   };
   if (OP == kOpJacobiZero) {
@@ -551,7 +563,7 @@ static bool launch_rows(const R* v, const R* f, O* out, const TileCoef<R>& T, R 
 
 template <typename R, typename O, int OP, bool FAST>
 static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, R omega, R keep,
-                         hipStream_t s) {
+                         hipStream_t s, double* acc = nullptr) {
   const int xlen = tiled_xlen(G.n);
   const unsigned grid = static_cast<unsigned>((G.n[2] / kTileZ) * (G.n[1] / kTileY) *
                                               ((G.n[0] + xlen - 1) / xlen));
@@ -559,6 +571,7 @@ static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, 
   T.G = G;
   T.K = G.local ? GlobalCoef<R>() : GlobalCoef<R>::make(G);
   T.dinv = G.local ? R(0) : R(1) / T.K.d;
+  T.acc = acc;
   if (launch_rows<R, O, OP, FAST>(v, f, out, T, omega, keep, s)) return;
   if (G.local)
     k_stencil_tiled<R, O, OP, kTileY, true, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
@@ -1199,13 +1212,20 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vcycle_coarse(CoarseArgs<R> 
 static size_t ncells(const size_t* n) { return n[0] * n[1] * n[2]; }
 
 template <typename R>
+// acc (FP32 fast sweeps on blocked levels only): add the result into this FP64
+// mesh instead of storing it; *fused tells whether that was done.
 static int launch_jacobi(const R* v, const R* f, R* out, const LevelGeom<R>& G, R omega, R keep,
-                         bool zero_in, hipStream_t s, int timer = -1, bool fast = false) {
+                         bool zero_in, hipStream_t s, int timer = -1, bool fast = false,
+                         double* acc = nullptr, bool* fused = nullptr) {
   const size_t total = static_cast<size_t>(G.n[0]) * G.n[1] * G.n[2];
   dim3 block, grid;
   generic_shape(G.n, block, grid);
+  if (fused) *fused = false;
   timer_before(timer, s);
-  if (zero_in && tiled_ok(G.n) && fast)
+  if (acc && !zero_in && tiled_ok(G.n) && fast && std::is_same<R, float>::value) {
+    launch_tiled<R, R, kOpJacobiAcc, std::is_same<R, float>::value>(v, f, out, G, omega, keep, s, acc);
+    if (fused) *fused = true;
+  } else if (zero_in && tiled_ok(G.n) && fast)
     launch_tiled<R, R, kOpJacobiZero, std::is_same<R, float>::value>(v, f, out, G, omega, keep, s);
   else if (zero_in && tiled_ok(G.n))
     launch_tiled<R, R, kOpJacobiZero, false>(v, f, out, G, omega, keep, s);
@@ -1345,6 +1365,11 @@ struct Solver {
   int niter;
   hipStream_t s;
   bool fast = false;  // fast FP32 kernels on the blocked levels (mixed-precision corrections)
+  // When fuse_top is set, the last post-smoothing sweep of level 0 adds its result into the FP64
+  // mesh acc_top[j] instead of storing it; fused[j] tells whether it could (blocked levels).
+  bool fuse_top = false;
+  double* acc_top[kMaxGroup] = {};
+  bool fused[kMaxGroup] = {};
 
   int nlev() const { return H[0]->nlev; }
   const size_t* n(int l) const { return H[0]->n[l]; }
@@ -1392,7 +1417,10 @@ struct Solver {
     return A;
   }
 
-  int jacobi(int l, const Ptrs<R>& in, const Ptrs<R>& src, const Ptrs<R>& out, bool zero_in) {
+  int jacobi(int l, const Ptrs<R>& in, const Ptrs<R>& src, const Ptrs<R>& out, bool zero_in,
+             bool fuse = false) {
+    if (fuse)
+      for (int j = 0; j < count; j++) fused[j] = false;
     if (grouped(l)) {
       const int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]),
                          static_cast<int>(n(l)[2])};
@@ -1408,7 +1436,8 @@ struct Solver {
     }
     for (int j = 0; j < count; j++)
       PRH_TRY(launch_jacobi(in[j], src[j], out[j], G[j][l], omega, keep, zero_in, s,
-                            l == 0 ? PRH_TIMER_JACOBI_FINEST : -1, fast));
+                            l == 0 ? PRH_TIMER_JACOBI_FINEST : -1, fast, fuse ? acc_top[j] : nullptr,
+                            fuse ? &fused[j] : nullptr));
     return PRH_OK;
   }
 
@@ -1473,13 +1502,14 @@ struct Solver {
   }
 
   // niter damped-Jacobi sweeps; the iterate moves between cur and alt.
-  int smooth(int l, Ptrs<R>& cur, Ptrs<R>& alt, const Ptrs<R>& src, bool zero_start) {
+  int smooth(int l, Ptrs<R>& cur, Ptrs<R>& alt, const Ptrs<R>& src, bool zero_start, bool post = false) {
     if (niter <= 0) {
       if (zero_start) PRH_TRY(fill_zero(l, cur));
       return PRH_OK;
     }
     for (int it = 0; it < niter; it++) {
-      PRH_TRY(jacobi(l, cur, src, alt, zero_start && it == 0));
+      const bool fuse = post && fuse_top && l == 0 && it == niter - 1;
+      PRH_TRY(jacobi(l, cur, src, alt, zero_start && it == 0, fuse));
       swap_all(cur, alt, count);
     }
     return PRH_OK;
@@ -1538,7 +1568,7 @@ struct Solver {
       PRH_TRY(vcycle(l + 1, cc, ca, fc, true));
       PRH_TRY(prolong_to(l, cc, cur, true));
     }
-    return smooth(l, cur, alt, src, false);
+    return smooth(l, cur, alt, src, false, true);
   }
 
   // src/multigrid.c:303-328, with level `top` as the finest level.  v0 is the
@@ -1808,8 +1838,16 @@ static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double*
       }
     }
     Ptrs<float> cur = e32, alt = S.tmp(0);
-    for (int c = 0; c < vpr; c++) PRH_TRY(S.vcycle(0, cur, alt, r32, c == 0));
+    // the last cycle's last sweep adds its correction into the FP64 iterate where it can
+    for (int j = 0; j < count; j++) S.acc_top[j] = v[j];
+    for (int c = 0; c < vpr; c++) {
+      S.fuse_top = c == vpr - 1;
+      for (int j = 0; j < count; j++) S.fused[j] = false;
+      PRH_TRY(S.vcycle(0, cur, alt, r32, c == 0));
+    }
+    S.fuse_top = false;
     for (int j = 0; j < count; j++) {
+      if (S.fused[j]) continue;
       k_add_f32_f64<<<grid, kThreads, 0, s>>>(v[j], cur[j], total);
       PRH_LAUNCHED();
     }

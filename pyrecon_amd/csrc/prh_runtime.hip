@@ -371,6 +371,7 @@ int prh_workspace_destroy(prh_workspace* ws) {
   for (int d = 0; d < 3; d++) {
     release(ws->smooth.tw[d]);
     release(ws->smooth.gain[d]);
+    release(ws->smooth.tw64[d]);
   }
   release(ws->red);
   release(ws->misc);

@@ -602,6 +602,95 @@ __global__ __launch_bounds__(64 * kReg512SWaves) void k_smooth_reg1024s(
   }
 }
 
+// Direct DFT along one axis, for lengths the radix passes do not cover (a prime
+// factor above 7, or a line too long for them): X[k] = sum_m x[m] w^(k m) with
+// w = exp(-2 pi i / n) (conjugated for the inverse), accumulated in double from
+// a double twiddle table, (k m) mod n kept by increments.  One block per line,
+// staged in LDS as double; O(n^2) per line.  Lines are addressed as in
+// k_smooth_axis.  OP: kDftFwd / kDftInv (IO as k_fft_axis), or kDftSmooth (a
+// real line in place: forward, times gain[k], inverse, real part).
+enum FftIO { kFftComplex = 0, kFftRealIn = 1, kFftRealOut = 2 };
+constexpr int kDirectMax = 4096;  // two lines of double complex in LDS
+constexpr int kDftThreads = 256;
+enum DftOp { kDftFwd = 0, kDftInv = 1, kDftSmooth = 2 };
+
+__device__ __forceinline__ void dft_line(const Cx<double>* __restrict__ in, Cx<double>* __restrict__ out,
+                                         int n, const Cx<double>* __restrict__ tw, bool inverse) {
+  for (int k = threadIdx.x; k < n; k += kDftThreads) {
+    double re = 0.0, im = 0.0;
+    int idx = 0;
+    for (int m = 0; m < n; m++) {
+      const Cx<double> x = in[m], w = tw[idx];
+      const double wi = inverse ? -w.im : w.im;
+      re += x.re * w.re - x.im * wi;
+      im += x.re * wi + x.im * w.re;
+      idx += k;
+      if (idx >= n) idx -= n;
+    }
+    out[k] = Cx<double>{re, im};
+  }
+}
+
+template <typename R, int OP, int IO>
+__global__ __launch_bounds__(kDftThreads) void k_dft_axis(const void* src, void* dst, int n, long long nlines,
+                                                          int I, long long ostride, long long istride,
+                                                          long long es, const Cx<double>* __restrict__ tw,
+                                                          const R* __restrict__ gain) {
+  extern __shared__ double smem[];
+  Cx<double>* a = reinterpret_cast<Cx<double>*>(smem);
+  Cx<double>* b = a + n;
+  const long long q = blockIdx.x;
+  if (q >= nlines) return;
+  const long long base = (q / I) * ostride + (q % I) * istride;
+  for (int e = threadIdx.x; e < n; e += kDftThreads) {
+    const long long at = base + e * es;
+    if (OP == kDftSmooth || IO == kFftRealIn) {
+      a[e] = Cx<double>{static_cast<double>(static_cast<const R*>(src)[at]), 0.0};
+    } else {
+      const Cx<R> v = static_cast<const Cx<R>*>(src)[at];
+      a[e] = Cx<double>{static_cast<double>(v.re), static_cast<double>(v.im)};
+    }
+  }
+  __syncthreads();
+  dft_line(a, b, n, tw, OP == kDftInv);
+  __syncthreads();
+  if (OP == kDftSmooth) {
+    // the gain in the mesh precision (with its 1 / n), applied as the radix kernels do
+    for (int k = threadIdx.x; k < n; k += kDftThreads) {
+      const R g = gain[k];
+      const Cx<R> v{static_cast<R>(b[k].re) * g, static_cast<R>(b[k].im) * g};
+      b[k] = Cx<double>{static_cast<double>(v.re), static_cast<double>(v.im)};
+    }
+    __syncthreads();
+    dft_line(b, a, n, tw, true);
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += kDftThreads)
+      static_cast<R*>(dst)[base + e * es] = static_cast<R>(a[e].re);
+    return;
+  }
+  for (int e = threadIdx.x; e < n; e += kDftThreads) {
+    const long long at = base + e * es;
+    if (IO == kFftRealOut)
+      static_cast<R*>(dst)[at] = static_cast<R>(b[e].re);
+    else
+      static_cast<Cx<R>*>(dst)[at] = Cx<R>{static_cast<R>(b[e].re), static_cast<R>(b[e].im)};
+  }
+}
+
+template <typename R, int OP, int IO>
+static int launch_dft_axis(const void* src, void* dst, int n, long long nlines, int I, long long ostride,
+                           long long istride, long long es, const double* tw, const R* gain,
+                           hipStream_t s) {
+  const size_t lds = 2 * sizeof(Cx<double>) * static_cast<size_t>(n);
+  const void* fn = reinterpret_cast<const void*>(&k_dft_axis<R, OP, IO>);
+  if (lds > 65536)
+    PRH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  k_dft_axis<R, OP, IO><<<static_cast<unsigned>(nlines), kDftThreads, lds, s>>>(
+      src, dst, n, nlines, I, ostride, istride, es, reinterpret_cast<const Cx<double>*>(tw), gain);
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
 // gain[k] = exp(-(r k_phys)^2 / 2) / n with k_phys = 2 pi fftfreq(n, L / n)[k]
 // (numpy's ordering: k < (n - 1) / 2 + 1 -> k, else k - n), in double.
 template <typename R>
@@ -677,9 +766,10 @@ int smooth_setup(prh_workspace* ws) {
   for (int d = 0; d < 3; d++) {
     const int n = static_cast<int>(ws->n[d]);
     AxisPlan P;
-    if (!factor_plan(n, P) || (padded_len(n) + n) * 2 * w > kSmoothLdsMax ||
-        plan_max_lm(P) > fly_log2()) {
-      S.have = false;  // smoothing refused at call time for this shape
+    S.direct[d] = !factor_plan(n, P) || (padded_len(n) + n) * 2 * w > kSmoothLdsMax ||
+                  plan_max_lm(P) > fly_log2();
+    if (S.direct[d] && n > kDirectMax) {
+      S.have = false;  // transforms refused at call time for this shape
       continue;
     }
     PRH_TRY(ws_alloc(ws, &S.tw[d], 2 * w * n));
@@ -689,6 +779,12 @@ int smooth_setup(prh_workspace* ws) {
       const double ang = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
       tw[2 * k] = std::cos(ang);
       tw[2 * k + 1] = std::sin(ang);
+    }
+    if (S.direct[d]) {
+      void* p;
+      PRH_TRY(ws_alloc(ws, &p, 2 * sizeof(double) * n));
+      S.tw64[d] = static_cast<double*>(p);
+      PRH_HIP(hipMemcpy(S.tw64[d], tw.data(), 2 * sizeof(double) * n, hipMemcpyHostToDevice));
     }
     if (ws->dtype == PRH_F64) {
       PRH_HIP(hipMemcpy(S.tw[d], tw.data(), 2 * w * n, hipMemcpyHostToDevice));
@@ -702,7 +798,14 @@ int smooth_setup(prh_workspace* ws) {
 
 size_t smooth_bytes(const size_t* n, int dtype) {
   size_t b = 0;
-  for (int d = 0; d < 3; d++) b += 3 * static_cast<size_t>(dtype) * n[d];
+  for (int d = 0; d < 3; d++) {
+    b += 3 * static_cast<size_t>(dtype) * n[d];
+    AxisPlan P;
+    const bool direct = !factor_plan(static_cast<int>(n[d]), P) ||
+                        (padded_len(n[d]) + n[d]) * 2 * static_cast<size_t>(dtype) > kSmoothLdsMax ||
+                        plan_max_lm(P) > fly_log2();
+    if (direct) b += 2 * sizeof(double) * n[d];
+  }
   return b;
 }
 
@@ -715,10 +818,8 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
     return PRH_ERR_STATE;
   }
   if (!ws->smooth.have) {
-    set_error("smoothing supports dimensions whose prime factors are 2, 3, 5 and 7, of at most %zu "
-              "cells (one line and its twiddles in LDS), and whose radix passes have at most %d "
-              "butterflies per line", (kSmoothLdsMax / (2 * sizeof(R)) - 1) * 16 / 33,
-              kMaxFly * kSmoothThreads);
+    set_error("smoothing supports dimensions of at most %d cells when they have a prime factor above 7 "
+              "or exceed the radix passes", kDirectMax);
     return PRH_ERR_ARG;
   }
   if (ws->dtype != Tag<R>::code) {
@@ -765,6 +866,11 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
       nreal = n0 * n2; I = static_cast<int>(n2); ostride = n1 * n2; istride = 1; es = n2;
     } else {
       nreal = n1 * n2; I = static_cast<int>(n2); ostride = n2; istride = 1; es = n1 * n2;
+    }
+    if (ws->smooth.direct[d]) {
+      PRH_TRY((launch_dft_axis<R, kDftSmooth, 0>(mesh, mesh, n, nreal, I, ostride, istride, es, ws->smooth.tw64[d],
+                                                 static_cast<const R*>(ws->smooth.gain[d]), s)));
+      continue;
     }
     const long long ncomplex = (nreal + 1) / 2;
     const unsigned grid = static_cast<unsigned>((ncomplex + LP - 1) / LP);
@@ -822,7 +928,6 @@ static int do_smooth(prh_workspace* ws, R* mesh, const size_t* nmesh, const R* b
 // 1 / N of the inverse is folded into the Fourier-space operator that precedes
 // it.
 // ---------------------------------------------------------------------------
-enum FftIO { kFftComplex = 0, kFftRealIn = 1, kFftRealOut = 2 };
 
 // The lines of one axis (same addressing as k_smooth_axis, in elements of the
 // mesh); block b takes complex lines [b LP, (b + 1) LP), remapped so that
@@ -941,6 +1046,17 @@ static int fft_pass(prh_workspace* ws, int d, const void* src, void* dst, int io
     nlines = n1 * n2; I = static_cast<int>(n2); ostride = n2; istride = 1; es = n1 * n2;
   }
   const Cx<R>* tw = static_cast<const Cx<R>*>(ws->smooth.tw[d]);
+  if (ws->smooth.direct[d]) {
+    const double* t64 = ws->smooth.tw64[d];
+    if (inverse) {
+      if (io == kFftRealOut)
+        return launch_dft_axis<R, kDftInv, kFftRealOut>(src, dst, n, nlines, I, ostride, istride, es, t64, nullptr, s);
+      return launch_dft_axis<R, kDftInv, kFftComplex>(src, dst, n, nlines, I, ostride, istride, es, t64, nullptr, s);
+    }
+    if (io == kFftRealIn)
+      return launch_dft_axis<R, kDftFwd, kFftRealIn>(src, dst, n, nlines, I, ostride, istride, es, t64, nullptr, s);
+    return launch_dft_axis<R, kDftFwd, kFftComplex>(src, dst, n, nlines, I, ostride, istride, es, t64, nullptr, s);
+  }
   const size_t lds = bytes(llp);
   if (d == 2) {
     if (io == kFftRealIn)
@@ -968,7 +1084,8 @@ static int check_transform_ws(prh_workspace* ws, const size_t* nmesh, int code) 
     return PRH_ERR_STATE;
   }
   if (!ws->smooth.have) {
-    set_error("Fourier transforms support dimensions whose prime factors are 2, 3, 5 and 7");
+    set_error("Fourier transforms support dimensions of at most %d cells when they have a prime factor "
+              "above 7 or exceed the radix passes", kDirectMax);
     return PRH_ERR_ARG;
   }
   if (ws->dtype != code) {

@@ -47,10 +47,10 @@ struct PaintScratch {
   // arrays alternate between paints, so the orders of the last two catalogs
   // painted (data and randoms) are both kept; `sorted` is the one of the last.
   uint32_t* sorted_prev = nullptr;  // chunk      the previous paint's order
-  // With each order, a paint of one chunk of Cartesian positions keeps a copy
-  // of the positions in that order (`spos`, 3 values of the positions dtype per
-  // point) and a 64-bit hash of the positions' bytes in input order.  A read of
-  // positions with the same pointer, count, dtype and hash streams the copy
+  // The FP32 tile paint lays the positions out in its (tile) order anyway
+  // (`spos`, 3 values of the positions dtype per point); painting one chunk, it
+  // also keeps a 64-bit hash of the positions' bytes in input order.  A read of
+  // positions with the same pointer, count, dtype and hash streams that copy
   // instead of gathering the positions at random.
   void* spos = nullptr;         // chunk x 3 doubles
   void* spos_prev = nullptr;    // chunk x 3 doubles
@@ -71,9 +71,11 @@ struct PaintScratch {
 // Tables of the separable Fourier-space smoothing: per axis, the twiddles
 // exp(-2 pi i k / n) and the per-call filter gains, in the mesh precision.
 struct SmoothState {
-  bool have = false;  // false when a dimension has a prime factor > 7 or is too long
+  bool have = false;  // false when a dimension is longer than the direct transform allows
+  bool direct[3] = {};  // axis transformed by the direct DFT (a prime factor > 7, or too long for the radix passes)
   void* tw[3] = {};
   void* gain[3] = {};
+  double* tw64[3] = {};  // exp(-2 pi i k / n), k < n, in double (direct DFT axes)
 };
 
 }  // namespace prh

@@ -11,6 +11,8 @@ reference's numpy-backed RealMesh keeps working.
 """
 import warnings
 
+import numbers
+
 import numpy as np
 
 from . import _lib
@@ -261,6 +263,95 @@ class RealMesh(object):
         if dtype is not None: value = value.astype(dtype)
         return value
 
+    # ------------------------------------------------------------------ arithmetic and indexing
+    # The reference's BaseMesh operators (pyrecon/mesh.py:156-226): *, / (both sides) and their
+    # in-place forms, numpy ufuncs through __array_ufunc__, and indexing of the value.  With a
+    # Python scalar, * and / run on the device; other operands take numpy's path on the host.
+    _HANDLED_TYPES = (np.ndarray, numbers.Number)
+
+    def _scalar_op(self, op, other, out):
+        """out (a RealMesh, possibly self) = self op other on the device; False if not applicable."""
+        if self.buffer is None or isinstance(other, (bool, np.bool_)) or not isinstance(other, (int, float)):
+            return False
+        fn = getattr(lib, 'prh_mesh_scalar_op_' + _suffix(self.dtype))
+        out._allocate(zero=False)
+        check(fn(self.buffer.ptr, out.buffer.ptr, self.size, op, float(other), stream_handle(self.stream)),
+              'mesh arithmetic')
+        return True
+
+    def _binary(self, op, other, reverse=False):
+        new = self.deepcopy(copy_value=False)
+        code = _lib.OP_RDIV if reverse else op
+        if not self._scalar_op(code, other, new):
+            value = self.value
+            if op == _lib.OP_MUL: result = value * other
+            elif reverse: result = other / value
+            else: result = value / other
+            new.value = result
+        return new
+
+    def _inplace(self, op, other):
+        if not self._scalar_op(op, other, self):
+            value = self.value
+            if op == _lib.OP_MUL: value *= other
+            else: value /= other
+            self.value = value
+        return self
+
+    def __mul__(self, other):
+        return self._binary(_lib.OP_MUL, other)
+
+    def __imul__(self, other):
+        return self._inplace(_lib.OP_MUL, other)
+
+    def __truediv__(self, other):
+        return self._binary(_lib.OP_DIV, other)
+
+    def __rtruediv__(self, other):
+        return self._binary(_lib.OP_DIV, other, reverse=True)
+
+    def __itruediv__(self, other):
+        return self._inplace(_lib.OP_DIV, other)
+
+    __div__, __rdiv__, __idiv__ = __truediv__, __rtruediv__, __itruediv__
+
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        out = kwargs.get('out', ())
+        for x in inputs + out:
+            if not isinstance(x, self._HANDLED_TYPES + (RealMesh,)):
+                return NotImplemented
+        values = tuple(x.value if isinstance(x, RealMesh) else x for x in inputs)
+        if out:
+            kwargs['out'] = tuple(x.value if isinstance(x, RealMesh) else x for x in out)
+        result = getattr(ufunc, method)(*values, **kwargs)
+        for target, host in zip(out, kwargs.get('out', ())):
+            if isinstance(target, RealMesh):
+                target.value = host
+
+        def cast(res):
+            # booleans and other shapes are no meshes: plain arrays
+            if res.dtype == '?' or res.shape != self.shape:
+                return res
+            new = self.deepcopy(copy_value=False)
+            new.value = res
+            return new
+
+        if method == 'at':
+            return None
+        if out:
+            return out[0] if len(out) == 1 else out
+        if type(result) is tuple:
+            return tuple(cast(r) for r in result)
+        return cast(result)
+
+    def __getitem__(self, index):
+        return self.value[index]
+
+    def __setitem__(self, index, value):
+        array = self.value
+        array[index] = value
+        self.value = array
+
     def __repr__(self):
         return 'RealMesh(dtype={}, nmesh={}, boxsize={}, boxcenter={})'.format(self.dtype, self.nmesh, self.boxsize,
                                                                                self.boxcenter)
@@ -270,9 +361,9 @@ class RealMesh(object):
         new._allocate(zero=True)
         return new
 
-    def deepcopy(self):
+    def deepcopy(self, copy_value=True):
         new = RealMesh(info=self.info.clone(), workspace=self._workspace, stream=self.stream)
-        if self.buffer is not None:
+        if copy_value and self.buffer is not None:
             new._allocate(zero=False).copy_(self.buffer, stream=self.stream)
         return new
 
@@ -356,7 +447,12 @@ class RealMesh(object):
         check(fn(self.workspace.handle, self.buffer.ptr, nm_ptr, bs_ptr, radius_.ctypes.data,
                  stream_handle(self.stream)), 'smooth_gaussian')
 
-    def sum(self):
+    def sum(self, *args, **kwargs):
+        """numpy's sum of the value (what np.sum does with the reference's mesh)."""
+        return np.sum(self.value, *args, **kwargs)
+
+    def device_sum(self):
+        """Deterministic FP64 sum of the cells on the device (one number back to the host)."""
         import ctypes
         sums = (ctypes.c_double * 2)()
         fn = getattr(lib, 'prh_mesh_sums_' + _suffix(self.dtype))

@@ -23,7 +23,8 @@ from catalogs import make_catalogs, RECON_KWARGS, SMOOTHING_RADIUS
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(8, 12, 20), (16, 16, 16), (6, 10, 14), (15, 9, 25), (4, 6, 512), (512, 2, 16), (2, 512, 8)]
+SHAPES = [(8, 12, 20), (16, 16, 16), (6, 10, 14), (15, 9, 25), (4, 6, 512), (512, 2, 16), (2, 512, 8),
+          (11, 13, 6), (22, 4, 17)]  # the last two: direct DFT along axes with prime factors above 7
 
 
 def rel_l2(a, b):

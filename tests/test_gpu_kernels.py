@@ -159,7 +159,10 @@ SMOOTH_SHAPES = [(8, 8, 8), (8, 12, 20), (6, 10, 14), (15, 9, 25), (10, 32, 128)
                  # 1024-cell lines: in registers along z (an even and an odd number of rows) and
                  # along x and y (16 adjacent rows per tile), and the LDS transform along x and y
                  # with a z extent that is not a multiple of 16
-                 (2, 4, 1024), (3, 5, 1024), (1024, 2, 16), (2, 1024, 32), (1024, 2, 8), (4, 1024, 2)]
+                 (2, 4, 1024), (3, 5, 1024), (1024, 2, 16), (2, 1024, 32), (1024, 2, 8), (4, 1024, 2),
+                 # prime factors above 7: the direct DFT along those axes (11, 13, 17, 2 x 11 x 13,
+                 # 131), next to radix axes
+                 (11, 13, 17), (286, 4, 6), (8, 131, 10)]
 
 
 @pytest.mark.parametrize('dtype', ['f4', 'f8'])
@@ -185,9 +188,10 @@ def test_smooth_gaussian(dtype, shape):
 
 
 def test_smooth_gaussian_unsupported_size():
-    # a dimension with a prime factor above 7 has no transform plan: the call fails cleanly
+    # a dimension longer than the direct DFT's LDS line (4096) with a prime factor above 7: the
+    # call fails cleanly
     from pyrecon_amd import RealMesh, MeshInfo
-    mesh = RealMesh(info=MeshInfo(nmesh=(8, 11, 8), boxsize=100., boxcenter=0., dtype='f8'))
-    mesh.value = np.ones((8, 11, 8))
+    mesh = RealMesh(info=MeshInfo(nmesh=(4099, 2, 2), boxsize=100., boxcenter=0., dtype='f8'))
+    mesh.value = np.ones((4099, 2, 2))
     with pytest.raises(ValueError):
         mesh.smooth_gaussian(10.)
