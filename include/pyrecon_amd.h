@@ -176,11 +176,11 @@ int prh_paint_f64(prh_workspace *ws, double *mesh, const size_t *nmesh, const do
                   void *stream);
 
 /* the mixed-precision variant of prh_paint_f64 (same arguments and checks;
- * replaces the same assign_cic, mesh.c:26, with FP32 corner weights): points
- * are sorted by 8^3-cell tile, each tile sums its FP32 contributions exactly
- * as 64-bit integers at the scale 2^(62 - e) of its own weight bound 2^e, and
- * tiles are combined in FP64 in a fixed order, so repeated calls give identical
- * bits (not the reference's).  Needs PRH_WS_PAINT | PRH_WS_MIXED.          */
+ * replaces the same assign_cic, mesh.c:26): points are sorted by 8^3-cell
+ * tile, each tile sums its FP64 corner weights as 64-bit integers at the scale
+ * 2^(62 - e) of its own weight bound 2^e, and tiles are combined in FP64 in a
+ * fixed order: the reference's sums to within their own rounding (not its
+ * bits), identical bits on repeated calls.  Needs PRH_WS_PAINT | PRH_WS_MIXED. */
 int prh_paint_mixed_f64(prh_workspace *ws, double *mesh, const size_t *nmesh,
                         const double *boxsize, const double *boxcenter, const void *positions,
                         int pos_dtype, const void *weights, int weight_dtype, size_t npositions,

@@ -417,11 +417,13 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 // equal-digit pairs before it in the wave (the wave's running count in LDS
 // plus the equal lanes below it).  The tile then stages its pairs in LDS in
 // (digit, rank) order and writes each digit's run to its scanned offset.
-// `vin` null: the values are the input positions (first pass).
+// `vin` null: the values are the input positions (first pass).  With `dest`,
+// each pair's output position is also written at its input position (coalesced),
+// so that the passes' destinations compose into the inverse permutation.
 __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, size_t m, int shift,
     int bits, uint32_t ntiles, uint32_t per, const uint32_t* __restrict__ goff,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t* __restrict__ dest = nullptr) {
   __shared__ uint32_t s_wcnt[kRadixWaves][kRadixMaxBins];  // per-wave counts, then prefixes
   __shared__ uint32_t s_lbeg[kRadixMaxBins];               // digit's first slot in the tile
   __shared__ uint32_t s_gbeg[kRadixMaxBins];               // digit's first output position
@@ -502,6 +504,7 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
         s_key[slot] = k[j];
         s_val[slot] = v[j];
       }
+      if (dest) dest[wbase + j * 64 + lane] = s_gbeg[d] + s_wcnt[w][d] + r[j];
     }
   }
   __syncthreads();
@@ -861,18 +864,23 @@ __global__ void k_zero_misc(uint32_t* misc) {
 }
 
 // ---------------------------------------------------------------------------
-// FP32 paint with exact accumulation per tile (the mixed-precision path)
+// Paint with exact accumulation per tile (the mixed-precision path)
 // ---------------------------------------------------------------------------
 // The paint above reproduces the reference's serial sum bit for bit.  The
-// mixed-precision reconstruction instead paints with FP32 corner weights, and
-// only needs every repeated run to give the same bits.  Points are sorted by
+// mixed-precision reconstruction instead sums the FP64 corner weights in fixed
+// point per tile: a cell's k contributions are each rounded by at most 2^-62 B
+// (B the tile's weight bound, below), against about 2^-53 of the cell's value
+// for each term of the serial FP64 sum, so the two meshes agree to the serial
+// sum's own rounding (a cell lands on the other side of the density rule's
+// threshold only if it lies within that rounding of it), and every repeated run
+// gives the same bits.  Points are sorted by
 // 8^3-cell tile alone (the key's low 9 bits carry the cell inside the tile and
 // ride along), so a 2- or 3-digit radix sort replaces the cell sort and the
 // point-order merge.  One workgroup per tile:
-//   - loads its points (positions in tile order, their FP32 weights),
+//   - loads its points (positions in tile order, their FP64 weights),
 //   - bounds B = sum |w| prod_d max(|1 - d_d|, |d_d|) over them (an FP64
 //     reduction in a fixed order) and takes the scale 2^(62 - e), B < 2^e,
-//   - rounds each FP32 contribution ((wx*wy)*wz)*w times the scale to an
+//   - rounds each FP64 contribution ((wx*wy)*wz)*w times the scale to an
 //     integer and adds it into a 9^3 array of 64-bit integers in LDS: integer
 //     sums do not depend on their order, and the scaled sum stays below 2^62,
 //   - converts to FP64 (the scale is a power of two) and adds the cells that
@@ -880,8 +888,8 @@ __global__ void k_zero_misc(uint32_t* misc) {
 //     into the mesh; the others go to a per-tile boundary array.
 // A second pass adds, for each cell with a local index 0, its own tile's value
 // and the upper halos of the (up to 7) lower neighbour tiles in a fixed order.
-// The contributions are rounded to 2^-62 B (relative to the tile's own
-// weight bound), far below the FP32 weights' rounding.
+// The contributions are rounded to within 2^-62 B (B the tile's own weight
+// bound), about the rounding of the FP64 products themselves.
 constexpr int kPT = 8;                           // tile edge (cells)
 constexpr int kPTH = kPT + 1;                    // with the upper halo
 constexpr int kPTCells = kPTH * kPTH * kPTH;     // 729
@@ -892,13 +900,14 @@ struct TileGrid {
   int t[3];  // tiles per axis
 };
 
-// A point's position (in the positions dtype) and FP32 weight, packed by the key
-// step into one aligned record so that the permute gathers a single record per point.
+// A point's position (in the positions dtype) and FP64 weight, packed by the key
+// step into one aligned 32-byte record so that the permute gathers a single record per point.
 template <typename P>
-struct alignas(4 * sizeof(P) > 16 ? 32 : 16) TileRec {
+struct alignas(32) TileRec {
   P x[3];
-  float w;
+  double w;
 };
+static_assert(sizeof(TileRec<double>) == 32 && sizeof(TileRec<float>) == 32, "tile records are 32 bytes");
 
 template <typename P, typename W, int MODE>
 __global__ __launch_bounds__(kCicThreads) void k_tile_key(const P* __restrict__ pos,
@@ -925,11 +934,18 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_key(const P* __restrict__ 
     r.x[0] = x[0];
     r.x[1] = x[1];
     r.x[2] = x[2];
-    r.w = w ? static_cast<float>(w[i]) : 1.f;
+    r.w = w ? static_cast<double>(w[i]) : 1.0;
     rec[i] = r;
   }
   flag_paint_errors(err, misc);
   if (hpart) block_hash_partial(hash, hpart);
+}
+
+// inv[i] = step[inv[i]]: the inverse permutation after one more sort pass.
+__global__ __launch_bounds__(kCicThreads) void k_compose(uint32_t* __restrict__ inv,
+                                                         const uint32_t* __restrict__ step, size_t m) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (i < m) inv[i] = step[inv[i]];
 }
 
 // start[t] = first sorted position whose tile (key >> kPTShift) is >= t, for
@@ -949,12 +965,12 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_starts(const uint32_t* __r
   start[t] = static_cast<uint32_t>(lo);
 }
 
-// Positions and FP32 weights in tile order.
+// Positions and weights in tile order.
 template <typename P>
 __global__ __launch_bounds__(kCicThreads) void k_tile_permute(const uint32_t* __restrict__ sorted,
                                                               const TileRec<P>* __restrict__ rec, size_t m,
                                                               P* __restrict__ spos,
-                                                              float* __restrict__ sw) {
+                                                              double* __restrict__ sw) {
   const size_t s = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
   if (s >= m) return;
   const TileRec<P> r = rec[sorted[s]];
@@ -964,18 +980,18 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_permute(const uint32_t* __
   spos[3 * s + 2] = r.x[2];
 }
 
-// FP32 offsets of a point from its base cell (recomputed from its position as
+// Offsets of a point from its base cell (recomputed from its position as
 // k_tile_key computed the cell).
 template <typename P>
 __device__ __forceinline__ void tile_offsets(const P* __restrict__ spos, size_t s,
-                                             const Frame<double>& F, int wrap, float d[3]) {
+                                             const Frame<double>& F, int wrap, double d[3]) {
   const P x[3] = {spos[3 * s], spos[3 * s + 1], spos[3 * s + 2]};
   double p[3];
   uint32_t err = 0;
   to_mesh_units<double, P>(x, F, wrap, p, err);
   for (int k = 0; k < 3; k++) {
     const int b = base_index(static_cast<int>(p[k]), F.n[k]);
-    d[k] = static_cast<float>(p[k] - static_cast<double>(b));
+    d[k] = p[k] - static_cast<double>(b);
   }
 }
 
@@ -997,7 +1013,7 @@ template <typename P>
 __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
     double* __restrict__ mesh, Frame<double> F, TileGrid TG, int wrap, uint32_t ntiles,
     uint32_t per, const uint32_t* __restrict__ start, const uint32_t* __restrict__ skey,
-    const P* __restrict__ spos, const float* __restrict__ sw, double* __restrict__ bnd,
+    const P* __restrict__ spos, const double* __restrict__ sw, double* __restrict__ bnd,
     const uint32_t* __restrict__ misc) {
   __shared__ unsigned long long acc[kPTCells];
   __shared__ double s_red[kCicThreads / 64];
@@ -1011,11 +1027,10 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
   // weight bound of the tile (fixed order: strided per thread, then a fixed tree)
   double bsum = 0.0;
   for (uint32_t q = beg + threadIdx.x; q < end; q += kCicThreads) {
-    float d[3];
+    double d[3];
     tile_offsets<P>(spos, q, F, wrap, d);
-    double pw = fabs(static_cast<double>(sw[q]));
-    for (int k = 0; k < 3; k++)
-      pw *= fmax(fabs(1.0 - static_cast<double>(d[k])), fabs(static_cast<double>(d[k])));
+    double pw = fabs(sw[q]);
+    for (int k = 0; k < 3; k++) pw *= fmax(fabs(1.0 - d[k]), fabs(d[k]));
     bsum += pw;
   }
   for (int off = 32; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
@@ -1027,20 +1042,19 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
   if (B > 0.0 && isfinite(B)) frexp(B, &ex);  // B < 2^ex
   const double scale = ldexp(1.0, 62 - ex), inv = ldexp(1.0, ex - 62);
   for (uint32_t q = beg + threadIdx.x; q < end; q += kCicThreads) {
-    float d[3];
+    double d[3];
     tile_offsets<P>(spos, q, F, wrap, d);
-    const float wq = sw[q];
+    const double wq = sw[q];
     const uint32_t cell = skey[q] & ((1u << kPTShift) - 1u);
     const int cx = static_cast<int>(cell >> 6), cy = static_cast<int>((cell >> 3) & 7u),
               cz = static_cast<int>(cell & 7u);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const int a = (k >> 2) & 1, b = (k >> 1) & 1, c = k & 1;
-      const float wx = a ? d[0] : 1.f - d[0];
-      const float wy = b ? d[1] : 1.f - d[1];
-      const float wz = c ? d[2] : 1.f - d[2];
-      const float contrib = ((wx * wy) * wz) * wq;
-      const long long fixed = __double2ll_rn(static_cast<double>(contrib) * scale);
+      const double wx = a ? d[0] : 1.0 - d[0];
+      const double wy = b ? d[1] : 1.0 - d[1];
+      const double wz = c ? d[2] : 1.0 - d[2];
+      const long long fixed = __double2ll_rn(((wx * wy) * wz) * wq * scale);
       atomicAdd(&acc[((cx + a) * kPTH + (cy + b)) * kPTH + (cz + c)], static_cast<unsigned long long>(fixed));
     }
   }
@@ -1373,9 +1387,23 @@ __global__ __launch_bounds__(kCicThreads) void k_read_sorted(const R* __restrict
   uint32_t err = 0;
   if (t < m) {
     const P x[3] = {spos[3 * t], spos[3 * t + 1], spos[3 * t + 2]};
-    err = read_value<R, P, O>(order[t], x, phi, A, out);
+    // without `order` the results stay in the paint's order (sequential writes), for k_unpermute
+    err = read_value<R, P, O>(order ? order[t] : t, x, phi, A, out);
   }
   flag_errors(err, misc);
+}
+
+// out[i] = tmp[inv[i]] (S values per point): results back into point order, with
+// sequential writes and scattered reads.
+template <typename O, int S>
+__global__ __launch_bounds__(kCicThreads) void k_unpermute(const O* __restrict__ tmp,
+                                                           const uint32_t* __restrict__ inv, size_t m,
+                                                           O* __restrict__ out) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (i >= m) return;
+  const size_t t = inv[i];
+#pragma unroll
+  for (int c = 0; c < S; c++) out[S * i + c] = tmp[S * t + c];
 }
 
 // Read binning: a counting sort of the points by 8^3-cell tile (the order
@@ -1531,6 +1559,7 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
   // this paint's order goes to the older of the two order arrays
   std::swap(S.sorted, S.sorted_prev);
   std::swap(S.spos, S.spos_prev);
+  std::swap(S.inv, S.inv_prev);
   S.order_prev = S.order;
   S.order = PaintScratch::Order();
   k_zero_misc<<<1, 64, 0, s>>>(S.misc);
@@ -1567,7 +1596,7 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
   return PRH_OK;
 }
 
-// The FP32 tile paint of one chunk (see k_tile_accumulate).  `keep`: also hash the
+// The tile paint of one chunk (see k_tile_accumulate).  `keep`: also hash the
 // positions into S.hash[0], so that a later read of the same catalog may stream
 // the positions this paint lays out in tile order (S.spos).
 template <typename P, typename W, int MODE>
@@ -1609,17 +1638,24 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
     k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.hist, nh, S.partial, S.goff);
     PRH_LAUNCHED();
     uint32_t* vout = vbuf[(RP.npass - 1 - p) & 1];
+    // destinations: of the first pass straight into inv, of later passes composed into it
+    uint32_t* dest = keep ? (p == 0 ? S.inv : S.dtmp) : nullptr;
     k_radix_scatter<<<rper * kXcds, kRadixThreads, 0, s>>>(kbuf[p & 1], vin, m, RP.shift[p], RP.bits[p],
-                                                           rtiles, rper, S.goff, kbuf[(p + 1) & 1], vout);
+                                                           rtiles, rper, S.goff, kbuf[(p + 1) & 1], vout,
+                                                           dest);
     PRH_LAUNCHED();
+    if (keep && p > 0) {
+      k_compose<<<grid, kCicThreads, 0, s>>>(S.inv, S.dtmp, m);
+      PRH_LAUNCHED();
+    }
     vin = vout;
   }
   const uint32_t* skey = kbuf[RP.npass & 1];
   k_tile_starts<<<ceil_div(ntiles + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, static_cast<uint32_t>(ntiles),
                                                                          S.start);
   PRH_LAUNCHED();
-  // FP32 weights in tile order go to the sorted-payload memory (free in this paint)
-  float* sw = static_cast<float*>(S.spay);
+  // the weights in tile order go to the sorted-payload memory (free in this paint)
+  double* sw = static_cast<double*>(S.spay);
   k_tile_permute<P><<<grid, kCicThreads, 0, s>>>(S.sorted, rec, m, static_cast<P*>(S.spos), sw);
   PRH_LAUNCHED();
   const uint32_t per = xcd_per(ntiles);
@@ -1629,7 +1665,7 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
                                                            S.tbnd, S.misc);
   // algorithmic: tile starts, key, position and weight per point, the mesh read and written
   timer_after(PRH_TIMER_PAINT_GATHER, s,
-              static_cast<double>(ntiles) * 4 + static_cast<double>(m) * (4 + 3 * sizeof(P) + 4) +
+              static_cast<double>(ntiles) * 4 + static_cast<double>(m) * (4 + 3 * sizeof(P) + 8) +
                   static_cast<double>(S.ncell) * 16);
   PRH_LAUNCHED();
   k_tile_boundary<<<per * kXcds, kCicThreads, 0, s>>>(mesh, TG, static_cast<uint32_t>(ntiles), per, S.start,
@@ -1649,6 +1685,7 @@ static int tile_paint_all(prh_workspace* ws, double* mesh, const Frame<double>& 
   }
   std::swap(S.sorted, S.sorted_prev);
   std::swap(S.spos, S.spos_prev);
+  std::swap(S.inv, S.inv_prev);
   S.order_prev = S.order;
   S.order = PaintScratch::Order();
   k_zero_misc<<<1, 64, 0, s>>>(S.misc);
@@ -1780,7 +1817,7 @@ static int do_paint_mixed(prh_workspace* ws, double* mesh, const size_t* nmesh, 
   PRH_TRY(check_paint_ws(ws, nmesh, PRH_F64));
   if (!boxsize || !boxcenter || (n && (!mesh || !pos))) return PRH_ERR_ARG;
   if (!ws->paint.tbnd) {
-    set_error("the FP32 tile paint needs a workspace with PRH_WS_PAINT | PRH_WS_MIXED");
+    set_error("the tile paint needs a workspace with PRH_WS_PAINT | PRH_WS_MIXED");
     return PRH_ERR_STATE;
   }
   size_t ntiles = 1;
@@ -1874,11 +1911,23 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
       std::memcpy(&h, ws->red_host, sizeof(uint64_t));
       if (h == po->hash) {
         const P* spos = static_cast<const P*>(last ? S.spos : S.spos_prev);
+        const uint32_t* inv = last ? S.inv : S.inv_prev;
+        // results in the paint's order into the sorted-payload memory (free after a paint,
+        // 4 mesh words per point), then back into point order
+        O* tmp = static_cast<O*>(S.spay);
+        const bool two = inv && ostride * sizeof(O) <= 4 * static_cast<size_t>(ws->dtype);
         timer_before(PRH_TIMER_READ_SHIFTS, s);
-        k_read_sorted<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, spos, n, out,
-                                                                               misc, porder);
+        k_read_sorted<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(
+            phi, A, spos, n, two ? tmp : out, misc, two ? nullptr : porder);
         timer_after(PRH_TIMER_READ_SHIFTS, s, bytes);
         PRH_LAUNCHED();
+        if (two) {
+          if (ostride == 3)
+            k_unpermute<O, 3><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(tmp, inv, n, out);
+          else
+            k_unpermute<O, 1><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(tmp, inv, n, out);
+          PRH_LAUNCHED();
+        }
         return PRH_OK;
       }
     }

@@ -58,6 +58,12 @@ struct PaintScratch {
   size_t nhpart = 0;
   uint64_t* hash = nullptr;     // [2]      hash of the last paint's positions; of a read's
   double* tbnd = nullptr;       // ntile x 729  boundary cells of the FP32 tile paint (PRH_WS_MIXED)
+  // The tile paint's inverse permutation (inv[i]: position of point i in tile
+  // order), composed from the sort passes' destinations (dtmp), for reads that
+  // write their results in tile order and gather them back into point order.
+  uint32_t* inv = nullptr;       // chunk
+  uint32_t* inv_prev = nullptr;  // chunk
+  uint32_t* dtmp = nullptr;      // chunk
   struct Order {
     const void* pos = nullptr;
     size_t n = 0;

@@ -46,10 +46,22 @@ def synchronize(stream=None):
     check(lib.prh_synchronize(stream_handle(stream)), 'synchronize')
 
 
+def block_bytes(nbytes):
+    """Allocation size of an array of ``nbytes``: up to 1 MiB a multiple of 4 KiB, above it
+    rounded up to one of eight sizes per power of two (at most 12.5% more), so that arrays
+    of similar sizes (catalogs of different lengths) share cached blocks."""
+    nbytes = int(nbytes)
+    if nbytes <= 1 << 20:
+        step = 4096
+    else:
+        step = 1 << (nbytes.bit_length() - 4)
+    return -(-nbytes // step) * step
+
+
 class _BlockCache(object):
-    """Exact-size, stream-ordered cache of freed device blocks, so that repeated
-    reconstructions of the same shapes reuse their allocations instead of calling
-    hipMalloc/hipFree each time.
+    """Stream-ordered cache of freed device blocks, by size class (:func:`block_bytes`), so
+    that repeated reconstructions of the same or similar shapes reuse their allocations
+    instead of calling hipMalloc/hipFree (which waits for the whole device) each time.
 
     A block is returned to the cache of the stream its array was allocated for and is
     handed out again only for that stream: work queued on the stream before the release
@@ -115,20 +127,22 @@ class DeviceArray(object):
         self.base = base
         self.stream = stream_handle(stream)
         if ptr is None:
-            cached = _cache.take(self.nbytes, self.stream) if self.nbytes else None
+            self._block = block_bytes(self.nbytes)
+            cached = _cache.take(self._block, self.stream) if self.nbytes else None
             if cached is None:
                 p = ctypes.c_void_p()
-                status = lib.prh_malloc(ctypes.byref(p), self.nbytes)
+                status = lib.prh_malloc(ctypes.byref(p), self._block)
                 if status == _lib.ERR_NOMEM and _cache.cached:
                     _cache.empty()
-                    status = lib.prh_malloc(ctypes.byref(p), self.nbytes)
-                check(status, 'device allocation of {} bytes'.format(self.nbytes))
+                    status = lib.prh_malloc(ctypes.byref(p), self._block)
+                check(status, 'device allocation of {} bytes'.format(self._block))
                 cached = p.value or 0
             self.ptr = cached
             self.owner = True if owner is None else owner
         else:
             self.ptr = int(ptr)
             self.owner = bool(owner)
+            self._block = None  # an adopted block is cached under its exact size
 
     @property
     def size(self):
@@ -208,7 +222,7 @@ class DeviceArray(object):
 
     def free(self):
         if self.owner and self.ptr:
-            _cache.give(self.ptr, self.nbytes, self.stream)
+            _cache.give(self.ptr, getattr(self, '_block', None) or self.nbytes, self.stream)
         self.ptr = 0
         self.owner = False
 

@@ -371,11 +371,12 @@ class RealMesh(object):
         return tuple(o + d * np.arange(n) for n, o, d in zip(self.nmesh, self.offset, self.boxsize / self.nmesh))
 
     # ------------------------------------------------------------------ kernels
-    def assign_cic(self, positions, weights=None, wrap=False, fp32_weights=False):
+    def assign_cic(self, positions, weights=None, wrap=False, tile_sums=False):
         """Paint positions with CIC (pyrecon/mesh.py:529-568); accumulates into the mesh.
 
-        ``fp32_weights`` (FP64 meshes): the mixed-precision paint, with FP32 corner weights summed
-        exactly per 8^3-cell tile and in FP64 across tiles (deterministic, not the reference's bits).
+        ``tile_sums`` (FP64 meshes): the mixed-precision paint, with the FP64 corner weights summed
+        in fixed point per 8^3-cell tile and in FP64 across tiles (deterministic; the reference's
+        sums to within their own rounding, not its bits).
         """
         pos, _ = as_device(positions, stream=self.stream)
         if pos.ndim != 2 or pos.shape[1] != 3:
@@ -390,9 +391,9 @@ class RealMesh(object):
         nm, nm_ptr = _lib.size3(self.shape)
         bs, bs_ptr = _lib.vec3(self.boxsize, self.dtype)
         bc, bc_ptr = _lib.vec3(self.boxcenter, self.dtype)
-        if fp32_weights and self.dtype != np.float64:
-            raise ValueError('the FP32-weight paint writes FP64 meshes (dtype="f8")')
-        fn = lib.prh_paint_mixed_f64 if fp32_weights else getattr(lib, 'prh_paint_' + _suffix(self.dtype))
+        if tile_sums and self.dtype != np.float64:
+            raise ValueError('the tile-sum paint writes FP64 meshes (dtype="f8")')
+        fn = lib.prh_paint_mixed_f64 if tile_sums else getattr(lib, 'prh_paint_' + _suffix(self.dtype))
         check(fn(self.workspace.handle, self.buffer.ptr, nm_ptr, bs_ptr, bc_ptr, pos.ptr,
                  _lib.dtype_code(pos.dtype), wdev.ptr if wdev is not None else None, wcode, len(pos), int(bool(wrap)),
                  stream_handle(self.stream)), 'assign_cic')

@@ -46,7 +46,7 @@ class BaseReconstruction(object):
         always uses its own transforms.
     wrap : bool
     precision : 'plain' (default: the reference's algorithm in the mesh dtype, bit for bit where the
-        reference is deterministic C) or 'mixed' (FP64 meshes: FP32 CIC weights summed exactly per tile
+        reference is deterministic C) or 'mixed' (FP64 meshes: CIC weights summed in fixed point per tile
         and in FP64 across tiles, and the mixed-precision solve of ``run``).
     kwargs : MeshInfo arguments (nmesh, boxsize, boxcenter, cellsize, positions, boxpad, dtype);
         ``nthreads`` is accepted and ignored.  Device-specific: ``stream``, ``paint_chunk``,
@@ -107,13 +107,13 @@ class BaseReconstruction(object):
             self.los = los / distance(los)
 
     def assign_data(self, positions, weights=None):
-        self.mesh_data.assign_cic(positions, weights=weights, wrap=self.wrap, fp32_weights=self.precision == 'mixed')
+        self.mesh_data.assign_cic(positions, weights=weights, wrap=self.wrap, tile_sums=self.precision == 'mixed')
 
     def assign_randoms(self, positions, weights=None):
         if self.mesh_randoms.buffer is None:
             self._size_randoms = 0
         self.mesh_randoms.assign_cic(positions, weights=weights, wrap=self.wrap,
-                                     fp32_weights=self.precision == 'mixed')
+                                     tile_sums=self.precision == 'mixed')
         self._size_randoms += len(positions)
 
     @property

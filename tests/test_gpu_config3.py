@@ -11,7 +11,7 @@ check does on full catalogs (pyrecon/tests/test_multigrid.py:183-239).  Against 
 - potential (mean removed): relative L2 <= 1e-10; shifted positions <= 1e-8 Mpc/h;
 - the GPU multigrid given the oracle's density contrast: the oracle's potential bit for bit;
 - the GPU read given the oracle's potential: the oracle's shifted positions bit for bit;
-- the mixed path (FP32 CIC weights summed exactly per tile, FP32 V-cycles inside FP64 defect
+- the mixed path (CIC weights summed in fixed point per tile, FP32 V-cycles inside FP64 defect
   correction): paint meshes identical between two runs and within 1e-6 (relative to the largest
   cell) of the FP64 paint; shifted positions within 1e-3 Mpc/h of the oracle,
   over all data and a 1e6-point randoms sample.
@@ -146,16 +146,20 @@ def test_config3_fmg_and_read_bitexact(c3):
 def test_config3_mixed(c3, device_catalogs):
     cat, geo, sample, ref = c3
     gpu = _painted(geo, device_catalogs, 'mixed')
-    # the FP32-weight paint: identical bits on a repeated run, within FP32 rounding of the FP64 paint
+    # the tile-sum paint: identical bits on a repeated run, within the FP64 rounding of the serial paint
     again = _painted(geo, device_catalogs, 'mixed')
     for name, exact in zip(('data', 'randoms'), ref['meshes']):
         a, b = getattr(gpu, 'mesh_' + name).value, getattr(again, 'mesh_' + name).value
         assert np.array_equal(bits(a), bits(b)), 'configuration 3: mixed {} mesh differs between two paints'.format(name)
         err = np.max(np.abs(a - exact)) / np.max(np.abs(exact))
         say('mixed paint, {}: max |mesh - FP64 mesh| / max |mesh| = {:.3e}'.format(name, err))
-        assert err < 1e-6
+        assert err < 1e-13
     del again
     gpu.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+    # the density rule's mask (randoms above the threshold) is the oracle's, cell for cell
+    flips = int(np.count_nonzero((gpu.mesh_delta.value == 0.) != (ref['delta'] == 0.)))
+    say('mixed density contrast: {:d} cells with the other mask value'.format(flips))
+    assert flips == 0
     gpu.run()
     err_phi = rel_l2(gpu.mesh_phi.value, ref['phi'], remove_mean=True)
     xd = gpu.read_shifted_positions(device_catalogs['data']).numpy()

@@ -98,14 +98,16 @@ def test_config5_mixed_vs_plain(stream, plain_alone):
     assert worst[0] <= 1e-3
 
 
-def test_sparse_job_mixed_vs_plain():
-    """The sparsest field of the 256-job stream (job 191: nmesh 512, 3.3e5 data, 0.025 randoms per
-    cell), where the mean of the density contrast dominates the residual: the mixed path stays within
-    1e-3 Mpc/h of plain FP64."""
-    spec = serving_stream(256, rate=4.)[191]
+@pytest.mark.parametrize('index', [191, 233])
+def test_sparse_job_mixed_vs_plain(index):
+    """Sparse fields of the 256-job stream, nmesh 512: job 191 (3.3e5 data, 0.025 randoms per cell),
+    where the mean of the density contrast dominates the residual, and job 233 (1.8e5 data), where
+    FP32 corner weights would put one randoms cell on the other side of the density rule's threshold.  The
+    mixed path stays within 1e-3 Mpc/h of plain FP64."""
+    spec = serving_stream(256, rate=4.)[index]
     assert spec['nmesh'] == 512 and spec['ndata'] < 400000
     pd, pr = alone(spec, 'plain')
     md, mr = alone(spec, 'mixed')
     dx = max(np.max(np.abs(md - pd)), np.max(np.abs(mr - pr)))
-    say('sparse job 191, mixed against plain FP64: max |dx| {:.3e} Mpc/h'.format(dx))
+    say('sparse job {:d}, mixed against plain FP64: max |dx| {:.3e} Mpc/h'.format(index, dx))
     assert dx <= 1e-3

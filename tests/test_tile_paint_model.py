@@ -1,4 +1,4 @@
-"""CPU model of the index arithmetic of the FP32 tile paint (k_tile_key, k_tile_accumulate,
+"""CPU model of the index arithmetic of the tile paint (k_tile_key, k_tile_accumulate,
 k_tile_boundary in pyrecon_amd/csrc/prh_cic.hip).
 
 The model follows the kernels' integer arithmetic step by step (tile keys, the 9^3 local array,
