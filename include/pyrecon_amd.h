@@ -156,6 +156,19 @@ int prh_fmg_f64(prh_workspace *ws, const double *f1h, double *v1h, const size_t 
                 double jacobi_damping_factor, int jacobi_niterations, int vcycle_niterations,
                 const double *los, void *stream);
 
+/* Allocation-free full multigrid.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * result as prh_fmg_f64 without a workspace: the multigrid hierarchy lives in
+ * a file-scope device array sized for meshes of at most 2^18 (64^3) cells
+ * (3.0 MB, held by every GPU the library runs on until the process ends);
+ * larger meshes return PRH_ERR_CAPACITY.  Calls no allocating routine.  All
+ * calls on one GPU share that array: calls that overlap on one GPU race.   */
+int prh_fmg_noalloc_f64(const double *f1h, double *v1h, const size_t *nmesh,
+                        const double *boxsize, const double *boxcenter, double beta,
+                        double jacobi_damping_factor, int jacobi_niterations,
+                        int vcycle_niterations, const double *los, void *stream);
+
 /* ---- fused pipeline entry points (what the drop-in Python class calls) ----
  * positions: device (N, 3) array of pos_dtype (PRH_F32/PRH_F64) in
  * Cartesian coordinates; the mesh-unit transform, the wrap or range check and

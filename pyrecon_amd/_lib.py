@@ -98,6 +98,7 @@ def _signatures():
         'prh_fmg_mixed_group': (_i, [_vp, _i, _vp, _vp, _szp, _vp, _vp, _vp, _d, _i, _i, _vp, _i, _i, _vp]),
         'prh_residual_norm_f64': (_i, [_vp, _vp, _vp, _szp, _vp, _vp, _d, _vp, ctypes.POINTER(_d), _vp]),
         'prh_paint_mixed_f64': (_i, [_vp, _vp, _szp, _vp, _vp, _vp, _i, _vp, _i, _sz, _i, _vp]),
+        'prh_fmg_noalloc_f64': (_i, [_vp, _vp, _szp, _vp, _vp, _d, _d, _i, _i, _vp, _vp]),
     }
     for suffix, real in [('f32', _f), ('f64', _d)]:
         sig.update({

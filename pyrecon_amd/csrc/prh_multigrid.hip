@@ -10,6 +10,7 @@
 // as one pass per sweep into a second iterate (ping-pong) instead of the
 // reference's two passes through a temporary; the arithmetic per cell is the
 // same, so results are bit-identical.
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -1674,6 +1675,90 @@ static int do_fmg(prh_workspace* ws, const R* f, R* v, const size_t* nmesh, cons
   return S.fmg(one(const_cast<R*>(f)), one(v), ncycle);
 }
 
+// ---------------------------------------------------------------------------
+// Full multigrid without a workspace (prh_fmg_noalloc_f64), for callers that
+// cannot allocate device memory (HIP graph capture, tight serving loops): the
+// hierarchy of meshes of at most kNoAllocCells cells lives in a file-scope
+// device array.  Every GPU holds its own copy of the array until the process
+// ends; its address is looked up once per device (hipGetSymbolAddress) and
+// only that address is passed to kernels.  All calls on one GPU share the
+// array: calls that overlap on one GPU (other streams, other threads) race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocCells = size_t(1) << 18;  // 64^3
+constexpr size_t kNoAllocAlign = 32;                // doubles (256 bytes) per sub-buffer boundary
+// level-0 scratch iterate, plus v, f and the scratch iterate of the coarser levels (each level has
+// at most 1/8 of the previous one's cells), each rounded up to kNoAllocAlign
+constexpr size_t kNoAllocDoubles =
+    kNoAllocCells + 3 * (kNoAllocCells / 7 + 1) + (1 + 3 * kMaxLevels) * kNoAllocAlign;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kNoAllocDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+static int noalloc_scratch(double** out) {
+  int dev = 0;
+  PRH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error("device %d: at most %d devices", dev, kNoAllocDevices);
+    return PRH_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress of the multigrid scratch failed: %s", hipGetErrorString(e));
+      return PRH_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PRH_OK;
+}
+
+static int do_fmg_noalloc(const double* f, double* v, const size_t* nmesh, const double* boxsize,
+                          const double* boxcenter, double beta, double damping, int niter,
+                          int ncycle, const double* los, void* stream) {
+  if (!f || !v || !nmesh || !boxsize || !boxcenter) return PRH_ERR_ARG;
+  const size_t cells = nmesh[0] * nmesh[1] * nmesh[2];
+  if (cells == 0 || cells > kNoAllocCells) {
+    set_error("prh_fmg_noalloc_f64 takes meshes of 1 to %zu cells, got %zu", kNoAllocCells, cells);
+    return PRH_ERR_CAPACITY;
+  }
+  double* base = nullptr;
+  PRH_TRY(noalloc_scratch(&base));
+  Hierarchy H;
+  size_t m[3] = {nmesh[0], nmesh[1], nmesh[2]};
+  size_t off = 0;
+  auto take = [&](size_t count) {
+    double* p = base + off;
+    off += (count + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign;
+    return static_cast<void*>(p);
+  };
+  for (;;) {
+    const int l = H.nlev;
+    for (int d = 0; d < 3; d++) H.n[l][d] = m[d];
+    const size_t c = m[0] * m[1] * m[2];
+    H.tmp[l] = take(c);
+    if (l > 0) {
+      H.v[l] = take(c);
+      H.f[l] = take(c);
+    }
+    H.nlev++;
+    if (!coarsenable(m) || H.nlev >= kMaxLevels) break;
+    for (int d = 0; d < 3; d++) m[d] /= 2;
+  }
+  if (off > kNoAllocDoubles) {
+    set_error("multigrid scratch overflow (%zu > %zu doubles)", off, kNoAllocDoubles);
+    return PRH_ERR_CAPACITY;
+  }
+  Solver<double> S;
+  S.setup(&H, boxsize, boxcenter, beta, damping, niter, los, as_stream(stream));
+  return S.fmg(one(const_cast<double*>(f)), one(v), ncycle);
+}
+
 // Checks a group of jobs: 1..kMaxGroup workspaces of the same shape and
 // precision, each with its buffers and geometry.
 template <typename T>
@@ -1939,6 +2024,14 @@ int prh_fmg_f64(prh_workspace* ws, const double* f1h, double* v1h, const size_t*
                 double jacobi_damping_factor, int jacobi_niterations, int vcycle_niterations,
                 const double* los, void* stream) {
   return do_fmg<double>(ws, f1h, v1h, nmesh, boxsize, boxcenter, beta, jacobi_damping_factor,
+                        jacobi_niterations, vcycle_niterations, los, stream);
+}
+
+int prh_fmg_noalloc_f64(const double* f1h, double* v1h, const size_t* nmesh, const double* boxsize,
+                        const double* boxcenter, double beta, double jacobi_damping_factor,
+                        int jacobi_niterations, int vcycle_niterations, const double* los,
+                        void* stream) {
+  return do_fmg_noalloc(f1h, v1h, nmesh, boxsize, boxcenter, beta, jacobi_damping_factor,
                         jacobi_niterations, vcycle_niterations, los, stream);
 }
 
