@@ -417,13 +417,11 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 // equal-digit pairs before it in the wave (the wave's running count in LDS
 // plus the equal lanes below it).  The tile then stages its pairs in LDS in
 // (digit, rank) order and writes each digit's run to its scanned offset.
-// `vin` null: the values are the input positions (first pass).  With `dest`,
-// each pair's output position is also written at its input position (coalesced),
-// so that the passes' destinations compose into the inverse permutation.
+// `vin` null: the values are the input positions (first pass).
 __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, size_t m, int shift,
     int bits, uint32_t ntiles, uint32_t per, const uint32_t* __restrict__ goff,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t* __restrict__ dest = nullptr) {
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
   __shared__ uint32_t s_wcnt[kRadixWaves][kRadixMaxBins];  // per-wave counts, then prefixes
   __shared__ uint32_t s_lbeg[kRadixMaxBins];               // digit's first slot in the tile
   __shared__ uint32_t s_gbeg[kRadixMaxBins];               // digit's first output position
@@ -504,7 +502,6 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
         s_key[slot] = k[j];
         s_val[slot] = v[j];
       }
-      if (dest) dest[wbase + j * 64 + lane] = s_gbeg[d] + s_wcnt[w][d] + r[j];
     }
   }
   __syncthreads();
@@ -941,13 +938,6 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_key(const P* __restrict__ 
   if (hpart) block_hash_partial(hash, hpart);
 }
 
-// inv[i] = step[inv[i]]: the inverse permutation after one more sort pass.
-__global__ __launch_bounds__(kCicThreads) void k_compose(uint32_t* __restrict__ inv,
-                                                         const uint32_t* __restrict__ step, size_t m) {
-  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
-  if (i < m) inv[i] = step[inv[i]];
-}
-
 // start[t] = first sorted position whose tile (key >> kPTShift) is >= t, for
 // t in [0, ntiles]; one thread per tile, by binary search over the sorted keys.
 __global__ __launch_bounds__(kCicThreads) void k_tile_starts(const uint32_t* __restrict__ skey, size_t m,
@@ -1244,6 +1234,55 @@ __global__ __launch_bounds__(kCicThreads) void k_read_fd(const R* __restrict__ m
   out[3 * i + 2] = g[2];
 }
 
+// The potential as a read sees it: global memory with periodic indices.
+template <typename R>
+struct GlobalMesh {
+  const R* __restrict__ phi;
+  int n0, n1, n2;
+  __device__ __forceinline__ R value(const R p[3]) const { return cic_value(phi, n0, n1, n2, p); }
+  __device__ __forceinline__ void gradient(const R cell[3], const R p[3], R g[3]) const {
+    fd_gradient(phi, n0, n1, n2, cell, p, g);
+  }
+};
+
+// The 11^3 cells o - 1 .. o + 9 of an 8^3 tile with origin o, staged in LDS
+// (periodic indices): every cell the value (base .. base + 1) or the central
+// differences (base - 1 .. base + 2) of a point whose base cell lies in the tile
+// reads; other points read global memory.  Same values, same arithmetic: the
+// results are GlobalMesh's bit for bit.
+constexpr int kTS = kPT + 3;  // staged cells per axis
+template <typename R>
+struct TileMesh {
+  const R* lds;
+  int o[3];
+  GlobalMesh<R> g;
+  __device__ __forceinline__ bool local(const R p[3], int u[3], R t[3]) const {
+    int b[3];
+    const int nn[3] = {g.n0, g.n1, g.n2};
+    cic_base(p, nn, b, t);
+    bool in = true;
+    for (int d = 0; d < 3; d++) {
+      u[d] = b[d] - o[d];
+      in = in && u[d] >= 0 && u[d] < kPT;
+    }
+    return in;
+  }
+  __device__ __forceinline__ R value(const R p[3]) const {
+    int u[3];
+    R t[3];
+    if (!local(p, u, t)) return g.value(p);
+    const R* c = lds + ((u[0] + 1) * kTS + (u[1] + 1)) * kTS + (u[2] + 1);
+    return cic_core<R>([&](int a, int bb, int e) { return c[(a * kTS + bb) * kTS + e]; }, t);
+  }
+  __device__ __forceinline__ void gradient(const R cell[3], const R p[3], R gr[3]) const {
+    int u[3];
+    R t[3];
+    if (!local(p, u, t)) return g.gradient(cell, p, gr);
+    const R* c = lds + (u[0] * kTS + u[1]) * kTS + u[2];
+    fd_core<R>([&](int i, int j, int k) { return c[(i * kTS + j) * kTS + k]; }, t, cell, gr);
+  }
+};
+
 // Parameters of a fused read shared by all points.
 template <typename R>
 struct ReadArgs {
@@ -1261,8 +1300,8 @@ constexpr int kRT = 8;  // read binning tile edge (cells)
 // positions dtype P; the local line of sight x/|x| is in P, computed from
 // the unwrapped input; with a global line of sight (mesh dtype R) the RSD term
 // is in Q = promote(P, R).  O is the output dtype.
-template <typename R, typename P, typename O>
-__device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const R* __restrict__ phi,
+template <typename R, typename P, typename O, typename M>
+__device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& mesh,
                                                const ReadArgs<R>& A, O* __restrict__ out) {
   using Q = promote_t<P, R>;
   const Frame<R>& F = A.F;
@@ -1271,11 +1310,11 @@ __device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const R* 
   to_mesh_units<R, P>(x, F, A.wrap, p, err);
   if (A.field == PRH_FIELD_VALUE) {
     // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
-    out[i] = static_cast<O>(static_cast<P>(cic_value(phi, F.n[0], F.n[1], F.n[2], p)));
+    out[i] = static_cast<O>(static_cast<P>(mesh.value(p)));
     return err;
   }
   R g[3];
-  fd_gradient(phi, F.n[0], F.n[1], F.n[2], A.cell, p, g);
+  mesh.gradient(A.cell, p, g);
   P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
   const int field = A.field, shifted = A.shifted;
   // shifts in P (disp, disp+rsd, local rsd) or in Q (global rsd)
@@ -1330,7 +1369,7 @@ __device__ __forceinline__ uint32_t read_point(size_t i, const R* __restrict__ p
                                                const ReadArgs<R>& A, const P* __restrict__ pos,
                                                O* __restrict__ out) {
   const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-  return read_value<R, P, O>(i, x, phi, A, out);
+  return read_value<R, P, O>(i, x, GlobalMesh<R>{phi, A.F.n[0], A.F.n[1], A.F.n[2]}, A, out);
 }
 
 // Wave-level OR of the status flags, one atomic per wave.
@@ -1387,23 +1426,56 @@ __global__ __launch_bounds__(kCicThreads) void k_read_sorted(const R* __restrict
   uint32_t err = 0;
   if (t < m) {
     const P x[3] = {spos[3 * t], spos[3 * t + 1], spos[3 * t + 2]};
-    // without `order` the results stay in the paint's order (sequential writes), for k_unpermute
-    err = read_value<R, P, O>(order ? order[t] : t, x, phi, A, out);
+    err = read_value<R, P, O>(order[t], x, GlobalMesh<R>{phi, A.F.n[0], A.F.n[1], A.F.n[2]}, A, out);
   }
   flag_errors(err, misc);
 }
 
-// out[i] = tmp[inv[i]] (S values per point): results back into point order, with
-// sequential writes and scattered reads.
-template <typename O, int S>
-__global__ __launch_bounds__(kCicThreads) void k_unpermute(const O* __restrict__ tmp,
-                                                           const uint32_t* __restrict__ inv, size_t m,
-                                                           O* __restrict__ out) {
-  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
-  if (i >= m) return;
-  const size_t t = inv[i];
-#pragma unroll
-  for (int c = 0; c < S; c++) out[S * i + c] = tmp[S * t + c];
+// The painted positions tile by tile (one workgroup per 8^3 tile of the tile
+// paint, tiles dealt to the XCDs in contiguous ranges): the tile's potential is
+// staged in LDS once and each point's 32 mesh reads come from there; results
+// go to the points' input positions (order).
+template <typename R, typename P, typename O>
+__global__ __launch_bounds__(kCicThreads) void k_read_tiles(const R* __restrict__ phi, ReadArgs<R> A,
+                                                            TileGrid TG, uint32_t ntiles, uint32_t per,
+                                                            const uint32_t* __restrict__ tstart,
+                                                            const P* __restrict__ spos,
+                                                            const uint32_t* __restrict__ order,
+                                                            O* __restrict__ out, uint32_t* __restrict__ misc) {
+  __shared__ R s_m[kTS * kTS * kTS];
+  const uint32_t t = xcd_block(per);
+  if (t >= ntiles) return;
+  const uint32_t beg = tstart[t], end = tstart[t + 1];
+  if (beg >= end) return;
+  const TilePos T = tile_pos(t, TG);
+  const int n0 = A.F.n[0], n1 = A.F.n[1], n2 = A.F.n[2];
+  // the loads of this thread's first point are issued before the staging, to overlap with it
+  const uint32_t q0 = beg + threadIdx.x;
+  P x0[3] = {P(0), P(0), P(0)};
+  uint32_t i0 = 0;
+  if (q0 < end) {
+    x0[0] = spos[3 * static_cast<size_t>(q0)];
+    x0[1] = spos[3 * static_cast<size_t>(q0) + 1];
+    x0[2] = spos[3 * static_cast<size_t>(q0) + 2];
+    i0 = order[q0];
+  }
+  for (int c = threadIdx.x; c < kTS * kTS * kTS; c += kCicThreads) {
+    const int u = c / (kTS * kTS), v = (c / kTS) % kTS, w = c % kTS;
+    const int gx = ((T.o[0] - 1 + u) % n0 + n0) % n0;
+    const int gy = ((T.o[1] - 1 + v) % n1 + n1) % n1;
+    const int gz = ((T.o[2] - 1 + w) % n2 + n2) % n2;
+    s_m[c] = phi[(static_cast<size_t>(gx) * n1 + gy) * n2 + gz];
+  }
+  __syncthreads();
+  uint32_t err = 0;
+  const TileMesh<R> M{s_m, {T.o[0], T.o[1], T.o[2]}, GlobalMesh<R>{phi, n0, n1, n2}};
+  if (q0 < end) err = read_value<R, P, O>(i0, x0, M, A, out);
+  for (uint32_t q = q0 + kCicThreads; q < end; q += kCicThreads) {
+    const P x[3] = {spos[3 * static_cast<size_t>(q)], spos[3 * static_cast<size_t>(q) + 1],
+                    spos[3 * static_cast<size_t>(q) + 2]};
+    err |= read_value<R, P, O>(order[q], x, M, A, out);
+  }
+  flag_errors(err, misc);
 }
 
 // Read binning: a counting sort of the points by 8^3-cell tile (the order
@@ -1559,7 +1631,7 @@ static int paint_all(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos
   // this paint's order goes to the older of the two order arrays
   std::swap(S.sorted, S.sorted_prev);
   std::swap(S.spos, S.spos_prev);
-  std::swap(S.inv, S.inv_prev);
+  std::swap(S.tstart, S.tstart_prev);
   S.order_prev = S.order;
   S.order = PaintScratch::Order();
   k_zero_misc<<<1, 64, 0, s>>>(S.misc);
@@ -1638,21 +1710,14 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
     k_scan_final<<<nscan, kCicThreads, 0, s>>>(S.hist, nh, S.partial, S.goff);
     PRH_LAUNCHED();
     uint32_t* vout = vbuf[(RP.npass - 1 - p) & 1];
-    // destinations: of the first pass straight into inv, of later passes composed into it
-    uint32_t* dest = keep ? (p == 0 ? S.inv : S.dtmp) : nullptr;
     k_radix_scatter<<<rper * kXcds, kRadixThreads, 0, s>>>(kbuf[p & 1], vin, m, RP.shift[p], RP.bits[p],
-                                                           rtiles, rper, S.goff, kbuf[(p + 1) & 1], vout,
-                                                           dest);
+                                                           rtiles, rper, S.goff, kbuf[(p + 1) & 1], vout);
     PRH_LAUNCHED();
-    if (keep && p > 0) {
-      k_compose<<<grid, kCicThreads, 0, s>>>(S.inv, S.dtmp, m);
-      PRH_LAUNCHED();
-    }
     vin = vout;
   }
   const uint32_t* skey = kbuf[RP.npass & 1];
   k_tile_starts<<<ceil_div(ntiles + 1, kCicThreads), kCicThreads, 0, s>>>(skey, m, static_cast<uint32_t>(ntiles),
-                                                                         S.start);
+                                                                         S.tstart);
   PRH_LAUNCHED();
   // the weights in tile order go to the sorted-payload memory (free in this paint)
   double* sw = static_cast<double*>(S.spay);
@@ -1661,14 +1726,14 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
   const uint32_t per = xcd_per(ntiles);
   timer_before(PRH_TIMER_PAINT_GATHER, s);
   k_tile_accumulate<P><<<per * kXcds, kCicThreads, 0, s>>>(mesh, F, TG, MODE == 2, static_cast<uint32_t>(ntiles),
-                                                           per, S.start, skey, static_cast<const P*>(S.spos), sw,
+                                                           per, S.tstart, skey, static_cast<const P*>(S.spos), sw,
                                                            S.tbnd, S.misc);
   // algorithmic: tile starts, key, position and weight per point, the mesh read and written
   timer_after(PRH_TIMER_PAINT_GATHER, s,
               static_cast<double>(ntiles) * 4 + static_cast<double>(m) * (4 + 3 * sizeof(P) + 8) +
                   static_cast<double>(S.ncell) * 16);
   PRH_LAUNCHED();
-  k_tile_boundary<<<per * kXcds, kCicThreads, 0, s>>>(mesh, TG, static_cast<uint32_t>(ntiles), per, S.start,
+  k_tile_boundary<<<per * kXcds, kCicThreads, 0, s>>>(mesh, TG, static_cast<uint32_t>(ntiles), per, S.tstart,
                                                       S.tbnd, S.misc);
   PRH_LAUNCHED();
   return PRH_OK;
@@ -1685,7 +1750,7 @@ static int tile_paint_all(prh_workspace* ws, double* mesh, const Frame<double>& 
   }
   std::swap(S.sorted, S.sorted_prev);
   std::swap(S.spos, S.spos_prev);
-  std::swap(S.inv, S.inv_prev);
+  std::swap(S.tstart, S.tstart_prev);
   S.order_prev = S.order;
   S.order = PaintScratch::Order();
   k_zero_misc<<<1, 64, 0, s>>>(S.misc);
@@ -1721,6 +1786,7 @@ static int tile_paint_all(prh_workspace* ws, double* mesh, const Frame<double>& 
   S.order.n = n;
   S.order.pdt = Tag<P>::code;
   S.order.copy = ceil_div(n, kCicThreads) <= S.nhpart;
+  S.order.tiles = S.order.copy;
   S.order.hash = hash;
   return PRH_OK;
 }
@@ -1911,23 +1977,24 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
       std::memcpy(&h, ws->red_host, sizeof(uint64_t));
       if (h == po->hash) {
         const P* spos = static_cast<const P*>(last ? S.spos : S.spos_prev);
-        const uint32_t* inv = last ? S.inv : S.inv_prev;
-        // results in the paint's order into the sorted-payload memory (free after a paint,
-        // 4 mesh words per point), then back into point order
-        O* tmp = static_cast<O*>(S.spay);
-        const bool two = inv && ostride * sizeof(O) <= 4 * static_cast<size_t>(ws->dtype);
+        const uint32_t* ts = last ? S.tstart : S.tstart_prev;
         timer_before(PRH_TIMER_READ_SHIFTS, s);
-        k_read_sorted<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(
-            phi, A, spos, n, two ? tmp : out, misc, two ? nullptr : porder);
+        if (po->tiles && ts) {
+          TileGrid TG;
+          for (int d = 0; d < 3; d++) {
+            TG.n[d] = F.n[d];
+            TG.t[d] = (F.n[d] + kPT - 1) / kPT;
+          }
+          const size_t ntiles = static_cast<size_t>(TG.t[0]) * TG.t[1] * TG.t[2];
+          const uint32_t per = xcd_per(ntiles);
+          k_read_tiles<R, P, O><<<per * kXcds, kCicThreads, 0, s>>>(phi, A, TG, static_cast<uint32_t>(ntiles),
+                                                                    per, ts, spos, porder, out, misc);
+        } else {
+          k_read_sorted<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, spos, n, out,
+                                                                                 misc, porder);
+        }
         timer_after(PRH_TIMER_READ_SHIFTS, s, bytes);
         PRH_LAUNCHED();
-        if (two) {
-          if (ostride == 3)
-            k_unpermute<O, 3><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(tmp, inv, n, out);
-          else
-            k_unpermute<O, 1><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(tmp, inv, n, out);
-          PRH_LAUNCHED();
-        }
         return PRH_OK;
       }
     }

@@ -127,7 +127,7 @@ static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags)
     p.paint = z.ntile * 4 + (z.ncell + 1) * 4 + (z.npartial + 1) * 4 + chunk * 4 * 5 +
               (2 * z.nhist + 1) * 4 + (z.nbnd + 1) * 4 + 2 * chunk * 4 * w + 4 * 4 +
               2 * chunk * 3 * sizeof(double) + (z.nhpart + 2) * sizeof(uint64_t);
-    if ((flags & PRH_WS_MIXED) && dtype == PRH_F64) p.paint += z.ntile * 729 * sizeof(double) + 3 * chunk * 4;
+    if ((flags & PRH_WS_MIXED) && dtype == PRH_F64) p.paint += z.ntile * 729 * sizeof(double) + 2 * (z.ntile + 1) * 4;
   }
   if (flags & PRH_WS_FFT) p.fft = smooth_bytes(n, dtype);
   p.red = (kReduceBlocks * 2 + 16) * sizeof(double) + 16 * sizeof(uint32_t);
@@ -177,9 +177,8 @@ static int paint_alloc(prh_workspace* ws, size_t chunk) {
   if ((ws->flags & PRH_WS_MIXED) && ws->dtype == PRH_F64) {
     PRH_TRY(ws_alloc(ws, &p, z.ntile * 729 * sizeof(double)));
     P.tbnd = static_cast<double*>(p);
-    PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.inv = static_cast<uint32_t*>(p);
-    PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.inv_prev = static_cast<uint32_t*>(p);
-    PRH_TRY(ws_alloc(ws, &p, chunk * 4)); P.dtmp = static_cast<uint32_t*>(p);
+    PRH_TRY(ws_alloc(ws, &p, (z.ntile + 1) * 4)); P.tstart = static_cast<uint32_t*>(p);
+    PRH_TRY(ws_alloc(ws, &p, (z.ntile + 1) * 4)); P.tstart_prev = static_cast<uint32_t*>(p);
   }
   PRH_HIP(hipMemset(P.count, 0, z.ntile * 4));
   PRH_HIP(hipMemset(P.misc, 0, 16));
@@ -369,8 +368,8 @@ int prh_workspace_destroy(prh_workspace* ws) {
   for (void* p : {(void*)P.count, (void*)P.start, (void*)P.partial, (void*)P.key, (void*)P.rank,
                   (void*)P.vtmp, (void*)P.hist, (void*)P.goff, (void*)P.bnd, (void*)P.sorted,
                   (void*)P.sorted_prev, P.payload, P.spay, (void*)P.misc, P.spos, P.spos_prev,
-                  (void*)P.hpart, (void*)P.hash, (void*)P.tbnd, (void*)P.inv, (void*)P.inv_prev,
-                  (void*)P.dtmp})
+                  (void*)P.hpart, (void*)P.hash, (void*)P.tbnd, (void*)P.tstart,
+                  (void*)P.tstart_prev})
     release(p);
   for (int d = 0; d < 3; d++) {
     release(ws->smooth.tw[d]);

@@ -58,17 +58,16 @@ struct PaintScratch {
   size_t nhpart = 0;
   uint64_t* hash = nullptr;     // [2]      hash of the last paint's positions; of a read's
   double* tbnd = nullptr;       // ntile x 729  boundary cells of the FP32 tile paint (PRH_WS_MIXED)
-  // The tile paint's inverse permutation (inv[i]: position of point i in tile
-  // order), composed from the sort passes' destinations (dtmp), for reads that
-  // write their results in tile order and gather them back into point order.
-  uint32_t* inv = nullptr;       // chunk
-  uint32_t* inv_prev = nullptr;  // chunk
-  uint32_t* dtmp = nullptr;      // chunk
+  // first sorted position of each 8^3 tile (and the end) of the last two tile
+  // paints, for reads that walk the painted positions tile by tile
+  uint32_t* tstart = nullptr;       // ntile + 1 (PRH_WS_MIXED)
+  uint32_t* tstart_prev = nullptr;  // ntile + 1
   struct Order {
     const void* pos = nullptr;
     size_t n = 0;
     int pdt = 0;
-    bool copy = false;  // spos holds the positions in this order
+    bool copy = false;   // spos holds the positions in this order
+    bool tiles = false;  // ... in 8^3-tile order, with the tile starts in tstart
     uint64_t hash = 0;
   };
   Order order, order_prev;  // tags of `sorted` and `sorted_prev`

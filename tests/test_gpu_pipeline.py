@@ -111,6 +111,31 @@ def test_read_reuses_paint_order(c1):
     assert np.array_equal(bits(gpu.read_shifted_positions(drand).numpy()), bits(cpu.read_shifted_positions(moved)))
 
 
+def test_mixed_read_tiles(c1):
+    # precision='mixed' paints by 8^3 tile; a read of a painted catalog whose positions hash to the
+    # painted ones walks the paint's copy tile by tile with the potential staged in LDS.  Same
+    # arithmetic as the global-memory read: the oracle's bits for the oracle's potential, for each
+    # field, for both catalogs, and after the positions change in place (hash mismatch)
+    from pyrecon_amd import DeviceArray
+    gpu, cpu = make_pair(c1, precision='mixed')
+    rand = c1['randoms'][:300000]
+    drand, ddata = DeviceArray.from_numpy(rand), DeviceArray.from_numpy(c1['data'])
+    for r, d, rr in ((gpu, ddata, drand), (cpu, c1['data'], rand)):
+        r.assign_data(d, c1['data_weights'])
+        r.assign_randoms(rr, c1['randoms_weights'][:300000])
+        r.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
+        r.run()
+    gpu.mesh_phi.value = cpu.mesh_phi
+    for field in ('disp+rsd', 'rsd', 'disp'):
+        for dev, host in ((drand, rand), (ddata, c1['data'])):
+            got = gpu.read_shifted_positions(dev, field=field).numpy()
+            assert np.array_equal(bits(got), bits(cpu.read_shifted_positions(host, field=field))), field
+    assert np.array_equal(bits(gpu.read_shifts(drand, field='disp').numpy()), bits(cpu.read_shifts(rand, field='disp')))
+    moved = np.ascontiguousarray(rand[np.random.RandomState(3).permutation(len(rand))])
+    drand.upload(moved)
+    assert np.array_equal(bits(gpu.read_shifted_positions(drand).numpy()), bits(cpu.read_shifted_positions(moved)))
+
+
 def test_fmg_of_oracle_delta_bitexact(c1):
     # same density contrast in -> bit-identical potential out (the multigrid is exact)
     gpu, cpu = make_pair(c1)
