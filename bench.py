@@ -505,23 +505,23 @@ def main():
     def timed(precision, steps, warmup):
         for _ in range(warmup): step(precision)
         P.synchronize()
-        with _lib.KernelTimer(_lib.TIMER_JACOBI_FINEST) as kt:
+        with _lib.KernelTimer(_lib.TIMER_JACOBI_FINEST) as kt, _lib.KernelTimer(_lib.TIMER_PAIR_FINEST) as kp:
             t0 = time.perf_counter()
             for _ in range(steps): step(precision)
             P.synchronize()
             elapsed = time.perf_counter() - t0
-            jac = kt.read()
-        return elapsed, jac
+            jac, pair = kt.read(), kp.read()
+        return elapsed, jac, pair
 
     log('mixed: warmup {:d}, timed {:d}'.format(args.warmup, args.steps))
-    elapsed, jac = timed('mixed', args.steps, args.warmup)
+    elapsed, jac, pair = timed('mixed', args.steps, args.warmup)
     value = args.steps / elapsed
     log('mixed: {:.3f} recon/s ({:.1f} ms/step)'.format(value, 1e3 * elapsed / args.steps))
     stages_mixed = {}
     step('mixed', stages=stages_mixed)
     nfp64 = args.fp64_steps if args.fp64_steps is not None else max(1, args.steps // 2)
     log('plain fp64: timed {:d}'.format(nfp64))
-    elapsed64, jac64 = timed('plain', nfp64, 1)
+    elapsed64, jac64, _ = timed('plain', nfp64, 1)
     value64 = nfp64 / elapsed64
     log('plain fp64: {:.3f} recon/s ({:.1f} ms/step)'.format(value64, 1e3 * elapsed64 / nfp64))
     stages_plain = {}
@@ -567,20 +567,35 @@ def main():
         log('accuracy', accuracy)
         del phi_p, xd_p, xr_p, phi_m, xd_m, xr_m, a, b, dx
 
+    # the dominant kernel: the finest-level sweeps of the mixed path, two per launch where paired
+    ms_p, np_, bytes_p = pair
     ms_j, nj, bytes_j = jac
-    achieved = bytes_j / (ms_j * 1e-3) / 1e9 if ms_j > 0 else 0.
+    single = dict(launches=nj, avg_launch_ms=round(ms_j / max(nj, 1), 5),
+                  achieved_gbs=round(bytes_j / (ms_j * 1e-3) / 1e9, 1) if ms_j > 0 else None,
+                  kernel='k_stencil_rows<float, fast> (one sweep per launch)')
+    paired = ms_p > ms_j
+    ms_d, nd_, bytes_d = pair if paired else jac
+    achieved = bytes_d / (ms_d * 1e-3) / 1e9 if ms_d > 0 else 0.
     traffic = None
-    tfile = os.path.join(ROOT, 'profiles', 'jacobi_f32_512_traffic.json')
+    tfile = os.path.join(ROOT, 'profiles', 'stencil_pair_f32_512_traffic.json' if paired else 'jacobi_f32_512_traffic.json')
     if os.path.isfile(tfile):
         with open(tfile) as fh: traffic = json.load(fh).get('hbm_bytes_per_launch')
+    if paired:
+        kernel = ('k_stencil_pair<float, fast>: two damped-Jacobi sweeps per launch (blocks of 8 whole 512-cell rows, '
+                  'the intermediate iterate in LDS), finest level 512^3 (mixed path); achieved = algorithmic bytes '
+                  '(3 x 512^3 x 4 B per pair, 2 x 512^3 x 4 B from a zero start) / average launch time (HIP events); '
+                  'traffic = FETCH_SIZE + WRITE_SIZE per launch, calibrated on a copy kernel')
+    else:
+        kernel = ('k_stencil_rows<float, fast> damped-Jacobi sweep (blocks of 8 whole 512-cell rows), '
+                  'finest level 512^3 (mixed path); achieved = algorithmic bytes (3 x 512^3 x 4 B per '
+                  'sweep, 2 x 512^3 x 4 B for the zero-start sweep) / average launch time (HIP events); '
+                  'traffic = FETCH_SIZE + WRITE_SIZE per launch, calibrated on a copy kernel')
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel='k_stencil_rows<float, fast> damped-Jacobi sweep (blocks of 8 whole 512-cell rows), '
-                           'finest level 512^3 (mixed path); achieved = algorithmic bytes (3 x 512^3 x 4 B per '
-                           'sweep, 2 x 512^3 x 4 B for the zero-start sweep) / average launch time (HIP events); '
-                           'traffic = FETCH_SIZE + WRITE_SIZE per launch, calibrated on a copy kernel',
-                    launches=nj, avg_launch_ms=round(ms_j / max(nj, 1), 5),
-                    bytes_per_launch=(bytes_j / nj) if nj else None)
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=kernel,
+                    launches=nd_, avg_launch_ms=round(ms_d / max(nd_, 1), 5),
+                    bytes_per_launch=(bytes_d / nd_) if nd_ else None)
+    if paired:
+        roofline['single_sweeps'] = single
     cpu = None
     if not args.no_cpu_baseline:
         log('cpu baseline')

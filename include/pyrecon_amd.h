@@ -122,6 +122,13 @@ int prh_read_finite_difference_cic_f64(const double *mesh, const size_t *nmesh,
 int prh_jacobi_f32(prh_workspace *ws, float *v, const float *f, const size_t *nmesh,
                    const float *boxsize, const float *boxcenter, float beta, float damping_factor,
                    int niterations, const float *los, void *stream);
+/* The mixed-precision solve's FP32 smoother (no reference counterpart; for
+ * tests): niterations fast damped-Jacobi sweeps (hardware reciprocal,
+ * explicit FMA; not the reference's bits), two per launch on levels with whole
+ * rows of 256 or 512 cells (PRH_TUNE_PAIRED_SWEEPS), from zero if zero_start. */
+int prh_jacobi_fast_f32(prh_workspace *ws, float *v, const float *f, const size_t *nmesh,
+                        const float *boxsize, const float *boxcenter, float beta, float damping_factor,
+                        int niterations, const float *los, int zero_start, void *stream);
 int prh_jacobi_f64(prh_workspace *ws, double *v, const double *f, const size_t *nmesh,
                    const double *boxsize, const double *boxcenter, double beta,
                    double damping_factor, int niterations, const double *los, void *stream);
@@ -390,10 +397,20 @@ int prh_fmg_mixed(prh_workspace *ws, const double *f1h, double *v1h, const size_
 #define PRH_TIMER_JACOBI_FINEST 0 /* damped-Jacobi sweeps on the finest level */
 #define PRH_TIMER_PAINT_GATHER 1  /* CIC paint merge-gather kernel */
 #define PRH_TIMER_READ_SHIFTS 2   /* fused finite-difference read + shift kernel */
-#define PRH_TIMER_COUNT 3
+#define PRH_TIMER_PAIR_FINEST 3   /* two fused FP32 sweeps on the finest level (mixed path) */
+#define PRH_TIMER_COUNT 4
 int prh_timer_enable(int which, int enable);
 int prh_timer_reset(int which);
 int prh_timer_read(int which, double *total_ms, long long *launches, double *bytes);
+/* Process-wide tuning switches (no reference counterpart); results do not
+ * depend on them.  PRH_TUNE_PAIRED_SWEEPS (default 0): 1 makes the mixed-
+ * precision solve run two Jacobi sweeps per launch on levels with whole rows
+ * of 256 or 512 cells (the same bits; slower on MI355X, where the sweep is
+ * nearly VALU-bound).  Returns the previous value, or PRH_ERR_ARG for an
+ * unknown option. */
+#define PRH_TUNE_PAIRED_SWEEPS 0
+int prh_set_tuning(int option, int value);
+
 /* a plain device-to-device copy kernel (HBM traffic calibration for profiles) */
 int prh_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 

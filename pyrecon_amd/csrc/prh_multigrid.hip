@@ -10,6 +10,7 @@
 // as one pass per sweep into a second iterate (ping-pong) instead of the
 // reference's two passes through a temporary; the arithmetic per cell is the
 // same, so results are bit-identical.
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 #include <utility>
@@ -504,6 +505,169 @@ __global__ __launch_bounds__(64 * kRowsY * WPR, PRH_ROWS_MIN_WAVES) void k_stenc
   }
 }
 
+// Two stencil operations fused along x (temporal blocking), FP32 fast kernels
+// on levels with whole rows of N2 = 256 or 512 cells and n1 % 8 == 0: OP1
+// (a Jacobi sweep, or one from zero) maps v to u and OP2 (a Jacobi sweep, or
+// one added into the FP64 mesh acc) maps u to the output, and u never leaves
+// LDS.  A block owns RY = 8 complete rows of a run of x planes.  For each plane
+// p it computes u on rows y0 - 1 .. y0 + RY (the halo rows OP2 needs; the
+// neighbouring blocks compute the same values) from the planes p - 1, p, p + 1
+// of v on rows y0 - 2 .. y0 + RY + 1, held in a ring of 3 LDS slots, then the
+// output at plane p - 1 from the ring of 3 u planes.  v plane p + 3 is in
+// flight in registers meanwhile.  Per cell, the two sweeps move 12 bytes
+// instead of 24 (8 from a zero start, 24 with the FP64 update instead of 40).
+// The arithmetic per cell is stencil_cell()'s: the same bits as two launches.
+constexpr int kPairRY = 8;
+template <int OP1, int OP2, int NT, int N2, bool LOCAL>
+__global__ __launch_bounds__(NT, 1) void k_stencil_pair(const float* __restrict__ v,
+                                                       const float* __restrict__ f,
+                                                       float* __restrict__ out, TileCoef<float> T,
+                                                       float omega, float keep, int xlen) {
+  constexpr int RY = kPairRY, VR = RY + 4, UR = RY + 2;
+  constexpr bool ZERO = OP1 == kOpJacobiZero;
+  constexpr int VPLANE = VR * N2, UPLANE = UR * N2, WPLANE = RY * N2;
+  constexpr int VPER = (VPLANE + NT - 1) / NT, UPER = (UPLANE + NT - 1) / NT, WPER = WPLANE / NT;
+  static_assert(WPLANE % NT == 0, "output rows split evenly over the threads");
+  __shared__ float sv[ZERO ? 1 : 3 * VPLANE];
+  __shared__ float su[3 * UPLANE];
+  const LevelGeom<float>& G = T.G;
+  const int n0 = G.n[0], n1 = G.n[1];
+  const int tid = threadIdx.x;
+  // XCD-aware order (see k_stencil_tiled): y-tile fastest, then x-run
+  const int ytiles = n1 / RY;
+  const unsigned nb = gridDim.x, bid = blockIdx.x, per = nb / 8;
+  const unsigned L = bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+  const int yt = static_cast<int>(L % ytiles), xr = static_cast<int>(L / ytiles);
+  const int y0 = yt * RY;
+  const int xb = xr * xlen;
+  const int xe = xb + xlen < n0 ? xb + xlen : n0;
+  const size_t s0 = static_cast<size_t>(n1) * N2;
+  auto wrapx = [n0](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
+  auto wrapy = [n1](int y) { return y < 0 ? y + n1 : (y >= n1 ? y - n1 : y); };
+  auto slot = [xb](int p) { return (p - xb + 3) % 3; };
+  // v rows y0 - 2 .. y0 + RY + 1 of a plane: element e = tid + NT q
+  int voff[VPER];
+  bool vhas[VPER];
+  float pre[VPER];
+#pragma unroll
+  for (int q = 0; q < VPER; q++) {
+    const int e = tid + NT * q;
+    vhas[q] = !ZERO && e < VPLANE;
+    voff[q] = wrapy(y0 - 2 + e / N2) * N2 + e % N2;
+  }
+  auto fetch = [&](int p) {
+    const float* src = v + static_cast<size_t>(wrapx(p)) * s0;
+#pragma unroll
+    for (int q = 0; q < VPER; q++)
+      if (vhas[q]) pre[q] = src[voff[q]];
+  };
+  auto stash = [&](int p) {
+    float* dst = sv + slot(p) * VPLANE;
+#pragma unroll
+    for (int q = 0; q < VPER; q++)
+      if (vhas[q]) dst[tid + NT * q] = pre[q];
+  };
+  // one cell of an operation: planes M, P, N (row stride N2), centre row rb / N2
+  auto cell = [&](auto op, const float* M, const float* P, const float* N, int rb, int c, float fv,
+                  int x, int iy) {
+    constexpr int OPC = decltype(op)::value;
+    const int cl = c == 0 ? N2 - 1 : c - 1;
+    const int cr = c == N2 - 1 ? 0 : c + 1;
+    Hood<float> h = zero_hood<float>();
+    if (OPC != kOpJacobiZero) {
+      const int rm = rb - N2, rp = rb + N2;
+      h.c = P[rb + c];
+      h.xp = N[rb + c];
+      h.xm = M[rb + c];
+      h.yp = P[rp + c];
+      h.ym = P[rm + c];
+      h.zp = P[rb + cr];
+      h.zm = P[rb + cl];
+      h.xpyp = N[rp + c];
+      h.xmym = M[rm + c];
+      h.xmyp = M[rp + c];
+      h.xpym = N[rm + c];
+      h.xpzp = N[rb + cr];
+      h.xmzm = M[rb + cl];
+      h.xmzp = M[rb + cr];
+      h.xpzm = N[rb + cl];
+      h.ypzp = P[rp + cr];
+      h.ymzm = P[rm + cl];
+      h.ymzp = P[rm + cr];
+      h.ypzm = P[rp + cl];
+    }
+    float px = 0.f, A = 0.f, py = 0.f, B = 0.f, pz = 0.f, C = 0.f, pypz = T.K.pypz;
+    if (LOCAL) {
+      px = static_cast<float>(x) + G.org[0];
+      A = G.h2[0] * px * px;
+      py = static_cast<float>(iy) + G.org[1];
+      B = G.h2[1] * py * py;
+      pz = static_cast<float>(c) + G.org[2];
+      C = G.h2[2] * pz * pz;
+      pypz = py * pz;
+    }
+    return stencil_cell<float, OPC, LOCAL, true>(T, h, fv, px, A, py, B, pz, C, pypz, omega, keep);
+  };
+  if (!ZERO) {
+    fetch(xb - 2);
+    stash(xb - 2);
+    fetch(xb - 1);
+    stash(xb - 1);
+    fetch(xb);
+    stash(xb);
+    fetch(xb + 1);
+  }
+  __syncthreads();
+  for (int p = xb - 1; p <= xe; p++) {
+    // u at plane p, rows y0 - 1 .. y0 + RY
+    {
+      const int xp = wrapx(p);
+      const float* fx = f + static_cast<size_t>(xp) * s0;
+      const float* Vm = sv + (ZERO ? 0 : slot(p - 1) * VPLANE);
+      const float* Vp = sv + (ZERO ? 0 : slot(p) * VPLANE);
+      const float* Vn = sv + (ZERO ? 0 : slot(p + 1) * VPLANE);
+      float* U = su + slot(p) * UPLANE;
+#pragma unroll
+      for (int q = 0; q < UPER; q++) {
+        const int e = tid + NT * q;
+        if (e < UPLANE) {
+          const int r = e / N2, c = e % N2;
+          const int iy = wrapy(y0 - 1 + r);
+          U[e] = cell(std::integral_constant<int, OP1>(), Vm, Vp, Vn, (r + 1) * N2, c,
+                      fx[static_cast<size_t>(iy) * N2 + c], xp, iy);
+        }
+      }
+    }
+    __syncthreads();
+    if (!ZERO && p + 2 <= xe + 1) {
+      stash(p + 2);  // into the slot of plane p - 1, which no later u reads
+      if (p + 3 <= xe + 1) fetch(p + 3);
+    }
+    // the output at plane x = p - 1, rows y0 .. y0 + RY - 1
+    const int x = p - 1;
+    if (x >= xb) {
+      const float* fx = f + static_cast<size_t>(x) * s0;
+      const float* Um = su + slot(x - 1) * UPLANE;
+      const float* Up = su + slot(x) * UPLANE;
+      const float* Un = su + slot(x + 1) * UPLANE;
+#pragma unroll
+      for (int q = 0; q < WPER; q++) {
+        const int e = tid + NT * q;
+        const int r = e / N2, c = e % N2;
+        const int iy = y0 + r;
+        const size_t gi = static_cast<size_t>(x) * s0 + static_cast<size_t>(iy) * N2 + c;
+        const float res = cell(std::integral_constant<int, OP2>(), Um, Up, Un, (r + 1) * N2, c,
+                               fx[static_cast<size_t>(iy) * N2 + c], x, iy);
+        if constexpr (OP2 == kOpJacobiAcc)
+          T.acc[gi] += static_cast<double>(res);
+        else
+          out[gi] = res;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // row length in units of 64 cells (1, 2, 4 or 8), 0 where the kernel does not apply
 inline int rows_cz(const int* n) {
   if (n[2] % 64 != 0 || n[1] % kRowsY != 0 || n[0] < 1) return 0;
@@ -521,6 +685,52 @@ inline int rows_xlen(const int* n) {
 }
 
 inline bool tiled_ok(const int* n) { return n[2] % kTileZ == 0 && n[1] % kTileY == 0 && n[0] >= 1; }
+
+// PRH_TUNE_PAIRED_SWEEPS, off by default: on MI355X the fused pair is slower
+// than two single sweeps (0.69 against 2 x 0.31 ms at 512^3), because a sweep
+// of this 19-point operator with per-cell coefficients is nearly as VALU-bound
+// (about 60 instructions per cell) as it is HBM-bound, and the pair recomputes
+// a quarter of the intermediate cells with one 132 KB block per CU.
+static std::atomic<int> g_paired_sweeps{0};
+
+inline bool pair_ok(const int* n) {
+  return (n[2] == 256 || n[2] == 512) && n[1] % kPairRY == 0 && n[0] >= 2;
+}
+#ifndef PRH_PAIR_BLOCKS
+#define PRH_PAIR_BLOCKS 512
+#endif
+// x-run per block of the fused pair so that the grid has about PRH_PAIR_BLOCKS blocks
+inline int pair_xlen(const int* n) {
+  const long tiles = n[1] / kPairRY;
+  long chunks = (PRH_PAIR_BLOCKS + tiles - 1) / tiles;
+  if (chunks > n[0]) chunks = n[0];
+  return static_cast<int>((n[0] + chunks - 1) / chunks);
+}
+
+// Two fused FP32 fast operations (k_stencil_pair): OP1 from v (kOpJacobi or
+// kOpJacobiZero), OP2 into out (kOpJacobi) or into acc (kOpJacobiAcc).
+template <int OP1, int OP2>
+static void launch_pair(const float* v, const float* f, float* out, const LevelGeom<float>& G,
+                        float omega, float keep, hipStream_t s, double* acc = nullptr) {
+  TileCoef<float> T;
+  T.G = G;
+  T.K = G.local ? GlobalCoef<float>() : GlobalCoef<float>::make(G);
+  T.dinv = G.local ? 0.f : 1.f / T.K.d;
+  T.acc = acc;
+  const int xlen = pair_xlen(G.n);
+  const unsigned grid = static_cast<unsigned>((G.n[1] / kPairRY) * ((G.n[0] + xlen - 1) / xlen));
+  if (G.n[2] == 512) {
+    if (G.local)
+      k_stencil_pair<OP1, OP2, 1024, 512, true><<<grid, 1024, 0, s>>>(v, f, out, T, omega, keep, xlen);
+    else
+      k_stencil_pair<OP1, OP2, 1024, 512, false><<<grid, 1024, 0, s>>>(v, f, out, T, omega, keep, xlen);
+  } else {
+    if (G.local)
+      k_stencil_pair<OP1, OP2, 512, 256, true><<<grid, 512, 0, s>>>(v, f, out, T, omega, keep, xlen);
+    else
+      k_stencil_pair<OP1, OP2, 512, 256, false><<<grid, 512, 0, s>>>(v, f, out, T, omega, keep, xlen);
+  }
+}
 
 // x-run per block so that the grid has roughly 2048+ blocks
 inline int tiled_xlen(const int* n) {
@@ -1366,6 +1576,7 @@ struct Solver {
   int niter;
   hipStream_t s;
   bool fast = false;  // fast FP32 kernels on the blocked levels (mixed-precision corrections)
+  bool pair_enabled = true;  // with fast: two sweeps per launch where the level allows
   // When fuse_top is set, the last post-smoothing sweep of level 0 adds its result into the FP64
   // mesh acc_top[j] instead of storing it; fused[j] tells whether it could (blocked levels).
   bool fuse_top = false;
@@ -1502,16 +1713,57 @@ struct Solver {
     return PRH_OK;
   }
 
-  // niter damped-Jacobi sweeps; the iterate moves between cur and alt.
+  // two sweeps in one launch (k_stencil_pair): FP32 fast kernels on levels with
+  // whole rows of 256 or 512 cells
+  bool pairable(int l) const {
+    if constexpr (std::is_same<R, float>::value) {
+      const int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]), static_cast<int>(n(l)[2])};
+      return fast && pair_enabled && pair_ok(nn);
+    }
+    return false;
+  }
+  int jacobi_pair(int l, const Ptrs<R>& in, const Ptrs<R>& src, const Ptrs<R>& out, bool zero_in, bool acc) {
+    if constexpr (std::is_same<R, float>::value) {
+      const double total = static_cast<double>(size(l));
+      // algorithmic traffic: v (unless zero) and f read, the result written (or added into FP64)
+      const double bytes = total * (acc ? 24.0 : zero_in ? 8.0 : 12.0);
+      for (int j = 0; j < count; j++) {
+        timer_before(l == 0 ? PRH_TIMER_PAIR_FINEST : -1, s);
+        if (acc) {
+          launch_pair<kOpJacobi, kOpJacobiAcc>(in[j], src[j], out[j], G[j][l], omega, keep, s, acc_top[j]);
+          fused[j] = true;
+        } else if (zero_in) {
+          launch_pair<kOpJacobiZero, kOpJacobi>(in[j], src[j], out[j], G[j][l], omega, keep, s);
+        } else {
+          launch_pair<kOpJacobi, kOpJacobi>(in[j], src[j], out[j], G[j][l], omega, keep, s);
+        }
+        timer_after(l == 0 ? PRH_TIMER_PAIR_FINEST : -1, s, bytes);
+        PRH_LAUNCHED();
+      }
+      return PRH_OK;
+    }
+    return PRH_ERR_STATE;
+  }
+
+  // niter damped-Jacobi sweeps; the iterate moves between cur and alt.  Where
+  // the level allows, the sweeps go in fused pairs (the last one alone when
+  // niter is odd).
   int smooth(int l, Ptrs<R>& cur, Ptrs<R>& alt, const Ptrs<R>& src, bool zero_start, bool post = false) {
     if (niter <= 0) {
       if (zero_start) PRH_TRY(fill_zero(l, cur));
       return PRH_OK;
     }
-    for (int it = 0; it < niter; it++) {
-      const bool fuse = post && fuse_top && l == 0 && it == niter - 1;
-      PRH_TRY(jacobi(l, cur, src, alt, zero_start && it == 0, fuse));
+    const bool last_fused = post && fuse_top && l == 0;
+    for (int it = 0; it < niter;) {
+      if (pairable(l) && niter - it >= 2) {
+        PRH_TRY(jacobi_pair(l, cur, src, alt, zero_start && it == 0, last_fused && it + 2 == niter));
+        swap_all(cur, alt, count);
+        it += 2;
+        continue;
+      }
+      PRH_TRY(jacobi(l, cur, src, alt, zero_start && it == 0, last_fused && it == niter - 1));
       swap_all(cur, alt, count);
+      it++;
     }
     return PRH_OK;
   }
@@ -1647,6 +1899,24 @@ static int do_jacobi(prh_workspace* ws, R* v, const R* f, const size_t* nmesh, c
   PRH_TRY(S.smooth(0, cur, alt, one(const_cast<R*>(f)), false));
   if (cur[0] != v)
     PRH_HIP(hipMemcpyAsync(v, cur[0], S.size(0) * sizeof(R), hipMemcpyDeviceToDevice, S.s));
+  return PRH_OK;
+}
+
+// The mixed path's FP32 smoother on its own: niter fast damped-Jacobi sweeps
+// (two per launch where the level allows), from zero if zero_start.
+static int do_jacobi_fast(prh_workspace* ws, float* v, const float* f, const size_t* nmesh,
+                          const float* boxsize, const float* boxcenter, float beta, float damping,
+                          int niter, const float* los, int zero_start, void* stream) {
+  PRH_TRY(check_ws(ws, nmesh, PRH_F32, PRH_WS_MULTIGRID));
+  if (!v || !f || !boxsize || !boxcenter) return PRH_ERR_ARG;
+  Solver<float> S;
+  S.setup(&ws->mg, boxsize, boxcenter, beta, damping, niter, los, as_stream(stream));
+  S.fast = true;
+  S.pair_enabled = g_paired_sweeps.load() != 0;
+  Ptrs<float> cur = one(v), alt = S.tmp(0);
+  PRH_TRY(S.smooth(0, cur, alt, one(const_cast<float*>(f)), zero_start != 0));
+  if (cur[0] != v)
+    PRH_HIP(hipMemcpyAsync(v, cur[0], S.size(0) * sizeof(float), hipMemcpyDeviceToDevice, S.s));
   return PRH_OK;
 }
 
@@ -1878,6 +2148,7 @@ static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double*
   }
   S.params(static_cast<float>(damping), niter, s);
   S.fast = true;
+  S.pair_enabled = g_paired_sweeps.load() != 0;
   const Ptrs<float> r32 = S.f(0), e32 = S.v(0);
   // 0. the centred source.  The mean of f is a null mode of the operator: the
   // iterate drifts by a constant per sweep, and in an FP32 correction that
@@ -1902,10 +2173,13 @@ static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double*
     PRH_LAUNCHED();
     src[j] = fc;
   }
-  // 1. coarse FMG stages in FP32, prolonged into the FP64 iterate
+  // 1. coarse FMG stages in FP32, prolonged into the FP64 iterate.  The full
+  // multigrid starts its coarsest level from zero; when level 1 is that level,
+  // Solver::fmg starts from the iterate it is given, so that is zeroed first.
   if (S.nlev() > 1) {
     const Ptrs<float> f1 = S.f(1), v1 = S.v(1);
     for (int j = 0; j < count; j++) PRH_TRY((launch_restrict<double, float>(src[j], f1[j], nmesh, s)));
+    if (S.nlev() == 2) PRH_TRY(S.fill_zero(1, v1));
     PRH_TRY(S.fmg(f1, v1, ncycle, 1));
     for (int j = 0; j < count; j++)
       PRH_TRY((launch_prolong<float, double>(v1[j], v[j], S.n(1), false, s)));
@@ -1957,11 +2231,24 @@ using namespace prh;
 
 extern "C" {
 
+int prh_set_tuning(int option, int value) {
+  if (option == PRH_TUNE_PAIRED_SWEEPS) return g_paired_sweeps.exchange(value ? 1 : 0);
+  set_error("unknown tuning option %d", option);
+  return PRH_ERR_ARG;
+}
+
+
 int prh_jacobi_f32(prh_workspace* ws, float* v, const float* f, const size_t* nmesh,
                    const float* boxsize, const float* boxcenter, float beta, float damping_factor,
                    int niterations, const float* los, void* stream) {
   return do_jacobi<float>(ws, v, f, nmesh, boxsize, boxcenter, beta, damping_factor, niterations,
                           los, stream);
+}
+int prh_jacobi_fast_f32(prh_workspace* ws, float* v, const float* f, const size_t* nmesh,
+                        const float* boxsize, const float* boxcenter, float beta, float damping_factor,
+                        int niterations, const float* los, int zero_start, void* stream) {
+  return do_jacobi_fast(ws, v, f, nmesh, boxsize, boxcenter, beta, damping_factor, niterations, los,
+                        zero_start, stream);
 }
 int prh_jacobi_f64(prh_workspace* ws, double* v, const double* f, const size_t* nmesh,
                    const double* boxsize, const double* boxcenter, double beta,
