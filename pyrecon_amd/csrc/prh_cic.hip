@@ -1436,7 +1436,7 @@ __global__ __launch_bounds__(kCicThreads) void k_read_sorted(const R* __restrict
 // staged in LDS once and each point's 32 mesh reads come from there; results
 // go to the points' input positions (order).
 template <typename R, typename P, typename O>
-__global__ __launch_bounds__(kCicThreads) void k_read_tiles(const R* __restrict__ phi, ReadArgs<R> A,
+__global__ __launch_bounds__(kCicThreads, 4) void k_read_tiles(const R* __restrict__ phi, ReadArgs<R> A,
                                                             TileGrid TG, uint32_t ntiles, uint32_t per,
                                                             const uint32_t* __restrict__ tstart,
                                                             const P* __restrict__ spos,

@@ -351,12 +351,12 @@ constexpr int kRowsY = 8;
 #define PRH_ROWS_MIN_WAVES 1
 #endif
 
-template <typename R, typename O, int OP, int WPR, int CZ, bool LOCAL, bool FAST>
-__global__ __launch_bounds__(64 * kRowsY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_rows(const R* __restrict__ v,
+template <typename R, typename O, int OP, int WPR, int CZ, bool LOCAL, bool FAST, int RY = kRowsY>
+__global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_rows(const R* __restrict__ v,
                                                               const R* __restrict__ f,
                                                               O* __restrict__ out, TileCoef<R> T,
                                                               R omega, R keep, int xlen) {
-  constexpr int RY = kRowsY, N2 = 64 * WPR * CZ, PLANE = (RY + 2) * N2, NT = 64 * RY * WPR;
+  constexpr int N2 = 64 * WPR * CZ, PLANE = (RY + 2) * N2, NT = 64 * RY * WPR;
   constexpr int PER = (PLANE + NT - 1) / NT;
   __shared__ R lds[4 * PLANE];
   const LevelGeom<R>& G = T.G;
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(64 * kRowsY * WPR, PRH_ROWS_MIN_WAVES) void k_stenc
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int e = tid + NT * q;
-    has[q] = e < PLANE;
+    has[q] = (PLANE % NT == 0) || e < PLANE;
     const int row = e / N2, col = e % N2;
     int gy = y0 - 1 + row;
     gy += gy < 0 ? n1 : 0;
@@ -668,16 +668,23 @@ __global__ __launch_bounds__(NT, 1) void k_stencil_pair(const float* __restrict_
   }
 }
 
-// row length in units of 64 cells (1, 2, 4 or 8), 0 where the kernel does not apply
+// Rows per block of the full-row kernel: 8, or 4 for rows of 1024 cells (four
+// wavefronts per row, four cells per lane, 96 KB of LDS for the ring of 4
+// planes).  Measured on the 1024^3 FP32 sweep: 2.71 ms (59% of 8 TB/s) against
+// 2.88 ms with 2 rows of 8 wavefronts and 2.99 ms with the 64-wide tiles.
+constexpr int kRowsY1024 = 4;
+inline int rows_ry(const int* n) { return n[2] == 1024 ? kRowsY1024 : kRowsY; }
+
+// row length in units of 64 cells (1, 2, 4, 8 or 16), 0 where the kernel does not apply
 inline int rows_cz(const int* n) {
-  if (n[2] % 64 != 0 || n[1] % kRowsY != 0 || n[0] < 1) return 0;
+  if (n[2] % 64 != 0 || n[1] % rows_ry(n) != 0 || n[0] < 1) return 0;
   const int cz = n[2] / 64;
-  return (cz == 1 || cz == 2 || cz == 4 || cz == 8) ? cz : 0;
+  return (cz == 1 || cz == 2 || cz == 4 || cz == 8 || cz == 16) ? cz : 0;
 }
 
 // x-run per block so that the grid has roughly PRH_ROWS_BLOCKS blocks
 inline int rows_xlen(const int* n) {
-  const long tiles = n[1] / kRowsY;
+  const long tiles = n[1] / rows_ry(n);
   long chunks = (PRH_ROWS_BLOCKS + tiles - 1) / tiles;
   if (chunks > n[0]) chunks = n[0];
   int xlen = static_cast<int>((n[0] + chunks - 1) / chunks);
@@ -741,17 +748,17 @@ inline int tiled_xlen(const int* n) {
   return xlen < 4 ? (n[0] < 4 ? n[0] : 4) : xlen;
 }
 
-template <typename R, typename O, int OP, bool FAST, int WPR, int CZ>
+template <typename R, typename O, int OP, bool FAST, int WPR, int CZ, int RY = kRowsY>
 static void launch_rows_cz(const R* v, const R* f, O* out, const TileCoef<R>& T, R omega, R keep,
                            hipStream_t s) {
   const int xlen = rows_xlen(T.G.n);
   const unsigned grid =
-      static_cast<unsigned>((T.G.n[1] / kRowsY) * ((T.G.n[0] + xlen - 1) / xlen));
-  constexpr int nt = 64 * kRowsY * WPR;
+      static_cast<unsigned>((T.G.n[1] / RY) * ((T.G.n[0] + xlen - 1) / xlen));
+  constexpr int nt = 64 * RY * WPR;
   if (T.G.local)
-    k_stencil_rows<R, O, OP, WPR, CZ, true, FAST><<<grid, nt, 0, s>>>(v, f, out, T, omega, keep, xlen);
+    k_stencil_rows<R, O, OP, WPR, CZ, true, FAST, RY><<<grid, nt, 0, s>>>(v, f, out, T, omega, keep, xlen);
   else
-    k_stencil_rows<R, O, OP, WPR, CZ, false, FAST><<<grid, nt, 0, s>>>(v, f, out, T, omega, keep, xlen);
+    k_stencil_rows<R, O, OP, WPR, CZ, false, FAST, RY><<<grid, nt, 0, s>>>(v, f, out, T, omega, keep, xlen);
 }
 
 // The full-row kernel where it applies (FP32 levels with whole rows of 64, 128,
@@ -766,6 +773,8 @@ static bool launch_rows(const R* v, const R* f, O* out, const TileCoef<R>& T, R 
       case 4: launch_rows_cz<R, O, OP, FAST, 1, 4>(v, f, out, T, omega, keep, s); return true;
       // 512-cell rows: two wavefronts per row keep the registers per lane near those of 256
       case 8: launch_rows_cz<R, O, OP, FAST, 2, 4>(v, f, out, T, omega, keep, s); return true;
+      // 1024-cell rows: four wavefronts per row, 4 rows per block
+      case 16: launch_rows_cz<R, O, OP, FAST, 4, 4, kRowsY1024>(v, f, out, T, omega, keep, s); return true;
       default: return false;
     }
   }

@@ -13,10 +13,10 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 8, 8), (8, 12, 16), (16, 16, 16), (6, 10, 12), (32, 32, 32), (24, 40, 20),
           # shapes taking the blocked kernels: 64-wide tiles (n2 % 64 == 0, n1 % 8 == 0, incl. a partial
-          # x run) and, in FP32, whole rows of 64, 128, 256 and 512 cells (n1 % 8 == 0)
+          # x run) and, in FP32, whole rows of 64, 128, 256 and 512 cells (n1 % 8 == 0) and of 1024 cells
+          # (n1 % 4 == 0)
           (6, 16, 64), (10, 32, 128), (64, 64, 64), (6, 16, 256), (4, 8, 512), (9, 24, 512),
-          # 1024-cell rows (the 1024^3 workload's z extent) take the 64-wide tiles
-          (4, 8, 1024)]
+          (4, 8, 1024), (3, 4, 1024), (5, 12, 1024)]
 LOS = [None, [0., 0., 1.], [0.3, 0.4, np.sqrt(1 - 0.25)]]
 BOX = dict(boxsize=np.array([1000., 1200., 900.]), boxcenter=np.array([0., 100., 1500.]))
 
