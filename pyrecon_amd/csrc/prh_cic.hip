@@ -999,6 +999,7 @@ __device__ __forceinline__ TilePos tile_pos(uint32_t t, const TileGrid& TG) {
   return T;
 }
 
+constexpr int kAccKeep = 2;  // points per thread whose offsets stay in registers between the passes
 template <typename P>
 __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
     double* __restrict__ mesh, Frame<double> F, TileGrid TG, int wrap, uint32_t ntiles,
@@ -1014,9 +1015,25 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
   if (beg >= end) return;  // empty: the boundary pass skips it
   const TilePos T = tile_pos(t, TG);
   for (int c = threadIdx.x; c < kPTCells; c += kCicThreads) acc[c] = 0ull;
-  // weight bound of the tile (fixed order: strided per thread, then a fixed tree)
+  // weight bound of the tile (fixed order: strided per thread, then a fixed tree).  The offsets
+  // and weights of a thread's first kAccKeep points stay in registers for the second pass, so
+  // that tiles of up to kAccKeep x 256 points read their positions and weights once.
   double bsum = 0.0;
-  for (uint32_t q = beg + threadIdx.x; q < end; q += kCicThreads) {
+  double dk[kAccKeep][3], wk[kAccKeep];
+#pragma unroll
+  for (int j = 0; j < kAccKeep; j++) {
+    const uint32_t q = beg + threadIdx.x + j * kCicThreads;
+    dk[j][0] = dk[j][1] = dk[j][2] = 0.0;
+    wk[j] = 0.0;
+    if (q < end) {
+      tile_offsets<P>(spos, q, F, wrap, dk[j]);
+      wk[j] = sw[q];
+      double pw = fabs(wk[j]);
+      for (int k = 0; k < 3; k++) pw *= fmax(fabs(1.0 - dk[j][k]), fabs(dk[j][k]));
+      bsum += pw;
+    }
+  }
+  for (uint32_t q = beg + threadIdx.x + kAccKeep * kCicThreads; q < end; q += kCicThreads) {
     double d[3];
     tile_offsets<P>(spos, q, F, wrap, d);
     double pw = fabs(sw[q]);
@@ -1031,11 +1048,8 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
   int ex = 0;
   if (B > 0.0 && isfinite(B)) frexp(B, &ex);  // B < 2^ex
   const double scale = ldexp(1.0, 62 - ex), inv = ldexp(1.0, ex - 62);
-  for (uint32_t q = beg + threadIdx.x; q < end; q += kCicThreads) {
-    double d[3];
-    tile_offsets<P>(spos, q, F, wrap, d);
-    const double wq = sw[q];
-    const uint32_t cell = skey[q] & ((1u << kPTShift) - 1u);
+  auto add_point = [&](const double d[3], double wq, uint32_t key) {
+    const uint32_t cell = key & ((1u << kPTShift) - 1u);
     const int cx = static_cast<int>(cell >> 6), cy = static_cast<int>((cell >> 3) & 7u),
               cz = static_cast<int>(cell & 7u);
 #pragma unroll
@@ -1047,6 +1061,16 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
       const long long fixed = __double2ll_rn(((wx * wy) * wz) * wq * scale);
       atomicAdd(&acc[((cx + a) * kPTH + (cy + b)) * kPTH + (cz + c)], static_cast<unsigned long long>(fixed));
     }
+  };
+#pragma unroll
+  for (int j = 0; j < kAccKeep; j++) {
+    const uint32_t q = beg + threadIdx.x + j * kCicThreads;
+    if (q < end) add_point(dk[j], wk[j], skey[q]);
+  }
+  for (uint32_t q = beg + threadIdx.x + kAccKeep * kCicThreads; q < end; q += kCicThreads) {
+    double d[3];
+    tile_offsets<P>(spos, q, F, wrap, d);
+    add_point(d, sw[q], skey[q]);
   }
   __syncthreads();
   const size_t n1 = static_cast<size_t>(TG.n[1]), n2 = static_cast<size_t>(TG.n[2]);
@@ -1435,8 +1459,12 @@ __global__ __launch_bounds__(kCicThreads) void k_read_sorted(const R* __restrict
 // paint, tiles dealt to the XCDs in contiguous ranges): the tile's potential is
 // staged in LDS once and each point's 32 mesh reads come from there; results
 // go to the points' input positions (order).
+#ifndef PRH_READ_TILE_THREADS
+#define PRH_READ_TILE_THREADS 128
+#endif
+constexpr int kReadTileThreads = PRH_READ_TILE_THREADS;
 template <typename R, typename P, typename O>
-__global__ __launch_bounds__(kCicThreads, 4) void k_read_tiles(const R* __restrict__ phi, ReadArgs<R> A,
+__global__ __launch_bounds__(kReadTileThreads, 4) void k_read_tiles(const R* __restrict__ phi, ReadArgs<R> A,
                                                             TileGrid TG, uint32_t ntiles, uint32_t per,
                                                             const uint32_t* __restrict__ tstart,
                                                             const P* __restrict__ spos,
@@ -1459,7 +1487,7 @@ __global__ __launch_bounds__(kCicThreads, 4) void k_read_tiles(const R* __restri
     x0[2] = spos[3 * static_cast<size_t>(q0) + 2];
     i0 = order[q0];
   }
-  for (int c = threadIdx.x; c < kTS * kTS * kTS; c += kCicThreads) {
+  for (int c = threadIdx.x; c < kTS * kTS * kTS; c += kReadTileThreads) {
     const int u = c / (kTS * kTS), v = (c / kTS) % kTS, w = c % kTS;
     const int gx = ((T.o[0] - 1 + u) % n0 + n0) % n0;
     const int gy = ((T.o[1] - 1 + v) % n1 + n1) % n1;
@@ -1470,7 +1498,7 @@ __global__ __launch_bounds__(kCicThreads, 4) void k_read_tiles(const R* __restri
   uint32_t err = 0;
   const TileMesh<R> M{s_m, {T.o[0], T.o[1], T.o[2]}, GlobalMesh<R>{phi, n0, n1, n2}};
   if (q0 < end) err = read_value<R, P, O>(i0, x0, M, A, out);
-  for (uint32_t q = q0 + kCicThreads; q < end; q += kCicThreads) {
+  for (uint32_t q = q0 + kReadTileThreads; q < end; q += kReadTileThreads) {
     const P x[3] = {spos[3 * static_cast<size_t>(q)], spos[3 * static_cast<size_t>(q) + 1],
                     spos[3 * static_cast<size_t>(q) + 2]};
     err |= read_value<R, P, O>(order[q], x, M, A, out);
@@ -1987,7 +2015,7 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
           }
           const size_t ntiles = static_cast<size_t>(TG.t[0]) * TG.t[1] * TG.t[2];
           const uint32_t per = xcd_per(ntiles);
-          k_read_tiles<R, P, O><<<per * kXcds, kCicThreads, 0, s>>>(phi, A, TG, static_cast<uint32_t>(ntiles),
+          k_read_tiles<R, P, O><<<per * kXcds, kReadTileThreads, 0, s>>>(phi, A, TG, static_cast<uint32_t>(ntiles),
                                                                     per, ts, spos, porder, out, misc);
         } else {
           k_read_sorted<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, spos, n, out,
