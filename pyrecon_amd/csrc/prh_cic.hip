@@ -871,14 +871,14 @@ __global__ void k_zero_misc(uint32_t* misc) {
 // sum's own rounding (a cell lands on the other side of the density rule's
 // threshold only if it lies within that rounding of it), and every repeated run
 // gives the same bits.  Points are sorted by
-// 8^3-cell tile alone (the key's low 9 bits carry the cell inside the tile and
-// ride along), so a 2- or 3-digit radix sort replaces the cell sort and the
-// point-order merge.  One workgroup per tile:
+// kPT^3-cell tile alone (kPT = 16; the key's low 12 bits carry the cell inside
+// the tile and ride along), so a 2-digit radix sort replaces the cell sort and
+// the point-order merge.  One workgroup per tile:
 //   - loads its points (positions in tile order, their FP64 weights),
 //   - bounds B = sum |w| prod_d max(|1 - d_d|, |d_d|) over them (an FP64
 //     reduction in a fixed order) and takes the scale 2^(62 - e), B < 2^e,
 //   - rounds each FP64 contribution ((wx*wy)*wz)*w times the scale to an
-//     integer and adds it into a 9^3 array of 64-bit integers in LDS: integer
+//     integer and adds it into a 17^3 array of 64-bit integers in LDS: integer
 //     sums do not depend on their order, and the scaled sum stays below 2^62,
 //   - converts to FP64 (the scale is a power of two) and adds the cells that
 //     only this tile reaches (local indices 1 .. extent - 1 on every axis)
@@ -887,10 +887,12 @@ __global__ void k_zero_misc(uint32_t* misc) {
 // and the upper halos of the (up to 7) lower neighbour tiles in a fixed order.
 // The contributions are rounded to within 2^-62 B (B the tile's own weight
 // bound), about the rounding of the FP64 products themselves.
-constexpr int kPT = 8;                           // tile edge (cells)
+constexpr int kPTB = kPaintTileBits;            // log2 of the tile edge
+constexpr int kPT = kPaintTile;                  // tile edge (cells)
 constexpr int kPTH = kPT + 1;                    // with the upper halo
-constexpr int kPTCells = kPTH * kPTH * kPTH;     // 729
-constexpr int kPTShift = 9;                      // key = tile << 9 | ((cx * 8 + cy) * 8 + cz)
+constexpr int kPTCells = kPTH * kPTH * kPTH;     // accumulation cells per tile
+constexpr int kPTShift = 3 * kPTB;               // key = tile << kPTShift | ((cx * kPT + cy) * kPT + cz)
+static_assert(kPTCells == kPaintTileCells, "tile record size");
 
 struct TileGrid {
   int n[3];  // mesh shape
@@ -925,8 +927,10 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_key(const P* __restrict__ 
     if (w && !isfinite(static_cast<double>(w[i]))) err |= kErrNonFinite;
     int b[3];
     for (int d = 0; d < 3; d++) b[d] = base_index(static_cast<int>(p[d]), F.n[d]);
-    const uint32_t tile = (static_cast<uint32_t>(b[0] >> 3) * TG.t[1] + (b[1] >> 3)) * TG.t[2] + (b[2] >> 3);
-    key[i] = (tile << kPTShift) | (((b[0] & 7) * 8 + (b[1] & 7)) * 8 + (b[2] & 7));
+    const uint32_t tile =
+        (static_cast<uint32_t>(b[0] >> kPTB) * TG.t[1] + (b[1] >> kPTB)) * TG.t[2] + (b[2] >> kPTB);
+    key[i] = (tile << kPTShift) |
+             (((b[0] & (kPT - 1)) * kPT + (b[1] & (kPT - 1))) * kPT + (b[2] & (kPT - 1)));
     TileRec<P> r;
     r.x[0] = x[0];
     r.x[1] = x[1];
@@ -999,30 +1003,31 @@ __device__ __forceinline__ TilePos tile_pos(uint32_t t, const TileGrid& TG) {
   return T;
 }
 
-constexpr int kAccKeep = 2;  // points per thread whose offsets stay in registers between the passes
+constexpr int kAccThreads = 512;  // threads per tile (a 16^3 tile holds about 3000 randoms at 512^3)
+constexpr int kAccKeep = 4;  // points per thread whose offsets stay in registers between the passes
 template <typename P>
-__global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
+__global__ __launch_bounds__(kAccThreads) void k_tile_accumulate(
     double* __restrict__ mesh, Frame<double> F, TileGrid TG, int wrap, uint32_t ntiles,
     uint32_t per, const uint32_t* __restrict__ start, const uint32_t* __restrict__ skey,
     const P* __restrict__ spos, const double* __restrict__ sw, double* __restrict__ bnd,
     const uint32_t* __restrict__ misc) {
   __shared__ unsigned long long acc[kPTCells];
-  __shared__ double s_red[kCicThreads / 64];
+  __shared__ double s_red[kAccThreads / 64];
   if (misc[0] != 0u) return;  // invalid input: leave the mesh untouched
   const uint32_t t = xcd_block(per);
   if (t >= ntiles) return;
   const uint32_t beg = start[t], end = start[t + 1];
   if (beg >= end) return;  // empty: the boundary pass skips it
   const TilePos T = tile_pos(t, TG);
-  for (int c = threadIdx.x; c < kPTCells; c += kCicThreads) acc[c] = 0ull;
+  for (int c = threadIdx.x; c < kPTCells; c += kAccThreads) acc[c] = 0ull;
   // weight bound of the tile (fixed order: strided per thread, then a fixed tree).  The offsets
   // and weights of a thread's first kAccKeep points stay in registers for the second pass, so
-  // that tiles of up to kAccKeep x 256 points read their positions and weights once.
+  // that tiles of up to kAccKeep x kAccThreads points read their positions and weights once.
   double bsum = 0.0;
   double dk[kAccKeep][3], wk[kAccKeep];
 #pragma unroll
   for (int j = 0; j < kAccKeep; j++) {
-    const uint32_t q = beg + threadIdx.x + j * kCicThreads;
+    const uint32_t q = beg + threadIdx.x + j * kAccThreads;
     dk[j][0] = dk[j][1] = dk[j][2] = 0.0;
     wk[j] = 0.0;
     if (q < end) {
@@ -1033,7 +1038,7 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
       bsum += pw;
     }
   }
-  for (uint32_t q = beg + threadIdx.x + kAccKeep * kCicThreads; q < end; q += kCicThreads) {
+  for (uint32_t q = beg + threadIdx.x + kAccKeep * kAccThreads; q < end; q += kAccThreads) {
     double d[3];
     tile_offsets<P>(spos, q, F, wrap, d);
     double pw = fabs(sw[q]);
@@ -1044,14 +1049,14 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = bsum;
   __syncthreads();
   double B = 0.0;
-  for (int u = 0; u < kCicThreads / 64; u++) B += s_red[u];
+  for (int u = 0; u < kAccThreads / 64; u++) B += s_red[u];
   int ex = 0;
   if (B > 0.0 && isfinite(B)) frexp(B, &ex);  // B < 2^ex
   const double scale = ldexp(1.0, 62 - ex), inv = ldexp(1.0, ex - 62);
   auto add_point = [&](const double d[3], double wq, uint32_t key) {
     const uint32_t cell = key & ((1u << kPTShift) - 1u);
-    const int cx = static_cast<int>(cell >> 6), cy = static_cast<int>((cell >> 3) & 7u),
-              cz = static_cast<int>(cell & 7u);
+    const int cx = static_cast<int>(cell >> (2 * kPTB)), cy = static_cast<int>((cell >> kPTB) & (kPT - 1u)),
+              cz = static_cast<int>(cell & (kPT - 1u));
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const int a = (k >> 2) & 1, b = (k >> 1) & 1, c = k & 1;
@@ -1064,17 +1069,17 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_accumulate(
   };
 #pragma unroll
   for (int j = 0; j < kAccKeep; j++) {
-    const uint32_t q = beg + threadIdx.x + j * kCicThreads;
+    const uint32_t q = beg + threadIdx.x + j * kAccThreads;
     if (q < end) add_point(dk[j], wk[j], skey[q]);
   }
-  for (uint32_t q = beg + threadIdx.x + kAccKeep * kCicThreads; q < end; q += kCicThreads) {
+  for (uint32_t q = beg + threadIdx.x + kAccKeep * kAccThreads; q < end; q += kAccThreads) {
     double d[3];
     tile_offsets<P>(spos, q, F, wrap, d);
     add_point(d, sw[q], skey[q]);
   }
   __syncthreads();
   const size_t n1 = static_cast<size_t>(TG.n[1]), n2 = static_cast<size_t>(TG.n[2]);
-  for (int c = threadIdx.x; c < kPTCells; c += kCicThreads) {
+  for (int c = threadIdx.x; c < kPTCells; c += kAccThreads) {
     const int i = c / (kPTH * kPTH), j = (c / kPTH) % kPTH, k = c % kPTH;
     if (i > T.e[0] || j > T.e[1] || k > T.e[2]) continue;
     const double v = static_cast<double>(static_cast<long long>(acc[c])) * inv;
@@ -1117,7 +1122,7 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_boundary(double* __restric
   }
   const size_t n1 = static_cast<size_t>(TG.n[1]), n2 = static_cast<size_t>(TG.n[2]);
   for (int q = threadIdx.x; q < kPT * kPT * kPT; q += kCicThreads) {
-    const int i = q >> 6, j = (q >> 3) & 7, k = q & 7;
+    const int i = q >> (2 * kPTB), j = (q >> kPTB) & (kPT - 1), k = q & (kPT - 1);
     if (i >= T.e[0] || j >= T.e[1] || k >= T.e[2]) continue;
     const int zero = (i == 0 ? 4 : 0) | (j == 0 ? 2 : 0) | (k == 0 ? 1 : 0);
     if (zero == 0) continue;
@@ -1269,7 +1274,7 @@ struct GlobalMesh {
   }
 };
 
-// The 11^3 cells o - 1 .. o + 9 of an 8^3 tile with origin o, staged in LDS
+// The (kPT + 3)^3 cells o - 1 .. o + kPT + 1 of a tile with origin o, staged in LDS
 // (periodic indices): every cell the value (base .. base + 1) or the central
 // differences (base - 1 .. base + 2) of a point whose base cell lies in the tile
 // reads; other points read global memory.  Same values, same arithmetic: the
@@ -1455,12 +1460,12 @@ __global__ __launch_bounds__(kCicThreads) void k_read_sorted(const R* __restrict
   flag_errors(err, misc);
 }
 
-// The painted positions tile by tile (one workgroup per 8^3 tile of the tile
+// The painted positions tile by tile (one workgroup per tile of the tile
 // paint, tiles dealt to the XCDs in contiguous ranges): the tile's potential is
 // staged in LDS once and each point's 32 mesh reads come from there; results
 // go to the points' input positions (order).
 #ifndef PRH_READ_TILE_THREADS
-#define PRH_READ_TILE_THREADS 128
+#define PRH_READ_TILE_THREADS 512
 #endif
 constexpr int kReadTileThreads = PRH_READ_TILE_THREADS;
 template <typename R, typename P, typename O>
@@ -1753,7 +1758,7 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
   PRH_LAUNCHED();
   const uint32_t per = xcd_per(ntiles);
   timer_before(PRH_TIMER_PAINT_GATHER, s);
-  k_tile_accumulate<P><<<per * kXcds, kCicThreads, 0, s>>>(mesh, F, TG, MODE == 2, static_cast<uint32_t>(ntiles),
+  k_tile_accumulate<P><<<per * kXcds, kAccThreads, 0, s>>>(mesh, F, TG, MODE == 2, static_cast<uint32_t>(ntiles),
                                                            per, S.tstart, skey, static_cast<const P*>(S.spos), sw,
                                                            S.tbnd, S.misc);
   // algorithmic: tile starts, key, position and weight per point, the mesh read and written
@@ -1917,7 +1922,7 @@ static int do_paint_mixed(prh_workspace* ws, double* mesh, const size_t* nmesh, 
   size_t ntiles = 1;
   for (int d = 0; d < 3; d++) ntiles *= (nmesh[d] + kPT - 1) / kPT;
   if (ntiles > (size_t(1) << (32 - kPTShift))) {
-    set_error("mesh has too many 8^3 tiles for 32-bit tile keys");
+    set_error("mesh has too many tiles for 32-bit tile keys");
     return PRH_ERR_ARG;
   }
   const Frame<double> F = make_frame<double>(nmesh, boxsize, boxcenter);

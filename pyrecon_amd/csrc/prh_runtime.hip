@@ -127,7 +127,8 @@ static Plan plan_bytes(const size_t* n, int dtype, size_t chunk, unsigned flags)
     p.paint = z.ntile * 4 + (z.ncell + 1) * 4 + (z.npartial + 1) * 4 + chunk * 4 * 5 +
               (2 * z.nhist + 1) * 4 + (z.nbnd + 1) * 4 + 2 * chunk * 4 * w + 4 * 4 +
               2 * chunk * 3 * sizeof(double) + (z.nhpart + 2) * sizeof(uint64_t);
-    if ((flags & PRH_WS_MIXED) && dtype == PRH_F64) p.paint += z.ntile * 729 * sizeof(double) + 2 * (z.ntile + 1) * 4;
+    if ((flags & PRH_WS_MIXED) && dtype == PRH_F64)
+      p.paint += paint_tiles(n) * kPaintTileCells * sizeof(double) + 2 * (paint_tiles(n) + 1) * 4;
   }
   if (flags & PRH_WS_FFT) p.fft = smooth_bytes(n, dtype);
   p.red = (kReduceBlocks * 2 + 16) * sizeof(double) + 16 * sizeof(uint32_t);
@@ -175,10 +176,11 @@ static int paint_alloc(prh_workspace* ws, size_t chunk) {
   PRH_TRY(ws_alloc(ws, &p, z.nhpart * sizeof(uint64_t))); P.hpart = static_cast<uint64_t*>(p);
   PRH_TRY(ws_alloc(ws, &p, 2 * sizeof(uint64_t))); P.hash = static_cast<uint64_t*>(p);
   if ((ws->flags & PRH_WS_MIXED) && ws->dtype == PRH_F64) {
-    PRH_TRY(ws_alloc(ws, &p, z.ntile * 729 * sizeof(double)));
+    const size_t nt = paint_tiles(ws->n);
+    PRH_TRY(ws_alloc(ws, &p, nt * kPaintTileCells * sizeof(double)));
     P.tbnd = static_cast<double*>(p);
-    PRH_TRY(ws_alloc(ws, &p, (z.ntile + 1) * 4)); P.tstart = static_cast<uint32_t*>(p);
-    PRH_TRY(ws_alloc(ws, &p, (z.ntile + 1) * 4)); P.tstart_prev = static_cast<uint32_t*>(p);
+    PRH_TRY(ws_alloc(ws, &p, (nt + 1) * 4)); P.tstart = static_cast<uint32_t*>(p);
+    PRH_TRY(ws_alloc(ws, &p, (nt + 1) * 4)); P.tstart_prev = static_cast<uint32_t*>(p);
   }
   PRH_HIP(hipMemset(P.count, 0, z.ntile * 4));
   PRH_HIP(hipMemset(P.misc, 0, 16));

@@ -21,6 +21,16 @@ struct Hierarchy {
 };
 
 // CIC paint scratch for one chunk of at most `chunk` points.
+// Tiles of the mixed path's paint and read: 2^kPaintTileBits cells per axis,
+// with the upper halo (kPaintTile + 1)^3 accumulation cells per tile.
+constexpr int kPaintTileBits = 4;
+constexpr int kPaintTile = 1 << kPaintTileBits;
+constexpr size_t kPaintTileCells = size_t(kPaintTile + 1) * (kPaintTile + 1) * (kPaintTile + 1);
+inline size_t paint_tiles(const size_t* n) {
+  return ((n[0] + kPaintTile - 1) / kPaintTile) * ((n[1] + kPaintTile - 1) / kPaintTile) *
+         ((n[2] + kPaintTile - 1) / kPaintTile);
+}
+
 struct PaintScratch {
   size_t chunk = 0;
   size_t ncell = 0;
@@ -58,7 +68,7 @@ struct PaintScratch {
   size_t nhpart = 0;
   uint64_t* hash = nullptr;     // [2]      hash of the last paint's positions; of a read's
   double* tbnd = nullptr;       // ntile x 729  boundary cells of the FP32 tile paint (PRH_WS_MIXED)
-  // first sorted position of each 8^3 tile (and the end) of the last two tile
+  // first sorted position of each kPaintTile^3 tile (and the end) of the last two tile
   // paints, for reads that walk the painted positions tile by tile
   uint32_t* tstart = nullptr;       // ntile + 1 (PRH_WS_MIXED)
   uint32_t* tstart_prev = nullptr;  // ntile + 1
@@ -67,7 +77,7 @@ struct PaintScratch {
     size_t n = 0;
     int pdt = 0;
     bool copy = false;   // spos holds the positions in this order
-    bool tiles = false;  // ... in 8^3-tile order, with the tile starts in tstart
+    bool tiles = false;  // ... in tile order, with the tile starts in tstart
     uint64_t hash = 0;
   };
   Order order, order_prev;  // tags of `sorted` and `sorted_prev`

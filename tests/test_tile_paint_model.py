@@ -1,10 +1,10 @@
 """CPU model of the index arithmetic of the tile paint (k_tile_key, k_tile_accumulate,
 k_tile_boundary in pyrecon_amd/csrc/prh_cic.hip).
 
-The model follows the kernels' integer arithmetic step by step (tile keys, the 9^3 local array,
+The model follows the kernels' integer arithmetic step by step (tile keys, the (PT + 1)^3 local array,
 the split into cells only one tile reaches and boundary cells, the boundary pass over the lower
 neighbours) with exact float sums, and checks that
-- every index stays inside its buffer (tile ids < ntiles, local cells < 729, mesh cells < n^3);
+- every index stays inside its buffer (tile ids < ntiles, local cells < (PT + 1)^3, mesh cells < n^3);
 - no mesh cell is written by both passes (cells of empty tiles that no neighbour reaches are
   not written at all);
 - the result equals a direct cloud-in-cell assignment (np.add.at) of the same weights,
@@ -15,7 +15,8 @@ import itertools
 import numpy as np
 import pytest
 
-PT, PTH = 8, 9
+PTB = 4                 # log2 of the tile edge (kPaintTileBits)
+PT, PTH = 1 << PTB, (1 << PTB) + 1
 
 
 def tile_grid(shape):
@@ -36,9 +37,9 @@ def model_paint(shape, p, w):
     ntiles = int(np.prod(tiles))
     b = np.floor(p).astype(int) % np.array(shape)
     d = p - b
-    tile = ((b[:, 0] >> 3) * tiles[1] + (b[:, 1] >> 3)) * tiles[2] + (b[:, 2] >> 3)
+    tile = ((b[:, 0] >> PTB) * tiles[1] + (b[:, 1] >> PTB)) * tiles[2] + (b[:, 2] >> PTB)
     assert tile.max() < ntiles and tile.min() >= 0
-    key = (tile << 9) | (((b[:, 0] & 7) * 8 + (b[:, 1] & 7)) * 8 + (b[:, 2] & 7))
+    key = (tile << (3 * PTB)) | (((b[:, 0] & (PT - 1)) * PT + (b[:, 1] & (PT - 1))) * PT + (b[:, 2] & (PT - 1)))
     assert key.max() < 2**32
     order = np.argsort(tile, kind='stable')
     mesh = np.zeros(shape)
@@ -52,8 +53,8 @@ def model_paint(shape, p, w):
         _, o, e = tile_pos(t, shape, tiles)
         acc = np.zeros(PTH**3)
         for i in sel:
-            cell = key[i] & 511
-            cx, cy, cz = cell >> 6, (cell >> 3) & 7, cell & 7
+            cell = key[i] & ((1 << (3 * PTB)) - 1)
+            cx, cy, cz = cell >> (2 * PTB), (cell >> PTB) & (PT - 1), cell & (PT - 1)
             assert cx < e[0] and cy < e[1] and cz < e[2]
             for a, bb, c in itertools.product((0, 1), repeat=3):
                 wt = (d[i, 0] if a else 1 - d[i, 0]) * (d[i, 1] if bb else 1 - d[i, 1]) * (d[i, 2] if c else 1 - d[i, 2])
@@ -61,7 +62,7 @@ def model_paint(shape, p, w):
                 assert 0 <= idx < PTH**3
                 acc[idx] += wt * w[i]
         for c in range(PTH**3):
-            i, j, k = c // 81, (c // 9) % 9, c % 9
+            i, j, k = c // (PTH * PTH), (c // PTH) % PTH, c % PTH
             if i > e[0] or j > e[1] or k > e[2]: continue
             own = 1 <= i < e[0] and 1 <= j < e[1] and 1 <= k < e[2]
             if own:
@@ -80,8 +81,8 @@ def model_paint(shape, p, w):
             cc = [(tiles[dd] - 1 if c[dd] == 0 else c[dd] - 1) if (sub >> (2 - dd)) & 1 else c[dd] for dd in range(3)]
             nb[sub] = (cc[0] * tiles[1] + cc[1]) * tiles[2] + cc[2]
             assert 0 <= nb[sub] < ntiles
-        for q in range(512):
-            i, j, k = q >> 6, (q >> 3) & 7, q & 7
+        for q in range(PT**3):
+            i, j, k = q >> (2 * PTB), (q >> PTB) & (PT - 1), q & (PT - 1)
             if i >= e[0] or j >= e[1] or k >= e[2]: continue
             zero = (4 if i == 0 else 0) | (2 if j == 0 else 0) | (1 if k == 0 else 0)
             if zero == 0: continue
@@ -117,7 +118,8 @@ def direct_cic(shape, p, w):
 
 
 @pytest.mark.parametrize('shape, extent', [((16, 16, 16), 1.), ((12, 20, 9), 1.), ((4, 8, 3), 1.), ((1, 2, 17), 1.),
-                                           ((8, 8, 8), 1.), ((24, 5, 16), 1.), ((32, 24, 16), 0.3)])
+                                           ((8, 8, 8), 1.), ((24, 5, 16), 1.), ((32, 24, 16), 0.3), ((40, 34, 18), 1.),
+                                           ((17, 33, 48), 0.5)])
 def test_tile_decomposition(shape, extent):
     rng = np.random.RandomState(sum(shape))
     n = 400
