@@ -1329,15 +1329,19 @@ constexpr int kRT = 8;  // read binning tile edge (cells)
 // positions dtype P; the local line of sight x/|x| is in P, computed from
 // the unwrapped input; with a global line of sight (mesh dtype R) the RSD term
 // is in Q = promote(P, R).  O is the output dtype.
-template <typename R, typename P, typename O, typename M>
+// SPEC: the shifted positions with the displacement and RSD terms along the
+// local line of sight, without wrap (RecSym's read, the common case), fixed at
+// compile time so that the registers of the other cases are not held.
+template <typename R, typename P, typename O, typename M, bool SPEC = false>
 __device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& mesh,
                                                const ReadArgs<R>& A, O* __restrict__ out) {
   using Q = promote_t<P, R>;
   const Frame<R>& F = A.F;
+  const int wrap = SPEC ? 0 : A.wrap, local = SPEC ? 1 : A.local;
   uint32_t err = 0;
   R p[3];
-  to_mesh_units<R, P>(x, F, A.wrap, p, err);
-  if (A.field == PRH_FIELD_VALUE) {
+  to_mesh_units<R, P>(x, F, wrap, p, err);
+  if (!SPEC && A.field == PRH_FIELD_VALUE) {
     // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
     out[i] = static_cast<O>(static_cast<P>(mesh.value(p)));
     return err;
@@ -1345,14 +1349,14 @@ __device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& 
   R g[3];
   mesh.gradient(A.cell, p, g);
   P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
-  const int field = A.field, shifted = A.shifted;
+  const int field = SPEC ? PRH_FIELD_DISP_RSD : A.field, shifted = SPEC ? 1 : A.shifted;
   // shifts in P (disp, disp+rsd, local rsd) or in Q (global rsd)
   O res[3];
   if (field == PRH_FIELD_DISP) {
     for (int d = 0; d < 3; d++) res[d] = static_cast<O>(s[d]);
     if (shifted)
       for (int d = 0; d < 3; d++) res[d] = static_cast<O>(x[d] - s[d]);
-  } else if (A.local) {
+  } else if (local) {
     const P dist = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
     const P l[3] = {x[0] / dist, x[1] / dist, x[2] / dist};
     const P dot = s[0] * l[0] + s[1] * l[1] + s[2] * l[2];
@@ -1379,7 +1383,7 @@ __device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& 
       }
     }
   }
-  if (shifted && A.wrap) {
+  if (shifted && wrap) {
     using QO = promote_t<O, R>;
     for (int d = 0; d < 3; d++)
       res[d] = static_cast<O>(np_mod(static_cast<QO>(res[d]) - static_cast<QO>(F.off[d]),
@@ -1468,7 +1472,7 @@ __global__ __launch_bounds__(kCicThreads) void k_read_sorted(const R* __restrict
 #define PRH_READ_TILE_THREADS 512
 #endif
 constexpr int kReadTileThreads = PRH_READ_TILE_THREADS;
-template <typename R, typename P, typename O>
+template <typename R, typename P, typename O, bool SPEC>
 __global__ __launch_bounds__(kReadTileThreads, 4) void k_read_tiles(const R* __restrict__ phi, ReadArgs<R> A,
                                                             TileGrid TG, uint32_t ntiles, uint32_t per,
                                                             const uint32_t* __restrict__ tstart,
@@ -1502,11 +1506,32 @@ __global__ __launch_bounds__(kReadTileThreads, 4) void k_read_tiles(const R* __r
   __syncthreads();
   uint32_t err = 0;
   const TileMesh<R> M{s_m, {T.o[0], T.o[1], T.o[2]}, GlobalMesh<R>{phi, n0, n1, n2}};
-  if (q0 < end) err = read_value<R, P, O>(i0, x0, M, A, out);
-  for (uint32_t q = q0 + kReadTileThreads; q < end; q += kReadTileThreads) {
-    const P x[3] = {spos[3 * static_cast<size_t>(q)], spos[3 * static_cast<size_t>(q) + 1],
-                    spos[3 * static_cast<size_t>(q) + 2]};
-    err |= read_value<R, P, O>(order[q], x, M, A, out);
+  if constexpr (SPEC) {
+    // the registers the general case leaves free hold the next point's loads while
+    // the current point is computed
+    for (uint32_t q = q0; q < end; q += kReadTileThreads) {
+      const uint32_t qn = q + kReadTileThreads;
+      P xn[3] = {P(0), P(0), P(0)};
+      uint32_t in = 0;
+      if (qn < end) {
+        xn[0] = spos[3 * static_cast<size_t>(qn)];
+        xn[1] = spos[3 * static_cast<size_t>(qn) + 1];
+        xn[2] = spos[3 * static_cast<size_t>(qn) + 2];
+        in = order[qn];
+      }
+      err |= read_value<R, P, O, TileMesh<R>, true>(i0, x0, M, A, out);
+      x0[0] = xn[0];
+      x0[1] = xn[1];
+      x0[2] = xn[2];
+      i0 = in;
+    }
+  } else {
+    if (q0 < end) err = read_value<R, P, O>(i0, x0, M, A, out);
+    for (uint32_t q = q0 + kReadTileThreads; q < end; q += kReadTileThreads) {
+      const P x[3] = {spos[3 * static_cast<size_t>(q)], spos[3 * static_cast<size_t>(q) + 1],
+                      spos[3 * static_cast<size_t>(q) + 2]};
+      err |= read_value<R, P, O>(order[q], x, M, A, out);
+    }
   }
   flag_errors(err, misc);
 }
@@ -2020,8 +2045,12 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
           }
           const size_t ntiles = static_cast<size_t>(TG.t[0]) * TG.t[1] * TG.t[2];
           const uint32_t per = xcd_per(ntiles);
-          k_read_tiles<R, P, O><<<per * kXcds, kReadTileThreads, 0, s>>>(phi, A, TG, static_cast<uint32_t>(ntiles),
-                                                                    per, ts, spos, porder, out, misc);
+          if (A.field == PRH_FIELD_DISP_RSD && A.local && A.shifted && !A.wrap)
+            k_read_tiles<R, P, O, true><<<per * kXcds, kReadTileThreads, 0, s>>>(
+                phi, A, TG, static_cast<uint32_t>(ntiles), per, ts, spos, porder, out, misc);
+          else
+            k_read_tiles<R, P, O, false><<<per * kXcds, kReadTileThreads, 0, s>>>(
+                phi, A, TG, static_cast<uint32_t>(ntiles), per, ts, spos, porder, out, misc);
         } else {
           k_read_sorted<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, spos, n, out,
                                                                                  misc, porder);
