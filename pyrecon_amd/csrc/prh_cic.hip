@@ -1121,11 +1121,28 @@ __global__ __launch_bounds__(kCicThreads) void k_tile_boundary(double* __restric
     full[sub] = start[nb[sub]] < start[nb[sub] + 1];
   }
   const size_t n1 = static_cast<size_t>(TG.n[1]), n2 = static_cast<size_t>(TG.n[2]);
-  for (int q = threadIdx.x; q < kPT * kPT * kPT; q += kCicThreads) {
-    const int i = q >> (2 * kPTB), j = (q >> kPTB) & (kPT - 1), k = q & (kPT - 1);
+  // the cells with a local index 0, listed face by face: i = 0 (kPT^2 cells),
+  // then i > 0 and j = 0 ((kPT - 1) kPT), then i, j > 0 and k = 0 ((kPT - 1)^2)
+  constexpr int kFaceI = kPT * kPT, kFaceJ = (kPT - 1) * kPT, kFaceK = (kPT - 1) * (kPT - 1);
+  for (int q = threadIdx.x; q < kFaceI + kFaceJ + kFaceK; q += kCicThreads) {
+    int i, j, k;
+    if (q < kFaceI) {
+      i = 0;
+      j = q >> kPTB;
+      k = q & (kPT - 1);
+    } else if (q < kFaceI + kFaceJ) {
+      const int r = q - kFaceI;
+      i = 1 + (r >> kPTB);
+      j = 0;
+      k = r & (kPT - 1);
+    } else {
+      const int r = q - kFaceI - kFaceJ;
+      i = 1 + r / (kPT - 1);
+      j = 1 + r % (kPT - 1);
+      k = 0;
+    }
     if (i >= T.e[0] || j >= T.e[1] || k >= T.e[2]) continue;
     const int zero = (i == 0 ? 4 : 0) | (j == 0 ? 2 : 0) | (k == 0 ? 1 : 0);
-    if (zero == 0) continue;
     bool any = false;
     double sum = 0.0;
     if (full[0]) {

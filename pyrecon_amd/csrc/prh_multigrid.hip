@@ -163,7 +163,7 @@ __device__ __forceinline__ R stencil_cell(const TileCoef<R>& T, const Hood<R>& h
 // Minimum waves per SIMD requested for the blocked kernel (a register cap);
 // 1 leaves the allocation to the compiler.
 #ifndef PRH_TILED_MIN_WAVES
-#define PRH_TILED_MIN_WAVES 1
+#define PRH_TILED_MIN_WAVES 4  // 128 VGPRs: the FP64 sweep keeps 4 waves per SIMD
 #endif
 
 template <typename R, typename O, int OP, int TY, bool LOCAL, bool FAST>
@@ -255,8 +255,13 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
     for (int q = 0; q < PER; q++)
       if (has[q]) dst[tid + 256 * q] = src[q];
   };
-  auto compute = [&](int x, const R* M, const R* P, const R* N) {
+  // the source values of plane x, loaded one plane ahead (see k_stencil_rows)
+  auto load_f = [&](R(&dst)[KY], int x) {
     const R* fx = f + static_cast<size_t>(x) * s0;
+#pragma unroll
+    for (int k = 0; k < KY; k++) dst[k] = fx[rowoff[k]];
+  };
+  auto compute = [&](int x, const R(&fvs)[KY], const R* M, const R* P, const R* N) {
     O* ox = out + static_cast<size_t>(x) * s0;
     R px = R(0), A = R(0);
     if (LOCAL) {
@@ -288,7 +293,7 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
       h.ymzp = P[o - W + 1];
       h.ypzm = P[o + W - 1];
       }
-      const R fv = fx[rowoff[k]];
+      const R fv = fvs[k];
       const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, fv, px, A, py[k], B[k], pz, C, pypz[k],
                                                      omega, keep);
       if constexpr (OP == kOpJacobiAcc)
@@ -297,25 +302,39 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
         ox[rowoff[k]] = static_cast<O>(res);
     }
   };
+  // Loads in the order of k_stencil_rows: plane x's sources during plane x - 1,
+  // after that plane's prefetch, every step issuing the same (clamped) loads.
+  R fA[KY], fB[KY];
   if (OP == kOpJacobiZero) {
     // zero iterate: no neighbourhood to stage, one pass over the run
-    for (int x = xb; x < xe; x++) compute(x, lds, lds, lds);
+    load_f(fA, xb);
+    for (int x0 = xb; x0 < xe; x0 += 2) {
+      load_f(fB, min(x0 + 1, xe - 1));
+      compute(x0, fA, lds, lds, lds);
+      load_f(fA, min(x0 + 2, xe - 1));
+      if (x0 + 1 < xe) compute(x0 + 1, fB, lds, lds, lds);
+    }
     return;
   }
-  if (xb + 1 < xe) fetch(preA, xb + 2);
+  fetch(preA, min(xb + 2, xe));
+  load_f(fA, xb);
   __syncthreads();
   for (int x0 = xb; x0 < xe; x0 += 4) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int x = x0 + u;
+      if ((u + 3) & 1)
+        fetch(preB, min(x + 3, xe));
+      else
+        fetch(preA, min(x + 3, xe));
+      if ((u + 1) & 1)
+        load_f(fB, min(x + 1, xe - 1));
+      else
+        load_f(fA, min(x + 1, xe - 1));
+      __builtin_amdgcn_sched_barrier(0);
       if (x < xe) {
-        if (x + 2 < xe) {
-          if ((u + 3) & 1)
-            fetch(preB, x + 3);
-          else
-            fetch(preA, x + 3);
-        }
-        compute(x, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE, lds + ((u + 1) & 3) * PLANE);
+        compute(x, (u & 1) ? fB : fA, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE,
+                lds + ((u + 1) & 3) * PLANE);
         if (x + 1 < xe) {
           if ((u + 2) & 1)
             stash(preB, (u + 2) & 3);
@@ -348,7 +367,7 @@ constexpr int kRowsY = 8;
 #define PRH_ROWS_MIN_XLEN 4  // shortest x run of a whole-row block
 #endif
 #ifndef PRH_ROWS_MIN_WAVES
-#define PRH_ROWS_MIN_WAVES 1
+#define PRH_ROWS_MIN_WAVES 4  // 128 VGPRs: two 512-thread blocks per CU on rows of 256 cells
 #endif
 
 template <typename R, typename O, int OP, int WPR, int CZ, bool LOCAL, bool FAST, int RY = kRowsY>
@@ -429,17 +448,19 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
       if (has[q]) dst[tid + NT * q] = src[q];
   };
   const int rm = ty * N2, r0 = (ty + 1) * N2, rp = (ty + 2) * N2;
-  auto compute = [&](int x, const R* M, const R* P, const R* N) {
+  // the source values of plane x, loaded one plane ahead (see the loop below)
+  auto load_f = [&](R(&dst)[CZ], int x) {
     const R* fx = f + static_cast<size_t>(x) * s0 + rowoff;
+#pragma unroll
+    for (int c = 0; c < CZ; c++) dst[c] = fx[zb + 64 * WPR * c];
+  };
+  auto compute = [&](int x, const R(&fv)[CZ], const R* M, const R* P, const R* N) {
     O* ox = out + static_cast<size_t>(x) * s0 + rowoff;
     R px = R(0), A = R(0);
     if (LOCAL) {
       px = static_cast<R>(x) + G.org[0];
       A = G.h2[0] * px * px;
     }
-    R fv[CZ];
-#pragma unroll
-    for (int c = 0; c < CZ; c++) fv[c] = fx[zb + 64 * WPR * c];
 #pragma unroll
     for (int c = 0; c < CZ; c++) {
       const int zc = zb + 64 * WPR * c;
@@ -475,24 +496,77 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
         ox[zc] = static_cast<O>(res);
     }
   };
+  // Global loads retire in issue order (one vmcnt), so a load issued after the
+  // prefetch of plane x + 3 could not be waited for without waiting for that
+  // prefetch too.  Plane x's source values are therefore loaded during plane
+  // x - 1, after that plane's prefetch: computing plane x waits only for loads
+  // issued a plane earlier, and plane x + 3 and the sources of x + 1 stay in
+  // flight meanwhile.  Every step issues the same loads, also past the end of
+  // the run (plane indices clamped to it: such a load re-reads a plane that is
+  // in L2 and its value is not used), so that the waits can count on them.
+  // Rows of 1024 cells keep the loads of the sources inside the plane's step:
+  // the registers of the loads in flight would spill there.
+  constexpr bool kAhead = N2 <= 512;
+  R fA[CZ], fB[CZ];
   if (OP == kOpJacobiZero) {
-    for (int x = xb; x < xe; x++) compute(x, lds, lds, lds);
+    load_f(fA, xb);
+    for (int x0 = xb; x0 < xe; x0 += 2) {
+      load_f(fB, min(x0 + 1, xe - 1));
+      compute(x0, fA, lds, lds, lds);
+      load_f(fA, min(x0 + 2, xe - 1));
+      if (x0 + 1 < xe) compute(x0 + 1, fB, lds, lds, lds);
+    }
     return;
   }
-  if (xb + 1 < xe) fetch(preA, xb + 2);
+  if constexpr (!kAhead) {
+    if (xb + 1 < xe) fetch(preA, xb + 2);
+    __syncthreads();
+    for (int x0 = xb; x0 < xe; x0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int x = x0 + u;
+        if (x < xe) {
+          if (x + 2 < xe) {
+            if ((u + 3) & 1)
+              fetch(preB, x + 3);
+            else
+              fetch(preA, x + 3);
+          }
+          load_f(fA, x);
+          compute(x, fA, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE, lds + ((u + 1) & 3) * PLANE);
+          if (x + 1 < xe) {
+            if ((u + 2) & 1)
+              stash(preB, (u + 2) & 3);
+            else
+              stash(preA, (u + 2) & 3);
+          }
+          __syncthreads();
+        }
+      }
+    }
+    return;
+  }
+  fetch(preA, min(xb + 2, xe));
+  load_f(fA, xb);
   __syncthreads();
   for (int x0 = xb; x0 < xe; x0 += 4) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int x = x0 + u;
+      if ((u + 3) & 1)
+        fetch(preB, min(x + 3, xe));
+      else
+        fetch(preA, min(x + 3, xe));
+      if ((u + 1) & 1)
+        load_f(fB, min(x + 1, xe - 1));
+      else
+        load_f(fA, min(x + 1, xe - 1));
+      // keep the loads ahead of the computation (the scheduler would otherwise
+      // sink them below the waits for this plane's sources)
+      __builtin_amdgcn_sched_barrier(0);
       if (x < xe) {
-        if (x + 2 < xe) {
-          if ((u + 3) & 1)
-            fetch(preB, x + 3);
-          else
-            fetch(preA, x + 3);
-        }
-        compute(x, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE, lds + ((u + 1) & 3) * PLANE);
+        compute(x, (u & 1) ? fB : fA, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE,
+                lds + ((u + 1) & 3) * PLANE);
         if (x + 1 < xe) {
           if ((u + 2) & 1)
             stash(preB, (u + 2) & 3);
