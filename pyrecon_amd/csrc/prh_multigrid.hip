@@ -448,13 +448,23 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
       if (has[q]) dst[tid + NT * q] = src[q];
   };
   const int rm = ty * N2, r0 = (ty + 1) * N2, rp = (ty + 2) * N2;
-  // the source values of plane x, loaded one plane ahead (see the loop below)
-  auto load_f = [&](R(&dst)[CZ], int x) {
+  // the source values of plane x (and, for kOpJacobiAcc, the FP64 values the
+  // results are added to), loaded one plane ahead (see the loop below)
+  struct Src {
+    R f[CZ];
+    double a[CZ];
+  };
+  auto load_f = [&](Src& dst, int x) {
     const R* fx = f + static_cast<size_t>(x) * s0 + rowoff;
 #pragma unroll
-    for (int c = 0; c < CZ; c++) dst[c] = fx[zb + 64 * WPR * c];
+    for (int c = 0; c < CZ; c++) dst.f[c] = fx[zb + 64 * WPR * c];
+    if constexpr (OP == kOpJacobiAcc && N2 <= 512) {
+      const double* ax = T.acc + static_cast<size_t>(x) * s0 + rowoff;
+#pragma unroll
+      for (int c = 0; c < CZ; c++) dst.a[c] = ax[zb + 64 * WPR * c];
+    }
   };
-  auto compute = [&](int x, const R(&fv)[CZ], const R* M, const R* P, const R* N) {
+  auto compute = [&](int x, const Src& src, const R* M, const R* P, const R* N) {
     O* ox = out + static_cast<size_t>(x) * s0 + rowoff;
     R px = R(0), A = R(0);
     if (LOCAL) {
@@ -488,11 +498,12 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
         h.ymzp = P[rm + zr];
         h.ypzm = P[rp + zl];
       }
-      const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, fv[c], px, A, py, B, pz[c], C[c],
+      const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, src.f[c], px, A, py, B, pz[c], C[c],
                                                      pypz[c], omega, keep);
-      if constexpr (OP == kOpJacobiAcc)
-        T.acc[static_cast<size_t>(x) * s0 + rowoff + zc] += static_cast<double>(res);
-      else
+      if constexpr (OP == kOpJacobiAcc) {
+        double* a = T.acc + static_cast<size_t>(x) * s0 + rowoff + zc;
+        *a = (N2 <= 512 ? src.a[c] : *a) + static_cast<double>(res);
+      } else
         ox[zc] = static_cast<O>(res);
     }
   };
@@ -507,7 +518,7 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
   // Rows of 1024 cells keep the loads of the sources inside the plane's step:
   // the registers of the loads in flight would spill there.
   constexpr bool kAhead = N2 <= 512;
-  R fA[CZ], fB[CZ];
+  Src fA, fB;
   if (OP == kOpJacobiZero) {
     load_f(fA, xb);
     for (int x0 = xb; x0 < xe; x0 += 2) {
