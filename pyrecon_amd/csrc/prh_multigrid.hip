@@ -1379,35 +1379,111 @@ struct CoarseArgs {
 // 4096 (1 + 1/8 + 1/64 + 1/512) cells: 150 KB in FP64).
 constexpr int kCoarseLds = 4682;
 
+// The cells of one level that a thread visits (i = threadIdx.x + kCoarseThreads u),
+// with their coordinates and the offsets of their periodic neighbours, worked
+// out once per visit of the level rather than once per sweep.  Offsets are
+// below the level's size in magnitude, so they pack in pairs of 16 bits.
+constexpr int kCoarsePer = static_cast<int>((kCoarseCells + kCoarseThreads - 1) / kCoarseThreads);
+static_assert(kCoarseCells <= 32767, "neighbour offsets are packed in 16 bits");
+struct CoarseCells {
+  int total;
+  int xy[kCoarsePer];  // ix | iy << 16
+  int z[kCoarsePer];
+  int ox[kCoarsePer];  // offset of x + 1 in the low half, of x - 1 in the high half
+  int oy[kCoarsePer];
+  int oz[kCoarsePer];
+};
+
+__device__ __forceinline__ int pack16(int lo, int hi) { return (lo & 0xffff) | (hi << 16); }
+__device__ __forceinline__ int lo16(int p) { return static_cast<int>(static_cast<int16_t>(p & 0xffff)); }
+__device__ __forceinline__ int hi16(int p) { return p >> 16; }
+
 template <typename R>
-__device__ __forceinline__ void cw_jacobi(const LevelGeom<R>& G, const R* in, const R* src, R* out,
-                                          const R* gtab, R omega, R keep, bool zero_in) {
-  const int n1 = G.n[1], n2 = G.n[2];
-  const int total = G.n[0] * n1 * n2;
-  for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
-    int ix, iy, iz;
-    unflatten(i, n1, n2, ix, iy, iz);
-    const Coef<R> k = cell_coef_g(G, ix, iy, iz, gtab[i]);
-    const Hood<R> h = zero_in ? zero_hood<R>() : gather_hood<R, int>(in, G.n[0], n1, n2, ix, iy, iz);
-    out[i] = jacobi_update(k, h, src[i], omega, keep, G.local);
+__device__ __forceinline__ CoarseCells coarse_cells(const LevelGeom<R>& G) {
+  const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2], s0 = n1 * n2;
+  CoarseCells C;
+  C.total = n0 * s0;
+#pragma unroll
+  for (int u = 0; u < kCoarsePer; u++) {
+    const int i = threadIdx.x + kCoarseThreads * u;
+    int ix = 0, iy = 0, iz = 0;
+    if (i < C.total) unflatten(i, n1, n2, ix, iy, iz);
+    const int xp = ix + 1 == n0 ? 0 : ix + 1, xm = ix == 0 ? n0 - 1 : ix - 1;
+    const int yp = iy + 1 == n1 ? 0 : iy + 1, ym = iy == 0 ? n1 - 1 : iy - 1;
+    const int zp = iz + 1 == n2 ? 0 : iz + 1, zm = iz == 0 ? n2 - 1 : iz - 1;
+    C.xy[u] = ix | (iy << 16);
+    C.z[u] = iz;
+    C.ox[u] = pack16((xp - ix) * s0, (xm - ix) * s0);
+    C.oy[u] = pack16((yp - iy) * n2, (ym - iy) * n2);
+    C.oz[u] = pack16(zp - iz, zm - iz);
+  }
+  return C;
+}
+
+// gather_hood() of cell u of C (the same neighbours)
+template <typename R>
+__device__ __forceinline__ Hood<R> coarse_hood(const R* v, const CoarseCells& C, int u, int i) {
+  const int xp = lo16(C.ox[u]), xm = hi16(C.ox[u]), yp = lo16(C.oy[u]), ym = hi16(C.oy[u]);
+  const int zp = lo16(C.oz[u]), zm = hi16(C.oz[u]);
+  const R* c = v + i;
+  Hood<R> h;
+  h.c = c[0];
+  h.xp = c[xp];
+  h.xm = c[xm];
+  h.yp = c[yp];
+  h.ym = c[ym];
+  h.zp = c[zp];
+  h.zm = c[zm];
+  h.xpyp = c[xp + yp];
+  h.xmym = c[xm + ym];
+  h.xmyp = c[xm + yp];
+  h.xpym = c[xp + ym];
+  h.xpzp = c[xp + zp];
+  h.xmzm = c[xm + zm];
+  h.xmzp = c[xm + zp];
+  h.xpzm = c[xp + zm];
+  h.ypzp = c[yp + zp];
+  h.ymzm = c[ym + zm];
+  h.ymzp = c[ym + zp];
+  h.ypzm = c[yp + zm];
+  return h;
+}
+
+template <typename R>
+__device__ __forceinline__ Coef<R> coarse_coef(const LevelGeom<R>& G, const CoarseCells& C, int u,
+                                               const R* gtab, int i) {
+  return cell_coef_g(G, C.xy[u] & 0xffff, C.xy[u] >> 16, C.z[u], gtab[i]);
+}
+
+template <typename R>
+__device__ __forceinline__ void cw_jacobi(const LevelGeom<R>& G, const CoarseCells& C, const R* in,
+                                          const R* src, R* out, const R* gtab, R omega, R keep,
+                                          bool zero_in) {
+#pragma unroll
+  for (int u = 0; u < kCoarsePer; u++) {
+    const int i = threadIdx.x + kCoarseThreads * u;
+    if (i < C.total) {
+      const Coef<R> k = coarse_coef(G, C, u, gtab, i);
+      const Hood<R> h = zero_in ? zero_hood<R>() : coarse_hood(in, C, u, i);
+      out[i] = jacobi_update(k, h, src[i], omega, keep, G.local);
+    }
   }
   __syncthreads();
 }
 
 template <typename R>
-__device__ __forceinline__ void cw_smooth(const LevelGeom<R>& G, R*& cur, R*& alt, const R* src,
-                                          const R* gtab, int niter, R omega, R keep,
-                                          bool zero_start) {
+__device__ __forceinline__ void cw_smooth(const LevelGeom<R>& G, const CoarseCells& C, R*& cur,
+                                          R*& alt, const R* src, const R* gtab, int niter,
+                                          R omega, R keep, bool zero_start) {
   if (niter <= 0) {
     if (zero_start) {
-      const int total = G.n[0] * G.n[1] * G.n[2];
-      for (int i = threadIdx.x; i < total; i += kCoarseThreads) cur[i] = R(0);
+      for (int i = threadIdx.x; i < C.total; i += kCoarseThreads) cur[i] = R(0);
       __syncthreads();
     }
     return;
   }
   for (int it = 0; it < niter; it++) {
-    cw_jacobi(G, cur, src, alt, gtab, omega, keep, zero_start && it == 0);
+    cw_jacobi(G, C, cur, src, alt, gtab, omega, keep, zero_start && it == 0);
     R* t = cur;
     cur = alt;
     alt = t;
@@ -1471,15 +1547,17 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vcycle_coarse(CoarseArgs<R> 
     const LevelGeom<R>& G = A.L[job][m].G;
     const R* src = sf + off[m];
     const R* gt = sg + off[m];
-    cw_smooth(G, cur[m], alt[m], src, gt, A.niter, A.omega, A.keep, m > 0 || A.zero_start);
+    const CoarseCells C = coarse_cells(G);
+    cw_smooth(G, C, cur[m], alt[m], src, gt, A.niter, A.omega, A.keep, m > 0 || A.zero_start);
     if (m + 1 < nl) {
-      const int total = G.n[0] * G.n[1] * G.n[2];
-      for (int i = threadIdx.x; i < total; i += kCoarseThreads) {
-        int ix, iy, iz;
-        unflatten(i, G.n[1], G.n[2], ix, iy, iz);
-        const Coef<R> k = cell_coef_g(G, ix, iy, iz, gt[i]);
-        const Hood<R> h = gather_hood<R, int>(cur[m], G.n[0], G.n[1], G.n[2], ix, iy, iz);
-        alt[m][i] = residual_value(k, h, src[i], G.local);
+#pragma unroll
+      for (int u = 0; u < kCoarsePer; u++) {
+        const int i = threadIdx.x + kCoarseThreads * u;
+        if (i < C.total) {
+          const Coef<R> k = coarse_coef(G, C, u, gt, i);
+          const Hood<R> h = coarse_hood(cur[m], C, u, i);
+          alt[m][i] = residual_value(k, h, src[i], G.local);
+        }
       }
       __syncthreads();
       const int ctotal = (G.n[0] / 2) * (G.n[1] / 2) * (G.n[2] / 2);
@@ -1500,7 +1578,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_vcycle_coarse(CoarseArgs<R> 
         prolong_at<R, R, true>(cur[m + 1], cur[m], Gc.n[0], Gc.n[1], Gc.n[2], i);
       __syncthreads();
     }
-    cw_smooth(G, cur[m], alt[m], sf + off[m], sg + off[m], A.niter, A.omega, A.keep, false);
+    cw_smooth(G, coarse_cells(G), cur[m], alt[m], sf + off[m], sg + off[m], A.niter, A.omega, A.keep,
+              false);
   }
   {
     const LevelGeom<R>& G = A.L[job][0].G;
