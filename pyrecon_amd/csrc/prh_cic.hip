@@ -123,6 +123,9 @@ __device__ __forceinline__ void to_mesh_units(const P x[3], const Frame<R>& F, i
 
 // ((int) p) % n with the reference's conversion of the int to size_t.
 __device__ __forceinline__ int base_index(int ip, int n) {
+  // the common cases without an integer division: in the mesh, or one period above
+  if (ip >= 0 && ip < n) return ip;
+  if (ip >= n && ip - n < n) return ip - n;
   if (ip >= 0) return ip % n;
   const unsigned long long u = static_cast<unsigned long long>(static_cast<long long>(ip));
   return static_cast<int>(u % static_cast<unsigned long long>(n));
@@ -1243,12 +1246,20 @@ __device__ __forceinline__ void fd_gradient(const R* __restrict__ mesh, int n0, 
   const int nn[3] = {n0, n1, n2};
   cic_base(p, nn, b, t);
   const size_t s0 = static_cast<size_t>(n1) * n2, s1 = static_cast<size_t>(n2);
+  // (b - 1 + k) mod n: b lies in [0, n), so for n >= 2 one correction by n is enough
+  auto wrapk = [](int b, int k, int n) {
+    if (n < 2) return (b + n - 1 + k) % n;
+    int v = b - 1 + k;
+    v += v < 0 ? n : 0;
+    v -= v >= n ? n : 0;
+    return v;
+  };
   size_t X[4], Y[4], Z[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    X[k] = static_cast<size_t>((b[0] + n0 - 1 + k) % n0) * s0;
-    Y[k] = static_cast<size_t>((b[1] + n1 - 1 + k) % n1) * s1;
-    Z[k] = static_cast<size_t>((b[2] + n2 - 1 + k) % n2);
+    X[k] = static_cast<size_t>(wrapk(b[0], k, n0)) * s0;
+    Y[k] = static_cast<size_t>(wrapk(b[1], k, n1)) * s1;
+    Z[k] = static_cast<size_t>(wrapk(b[2], k, n2));
   }
   fd_core<R>([&](int i, int j, int k) { return mesh[X[i] + Y[j] + Z[k]]; }, t, cell, g);
 }
@@ -1497,12 +1508,20 @@ __global__ __launch_bounds__(kReadTileThreads, 4) void k_read_tiles(const R* __r
                                                             const uint32_t* __restrict__ order,
                                                             O* __restrict__ out, uint32_t* __restrict__ misc) {
   __shared__ R s_m[kTS * kTS * kTS];
+  __shared__ size_t s_off[3 * kTS];  // periodic offsets of the staged planes, rows and columns
   const uint32_t t = xcd_block(per);
   if (t >= ntiles) return;
   const uint32_t beg = tstart[t], end = tstart[t + 1];
   if (beg >= end) return;
   const TilePos T = tile_pos(t, TG);
   const int n0 = A.F.n[0], n1 = A.F.n[1], n2 = A.F.n[2];
+  if (threadIdx.x < 3 * kTS) {
+    const int d = threadIdx.x / kTS, u = threadIdx.x % kTS;
+    const int n = d == 0 ? n0 : (d == 1 ? n1 : n2);
+    const int o = d == 0 ? T.o[0] : (d == 1 ? T.o[1] : T.o[2]);  // no dynamic index into T
+    const int g = ((o - 1 + u) % n + n) % n;
+    s_off[threadIdx.x] = static_cast<size_t>(g) * (d == 0 ? static_cast<size_t>(n1) * n2 : (d == 1 ? n2 : 1));
+  }
   // the loads of this thread's first point are issued before the staging, to overlap with it
   const uint32_t q0 = beg + threadIdx.x;
   P x0[3] = {P(0), P(0), P(0)};
@@ -1513,12 +1532,10 @@ __global__ __launch_bounds__(kReadTileThreads, 4) void k_read_tiles(const R* __r
     x0[2] = spos[3 * static_cast<size_t>(q0) + 2];
     i0 = order[q0];
   }
+  __syncthreads();
   for (int c = threadIdx.x; c < kTS * kTS * kTS; c += kReadTileThreads) {
     const int u = c / (kTS * kTS), v = (c / kTS) % kTS, w = c % kTS;
-    const int gx = ((T.o[0] - 1 + u) % n0 + n0) % n0;
-    const int gy = ((T.o[1] - 1 + v) % n1 + n1) % n1;
-    const int gz = ((T.o[2] - 1 + w) % n2 + n2) % n2;
-    s_m[c] = phi[(static_cast<size_t>(gx) * n1 + gy) * n2 + gz];
+    s_m[c] = phi[s_off[u] + s_off[kTS + v] + s_off[2 * kTS + w]];
   }
   __syncthreads();
   uint32_t err = 0;
