@@ -161,13 +161,18 @@ __device__ __forceinline__ R stencil_cell(const TileCoef<R>& T, const Hood<R>& h
 }
 
 // Minimum waves per SIMD requested for the blocked kernel (a register cap);
-// 1 leaves the allocation to the compiler.
+// 1 leaves the allocation to the compiler.  The FP64-to-FP32 residual keeps 4
+// waves per SIMD (128 VGPRs) with its sources loaded a plane ahead.
 #ifndef PRH_TILED_MIN_WAVES
-#define PRH_TILED_MIN_WAVES 4  // 128 VGPRs: the FP64 sweep keeps 4 waves per SIMD
+#define PRH_TILED_MIN_WAVES 1
 #endif
+template <typename R, typename O>
+constexpr int tiled_min_waves() {
+  return std::is_same<R, double>::value && std::is_same<O, float>::value ? 4 : PRH_TILED_MIN_WAVES;
+}
 
 template <typename R, typename O, int OP, int TY, bool LOCAL, bool FAST>
-__global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(const R* __restrict__ v,
+__global__ __launch_bounds__(256, (tiled_min_waves<R, O>())) void k_stencil_tiled(const R* __restrict__ v,
                                                        const R* __restrict__ f,
                                                        O* __restrict__ out, TileCoef<R> T,
                                                        R omega, R keep, int xlen) {
@@ -304,7 +309,46 @@ __global__ __launch_bounds__(256, PRH_TILED_MIN_WAVES) void k_stencil_tiled(cons
   };
   // Loads in the order of k_stencil_rows: plane x's sources during plane x - 1,
   // after that plane's prefetch, every step issuing the same (clamped) loads.
+  // The FP64 sweeps and residuals of the plain path keep the sources inside the
+  // plane's step: the early loads cost them 4.6% at 1024^3 and gained nothing
+  // at 512^3, while the FP64-to-FP32 residual gains 4-5% at both.
+  constexpr bool kAhead = !(std::is_same<R, double>::value && std::is_same<O, double>::value);
   R fA[KY], fB[KY];
+  if constexpr (!kAhead) {
+    if (OP == kOpJacobiZero) {
+      for (int x = xb; x < xe; x++) {
+        load_f(fA, x);
+        compute(x, fA, lds, lds, lds);
+      }
+      return;
+    }
+    if (xb + 1 < xe) fetch(preA, xb + 2);
+    __syncthreads();
+    for (int x0 = xb; x0 < xe; x0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int x = x0 + u;
+        if (x < xe) {
+          if (x + 2 < xe) {
+            if ((u + 3) & 1)
+              fetch(preB, x + 3);
+            else
+              fetch(preA, x + 3);
+          }
+          load_f(fA, x);
+          compute(x, fA, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE, lds + ((u + 1) & 3) * PLANE);
+          if (x + 1 < xe) {
+            if ((u + 2) & 1)
+              stash(preB, (u + 2) & 3);
+            else
+              stash(preA, (u + 2) & 3);
+          }
+          __syncthreads();
+        }
+      }
+    }
+    return;
+  }
   if (OP == kOpJacobiZero) {
     // zero iterate: no neighbourhood to stage, one pass over the run
     load_f(fA, xb);
@@ -458,7 +502,7 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
     const R* fx = f + static_cast<size_t>(x) * s0 + rowoff;
 #pragma unroll
     for (int c = 0; c < CZ; c++) dst.f[c] = fx[zb + 64 * WPR * c];
-    if constexpr (OP == kOpJacobiAcc && N2 <= 512) {
+    if constexpr (OP == kOpJacobiAcc && N2 == 512) {
       const double* ax = T.acc + static_cast<size_t>(x) * s0 + rowoff;
 #pragma unroll
       for (int c = 0; c < CZ; c++) dst.a[c] = ax[zb + 64 * WPR * c];
@@ -502,7 +546,7 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
                                                      pypz[c], omega, keep);
       if constexpr (OP == kOpJacobiAcc) {
         double* a = T.acc + static_cast<size_t>(x) * s0 + rowoff + zc;
-        *a = (N2 <= 512 ? src.a[c] : *a) + static_cast<double>(res);
+        *a = (N2 == 512 ? src.a[c] : *a) + static_cast<double>(res);
       } else
         ox[zc] = static_cast<O>(res);
     }
@@ -515,9 +559,11 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
   // flight meanwhile.  Every step issues the same loads, also past the end of
   // the run (plane indices clamped to it: such a load re-reads a plane that is
   // in L2 and its value is not used), so that the waits can count on them.
-  // Rows of 1024 cells keep the loads of the sources inside the plane's step:
-  // the registers of the loads in flight would spill there.
-  constexpr bool kAhead = N2 <= 512;
+  // Only rows of 512 cells load ahead: their blocks walk long runs (64 planes
+  // at 512^3; 0.322 -> 0.306 ms per sweep), while the short runs of the smaller
+  // levels lose a few percent to the clamped loads at their ends (64^3: 5.15 ->
+  // 5.35 us) and rows of 1024 cells would spill the registers in flight.
+  constexpr bool kAhead = N2 == 512;
   Src fA, fB;
   if (OP == kOpJacobiZero) {
     load_f(fA, xb);
