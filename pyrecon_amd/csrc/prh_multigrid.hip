@@ -65,10 +65,23 @@ __global__ __launch_bounds__(kThreads) void k_jacobi(const R* __restrict__ v,
   out[i] = jacobi_update(k, h, f[i], omega, keep, G.local);
 }
 
+// A source read with its mean removed: each value f becomes f - sum[0] / n
+// (sum on the device, n the number of cells), the same FP64 subtraction as a
+// centred copy of f would hold.  sum == nullptr: the source as it is.
+struct Centre {
+  const double* sum = nullptr;
+  double n = 1.0;
+};
+template <typename R>
+__device__ __forceinline__ R centre_mean(const Centre& C) {
+  return C.sum ? static_cast<R>(*C.sum / C.n) : R(0);
+}
+
 template <typename R, typename O>
 __global__ __launch_bounds__(kThreads) void k_residual(const R* __restrict__ v,
                                                        const R* __restrict__ f,
-                                                       O* __restrict__ r, LevelGeom<R> G) {
+                                                       O* __restrict__ r, LevelGeom<R> G,
+                                                       Centre C) {
   const int iz = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   const int ix = blockIdx.z;
@@ -76,7 +89,8 @@ __global__ __launch_bounds__(kThreads) void k_residual(const R* __restrict__ v,
   const size_t i = (static_cast<size_t>(ix) * G.n[1] + iy) * G.n[2] + iz;
   const Coef<R> k = cell_coef(G, ix, iy, iz);
   const Hood<R> h = gather_hood(v, G.n[0], G.n[1], G.n[2], ix, iy, iz);
-  r[i] = static_cast<O>(residual_value(k, h, f[i], G.local));
+  const R fv = C.sum ? f[i] - centre_mean<R>(C) : f[i];
+  r[i] = static_cast<O>(residual_value(k, h, fv, G.local));
 }
 
 // 2.5D-blocked kernel for large levels (n2 % 64 == 0, n1 % TY == 0).  A block
@@ -103,6 +117,8 @@ struct TileCoef {
   // being stored (the last sweep of a mixed-precision correction, whose FP32
   // result is only needed in the FP64 update v += e)
   double* acc = nullptr;
+  // kOpResidual: the source's mean to remove (the mixed path's centred source)
+  Centre C;
 };
 
 // One cell of a blocked sweep: the coefficients at (px, py, pz) (A, B, C are
@@ -180,6 +196,7 @@ __global__ __launch_bounds__(256, (tiled_min_waves<R, O>())) void k_stencil_tile
   constexpr int PER = (PLANE + 255) / 256;
   __shared__ R lds[4 * PLANE];
   const LevelGeom<R>& G = T.G;
+  const R fmean = OP == kOpResidual ? centre_mean<R>(T.C) : R(0);
   const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
   const int tid = threadIdx.x, tz = tid & 63, ty = tid >> 6;
   // XCD-aware tile order: the hardware deals blocks to the 8 XCDs round-robin;
@@ -298,7 +315,7 @@ __global__ __launch_bounds__(256, (tiled_min_waves<R, O>())) void k_stencil_tile
       h.ymzp = P[o - W + 1];
       h.ypzm = P[o + W - 1];
       }
-      const R fv = fvs[k];
+      const R fv = (OP == kOpResidual && T.C.sum) ? fvs[k] - fmean : fvs[k];
       const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, fv, px, A, py[k], B[k], pz, C, pypz[k],
                                                      omega, keep);
       if constexpr (OP == kOpJacobiAcc)
@@ -914,7 +931,7 @@ static bool launch_rows(const R* v, const R* f, O* out, const TileCoef<R>& T, R 
 
 template <typename R, typename O, int OP, bool FAST>
 static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, R omega, R keep,
-                         hipStream_t s, double* acc = nullptr) {
+                         hipStream_t s, double* acc = nullptr, Centre C = {}) {
   const int xlen = tiled_xlen(G.n);
   const unsigned grid = static_cast<unsigned>((G.n[2] / kTileZ) * (G.n[1] / kTileY) *
                                               ((G.n[0] + xlen - 1) / xlen));
@@ -923,7 +940,8 @@ static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, 
   T.K = G.local ? GlobalCoef<R>() : GlobalCoef<R>::make(G);
   T.dinv = G.local ? R(0) : R(1) / T.K.d;
   T.acc = acc;
-  if (launch_rows<R, O, OP, FAST>(v, f, out, T, omega, keep, s)) return;
+  T.C = C;
+  if (!C.sum && launch_rows<R, O, OP, FAST>(v, f, out, T, omega, keep, s)) return;
   if (G.local)
     k_stencil_tiled<R, O, OP, kTileY, true, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
   else
@@ -986,37 +1004,41 @@ __device__ __forceinline__ R restrict_sum(A at) {
 
 template <typename R>
 __device__ __forceinline__ R restrict_value(const R* __restrict__ u, int f0, int f1, int f2,
-                                            int jx, int jy, int jz) {
+                                            int jx, int jy, int jz, const Centre& C = {}) {
+  const R m = centre_mean<R>(C);
   const size_t s0 = static_cast<size_t>(f1) * f2, s1 = static_cast<size_t>(f2);
   const int fx = 2 * jx, fy = 2 * jy, fz = 2 * jz;
   const size_t xo[3] = {(fx == 0 ? f0 - 1 : fx - 1) * s0, fx * s0, (fx + 1 == f0 ? 0 : fx + 1) * s0};
   const size_t yo[3] = {(fy == 0 ? f1 - 1 : fy - 1) * s1, fy * s1, (fy + 1 == f1 ? 0 : fy + 1) * s1};
   const size_t zo[3] = {static_cast<size_t>(fz == 0 ? f2 - 1 : fz - 1), static_cast<size_t>(fz),
                         static_cast<size_t>(fz + 1 == f2 ? 0 : fz + 1)};
-  return restrict_sum<R>([&](int a, int b, int c) { return u[xo[a] + yo[b] + zo[c]]; });
+  return restrict_sum<R>([&](int a, int b, int c) {
+    const R x = u[xo[a] + yo[b] + zo[c]];
+    return C.sum ? x - m : x;
+  });
 }
 
 template <typename R, typename RO, typename I>
 __device__ __forceinline__ void restrict_at(const R* __restrict__ u, RO* __restrict__ c, int f0,
-                                            int f1, int f2, I j) {
+                                            int f1, int f2, I j, const Centre& C = {}) {
   int jx, jy, jz;
   unflatten(j, f1 / 2, f2 / 2, jx, jy, jz);
-  c[j] = static_cast<RO>(restrict_value(u, f0, f1, f2, jx, jy, jz));
+  c[j] = static_cast<RO>(restrict_value(u, f0, f1, f2, jx, jy, jz, C));
 }
 
 template <typename R, typename RO>
 __device__ __forceinline__ void restrict_body(const R* __restrict__ u, RO* __restrict__ c, int f0,
-                                              int f1, int f2) {
+                                              int f1, int f2, const Centre& C) {
   const size_t total = static_cast<size_t>(f0 / 2) * (f1 / 2) * (f2 / 2);
   const size_t j = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
-  if (j < total) restrict_at<R, RO>(u, c, f0, f1, f2, j);
+  if (j < total) restrict_at<R, RO>(u, c, f0, f1, f2, j, C);
 }
 
 template <typename R, typename RO>
 __global__ __launch_bounds__(kThreads) void k_restrict(const R* __restrict__ u,
                                                        RO* __restrict__ c, int f0, int f1,
-                                                       int f2) {
-  restrict_body<R, RO>(u, c, f0, f1, f2);
+                                                       int f2, Centre C) {
+  restrict_body<R, RO>(u, c, f0, f1, f2, C);
 }
 
 // FP32 transfers with two coarse cells per thread (16-byte fine accesses)
@@ -1035,7 +1057,7 @@ template <> struct Pair<double> { using type = double2; };
 template <typename R, typename RO>
 __global__ __launch_bounds__(kThreads) void k_restrict_rows(const R* __restrict__ u,
                                                             RO* __restrict__ c, int f0, int f1,
-                                                            int f2) {
+                                                            int f2, Centre C) {
   using V = typename Pair<R>::type;
   const int c1 = f1 / 2, c2 = f2 / 2;
   const int jz0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1062,6 +1084,15 @@ __global__ __launch_bounds__(kThreads) void k_restrict_rows(const R* __restrict_
       w[a][b][2] = pr.y;
       w[a][b][0] = first ? row[zm] : left;
     }
+  if (C.sum) {
+    const R m = centre_mean<R>(C);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) w[a][b][q] -= m;
+  }
   // fine z = 2 jz + 1 wraps to 0 only when f2 == 2 jz + 1, which cannot happen (f2 even)
   if (in)
     c[(static_cast<size_t>(jx) * c1 + jy) * c2 + jz] =
@@ -1314,16 +1345,6 @@ __global__ __launch_bounds__(kReduceThreads) void k_sum_f64(const double* __rest
   block_reduce_range<1>(n, [&](size_t i, double* out) { out[0] = a[i]; }, partial, 0);
 }
 
-// out = a - sum[0] / n, the mesh with its mean removed (sum from k_reduce_final).
-__global__ __launch_bounds__(kThreads) void k_subtract_mean(const double* __restrict__ a,
-                                                             double* __restrict__ out,
-                                                             const double* __restrict__ sum,
-                                                             size_t n) {
-  const size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
-  const double mean = sum[0] / static_cast<double>(n);
-  if (i < n) out[i] = a[i] - mean;
-}
-
 // ---------------------------------------------------------------------------
 // Grouped launches: one launch applies the same operation at the same level
 // shape to up to kMaxGroup independent problems (serving several
@@ -1375,7 +1396,7 @@ __global__ __launch_bounds__(kThreads) void k_residual_group(GroupArgs<R, R, R> 
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_restrict_group(GroupArgs<R, R, R> A, int f0, int f1,
                                                              int f2) {
-  restrict_body<R, R>(A.in[blockIdx.y], A.out[blockIdx.y], f0, f1, f2);
+  restrict_body<R, R>(A.in[blockIdx.y], A.out[blockIdx.y], f0, f1, f2, Centre{});
 }
 
 template <typename R, bool ADD>
@@ -1675,28 +1696,28 @@ static int launch_jacobi(const R* v, const R* f, R* out, const LevelGeom<R>& G, 
 
 template <typename R, typename O>
 static int launch_residual(const R* v, const R* f, O* r, const LevelGeom<R>& G, hipStream_t s,
-                           bool fast = false) {
+                           bool fast = false, Centre C = {}) {
   if (tiled_ok(G.n) && fast) {
     launch_tiled<R, O, kOpResidual, std::is_same<R, float>::value && std::is_same<O, float>::value>(
-        v, f, r, G, R(0), R(0), s);
+        v, f, r, G, R(0), R(0), s, nullptr, C);
   } else if (tiled_ok(G.n)) {
-    launch_tiled<R, O, kOpResidual, false>(v, f, r, G, R(0), R(0), s);
+    launch_tiled<R, O, kOpResidual, false>(v, f, r, G, R(0), R(0), s, nullptr, C);
   } else {
     dim3 block, grid;
     generic_shape(G.n, block, grid);
-    k_residual<R, O><<<grid, block, 0, s>>>(v, f, r, G);
+    k_residual<R, O><<<grid, block, 0, s>>>(v, f, r, G, C);
   }
   PRH_LAUNCHED();
   return PRH_OK;
 }
 
 template <typename R, typename RO>
-static int launch_restrict(const R* fine, RO* coarse, const size_t* nf, hipStream_t s) {
+static int launch_restrict(const R* fine, RO* coarse, const size_t* nf, hipStream_t s, Centre C = {}) {
   if (nf[2] % 2 != 0 || reinterpret_cast<uintptr_t>(fine) % (2 * sizeof(R)) != 0) {
     // rows that do not start on pair boundaries: one thread per coarse cell
     const size_t total = (nf[0] / 2) * (nf[1] / 2) * (nf[2] / 2);
     k_restrict<R, RO><<<ceil_div(total, kThreads), kThreads, 0, s>>>(
-        fine, coarse, static_cast<int>(nf[0]), static_cast<int>(nf[1]), static_cast<int>(nf[2]));
+        fine, coarse, static_cast<int>(nf[0]), static_cast<int>(nf[1]), static_cast<int>(nf[2]), C);
     PRH_LAUNCHED();
     return PRH_OK;
   }
@@ -1704,7 +1725,7 @@ static int launch_restrict(const R* fine, RO* coarse, const size_t* nf, hipStrea
                      static_cast<int>(nf[2] / 2)};
   dim3 block, grid;
   if constexpr (std::is_same<R, float>::value) {
-    if (PRH_TRANSFER_QUADS && nf[2] % 4 == 0 && reinterpret_cast<uintptr_t>(fine) % 16 == 0 &&
+    if (!C.sum && PRH_TRANSFER_QUADS && nf[2] % 4 == 0 && reinterpret_cast<uintptr_t>(fine) % 16 == 0 &&
         reinterpret_cast<uintptr_t>(coarse) % (2 * sizeof(RO)) == 0) {
       const int nq[3] = {nc[0], nc[1], nc[2] / 2};
       generic_shape(nq, block, grid);
@@ -1716,7 +1737,7 @@ static int launch_restrict(const R* fine, RO* coarse, const size_t* nf, hipStrea
   }
   generic_shape(nc, block, grid);
   k_restrict_rows<R, RO><<<grid, block, 0, s>>>(fine, coarse, static_cast<int>(nf[0]),
-                                                static_cast<int>(nf[1]), static_cast<int>(nf[2]));
+                                                static_cast<int>(nf[1]), static_cast<int>(nf[2]), C);
   PRH_LAUNCHED();
   return PRH_OK;
 }
@@ -2377,27 +2398,25 @@ static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double*
   // restriction or prolongation maps a constant to a constant, so each cycle
   // maps f - m to its result on f minus a constant: solving with f - m (m the
   // mean of f) changes the potential by a constant only, and its gradient not
-  // at all.  f - m is kept in the workspace's FP64 level-0 scratch.
-  const double* src[kMaxGroup];
+  // at all.  The kernels that read f (the restriction to level 1 and the FP64
+  // residuals) subtract m as they load it (Centre).
+  Centre cen[kMaxGroup];
   for (int j = 0; j < count; j++) {
-    src[j] = f[j];
     if (!centrable(G64[j]) || !ws[j]->mg.tmp[0]) continue;
-    double* fc = static_cast<double*>(ws[j]->mg.tmp[0]);
     double* sum = ws[j]->red + 2 * kReduceBlocks + 2;
     k_sum_f64<<<kReduceBlocks, kReduceThreads, 0, s>>>(f[j], total, ws[j]->red);
     PRH_LAUNCHED();
     k_reduce_final<1><<<1, 1024, 0, s>>>(ws[j]->red, kReduceBlocks, 0, sum);
     PRH_LAUNCHED();
-    k_subtract_mean<<<grid, kThreads, 0, s>>>(f[j], fc, sum, total);
-    PRH_LAUNCHED();
-    src[j] = fc;
+    cen[j].sum = sum;
+    cen[j].n = static_cast<double>(total);
   }
   // 1. coarse FMG stages in FP32, prolonged into the FP64 iterate.  The full
   // multigrid starts its coarsest level from zero; when level 1 is that level,
   // Solver::fmg starts from the iterate it is given, so that is zeroed first.
   if (S.nlev() > 1) {
     const Ptrs<float> f1 = S.f(1), v1 = S.v(1);
-    for (int j = 0; j < count; j++) PRH_TRY((launch_restrict<double, float>(src[j], f1[j], nmesh, s)));
+    for (int j = 0; j < count; j++) PRH_TRY((launch_restrict<double, float>(f[j], f1[j], nmesh, s, cen[j])));
     if (S.nlev() == 2) PRH_TRY(S.fill_zero(1, v1));
     PRH_TRY(S.fmg(f1, v1, ncycle, 1));
     for (int j = 0; j < count; j++)
@@ -2409,10 +2428,10 @@ static int do_fmg_mixed_group(prh_workspace* const* ws, int count, const double*
     for (int j = 0; j < count; j++) {
       if (norms) {
         // the norm is of the residual of the true source, f - A v
-        PRH_TRY(residual_rms(ws[j], v[j], f[j], src[j] == f[j] ? r32[j] : nullptr, G64[j], &norms[it], s));
-        if (src[j] != f[j]) PRH_TRY((launch_residual<double, float>(v[j], src[j], r32[j], G64[j], s)));
+        PRH_TRY(residual_rms(ws[j], v[j], f[j], cen[j].sum ? nullptr : r32[j], G64[j], &norms[it], s));
+        if (cen[j].sum) PRH_TRY((launch_residual<double, float>(v[j], f[j], r32[j], G64[j], s, false, cen[j])));
       } else {
-        PRH_TRY((launch_residual<double, float>(v[j], src[j], r32[j], G64[j], s)));
+        PRH_TRY((launch_residual<double, float>(v[j], f[j], r32[j], G64[j], s, false, cen[j])));
       }
     }
     Ptrs<float> cur = e32, alt = S.tmp(0);
