@@ -138,11 +138,15 @@ class OriginalMultiGridReconstruction(BaseReconstruction):
         return self.mesh_delta.buffer
 
     def _new_phi(self, zero_stream=_OWN):
-        """Zero-filled potential mesh; the fill is queued on ``zero_stream`` (default: this
-        reconstruction's stream), the stream of the solve that writes it."""
+        """Potential mesh for the solve, zero-filled where the solve starts from it: the full
+        multigrid overwrites the iterate with the prolongation from the coarser level whenever
+        the mesh can be coarsened (every extent even and above 4, src/multigrid.c:303-328), and
+        only starts from the given iterate otherwise.  The fill is queued on ``zero_stream``
+        (default: this reconstruction's stream), the stream of the solve that writes it."""
         phi = RealMesh(info=self.info, workspace=self.workspace, stream=self.stream)
         phi._allocate(zero=False)
-        phi.buffer.fill_zero(stream=self.stream if zero_stream is _OWN else zero_stream)
+        if not all(n > 4 and n % 2 == 0 for n in self.info.shape):
+            phi.buffer.fill_zero(stream=self.stream if zero_stream is _OWN else zero_stream)
         return phi
 
 
