@@ -588,7 +588,8 @@ def main():
     else:
         kernel = ('k_stencil_rows<float, fast> damped-Jacobi sweep (blocks of 8 whole 512-cell rows), '
                   'finest level 512^3 (mixed path); achieved = algorithmic bytes (3 x 512^3 x 4 B per '
-                  'sweep, 2 x 512^3 x 4 B for the zero-start sweep) / average launch time (HIP events); '
+                  'sweep) / average launch time (HIP events); the last sweep of each correction (added into the '
+                  'FP64 iterate) and the first two sweeps from zero (one launch) are other kernels, not timed; '
                   'traffic = FETCH_SIZE + WRITE_SIZE per launch, calibrated on a copy kernel')
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=kernel,

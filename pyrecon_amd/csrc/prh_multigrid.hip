@@ -103,7 +103,11 @@ __global__ __launch_bounds__(kThreads) void k_residual(const R* __restrict__ v,
 // kernels' (same bits).
 // kOpJacobiAcc: a Jacobi sweep whose result is added (as double) into TileCoef::acc
 // instead of being stored (the last sweep of a mixed-precision correction)
-enum StencilOp { kOpJacobi = 0, kOpResidual = 1, kOpJacobiZero = 2, kOpJacobiAcc = 3 };
+// kOpJacobiAfterZero: the first two sweeps from a zero iterate in one launch
+// (FP32 rows of 512 cells): the first sweep of a zero iterate is pointwise in
+// the source, so the second stages the first's value of each cell, computed
+// from the source as the zero-start sweep computes it, instead of reading it
+enum StencilOp { kOpJacobi = 0, kOpResidual = 1, kOpJacobiZero = 2, kOpJacobiAcc = 3, kOpJacobiAfterZero = 4 };
 constexpr int kTileZ = 64;
 
 // Coefficients of a level passed to the blocked kernels: the geometry for a
@@ -438,6 +442,11 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
                                                               R omega, R keep, int xlen) {
   constexpr int N2 = 64 * WPR * CZ, PLANE = (RY + 2) * N2, NT = 64 * RY * WPR;
   constexpr int PER = (PLANE + NT - 1) / NT;
+  // kOpJacobiAfterZero: v is the source, and the staged value of each cell is
+  // the zero-start sweep's; the computation is then a plain sweep
+  constexpr bool AZ = OP == kOpJacobiAfterZero;
+  constexpr int OPC = AZ ? kOpJacobi : OP;
+  static_assert(!AZ || (N2 == 512 && FAST), "two sweeps from zero: FP32 rows of 512 cells");
   __shared__ R lds[4 * PLANE];
   const LevelGeom<R>& G = T.G;
   const int n0 = G.n[0], n1 = G.n[1];
@@ -466,13 +475,42 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
   }
   auto wrapx = [n0](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
   auto slot = [xb](int x) { return (x - xb + 4) & 3; };
+  // the value staged for element q of plane xp (a plane index in [0, n0))
+  auto staged = [&](R val, int xp, int q) -> R {
+    if constexpr (!AZ) {
+      return val;
+    } else {
+      // the zero-start sweep at (xp, gy, gz), with the coefficients that
+      // sweep's own kernel computes for that cell.  They are recomputed at each
+      // plane: the empty asm hides that they do not change along x, which would
+      // otherwise hold all of them in registers across the loop (and spill).
+      int o = off[q];
+      asm volatile("" : "+v"(o));
+      const int gy = o / N2, gz = o - gy * N2;
+      R px = R(0), A = R(0), py = R(0), B = R(0), pz = R(0), C = R(0), pypz;
+      if (LOCAL) {
+        px = static_cast<R>(xp) + G.org[0];
+        A = G.h2[0] * px * px;
+        py = static_cast<R>(gy) + G.org[1];
+        B = G.h2[1] * py * py;
+        pz = static_cast<R>(gz) + G.org[2];
+        C = G.h2[2] * pz * pz;
+        pypz = py * pz;
+      } else {
+        pypz = T.K.pypz;
+      }
+      return stencil_cell<R, kOpJacobiZero, LOCAL, FAST>(T, zero_hood<R>(), val, px, A, py, B, pz, C,
+                                                         pypz, omega, keep);
+    }
+  };
   if (OP != kOpJacobiZero)
     for (int p = -1; p <= 1; p++) {
-      const R* src = v + static_cast<size_t>(wrapx(xb + p)) * s0;
+      const int xp = wrapx(xb + p);
+      const R* src = v + static_cast<size_t>(xp) * s0;
       R* dst = lds + slot(xb + p) * PLANE;
 #pragma unroll
       for (int q = 0; q < PER; q++)
-        if (has[q]) dst[tid + NT * q] = src[off[q]];
+        if (has[q]) dst[tid + NT * q] = staged(src[off[q]], xp, q);
     }
   // per-thread constants: row iy = y0 + ty, columns iz = zb + 64 WPR c
   const int iy = y0 + ty;
@@ -502,11 +540,21 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
     for (int q = 0; q < PER; q++)
       if (has[q]) dst[q] = src[off[q]];
   };
-  auto stash = [&](const R(&src)[PER], int sl) {
+  auto stash = [&](const R(&src)[PER], int sl, int xp) {
     R* dst = lds + sl * PLANE;
 #pragma unroll
     for (int q = 0; q < PER; q++)
       if (has[q]) dst[tid + NT * q] = src[q];
+    if constexpr (AZ) {
+      // the thread's own elements, in place, once the loaded values are out of
+      // registers (kept apart from the sweep's own code, which holds the rest)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        if (has[q]) dst[tid + NT * q] = staged(dst[tid + NT * q], xp, q);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
   };
   const int rm = ty * N2, r0 = (ty + 1) * N2, rp = (ty + 2) * N2;
   // the source values of plane x (and, for kOpJacobiAcc, the FP64 values the
@@ -559,7 +607,7 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
         h.ymzp = P[rm + zr];
         h.ypzm = P[rp + zl];
       }
-      const R res = stencil_cell<R, OP, LOCAL, FAST>(T, h, src.f[c], px, A, py, B, pz[c], C[c],
+      const R res = stencil_cell<R, OPC, LOCAL, FAST>(T, h, src.f[c], px, A, py, B, pz[c], C[c],
                                                      pypz[c], omega, keep);
       if constexpr (OP == kOpJacobiAcc) {
         double* a = T.acc + static_cast<size_t>(x) * s0 + rowoff + zc;
@@ -610,9 +658,9 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
           compute(x, fA, lds + ((u + 3) & 3) * PLANE, lds + (u & 3) * PLANE, lds + ((u + 1) & 3) * PLANE);
           if (x + 1 < xe) {
             if ((u + 2) & 1)
-              stash(preB, (u + 2) & 3);
+              stash(preB, (u + 2) & 3, wrapx(x + 2));
             else
-              stash(preA, (u + 2) & 3);
+              stash(preA, (u + 2) & 3, wrapx(x + 2));
           }
           __syncthreads();
         }
@@ -643,9 +691,9 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
                 lds + ((u + 1) & 3) * PLANE);
         if (x + 1 < xe) {
           if ((u + 2) & 1)
-            stash(preB, (u + 2) & 3);
+            stash(preB, (u + 2) & 3, wrapx(x + 2));
           else
-            stash(preA, (u + 2) & 3);
+            stash(preA, (u + 2) & 3, wrapx(x + 2));
         }
         __syncthreads();
       }
@@ -914,7 +962,12 @@ static void launch_rows_cz(const R* v, const R* f, O* out, const TileCoef<R>& T,
 template <typename R, typename O, int OP, bool FAST>
 static bool launch_rows(const R* v, const R* f, O* out, const TileCoef<R>& T, R omega, R keep,
                         hipStream_t s) {
-  if constexpr (std::is_same<R, float>::value && PRH_ROWS_BLOCKS > 0) {
+  if constexpr (OP == kOpJacobiAfterZero) {
+    if (!FAST || !std::is_same<R, float>::value || PRH_ROWS_BLOCKS <= 0 || rows_cz(T.G.n) != 8) return false;
+    if constexpr (FAST && std::is_same<R, float>::value)
+      launch_rows_cz<R, O, OP, FAST, 2, 4>(v, f, out, T, omega, keep, s);
+    return true;
+  } else if constexpr (std::is_same<R, float>::value && PRH_ROWS_BLOCKS > 0) {
     switch (rows_cz(T.G.n)) {
       case 1: launch_rows_cz<R, O, OP, FAST, 1, 1>(v, f, out, T, omega, keep, s); return true;
       case 2: launch_rows_cz<R, O, OP, FAST, 1, 2>(v, f, out, T, omega, keep, s); return true;
@@ -941,11 +994,16 @@ static void launch_tiled(const R* v, const R* f, O* out, const LevelGeom<R>& G, 
   T.dinv = G.local ? R(0) : R(1) / T.K.d;
   T.acc = acc;
   T.C = C;
-  if (!C.sum && launch_rows<R, O, OP, FAST>(v, f, out, T, omega, keep, s)) return;
-  if (G.local)
-    k_stencil_tiled<R, O, OP, kTileY, true, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
-  else
-    k_stencil_tiled<R, O, OP, kTileY, false, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
+  if constexpr (OP == kOpJacobiAfterZero) {
+    // only where after_zero_ok() holds: FP32 fast rows of 512 cells
+    launch_rows<R, O, OP, FAST>(v, f, out, T, omega, keep, s);
+  } else {
+    if (!C.sum && launch_rows<R, O, OP, FAST>(v, f, out, T, omega, keep, s)) return;
+    if (G.local)
+      k_stencil_tiled<R, O, OP, kTileY, true, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
+    else
+      k_stencil_tiled<R, O, OP, kTileY, false, FAST><<<grid, 256, 0, s>>>(v, f, out, T, omega, keep, xlen);
+  }
 }
 
 // Block shape of the generic kernels: up to 64 threads along z, the rest along y.
@@ -1672,8 +1730,11 @@ static int launch_jacobi(const R* v, const R* f, R* out, const LevelGeom<R>& G, 
   dim3 block, grid;
   generic_shape(G.n, block, grid);
   if (fused) *fused = false;
+  // a sweep added into the FP64 iterate is another kernel (kOpJacobiAcc): not in the timer
+  const bool into_acc = acc && !zero_in && tiled_ok(G.n) && fast && std::is_same<R, float>::value;
+  if (into_acc) timer = -1;
   timer_before(timer, s);
-  if (acc && !zero_in && tiled_ok(G.n) && fast && std::is_same<R, float>::value) {
+  if (into_acc) {
     launch_tiled<R, R, kOpJacobiAcc, std::is_same<R, float>::value>(v, f, out, G, omega, keep, s, acc);
     if (fused) *fused = true;
   } else if (zero_in && tiled_ok(G.n) && fast)
@@ -1985,9 +2046,27 @@ struct Solver {
     return PRH_ERR_STATE;
   }
 
+  // the first two sweeps from a zero iterate in one launch (kOpJacobiAfterZero)
+  bool after_zero_ok(int l) const {
+    if (!fast || !std::is_same<R, float>::value || grouped(l) || PRH_ROWS_BLOCKS <= 0) return false;
+    const int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]), static_cast<int>(n(l)[2])};
+    return tiled_ok(nn) && rows_cz(nn) == 8;
+  }
+  // both sweeps' result goes to out (where two single sweeps would leave it)
+  int jacobi_after_zero(int l, const Ptrs<R>& src, const Ptrs<R>& out) {
+    if constexpr (std::is_same<R, float>::value) {
+      for (int j = 0; j < count; j++) {
+        launch_tiled<R, R, kOpJacobiAfterZero, true>(src[j], src[j], out[j], G[j][l], omega, keep, s);
+        PRH_LAUNCHED();
+      }
+      return PRH_OK;
+    }
+    return PRH_ERR_STATE;
+  }
+
   // niter damped-Jacobi sweeps; the iterate moves between cur and alt.  Where
   // the level allows, the sweeps go in fused pairs (the last one alone when
-  // niter is odd).
+  // niter is odd), and a zero start's first two sweeps go in one launch.
   int smooth(int l, Ptrs<R>& cur, Ptrs<R>& alt, const Ptrs<R>& src, bool zero_start, bool post = false) {
     if (niter <= 0) {
       if (zero_start) PRH_TRY(fill_zero(l, cur));
@@ -1995,6 +2074,13 @@ struct Solver {
     }
     const bool last_fused = post && fuse_top && l == 0;
     for (int it = 0; it < niter;) {
+      if (zero_start && it == 0 && niter >= 2 && !(pairable(l)) && !(last_fused && niter == 2) &&
+          after_zero_ok(l)) {
+        // two sweeps: the iterate ends in cur, as after two single sweeps (two swaps)
+        PRH_TRY(jacobi_after_zero(l, src, cur));
+        it += 2;
+        continue;
+      }
       if (pairable(l) && niter - it >= 2) {
         PRH_TRY(jacobi_pair(l, cur, src, alt, zero_start && it == 0, last_fused && it + 2 == niter));
         swap_all(cur, alt, count);
