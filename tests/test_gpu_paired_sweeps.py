@@ -7,7 +7,8 @@ block with a partial last run (n0 = 1100: runs of 3), rows of 256 and 512 cells,
 that is left to the single sweeps, an odd number of sweeps (the last one alone) and an even one
 (the last pair adds into the FP64 iterate), zero starts, and a local and a global line of sight.
 Also: a mixed solve whose hierarchy has two levels starts its coarse stage from zero (repeatable,
-and within 1e-5 of the FP64 full multigrid).
+and within 1e-5 of the FP64 full multigrid); and the first two sweeps from a zero iterate, one
+launch on rows of 512 cells, give the bits of a zero-start sweep followed by a sweep.
 """
 import numpy as np
 import pytest
@@ -135,3 +136,21 @@ def test_tuning_switch():
     prev = _lib.lib.prh_set_tuning(_lib.TUNE_PAIRED_SWEEPS, 0)
     assert _lib.lib.prh_set_tuning(_lib.TUNE_PAIRED_SWEEPS, prev) == 0
     assert _lib.lib.prh_set_tuning(12345, 1) == _lib.ERR_ARG
+
+
+@pytest.mark.parametrize('shape', [(6, 8, 512), (16, 32, 512), (32, 16, 256)])
+@pytest.mark.parametrize('los', [None, (0., 0., 1.)])
+def test_two_sweeps_from_zero_same_bits(shape, los):
+    # the first two sweeps from a zero iterate run as one launch on rows of 512 cells
+    # (kOpJacobiAfterZero): the same bits as a zero-start sweep followed by a sweep
+    rng = np.random.RandomState(sum(shape))
+    v = rng.normal(size=shape).astype('f4')  # ignored by the zero start
+    f = rng.normal(size=shape).astype('f4')
+    args = ((300., 400., 1000.), (10., -20., 1500.), los)
+    together = _smooth(v, f, *args, 2, True)
+    first = _smooth(v, f, *args, 1, True)
+    both = _smooth(first, f, *args, 1, False)
+    assert np.array_equal(bits(together), bits(both))
+    # and three sweeps: the pair from zero, then one more
+    three = _smooth(v, f, *args, 3, True)
+    assert np.array_equal(bits(three), bits(_smooth(both, f, *args, 1, False)))
