@@ -446,7 +446,7 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
   // the zero-start sweep's; the computation is then a plain sweep
   constexpr bool AZ = OP == kOpJacobiAfterZero;
   constexpr int OPC = AZ ? kOpJacobi : OP;
-  static_assert(!AZ || (N2 == 512 && FAST), "two sweeps from zero: FP32 rows of 512 cells");
+  static_assert(!AZ || ((N2 == 512 || N2 == 1024) && FAST), "two sweeps from zero: FP32 rows of 512 or 1024 cells");
   __shared__ R lds[4 * PLANE];
   const LevelGeom<R>& G = T.G;
   const int n0 = G.n[0], n1 = G.n[1];
@@ -963,9 +963,14 @@ template <typename R, typename O, int OP, bool FAST>
 static bool launch_rows(const R* v, const R* f, O* out, const TileCoef<R>& T, R omega, R keep,
                         hipStream_t s) {
   if constexpr (OP == kOpJacobiAfterZero) {
-    if (!FAST || !std::is_same<R, float>::value || PRH_ROWS_BLOCKS <= 0 || rows_cz(T.G.n) != 8) return false;
-    if constexpr (FAST && std::is_same<R, float>::value)
-      launch_rows_cz<R, O, OP, FAST, 2, 4>(v, f, out, T, omega, keep, s);
+    const int cz = rows_cz(T.G.n);
+    if (!FAST || !std::is_same<R, float>::value || PRH_ROWS_BLOCKS <= 0 || (cz != 8 && cz != 16)) return false;
+    if constexpr (FAST && std::is_same<R, float>::value) {
+      if (cz == 8)
+        launch_rows_cz<R, O, OP, FAST, 2, 4>(v, f, out, T, omega, keep, s);
+      else
+        launch_rows_cz<R, O, OP, FAST, 4, 4, kRowsY1024>(v, f, out, T, omega, keep, s);
+    }
     return true;
   } else if constexpr (std::is_same<R, float>::value && PRH_ROWS_BLOCKS > 0) {
     switch (rows_cz(T.G.n)) {
@@ -2050,7 +2055,7 @@ struct Solver {
   bool after_zero_ok(int l) const {
     if (!fast || !std::is_same<R, float>::value || grouped(l) || PRH_ROWS_BLOCKS <= 0) return false;
     const int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]), static_cast<int>(n(l)[2])};
-    return tiled_ok(nn) && rows_cz(nn) == 8;
+    return tiled_ok(nn) && (rows_cz(nn) == 8 || rows_cz(nn) == 16);
   }
   // both sweeps' result goes to out (where two single sweeps would leave it)
   int jacobi_after_zero(int l, const Ptrs<R>& src, const Ptrs<R>& out) {
