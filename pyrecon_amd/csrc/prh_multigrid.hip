@@ -446,7 +446,7 @@ __global__ __launch_bounds__(64 * RY * WPR, PRH_ROWS_MIN_WAVES) void k_stencil_r
   // the zero-start sweep's; the computation is then a plain sweep
   constexpr bool AZ = OP == kOpJacobiAfterZero;
   constexpr int OPC = AZ ? kOpJacobi : OP;
-  static_assert(!AZ || ((N2 == 512 || N2 == 1024) && FAST), "two sweeps from zero: FP32 rows of 512 or 1024 cells");
+  static_assert(!AZ || FAST, "two sweeps from zero: the FP32 fast sweeps");
   __shared__ R lds[4 * PLANE];
   const LevelGeom<R>& G = T.G;
   const int n0 = G.n[0], n1 = G.n[1];
@@ -964,12 +964,15 @@ static bool launch_rows(const R* v, const R* f, O* out, const TileCoef<R>& T, R 
                         hipStream_t s) {
   if constexpr (OP == kOpJacobiAfterZero) {
     const int cz = rows_cz(T.G.n);
-    if (!FAST || !std::is_same<R, float>::value || PRH_ROWS_BLOCKS <= 0 || (cz != 8 && cz != 16)) return false;
+    if (!FAST || !std::is_same<R, float>::value || PRH_ROWS_BLOCKS <= 0 || cz == 0) return false;
     if constexpr (FAST && std::is_same<R, float>::value) {
-      if (cz == 8)
-        launch_rows_cz<R, O, OP, FAST, 2, 4>(v, f, out, T, omega, keep, s);
-      else
-        launch_rows_cz<R, O, OP, FAST, 4, 4, kRowsY1024>(v, f, out, T, omega, keep, s);
+      switch (cz) {
+        case 1: launch_rows_cz<R, O, OP, FAST, 1, 1>(v, f, out, T, omega, keep, s); break;
+        case 2: launch_rows_cz<R, O, OP, FAST, 1, 2>(v, f, out, T, omega, keep, s); break;
+        case 4: launch_rows_cz<R, O, OP, FAST, 1, 4>(v, f, out, T, omega, keep, s); break;
+        case 8: launch_rows_cz<R, O, OP, FAST, 2, 4>(v, f, out, T, omega, keep, s); break;
+        default: launch_rows_cz<R, O, OP, FAST, 4, 4, kRowsY1024>(v, f, out, T, omega, keep, s); break;
+      }
     }
     return true;
   } else if constexpr (std::is_same<R, float>::value && PRH_ROWS_BLOCKS > 0) {
@@ -2055,7 +2058,7 @@ struct Solver {
   bool after_zero_ok(int l) const {
     if (!fast || !std::is_same<R, float>::value || grouped(l) || PRH_ROWS_BLOCKS <= 0) return false;
     const int nn[3] = {static_cast<int>(n(l)[0]), static_cast<int>(n(l)[1]), static_cast<int>(n(l)[2])};
-    return tiled_ok(nn) && (rows_cz(nn) == 8 || rows_cz(nn) == 16);
+    return tiled_ok(nn) && rows_cz(nn) != 0;
   }
   // both sweeps' result goes to out (where two single sweeps would leave it)
   int jacobi_after_zero(int l, const Ptrs<R>& src, const Ptrs<R>& out) {

@@ -8,7 +8,7 @@ that is left to the single sweeps, an odd number of sweeps (the last one alone) 
 (the last pair adds into the FP64 iterate), zero starts, and a local and a global line of sight.
 Also: a mixed solve whose hierarchy has two levels starts its coarse stage from zero (repeatable,
 and within 1e-5 of the FP64 full multigrid); and the first two sweeps from a zero iterate, one
-launch on rows of 512 or 1024 cells, give the bits of a zero-start sweep followed by a sweep.
+launch on the whole-row levels, give the bits of a zero-start sweep followed by a sweep.
 """
 import numpy as np
 import pytest
@@ -138,10 +138,11 @@ def test_tuning_switch():
     assert _lib.lib.prh_set_tuning(12345, 1) == _lib.ERR_ARG
 
 
-@pytest.mark.parametrize('shape', [(6, 8, 512), (16, 32, 512), (32, 16, 256), (5, 8, 1024), (3, 4, 1024)])
+@pytest.mark.parametrize('shape', [(6, 8, 512), (16, 32, 512), (32, 16, 256), (5, 8, 1024), (3, 4, 1024),
+                                   (9, 16, 128), (7, 8, 64)])
 @pytest.mark.parametrize('los', [None, (0., 0., 1.)])
 def test_two_sweeps_from_zero_same_bits(shape, los):
-    # the first two sweeps from a zero iterate run as one launch on rows of 512 or 1024 cells
+    # the first two sweeps from a zero iterate run as one launch on the whole-row levels
     # (kOpJacobiAfterZero): the same bits as a zero-start sweep followed by a sweep
     rng = np.random.RandomState(sum(shape))
     v = rng.normal(size=shape).astype('f4')  # ignored by the zero start
