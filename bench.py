@@ -548,7 +548,8 @@ def main():
         th = (time.perf_counter() - t0) / args.host_io_steps
         host_io = dict(value=round(1. / th, 4), unit='recon/s', ms_per_step=round(1e3 * th, 2),
                        steps=args.host_io_steps, precision='mixed',
-                       note='catalogs uploaded from numpy (pageable host memory) and shifted positions of data '
+                       note='catalogs uploaded from numpy (pageable host memory, copied through the library\'s pinned '
+                            'staging buffers) and shifted positions of data '
                             'and randoms returned to numpy inside each step: PCIe-inclusive, not the headline value')
         log('host io', host_io)
 

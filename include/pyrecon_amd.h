@@ -63,6 +63,13 @@ int prh_host_alloc(void **ptr, size_t bytes); /* pinned host memory */
 int prh_host_free(void *ptr);
 int prh_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
 int prh_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+/* Copies between pageable host memory and the device through the library's
+ * pinned staging buffers (no reference counterpart: the reference's arrays
+ * stay on the host).  Host threads copy one chunk while the transfer of the
+ * previous one runs.  htod returns once src has been read (the transfers are
+ * queued on stream); dtoh returns once dst holds the data. */
+int prh_memcpy_htod_staged(void *dst, const void *src, size_t bytes, void *stream);
+int prh_memcpy_dtoh_staged(void *dst, const void *src, size_t bytes, void *stream);
 int prh_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
 int prh_memset(void *dst, int value, size_t bytes, void *stream);
 int prh_mem_info(size_t *free_bytes, size_t *total_bytes);

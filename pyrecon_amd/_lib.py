@@ -77,6 +77,8 @@ def _signatures():
         'prh_host_free': (_i, [_vp]),
         'prh_memcpy_htod': (_i, [_vp, _vp, _sz, _vp]),
         'prh_memcpy_dtoh': (_i, [_vp, _vp, _sz, _vp]),
+        'prh_memcpy_htod_staged': (_i, [_vp, _vp, _sz, _vp]),
+        'prh_memcpy_dtoh_staged': (_i, [_vp, _vp, _sz, _vp]),
         'prh_memcpy_dtod': (_i, [_vp, _vp, _sz, _vp]),
         'prh_memset': (_i, [_vp, _i, _sz, _vp]),
         'prh_mem_info': (_i, [_szp, _szp]),

@@ -3,6 +3,9 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "prh_workspace.hpp"
@@ -187,11 +190,104 @@ static int paint_alloc(prh_workspace* ws, size_t chunk) {
   return PRH_OK;
 }
 
+// ---- host copies through pinned staging buffers ---------------------------
+// The DMA engines cannot read pageable host memory: HIP stages such a copy
+// itself, one buffer at a time on the calling thread.  The staged copies below
+// pass chunks of kStageBytes through kStages pinned buffers instead: several
+// host threads copy chunk k between the user's memory and its buffer while the
+// transfer of the chunk before it runs, so the host copy and the PCIe transfer
+// overlap.  One set of buffers per process, taken by one call at a time.
+constexpr size_t kStageBytes = size_t(32) << 20;
+constexpr int kStages = 3;
+constexpr int kCopyThreads = 8;
+constexpr size_t kCopyMinPerThread = size_t(1) << 20;
+
+struct Staging {
+  std::mutex m;
+  void* buf[kStages] = {};
+  hipEvent_t done[kStages] = {};  // the last transfer from or into each buffer
+  bool ready = false;
+};
+static Staging g_stage;
+
+static int staging_init() {
+  if (g_stage.ready) return PRH_OK;
+  for (int b = 0; b < kStages; b++) {
+    if (!g_stage.buf[b]) PRH_HIP(hipHostMalloc(&g_stage.buf[b], kStageBytes, hipHostMallocDefault));
+    if (!g_stage.done[b]) PRH_HIP(hipEventCreateWithFlags(&g_stage.done[b], hipEventDisableTiming));
+  }
+  g_stage.ready = true;
+  return PRH_OK;
+}
+
+// memcpy split over up to kCopyThreads threads (the calling one included)
+static void parallel_copy(void* dst, const void* src, size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  size_t nt = std::min<size_t>({static_cast<size_t>(kCopyThreads), hw, n / kCopyMinPerThread});
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t per = ((n + nt - 1) / nt + 63) / 64 * 64;
+  std::thread th[kCopyThreads];
+  size_t used = 0;
+  for (size_t t = 1; t < nt && t * per < n; t++, used++) {
+    const size_t o = t * per, len = std::min(per, n - o);
+    th[t] = std::thread([=] { std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, len); });
+  }
+  std::memcpy(dst, src, std::min(per, n));
+  for (size_t t = 1; t <= used; t++) th[t].join();
+}
+
 }  // namespace prh
 
 using namespace prh;
 
 extern "C" {
+
+int prh_memcpy_htod_staged(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return PRH_OK;
+  if (!dst || !src) return PRH_ERR_ARG;
+  const hipStream_t s = as_stream(stream);
+  std::lock_guard<std::mutex> lock(g_stage.m);
+  PRH_TRY(staging_init());
+  size_t k = 0;
+  for (size_t o = 0; o < bytes; o += kStageBytes, k++) {
+    const size_t n = std::min(kStageBytes, bytes - o);
+    const int b = static_cast<int>(k % kStages);
+    PRH_HIP(hipEventSynchronize(g_stage.done[b]));
+    parallel_copy(g_stage.buf[b], static_cast<const char*>(src) + o, n);
+    PRH_HIP(hipMemcpyAsync(static_cast<char*>(dst) + o, g_stage.buf[b], n, hipMemcpyHostToDevice, s));
+    PRH_HIP(hipEventRecord(g_stage.done[b], s));
+  }
+  return PRH_OK;
+}
+
+int prh_memcpy_dtoh_staged(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return PRH_OK;
+  if (!dst || !src) return PRH_ERR_ARG;
+  const hipStream_t s = as_stream(stream);
+  std::lock_guard<std::mutex> lock(g_stage.m);
+  PRH_TRY(staging_init());
+  const size_t nchunk = (bytes + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](size_t k) -> int {
+    const size_t o = k * kStageBytes, n = std::min(kStageBytes, bytes - o);
+    const int b = static_cast<int>(k % kStages);
+    PRH_HIP(hipEventSynchronize(g_stage.done[b]));  // an earlier upload may still read it
+    PRH_HIP(hipMemcpyAsync(g_stage.buf[b], static_cast<const char*>(src) + o, n, hipMemcpyDeviceToHost, s));
+    PRH_HIP(hipEventRecord(g_stage.done[b], s));
+    return PRH_OK;
+  };
+  for (size_t k = 0; k < nchunk && k < static_cast<size_t>(kStages); k++) PRH_TRY(issue(k));
+  for (size_t k = 0; k < nchunk; k++) {
+    const size_t o = k * kStageBytes, n = std::min(kStageBytes, bytes - o);
+    const int b = static_cast<int>(k % kStages);
+    PRH_HIP(hipEventSynchronize(g_stage.done[b]));
+    parallel_copy(static_cast<char*>(dst) + o, g_stage.buf[b], n);
+    if (k + kStages < nchunk) PRH_TRY(issue(k + kStages));
+  }
+  return PRH_OK;
+}
 
 const char* prh_last_error(void) { return get_error(); }
 

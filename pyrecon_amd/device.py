@@ -14,6 +14,11 @@ import numpy as np
 from . import _lib
 from ._lib import lib, check
 
+# host copies of at least this many bytes go through the library's pinned
+# staging buffers (prh_memcpy_*_staged): host threads fill one chunk while the
+# previous one is in flight; smaller ones are left to HIP's own staging
+STAGED_COPY_BYTES = 8 << 20
+
 
 class Stream(object):
     """A non-blocking HIP stream (``None`` means the default stream)."""
@@ -183,12 +188,14 @@ class DeviceArray(object):
         array = np.ascontiguousarray(array, dtype=self.dtype)
         if array.size != self.size:
             raise ValueError('size mismatch: {} vs {}'.format(array.size, self.size))
-        check(lib.prh_memcpy_htod(self.ptr, array.ctypes.data, self.nbytes, stream_handle(stream)), 'upload')
+        copy = lib.prh_memcpy_htod_staged if self.nbytes >= STAGED_COPY_BYTES else lib.prh_memcpy_htod
+        check(copy(self.ptr, array.ctypes.data, self.nbytes, stream_handle(stream)), 'upload')
         check(lib.prh_synchronize(stream_handle(stream)), 'upload')
 
     def numpy(self, stream=None):
         out = np.empty(self.shape, dtype=self.dtype)
-        check(lib.prh_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes, stream_handle(stream)), 'download')
+        copy = lib.prh_memcpy_dtoh_staged if self.nbytes >= STAGED_COPY_BYTES else lib.prh_memcpy_dtoh
+        check(copy(out.ctypes.data, self.ptr, self.nbytes, stream_handle(stream)), 'download')
         check(lib.prh_synchronize(stream_handle(stream)), 'download')
         return out
 
