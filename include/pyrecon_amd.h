@@ -98,6 +98,12 @@ int prh_workspace_create(prh_workspace **ws, const size_t *nmesh, int dtype, siz
                          unsigned flags);
 int prh_workspace_destroy(prh_workspace *ws);
 int prh_workspace_bytes(const prh_workspace *ws, size_t *bytes);
+/* Grows the paint scratch to chunks of paint_chunk points (no reference
+ * counterpart: the reference paints every catalog in one pass from host
+ * memory), keeping the cell and tile orders of the last two paints for their
+ * reads.  Synchronizes the device.  PRH_ERR_NOMEM leaves the workspace as it
+ * was; a smaller paint_chunk than the current one changes nothing. */
+int prh_workspace_reserve_paint(prh_workspace *ws, size_t paint_chunk);
 /* bytes a workspace with these arguments would allocate (admission control) */
 int prh_workspace_footprint(const size_t *nmesh, int dtype, size_t paint_chunk, unsigned flags,
                             size_t *bytes);

@@ -92,6 +92,7 @@ def _signatures():
         'prh_workspace_destroy': (_i, [_vp]),
         'prh_workspace_bytes': (_i, [_vp, _szp]),
         'prh_workspace_footprint': (_i, [_szp, _i, _sz, ctypes.c_uint, _szp]),
+        'prh_workspace_reserve_paint': (_i, [_vp, _sz]),
         'prh_timer_enable': (_i, [_i, _i]),
         'prh_set_tuning': (_i, [_i, _i]),
         'prh_jacobi_fast_f32': (_i, [_vp, _vp, _vp, _szp, _vp, _vp, _f, _f, _i, _vp, _i, _vp]),

@@ -482,6 +482,67 @@ int prh_workspace_destroy(prh_workspace* ws) {
   return PRH_OK;
 }
 
+int prh_workspace_reserve_paint(prh_workspace* ws, size_t paint_chunk) {
+  if (!ws || !(ws->flags & PRH_WS_PAINT)) return PRH_ERR_ARG;
+  if (paint_chunk == 0 || paint_chunk >= (size_t(1) << 31)) {
+    set_error("paint_chunk must be in [1, 2^31)");
+    return PRH_ERR_ARG;
+  }
+  PaintScratch& P = ws->paint;
+  if (paint_chunk <= P.chunk) return PRH_OK;
+  const PaintSizes z = paint_sizes(ws->n, paint_chunk);
+  const size_t w = static_cast<size_t>(ws->dtype), c0 = P.chunk, c1 = paint_chunk;
+  // every array whose size follows the chunk: (slot, old bytes, new bytes, old bytes kept).
+  // The orders and tile-order copies of the last two paints are carried over.
+  struct Slot {
+    void** p;
+    size_t old_bytes, new_bytes, keep;
+  };
+  const Slot slots[] = {
+      {reinterpret_cast<void**>(&P.partial), (P.npartial + 1) * 4, (z.npartial + 1) * 4, 0},
+      {reinterpret_cast<void**>(&P.key), c0 * 4, c1 * 4, 0},
+      {reinterpret_cast<void**>(&P.rank), c0 * 4, c1 * 4, 0},
+      {reinterpret_cast<void**>(&P.vtmp), c0 * 4, c1 * 4, 0},
+      {reinterpret_cast<void**>(&P.hist), P.nhist * 4, z.nhist * 4, 0},
+      {reinterpret_cast<void**>(&P.goff), (P.nhist + 1) * 4, (z.nhist + 1) * 4, 0},
+      {reinterpret_cast<void**>(&P.sorted), c0 * 4, c1 * 4, c0 * 4},
+      {reinterpret_cast<void**>(&P.sorted_prev), c0 * 4, c1 * 4, c0 * 4},
+      {&P.payload, c0 * 4 * w, c1 * 4 * w, 0},
+      {&P.spay, c0 * 4 * w, c1 * 4 * w, 0},
+      {&P.spos, c0 * 3 * sizeof(double), c1 * 3 * sizeof(double), c0 * 3 * sizeof(double)},
+      {&P.spos_prev, c0 * 3 * sizeof(double), c1 * 3 * sizeof(double), c0 * 3 * sizeof(double)},
+      {reinterpret_cast<void**>(&P.hpart), P.nhpart * sizeof(uint64_t), z.nhpart * sizeof(uint64_t), 0},
+  };
+  constexpr int ns = sizeof(slots) / sizeof(slots[0]);
+  // queued work may still use the old arrays
+  PRH_HIP(hipDeviceSynchronize());
+  void* fresh[ns] = {};
+  for (int i = 0; i < ns; i++) {
+    hipError_t e = hipMalloc(&fresh[i], slots[i].new_bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      for (int j = 0; j < i; j++) (void)hipFree(fresh[j]);
+      set_error("hipMalloc of %zu bytes for a paint chunk of %zu points failed: %s", slots[i].new_bytes,
+                paint_chunk, hipGetErrorString(e));
+      return PRH_ERR_NOMEM;  // the workspace is unchanged
+    }
+  }
+  for (int i = 0; i < ns; i++)
+    if (slots[i].keep && *slots[i].p)
+      PRH_HIP(hipMemcpy(fresh[i], *slots[i].p, slots[i].keep, hipMemcpyDeviceToDevice));
+  for (int i = 0; i < ns; i++) {
+    if (*slots[i].p) PRH_HIP(hipFree(*slots[i].p));
+    *slots[i].p = fresh[i];
+    ws->bytes += slots[i].new_bytes;
+    ws->bytes -= slots[i].old_bytes;
+  }
+  P.chunk = paint_chunk;
+  P.npartial = z.npartial;
+  P.nhist = z.nhist;
+  P.nhpart = z.nhpart;
+  return PRH_OK;
+}
+
 int prh_workspace_create(prh_workspace** out, const size_t* nmesh, int dtype, size_t paint_chunk,
                          unsigned flags) {
   if (!out || !nmesh) return PRH_ERR_ARG;

@@ -17,9 +17,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import lib, check, MeshError
-from .device import DeviceArray, as_device, stream_handle
+from .device import DeviceArray, as_device, stream_handle, mem_info, empty_cache
 
 DEFAULT_PAINT_CHUNK = 1 << 24
+# a workspace created with paint_chunk=None grows its paint scratch up to this many points
+MAX_AUTO_PAINT_CHUNK = 1 << 28
 
 
 def _make_array(value, shape, dtype='f8'):
@@ -145,13 +147,21 @@ class MeshInfo(object):
 
 
 class Workspace(object):
-    """Owner of a ``prh_workspace``: multigrid hierarchy, paint scratch, FFT plans."""
+    """Owner of a ``prh_workspace``: multigrid hierarchy, paint scratch, FFT plans.
 
-    def __init__(self, nmesh, dtype, paint_chunk=DEFAULT_PAINT_CHUNK, flags=_lib.WS_ALL):
+    ``paint_chunk`` is the number of points painted per pass.  ``None`` starts at
+    ``DEFAULT_PAINT_CHUNK`` and grows the scratch for larger catalogs (``reserve``), up to
+    ``MAX_AUTO_PAINT_CHUNK`` points and while device memory allows, so that a catalog is painted
+    in one pass (and its read can walk the order that pass left); an integer fixes it.
+    ``grow`` overrides that choice.
+    """
+
+    def __init__(self, nmesh, dtype, paint_chunk=None, flags=_lib.WS_ALL, grow=None):
         import ctypes
         self.nmesh = tuple(int(n) for n in nmesh)
         self.dtype = np.dtype(dtype)
-        self.paint_chunk = int(paint_chunk)
+        self.grow = (paint_chunk is None) if grow is None else bool(grow)
+        self.paint_chunk = int(DEFAULT_PAINT_CHUNK if paint_chunk is None else paint_chunk)
         self.flags = int(flags)
         if self.dtype == np.float32: self.flags &= ~_lib.WS_MIXED
         handle = ctypes.c_void_p()
@@ -163,10 +173,45 @@ class Workspace(object):
         check(lib.prh_workspace_bytes(self.handle, ctypes.byref(nbytes)), 'workspace bytes')
         self.nbytes = nbytes.value
 
+    def _update_bytes(self):
+        import ctypes
+        nbytes = ctypes.c_size_t()
+        check(lib.prh_workspace_bytes(self.handle, ctypes.byref(nbytes)), 'workspace bytes')
+        self.nbytes = nbytes.value
+
+    def reserve(self, npoints):
+        """Grow the paint scratch (if this workspace grows) so that ``npoints`` points are painted
+        in one pass.  Returns whether they are; if not, the paint takes several passes."""
+        npoints = int(npoints)
+        if npoints <= self.paint_chunk: return True
+        if not self.grow or not (self.flags & _lib.WS_PAINT) or npoints > MAX_AUTO_PAINT_CHUNK: return False
+        step = 1 << 22
+        chunk = min(-(-npoints // step) * step, MAX_AUTO_PAINT_CHUNK)
+        # the new arrays are allocated before the old ones are freed; keep a margin for the meshes
+        need = Workspace.footprint(self.nmesh, self.dtype, chunk, self.flags) - \
+            Workspace.footprint(self.nmesh, self.dtype, 1, self.flags)
+        free, total = mem_info()
+        margin = max(2 << 30, total // 20)
+        if free < need + margin:
+            empty_cache()
+            free, total = mem_info()
+        if free < need + margin:
+            self.grow = False
+            return False
+        status = lib.prh_workspace_reserve_paint(self.handle, chunk)
+        if status == _lib.ERR_NOMEM:
+            self.grow = False
+            return False
+        check(status, 'paint scratch reserve')
+        self.paint_chunk = chunk
+        self._update_bytes()
+        return True
+
     @staticmethod
     def footprint(nmesh, dtype, paint_chunk=DEFAULT_PAINT_CHUNK, flags=_lib.WS_ALL):
         import ctypes
         dtype = np.dtype(dtype)
+        if paint_chunk is None: paint_chunk = DEFAULT_PAINT_CHUNK
         if dtype == np.float32: flags &= ~_lib.WS_MIXED
         nm, nm_ptr = _lib.size3(nmesh)
         out = ctypes.c_size_t()
@@ -190,8 +235,8 @@ class Workspace(object):
 _workspace_cache = {}
 
 
-def shared_workspace(nmesh, dtype, flags=_lib.WS_ALL, paint_chunk=DEFAULT_PAINT_CHUNK):
-    key = (tuple(int(n) for n in nmesh), np.dtype(dtype).str, int(flags), int(paint_chunk))
+def shared_workspace(nmesh, dtype, flags=_lib.WS_ALL, paint_chunk=None):
+    key = (tuple(int(n) for n in nmesh), np.dtype(dtype).str, int(flags), paint_chunk)
     ws = _workspace_cache.get(key)
     if ws is None:
         ws = _workspace_cache[key] = Workspace(nmesh, dtype, paint_chunk=paint_chunk, flags=flags)
@@ -388,6 +433,7 @@ class RealMesh(object):
                 raise ValueError('weights must have shape (N,)')
             wcode = _lib.dtype_code(wdev.dtype)
         self._allocate(zero=True)
+        self.workspace.reserve(len(pos))  # one pass for the whole catalog, where the workspace grows
         nm, nm_ptr = _lib.size3(self.shape)
         bs, bs_ptr = _lib.vec3(self.boxsize, self.dtype)
         bc, bc_ptr = _lib.vec3(self.boxcenter, self.dtype)

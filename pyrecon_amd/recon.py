@@ -13,7 +13,7 @@ import numpy as np
 from . import _lib
 from ._lib import lib, check, MeshError
 from .device import DeviceArray, as_device, stream_handle
-from .mesh import MeshInfo, RealMesh, Workspace, DEFAULT_PAINT_CHUNK
+from .mesh import MeshInfo, RealMesh, Workspace
 
 
 def numpy_mean(rtype, total, count):
@@ -49,8 +49,10 @@ class BaseReconstruction(object):
         reference is deterministic C) or 'mixed' (FP64 meshes: CIC weights summed in fixed point per tile
         and in FP64 across tiles, and the mixed-precision solve of ``run``).
     kwargs : MeshInfo arguments (nmesh, boxsize, boxcenter, cellsize, positions, boxpad, dtype);
-        ``nthreads`` is accepted and ignored.  Device-specific: ``stream``, ``paint_chunk``,
-        ``workspace`` (a :class:`~pyrecon_amd.mesh.Workspace` to share).
+        ``nthreads`` is accepted and ignored.  Device-specific: ``stream``, ``paint_chunk`` (points
+        painted per pass; ``None`` grows the scratch so that each catalog is painted in one pass,
+        see :class:`~pyrecon_amd.mesh.Workspace`), ``workspace`` (a
+        :class:`~pyrecon_amd.mesh.Workspace` to share).
     """
 
     @staticmethod
@@ -58,7 +60,7 @@ class BaseReconstruction(object):
         return nmesh.copy()
 
     def __init__(self, f=0., bias=1., los=None, fft_engine='numpy', fft_wisdom=None, save_fft_wisdom=None,
-                 fft_plan='measure', wrap=False, stream=None, paint_chunk=DEFAULT_PAINT_CHUNK, workspace=None,
+                 fft_plan='measure', wrap=False, stream=None, paint_chunk=None, workspace=None,
                  precision='plain', **kwargs):
         self.logger = logging.getLogger(self.__class__.__name__)
         self.set_cosmo(f=f, bias=bias)
