@@ -105,12 +105,15 @@ class _BlockCache(object):
 
 
 _cache = _BlockCache()
+# other holders of idle device memory (idle workspaces), released by empty_cache()
+_release_hooks = []
 
 
 def empty_cache():
-    """Return every cached device block to the HIP runtime."""
+    """Return every cached device block, and every idle workspace, to the HIP runtime."""
     synchronize()
     _cache.empty()
+    for hook in _release_hooks: hook()
 
 
 class DeviceArray(object):
@@ -137,8 +140,8 @@ class DeviceArray(object):
             if cached is None:
                 p = ctypes.c_void_p()
                 status = lib.prh_malloc(ctypes.byref(p), self._block)
-                if status == _lib.ERR_NOMEM and _cache.cached:
-                    _cache.empty()
+                if status == _lib.ERR_NOMEM:
+                    empty_cache()
                     status = lib.prh_malloc(ctypes.byref(p), self._block)
                 check(status, 'device allocation of {} bytes'.format(self._block))
                 cached = p.value or 0

@@ -9,14 +9,15 @@ the recommended-size warning, ``offset``, ``cellsize`` and ``wrap``.
 copy (and assigning ``.value`` uploads), so code written against the
 reference's numpy-backed RealMesh keeps working.
 """
-import warnings
-
 import numbers
+import threading
+import warnings
 
 import numpy as np
 
 from . import _lib
 from ._lib import lib, check, MeshError
+from . import device as _device
 from .device import DeviceArray, as_device, stream_handle, mem_info, empty_cache
 
 DEFAULT_PAINT_CHUNK = 1 << 24
@@ -146,6 +147,24 @@ class MeshInfo(object):
                                                                                self.boxcenter)
 
 
+# Workspaces marked ``reuse`` that no object references any more, by kind (one each, the last
+# _MAX_IDLE kinds): a new workspace of the same kind takes one instead of allocating and setting
+# up its own.
+_idle = {}
+_idle_lock = threading.Lock()
+_MAX_IDLE = 2
+
+
+def _drop_idle():
+    with _idle_lock:
+        handles = [entry[0] for entry in _idle.values()]
+        _idle.clear()
+    for handle in handles: lib.prh_workspace_destroy(handle)
+
+
+_device._release_hooks.append(_drop_idle)
+
+
 class Workspace(object):
     """Owner of a ``prh_workspace``: multigrid hierarchy, paint scratch, FFT plans.
 
@@ -154,9 +173,13 @@ class Workspace(object):
     ``MAX_AUTO_PAINT_CHUNK`` points and while device memory allows, so that a catalog is painted
     in one pass (and its read can walk the order that pass left); an integer fixes it.
     ``grow`` overrides that choice.
+
+    ``reuse``: for a workspace used on the default stream only.  When no object references it
+    any more it is kept (one per kind) for the next workspace of the same kind, instead of being
+    destroyed; ``empty_cache()`` destroys the kept ones.
     """
 
-    def __init__(self, nmesh, dtype, paint_chunk=None, flags=_lib.WS_ALL, grow=None):
+    def __init__(self, nmesh, dtype, paint_chunk=None, flags=_lib.WS_ALL, grow=None, reuse=False):
         import ctypes
         self.nmesh = tuple(int(n) for n in nmesh)
         self.dtype = np.dtype(dtype)
@@ -164,10 +187,22 @@ class Workspace(object):
         self.paint_chunk = int(DEFAULT_PAINT_CHUNK if paint_chunk is None else paint_chunk)
         self.flags = int(flags)
         if self.dtype == np.float32: self.flags &= ~_lib.WS_MIXED
+        self._reuse_key = (self.nmesh, self.dtype.str, self.flags, paint_chunk, self.grow) if reuse else None
+        if self._reuse_key is not None:
+            with _idle_lock:
+                entry = _idle.pop(self._reuse_key, None)
+            if entry is not None:
+                self.handle, self.nbytes, self.paint_chunk, self.grow = entry
+                return
         handle = ctypes.c_void_p()
         nm, nm_ptr = _lib.size3(self.nmesh)
-        check(lib.prh_workspace_create(ctypes.byref(handle), nm_ptr, _lib.dtype_code(self.dtype), self.paint_chunk,
-                                       self.flags), 'workspace create')
+        create = lambda: lib.prh_workspace_create(ctypes.byref(handle), nm_ptr, _lib.dtype_code(self.dtype),
+                                                  self.paint_chunk, self.flags)
+        status = create()
+        if status == _lib.ERR_NOMEM:  # cached blocks or kept workspaces may hold the memory
+            empty_cache()
+            status = create()
+        check(status, 'workspace create')
         self.handle = handle.value
         nbytes = ctypes.c_size_t()
         check(lib.prh_workspace_bytes(self.handle, ctypes.byref(nbytes)), 'workspace bytes')
@@ -227,6 +262,18 @@ class Workspace(object):
 
     def __del__(self):
         try:
+            handle = getattr(self, 'handle', None)
+            if handle and getattr(self, '_reuse_key', None) is not None:
+                lib.prh_synchronize(None)  # its work on the default stream has finished
+                evicted = []
+                with _idle_lock:
+                    if self._reuse_key not in _idle:
+                        _idle[self._reuse_key] = (handle, self.nbytes, self.paint_chunk, self.grow)
+                        self.handle = None
+                        while len(_idle) > _MAX_IDLE:
+                            evicted.append(_idle.pop(next(iter(_idle)))[0])
+                for old in evicted: lib.prh_workspace_destroy(old)
+                if self.handle is None: return
             self.close()
         except Exception:
             pass

@@ -74,7 +74,8 @@ class BaseReconstruction(object):
             raise ValueError('precision="mixed" needs an FP64 mesh (dtype="f8")')
         self.precision = precision
         if workspace is None:
-            workspace = Workspace(self.info.shape, self.info.dtype, paint_chunk=paint_chunk)
+            # on the default stream, the workspace of a released reconstruction of the same kind
+            workspace = Workspace(self.info.shape, self.info.dtype, paint_chunk=paint_chunk, reuse=stream is None)
         elif tuple(workspace.nmesh) != self.info.shape or workspace.dtype != self.info.dtype:
             raise ValueError('workspace does not match the mesh')
         self.workspace = workspace

@@ -1,8 +1,12 @@
-"""Growth of the paint scratch (Workspace.reserve, prh_workspace_reserve_paint): a workspace that
+"""Workspaces of the drop-in classes.
+
+Growth of the paint scratch (Workspace.reserve, prh_workspace_reserve_paint): a workspace that
 starts with chunks of 1000 points and grows when the randoms (4000 points) are painted gives the
 bits of one sized for both catalogs from the start -- meshes, potential and the shifted positions of
 both catalogs, the data's read walking the paint order (and, mixed, the tile-order copy) that the
-growth carried over.  A workspace with a fixed chunk does not grow and paints in several passes."""
+growth carried over.  A workspace with a fixed chunk does not grow and paints in several passes.
+A released reconstruction's workspace is taken by the next one of the same kind, with the results
+of a fresh workspace."""
 import numpy as np
 import pytest
 
@@ -60,3 +64,39 @@ def test_fixed_chunk_does_not_grow():
     ref.assign_randoms(randoms)
     assert ws.paint_chunk == 1000
     assert np.array_equal(bits(rec.mesh_randoms.value), bits(ref.mesh_randoms.value))
+
+
+def test_released_workspace_reused():
+    # a reconstruction on the default stream that is no longer referenced leaves its workspace for
+    # the next one of the same kind (scratch only: the results are those of a fresh workspace)
+    import gc
+    from pyrecon_amd import MultiGridReconstruction, empty_cache
+    from pyrecon_amd.mesh import Workspace
+    data, wdata, randoms = _catalogs()
+    kwargs = dict(f=0.8, bias=2., nmesh=32, boxsize=900., boxcenter=(0., 0., 1000.), precision='mixed')
+
+    def reconstruct(rec):
+        rec.assign_data(data, wdata)
+        rec.assign_randoms(randoms)
+        rec.set_density_contrast(smoothing_radius=15.)
+        rec.run()
+        return rec.read_shifted_positions(data), rec.read_shifted_positions(randoms)
+
+    empty_cache()
+    first = MultiGridReconstruction(**kwargs)
+    handle = first.workspace.handle
+    out_first = reconstruct(first)
+    del first
+    gc.collect()
+    second = MultiGridReconstruction(**kwargs)
+    assert second.workspace.handle == handle
+    out_second = reconstruct(second)
+    fresh = MultiGridReconstruction(workspace=Workspace((32, 32, 32), 'f8'), **kwargs)
+    out_fresh = reconstruct(fresh)
+    for a, b, c in zip(out_first, out_second, out_fresh):
+        assert np.array_equal(bits(a), bits(b)) and np.array_equal(bits(a), bits(c))
+    del second
+    gc.collect()
+    empty_cache()  # destroys the kept workspace
+    third = MultiGridReconstruction(**kwargs)
+    assert third.workspace.handle is not None
