@@ -529,11 +529,12 @@ def main():
 
     host_io = None
     if args.host_io_steps > 0:
-        # end to end from numpy: catalogs uploaded and shifted positions returned inside each step
+        # end to end from numpy, as a drop-in user calls it: a new reconstruction object per step with
+        # the default workspace settings, catalogs uploaded and shifted positions returned inside each step
         from catalogs import RECON_KWARGS, SMOOTHING_RADIUS
 
         def host_step():
-            recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, workspace=ws, precision='mixed')
+            recon = P.MultiGridReconstruction(**RECON_KWARGS, **geo, precision='mixed')
             recon.assign_data(cat['data'], cat['data_weights'])
             recon.assign_randoms(cat['randoms'], cat['randoms_weights'])
             recon.set_density_contrast(smoothing_radius=SMOOTHING_RADIUS)
@@ -548,9 +549,10 @@ def main():
         th = (time.perf_counter() - t0) / args.host_io_steps
         host_io = dict(value=round(1. / th, 4), unit='recon/s', ms_per_step=round(1e3 * th, 2),
                        steps=args.host_io_steps, precision='mixed',
-                       note='catalogs uploaded from numpy (pageable host memory, copied through the library\'s pinned '
-                            'staging buffers) and shifted positions of data '
-                            'and randoms returned to numpy inside each step: PCIe-inclusive, not the headline value')
+                       note='a new MultiGridReconstruction per step with default settings (its workspace kept '
+                            'between steps), catalogs uploaded from numpy (pageable host memory, copied through the '
+                            'library\'s pinned staging buffers) and shifted positions of data and randoms returned '
+                            'to numpy inside each step: PCIe-inclusive, not the headline value')
         log('host io', host_io)
 
     accuracy = None
