@@ -230,13 +230,18 @@ static void parallel_copy(void* dst, const void* src, size_t n) {
   }
   const size_t per = ((n + nt - 1) / nt + 63) / 64 * 64;
   std::thread th[kCopyThreads];
-  size_t used = 0;
-  for (size_t t = 1; t < nt && t * per < n; t++, used++) {
+  for (size_t t = 1; t < nt && t * per < n; t++) {
     const size_t o = t * per, len = std::min(per, n - o);
-    th[t] = std::thread([=] { std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, len); });
+    auto part = [=] { std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, len); };
+    try {
+      th[t] = std::thread(part);
+    } catch (...) {  // no thread to be had: this part on the calling thread
+      part();
+    }
   }
   std::memcpy(dst, src, std::min(per, n));
-  for (size_t t = 1; t <= used; t++) th[t].join();
+  for (size_t t = 1; t < static_cast<size_t>(kCopyThreads); t++)
+    if (th[t].joinable()) th[t].join();
 }
 
 }  // namespace prh
