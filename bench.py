@@ -60,6 +60,8 @@ def parse_args():
     p.add_argument('--serving-only', action='store_true', help='run only the serving stream')
     p.add_argument('--host-io-steps', type=int, default=2,
                    help='timed mixed steps with numpy inputs and outputs (PCIe-inclusive rate; 0: skip)')
+    p.add_argument('--fft-recon', action='store_true',
+                   help='the FFT reconstructions (iterative, plane-parallel, iterative particle) on the workload')
     p.add_argument('--catalog-cache', default=None,
                    help='directory where the generated catalogs are saved and, on later runs, loaded from')
     return p.parse_args()
@@ -442,6 +444,128 @@ def cpu_baseline(cat, threads, full=False):
     return out
 
 
+FFT_METRIC = 'recon/s at nmesh=512, IterativeFFTReconstruction (FP64, 1 GPU)'
+
+
+def fft_cpu_baseline(cat, threads, transforms):
+    """The reference's path for the FFT reconstructions on bounded samples: its C paint (oracle/_ref)
+    of 4e6 randoms scaled to every point, numpy's c2c transform of the 512^3 mesh (one, timed) times
+    the transforms of one step (the smoothing's four and those of the algorithm), and its shifts (the
+    three CIC reads of oracle/_ref plus the numpy algebra) of 4e6 randoms scaled."""
+    from oracle.cpu import RefKernels, OracleKernels
+    from oracle.recon_numpy import OracleIterativeFFTReconstruction
+    kind = 'reference'
+    try:
+        kern = RefKernels()
+    except OSError:
+        kind, kern = 'port', OracleKernels()
+    os.environ['OMP_NUM_THREADS'] = str(threads)
+    if kind == 'reference': kern.set_num_threads(threads)
+    cfg = cat['config']
+    n = cfg['nmesh']
+    recon = OracleIterativeFFTReconstruction(f=0.8, bias=2., los=None, nmesh=n, boxsize=cfg['boxsize'],
+                                             boxcenter=cfg['boxcenter'], dtype='f8', kernels=kern)
+    ntot = len(cat['data']) + len(cat['randoms'])
+    ns = min(4000000, len(cat['randoms']))
+    stages = {}
+    t = time.perf_counter()
+    recon.assign_randoms(cat['randoms'][:ns], cat['randoms_weights'][:ns])
+    stages['paint'] = (time.perf_counter() - t) * ntot / ns
+    t = time.perf_counter()
+    spec = np.fft.fftn(recon.mesh_randoms)
+    t_fft = time.perf_counter() - t
+    del spec
+    stages['transforms'] = (4 + transforms) * t_fft
+    rng = np.random.default_rng(1)
+    recon.mesh_psi = [rng.standard_normal((n, n, n)) for _ in range(3)]
+    t = time.perf_counter()
+    recon.read_shifted_positions(cat['randoms'][:ns])
+    stages['shifts'] = (time.perf_counter() - t) * ntot / ns
+    usable, physical = host_cores()
+    return dict(value=1. / sum(stages.values()), unit='recon/s', cores=threads, kind=kind,
+                host_cores=dict(usable=usable, physical=physical), stages_s={k: round(v, 3) for k, v in stages.items()},
+                sample=('one IterativeFFTReconstruction step of the same workload on samples: paint and shifts (three '
+                        'CIC reads + numpy algebra) of {:d} randoms scaled to {:d} points, one {n}^3 numpy c2c '
+                        'transform x {:d} (4 for the smoothing, {:d} for the algorithm); reference src/*.c via '
+                        'oracle/_ref with {:d} OpenMP threads, numpy single-threaded').format(
+                            ns, ntot, 4 + transforms, transforms, threads, n=n))
+
+
+def run_fft(args):
+    """The FFT reconstructions on the workload's catalogs (FP64 meshes): IterativeFFTReconstruction
+    (niterations=3, the value), PlaneParallelFFTReconstruction (global line of sight along z) and
+    IterativeFFTParticleReconstruction (niterations=3).  A step: paint, smoothing and density contrast,
+    run, and the shifted positions of data and randoms.  Their shifts take and return host arrays
+    (the particle variant also keeps the data on the host), so a step includes those transfers; the
+    paint reads the device-resident catalogs.  The roofline is that of the 3-D transform passes."""
+    import pyrecon_amd as P
+    from pyrecon_amd import _lib
+    from pyrecon_amd.device import device_name
+    from catalogs import make_catalogs
+    log('device', device_name())
+    cat = load_catalogs(args, make_catalogs)
+    cfg = cat['config']
+    nd, nr = len(cat['data']), len(cat['randoms'])
+    d, dw = P.DeviceArray.from_numpy(cat['data']), P.DeviceArray.from_numpy(cat['data_weights'])
+    r, rw = P.DeviceArray.from_numpy(cat['randoms']), P.DeviceArray.from_numpy(cat['randoms_weights'])
+    geo = dict(nmesh=cfg['nmesh'], boxsize=cfg['boxsize'], boxcenter=cfg['boxcenter'], dtype='f8')
+    steps = max(1, min(args.steps, 3))
+    algorithms = [('IterativeFFTReconstruction', P.IterativeFFTReconstruction, {}, dict(niterations=3), True),
+                  ('PlaneParallelFFTReconstruction', P.PlaneParallelFFTReconstruction, dict(los='z'), {}, True),
+                  ('IterativeFFTParticleReconstruction', P.IterativeFFTParticleReconstruction, {},
+                   dict(niterations=3), False)]
+    results = {}
+    for name, cls, kwargs, run_kwargs, device_data in algorithms:
+        def step():
+            rec = cls(f=0.8, bias=2., **geo, **kwargs)
+            if device_data:
+                rec.assign_data(d, dw)
+            else:
+                rec.assign_data(cat['data'], cat['data_weights'])
+            rec.assign_randoms(r, rw)
+            rec.set_density_contrast(smoothing_radius=15.)
+            rec.run(**run_kwargs)
+            return rec.read_shifted_positions(cat['data']), rec.read_shifted_positions(cat['randoms'])
+
+        step()
+        P.synchronize()
+        with _lib.KernelTimer(_lib.TIMER_FFT_PASS) as kt:
+            t0 = time.perf_counter()
+            for _ in range(steps): step()
+            P.synchronize()
+            elapsed = time.perf_counter() - t0
+            ms, nl, nb = kt.read()
+        results[name] = dict(value=round(steps / elapsed, 4), unit='recon/s', ms_per_step=round(1e3 * elapsed / steps, 2),
+                             fft_passes_per_step=nl // steps, fft_ms_per_step=round(ms / steps, 2),
+                             fft_pass_avg_ms=round(ms / max(nl, 1), 4),
+                             fft_pass_gbs=round(nb / (ms * 1e-3) / 1e9, 1) if ms else None)
+        log(name, results[name])
+    main = results['IterativeFFTReconstruction']
+    achieved = main['fft_pass_gbs']
+    roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s',
+                    frac=round(achieved / HBM_PEAK_GBS, 4) if achieved else None, traffic=None,
+                    kernel='k_fft_axis: one axis pass of a 3-D c2c transform, 512^3 FP64, LDS Stockham passes; '
+                           'achieved = algorithmic bytes (4 n^3 x 8 B per complex pass, 3 n^3 x 8 B for the real-in '
+                           'and real-out passes) / average pass time (HIP events) over the IterativeFFTReconstruction '
+                           'steps', avg_launch_ms=main['fft_pass_avg_ms'])
+    cpu = None
+    if not args.no_cpu_baseline:
+        ntrans = main['fft_passes_per_step'] // 3
+        log('cpu baseline ({:d} transforms per step)'.format(ntrans))
+        cpu = fft_cpu_baseline(cat, threads=args.workers, transforms=ntrans)
+        log('cpu baseline', cpu)
+    line = dict(metric=FFT_METRIC, value=main['value'], unit='recon/s', n_gpus=1, steps=steps, warmup=1,
+                ms_per_step=main['ms_per_step'], higher_is_better=True, scaling='weak', vs_baseline=None,
+                dtype='f64', data='synthetic',
+                config=dict(workload='configuration 3 catalogs ({:d} data + {:d} randoms, nmesh=512, boxsize=4000, '
+                                     'boxcenter=(0,0,3000)), f=0.8, bias=2, R=15; IterativeFFTReconstruction '
+                                     'niterations=3, local LOS'.format(nd, nr),
+                            nmesh=cfg['nmesh'], ndata=nd, nrandoms=nr, inputs='catalogs device-resident for the '
+                            'paint; shifts from and to host arrays'),
+                roofline=roofline, cpu_baseline=cpu, algorithms=results)
+    print(json.dumps(line), flush=True)
+
+
 def load_catalogs(args, make_catalogs):
     """The seeded catalogs of the configuration, generated or read back from --catalog-cache."""
     keys = ['data', 'data_weights', 'randoms', 'randoms_weights']
@@ -482,6 +606,8 @@ def main():
         return
     if args.config == 'c4':
         return run_large(args)
+    if args.fft_recon:
+        return run_fft(args)
     import pyrecon_amd as P
     from pyrecon_amd import _lib
     from pyrecon_amd.device import device_name

@@ -383,6 +383,18 @@ int prh_read_shifts_f64(prh_workspace *ws, const double *phi, const size_t *nmes
                         const double *boxcenter, const void *positions, int pos_dtype,
                         size_t npositions, double f, const double *los, int field, int wrap,
                         int shifted, void *out, int out_dtype, int *nonfinite, void *stream);
+/* The shift algebra of the FFT reconstructions (read_shifts / read_shifted_positions,
+ * pyrecon/recon.py:255-325, without wrap), after the three displacement meshes have been read
+ * by CIC at the positions (prh_read_shifts_* with PRH_FIELD_VALUE): s0, s1, s2 hold the
+ * components in the positions dtype.  field as above (not PRH_FIELD_VALUE); los in the mesh
+ * dtype, or NULL for the local line of sight; shifted returns positions - shifts; out (N, 3) in
+ * numpy's promotion of the reference's expressions. */
+int prh_shift_algebra_f32(const void *positions, int pos_dtype, const void *s0, const void *s1, const void *s2,
+                          size_t npositions, double f, const float *los, int field, int shifted, void *out,
+                          int out_dtype, void *stream);
+int prh_shift_algebra_f64(const void *positions, int pos_dtype, const void *s0, const void *s1, const void *s2,
+                          size_t npositions, double f, const double *los, int field, int shifted, void *out,
+                          int out_dtype, void *stream);
 
 /* Mixed-precision solve (no reference counterpart; same contract as
  * prh_fmg_f64, which it tracks).  The coarse stages of the full multigrid
@@ -411,7 +423,8 @@ int prh_fmg_mixed(prh_workspace *ws, const double *f1h, double *v1h, const size_
 #define PRH_TIMER_PAINT_GATHER 1  /* CIC paint merge-gather kernel */
 #define PRH_TIMER_READ_SHIFTS 2   /* fused finite-difference read + shift kernel */
 #define PRH_TIMER_PAIR_FINEST 3   /* two fused FP32 sweeps on the finest level (mixed path) */
-#define PRH_TIMER_COUNT 4
+#define PRH_TIMER_FFT_PASS 4      /* one axis pass of a 3-D transform (prh_fft3d_*) */
+#define PRH_TIMER_COUNT 5
 int prh_timer_enable(int which, int enable);
 int prh_timer_reset(int which);
 int prh_timer_read(int which, double *total_ms, long long *launches, double *bytes);

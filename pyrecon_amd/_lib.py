@@ -28,7 +28,7 @@ ERR_STATE = -8
 
 F32, F64 = 4, 8
 FIELD_DISP, FIELD_RSD, FIELD_DISP_RSD, FIELD_VALUE = 0, 1, 2, 3
-TIMER_JACOBI_FINEST, TIMER_PAINT_GATHER, TIMER_READ_SHIFTS, TIMER_PAIR_FINEST = 0, 1, 2, 3
+TIMER_JACOBI_FINEST, TIMER_PAINT_GATHER, TIMER_READ_SHIFTS, TIMER_PAIR_FINEST, TIMER_FFT_PASS = 0, 1, 2, 3, 4
 TUNE_PAIRED_SWEEPS = 0
 FFT_COMPLEX, FFT_REAL_IN, FFT_REAL_OUT = 0, 1, 2
 SPEC_PLANE_PARALLEL, SPEC_INV_K2, SPEC_LOS_SQ, SPEC_KA_KB, SPEC_GRAD, SPEC_I_KA = 0, 1, 2, 3, 4, 5
@@ -93,6 +93,8 @@ def _signatures():
         'prh_workspace_bytes': (_i, [_vp, _szp]),
         'prh_workspace_footprint': (_i, [_szp, _i, _sz, ctypes.c_uint, _szp]),
         'prh_workspace_reserve_paint': (_i, [_vp, _sz]),
+        'prh_shift_algebra_f32': (_i, [_vp, _i, _vp, _vp, _vp, _sz, _d, _vp, _i, _i, _vp, _i, _vp]),
+        'prh_shift_algebra_f64': (_i, [_vp, _i, _vp, _vp, _vp, _sz, _d, _vp, _i, _i, _vp, _i, _vp]),
         'prh_timer_enable': (_i, [_i, _i]),
         'prh_set_tuning': (_i, [_i, _i]),
         'prh_jacobi_fast_f32': (_i, [_vp, _vp, _vp, _szp, _vp, _vp, _f, _f, _i, _vp, _i, _vp]),

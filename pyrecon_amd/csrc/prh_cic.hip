@@ -1360,24 +1360,14 @@ constexpr int kRT = 8;  // read binning tile edge (cells)
 // SPEC: the shifted positions with the displacement and RSD terms along the
 // local line of sight, without wrap (RecSym's read, the common case), fixed at
 // compile time so that the registers of the other cases are not held.
-template <typename R, typename P, typename O, typename M, bool SPEC = false>
-__device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& mesh,
-                                               const ReadArgs<R>& A, O* __restrict__ out) {
+// The shifts (or shifted positions) of point i from its displacement s, with numpy's dtypes and
+// order (recon.py:255-325): s and the local line of sight x/|x| in P, a global line of sight's
+// RSD term in Q = promote(P, R); written at out[3 i .. 3 i + 2].
+template <typename R, typename P, typename O>
+__device__ __forceinline__ void shift_out(size_t i, const P x[3], const P s[3], const ReadArgs<R>& A, int field,
+                                          int shifted, int local, int wrap, O* __restrict__ out) {
   using Q = promote_t<P, R>;
   const Frame<R>& F = A.F;
-  const int wrap = SPEC ? 0 : A.wrap, local = SPEC ? 1 : A.local;
-  uint32_t err = 0;
-  R p[3];
-  to_mesh_units<R, P>(x, F, wrap, p, err);
-  if (!SPEC && A.field == PRH_FIELD_VALUE) {
-    // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
-    out[i] = static_cast<O>(static_cast<P>(mesh.value(p)));
-    return err;
-  }
-  R g[3];
-  mesh.gradient(A.cell, p, g);
-  P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
-  const int field = SPEC ? PRH_FIELD_DISP_RSD : A.field, shifted = SPEC ? 1 : A.shifted;
   // shifts in P (disp, disp+rsd, local rsd) or in Q (global rsd)
   O res[3];
   if (field == PRH_FIELD_DISP) {
@@ -1387,7 +1377,8 @@ __device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& 
   } else if (local) {
     const P dist = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
     const P l[3] = {x[0] / dist, x[1] / dist, x[2] / dist};
-    const P dot = s[0] * l[0] + s[1] * l[1] + s[2] * l[2];
+    // numpy's sum along the last axis starts from +0 (a sum of three -0 is +0)
+    const P dot = ((P(0) + s[0] * l[0]) + s[1] * l[1]) + s[2] * l[2];
     const P fd = static_cast<P>(A.fgrowth) * dot;
     P sh[3];
     for (int d = 0; d < 3; d++) {
@@ -1398,7 +1389,7 @@ __device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& 
       res[d] = static_cast<O>(shifted ? static_cast<P>(x[d] - sh[d]) : sh[d]);
   } else {
     const Q l[3] = {static_cast<Q>(A.los[0]), static_cast<Q>(A.los[1]), static_cast<Q>(A.los[2])};
-    const Q dot = static_cast<Q>(s[0]) * l[0] + static_cast<Q>(s[1]) * l[1] +
+    const Q dot = ((Q(0) + static_cast<Q>(s[0]) * l[0]) + static_cast<Q>(s[1]) * l[1]) +
                   static_cast<Q>(s[2]) * l[2];
     const Q fd = static_cast<Q>(A.fgrowth) * dot;
     for (int d = 0; d < 3; d++) {
@@ -1421,6 +1412,26 @@ __device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& 
   out[3 * i] = res[0];
   out[3 * i + 1] = res[1];
   out[3 * i + 2] = res[2];
+}
+
+template <typename R, typename P, typename O, typename M, bool SPEC = false>
+__device__ __forceinline__ uint32_t read_value(size_t i, const P x[3], const M& mesh,
+                                               const ReadArgs<R>& A, O* __restrict__ out) {
+  const Frame<R>& F = A.F;
+  const int wrap = SPEC ? 0 : A.wrap, local = SPEC ? 1 : A.local;
+  uint32_t err = 0;
+  R p[3];
+  to_mesh_units<R, P>(x, F, wrap, p, err);
+  if (!SPEC && A.field == PRH_FIELD_VALUE) {
+    // RealMesh.read_cic (mesh.py:570-610): the CIC value, cast to the positions dtype
+    out[i] = static_cast<O>(static_cast<P>(mesh.value(p)));
+    return err;
+  }
+  R g[3];
+  mesh.gradient(A.cell, p, g);
+  const P s[3] = {static_cast<P>(g[0]), static_cast<P>(g[1]), static_cast<P>(g[2])};
+  const int field = SPEC ? PRH_FIELD_DISP_RSD : A.field, shifted = SPEC ? 1 : A.shifted;
+  shift_out<R, P, O>(i, x, s, A, field, shifted, local, wrap, out);
   return err;
 }
 
@@ -2135,6 +2146,63 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
   return PRH_OK;
 }
 
+// Shifts of the FFT reconstructions (recon.py:255-325 without wrap): the displacement of point i
+// is (s0[i], s1[i], s2[i]), read by CIC from the three displacement meshes in the positions dtype.
+template <typename R, typename P, typename O>
+__global__ __launch_bounds__(kCicThreads) void k_shift_algebra(const P* __restrict__ pos, const P* __restrict__ s0,
+                                                               const P* __restrict__ s1, const P* __restrict__ s2,
+                                                               size_t n, ReadArgs<R> A, O* __restrict__ out) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kCicThreads) + threadIdx.x;
+  if (i >= n) return;
+  const P x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  const P s[3] = {s0[i], s1[i], s2[i]};
+  shift_out<R, P, O>(i, x, s, A, A.field, A.shifted, A.local, 0, out);
+}
+
+template <typename R>
+static int do_shift_algebra(const void* pos, int pdt, const void* s0, const void* s1, const void* s2, size_t n,
+                            double f, const R* los, int field, int shifted, void* out, int odt, void* stream) {
+  if (n && (!pos || !s0 || !s1 || !s2 || !out)) return PRH_ERR_ARG;
+  if (field != PRH_FIELD_DISP && field != PRH_FIELD_RSD && field != PRH_FIELD_DISP_RSD) {
+    set_error("unknown field %d", field);
+    return PRH_ERR_ARG;
+  }
+  if ((pdt != PRH_F32 && pdt != PRH_F64) || (odt != PRH_F32 && odt != PRH_F64)) {
+    set_error("positions/output dtype must be PRH_F32 or PRH_F64");
+    return PRH_ERR_ARG;
+  }
+  const int q = pdt > Tag<R>::code ? pdt : Tag<R>::code;
+  const int expect = (field == PRH_FIELD_RSD && los) ? q : pdt;
+  if (odt != expect) {
+    set_error("output dtype %d does not follow numpy promotion (expected %d)", odt, expect);
+    return PRH_ERR_ARG;
+  }
+  if (n == 0) return PRH_OK;
+  ReadArgs<R> A{};
+  A.fgrowth = f;
+  for (int d = 0; d < 3; d++) A.los[d] = los ? los[d] : R(0);
+  A.local = los ? 0 : 1;
+  A.field = field;
+  A.wrap = 0;
+  A.shifted = shifted;
+  const unsigned grid = static_cast<unsigned>(ceil_div(n, static_cast<size_t>(kCicThreads)));
+  hipStream_t s = as_stream(stream);
+  if (pdt == PRH_F64)
+    k_shift_algebra<R, double, double><<<grid, kCicThreads, 0, s>>>(
+        static_cast<const double*>(pos), static_cast<const double*>(s0), static_cast<const double*>(s1),
+        static_cast<const double*>(s2), n, A, static_cast<double*>(out));
+  else if (odt == PRH_F32)
+    k_shift_algebra<R, float, float><<<grid, kCicThreads, 0, s>>>(
+        static_cast<const float*>(pos), static_cast<const float*>(s0), static_cast<const float*>(s1),
+        static_cast<const float*>(s2), n, A, static_cast<float*>(out));
+  else
+    k_shift_algebra<R, float, double><<<grid, kCicThreads, 0, s>>>(
+        static_cast<const float*>(pos), static_cast<const float*>(s0), static_cast<const float*>(s1),
+        static_cast<const float*>(s2), n, A, static_cast<double*>(out));
+  PRH_LAUNCHED();
+  return PRH_OK;
+}
+
 template <typename R>
 static int do_read_shifts(prh_workspace* ws, const R* phi, const size_t* nmesh, const R* boxsize, const R* boxcenter,
                           const void* pos, int pdt, size_t n, double f, const R* los, int field,
@@ -2257,6 +2325,19 @@ int prh_paint_mixed_f64(prh_workspace* ws, double* mesh, const size_t* nmesh, co
                         int wdtype, size_t npositions, int wrap, void* stream) {
   return do_paint_mixed(ws, mesh, nmesh, boxsize, boxcenter, positions, pdtype, weights, wdtype, npositions,
                         wrap, stream);
+}
+
+int prh_shift_algebra_f32(const void* positions, int pos_dtype, const void* s0, const void* s1, const void* s2,
+                          size_t npositions, double f, const float* los, int field, int shifted, void* out,
+                          int out_dtype, void* stream) {
+  return do_shift_algebra<float>(positions, pos_dtype, s0, s1, s2, npositions, f, los, field, shifted, out,
+                                 out_dtype, stream);
+}
+int prh_shift_algebra_f64(const void* positions, int pos_dtype, const void* s0, const void* s1, const void* s2,
+                          size_t npositions, double f, const double* los, int field, int shifted, void* out,
+                          int out_dtype, void* stream) {
+  return do_shift_algebra<double>(positions, pos_dtype, s0, s1, s2, npositions, f, los, field, shifted, out,
+                                  out_dtype, stream);
 }
 
 int prh_read_shifts_f32(prh_workspace* ws, const float* phi, const size_t* nmesh, const float* boxsize,

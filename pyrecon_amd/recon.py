@@ -294,16 +294,48 @@ class SpectralReconstruction(BaseReconstruction):
         self._fft(spectrum, out.buffer, True, _lib.FFT_REAL_OUT)
         return out
 
-    def read_shifts(self, positions, field='disp+rsd'):
-        """Displacements at ``positions`` (pyrecon/recon.py:255-302): each component read by CIC from
-        :attr:`mesh_psi` on the device, positions wrapped first if :attr:`wrap`; the RSD algebra is the
-        reference's numpy.  Host arrays in and out (device inputs are copied to the host)."""
+    _FIELDS = {'disp': _lib.FIELD_DISP, 'rsd': _lib.FIELD_RSD, 'disp+rsd': _lib.FIELD_DISP_RSD}
+
+    def _check_field(self, field):
         field = field.lower()
-        allowed_fields = ['disp', 'rsd', 'disp+rsd']
-        if field not in allowed_fields:
-            raise ReconstructionError('Unknown field {}. Choices are {}'.format(field, allowed_fields))
+        if field not in self._FIELDS:
+            raise ReconstructionError('Unknown field {}. Choices are {}'.format(field, list(self._FIELDS)))
         if getattr(self, 'mesh_psi', None) is None:
             raise ReconstructionError('run the reconstruction before reading shifts')
+        return field
+
+    def _device_shifts(self, positions, field, shifted):
+        """Shifts (or, ``shifted``, shifted positions) without wrap, on the device: the three CIC reads
+        of :attr:`mesh_psi`, then the RSD algebra of pyrecon/recon.py:255-325 in numpy's dtypes and
+        order (``prh_shift_algebra_*``).  Host in -> host out, device in -> device out."""
+        pos, is_host = as_device(positions, stream=self.stream)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError('positions must have shape (N, 3)')
+        if pos.dtype not in (np.dtype('f4'), np.dtype('f8')):
+            pos, is_host = as_device(np.asarray(positions, dtype='f8'), stream=self.stream)
+        comps = [psi.read_cic(pos, wrap=False) for psi in self.mesh_psi]
+        los = None if self.los is None else np.ascontiguousarray(self.los, dtype=self.info.dtype)
+        odt = pos.dtype
+        if field == 'rsd' and los is not None: odt = np.promote_types(pos.dtype, self.info.dtype)
+        out = DeviceArray((len(pos), 3), odt, stream=self.stream)
+        fn = getattr(lib, 'prh_shift_algebra_' + _suffix(self.info.dtype))
+        check(fn(pos.ptr, _lib.dtype_code(pos.dtype), comps[0].ptr, comps[1].ptr, comps[2].ptr, len(pos),
+                 float(self.f), los.ctypes.data if los is not None else None, self._FIELDS[field], int(bool(shifted)),
+                 out.ptr, _lib.dtype_code(odt), stream_handle(self.stream)), 'shift algebra')
+        return out.numpy(stream=self.stream) if is_host else out
+
+    def read_shifts(self, positions, field='disp+rsd'):
+        """Displacements at ``positions`` (pyrecon/recon.py:255-302): each component read by CIC from
+        :attr:`mesh_psi` on the device.  Without :attr:`wrap` the RSD algebra runs on the device
+        (host in -> host out, device in -> device out); with it, positions are wrapped first and the
+        algebra is the reference's numpy on the host (host arrays out)."""
+        field = self._check_field(field)
+        if not self.wrap: return self._device_shifts(positions, field, False)
+        return self._host_shifts(positions, field)
+
+    def _host_shifts(self, positions, field):
+        """The shifts with the reference's numpy RSD algebra on the host, positions wrapped first if
+        :attr:`wrap` (host arrays out)."""
         if isinstance(positions, DeviceArray): positions = positions.numpy(stream=self.stream)
         positions = np.asarray(positions)
         shifts = np.empty_like(positions)
@@ -325,8 +357,10 @@ class SpectralReconstruction(BaseReconstruction):
 
     def read_shifted_positions(self, positions, field='disp+rsd'):
         """``positions - read_shifts(positions)``, wrapped if :attr:`wrap` (pyrecon/recon.py:304-325)."""
+        field = self._check_field(field)
+        if not self.wrap: return self._device_shifts(positions, field, True)
         if isinstance(positions, DeviceArray): positions = positions.numpy(stream=self.stream)
-        shifts = self.read_shifts(positions, field=field)
+        shifts = self._host_shifts(positions, field)
         positions = positions - shifts
         if self.wrap: positions = self.info.wrap(positions)
         return positions
