@@ -1637,6 +1637,7 @@ template <typename R, typename P, typename W, int MODE>
 static int paint_chunk(prh_workspace* ws, R* mesh, const Frame<R>& F, const P* pos, const W* w,
                        size_t m, bool check_only, hipStream_t s) {
   PaintScratch& S = ws->paint;
+  S.bins = PaintScratch::Bins();  // the payload memory is overwritten
   const unsigned grid = ceil_div(m, kCicThreads);
   k_paint_key<R, P, W, MODE><<<grid, kCicThreads, 0, s>>>(
       pos, w, m, F, S.key, static_cast<Payload<R>*>(S.payload), S.misc, check_only ? 1 : 0);
@@ -1779,6 +1780,7 @@ static int tile_paint_chunk(prh_workspace* ws, double* mesh, const Frame<double>
                             const TileGrid& TG, const P* pos, const W* w, size_t m, bool check_only,
                             hipStream_t s, bool keep) {
   PaintScratch& S = ws->paint;
+  S.bins = PaintScratch::Bins();  // the payload memory is overwritten
   const unsigned grid = ceil_div(m, kCicThreads);
   keep = keep && !check_only && grid <= S.nhpart;
   // the packed records go to the paint payload memory (32 bytes per point at most; chunk x 4 x 8 held)
@@ -2119,9 +2121,23 @@ static int read_shifts_all(prh_workspace* ws, const R* phi, const ReadArgs<R>& A
   // the tile order goes to the memory of the point-order payload (dead after a paint), so that
   // both paint orders are kept
   uint32_t* order = static_cast<uint32_t*>(S.payload);
+  if (n <= S.chunk && S.bins.pos == static_cast<const void*>(pos) && S.bins.n == n && S.bins.pdt == Tag<P>::code) {
+    // the same positions were binned for the previous read (e.g. another component's mesh)
+    timer_before(PRH_TIMER_READ_SHIFTS, s);
+    k_read_binned<R, P, O><<<ceil_div(n, kCicThreads), kCicThreads, 0, s>>>(phi, A, pos, n, out, misc, order);
+    timer_after(PRH_TIMER_READ_SHIFTS, s, bytes);
+    PRH_LAUNCHED();
+    return PRH_OK;
+  }
+  S.bins = PaintScratch::Bins();
   const int t0 = (F.n[0] + kRT - 1) / kRT, t1 = (F.n[1] + kRT - 1) / kRT, t2 = (F.n[2] + kRT - 1) / kRT;
   const size_t ntile = static_cast<size_t>(t0) * t1 * t2;
   const unsigned nscan = ceil_div(ntile, kScanBlock);
+  if (n <= S.chunk) {
+    S.bins.pos = pos;
+    S.bins.n = n;
+    S.bins.pdt = Tag<P>::code;
+  }
   for (size_t o = 0; o < n; o += S.chunk) {
     const size_t m = n - o < S.chunk ? n - o : S.chunk;
     const unsigned grid = ceil_div(m, kCicThreads);

@@ -541,6 +541,7 @@ int prh_workspace_reserve_paint(prh_workspace* ws, size_t paint_chunk) {
     ws->bytes += slots[i].new_bytes;
     ws->bytes -= slots[i].old_bytes;
   }
+  P.bins = PaintScratch::Bins();  // `payload` was reallocated
   P.chunk = paint_chunk;
   P.npartial = z.npartial;
   P.nhist = z.nhist;

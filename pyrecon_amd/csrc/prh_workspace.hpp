@@ -81,6 +81,16 @@ struct PaintScratch {
     uint64_t hash = 0;
   };
   Order order, order_prev;  // tags of `sorted` and `sorted_prev`
+  // The read binning of the last read of positions that no kept paint order covers lives in
+  // `payload` until a paint (or a growth of the scratch) overwrites it: a read of the same
+  // positions (pointer, count, dtype) in one pass reuses it.  Any permutation of [0, n) gives the
+  // same results; the bins only set the locality.
+  struct Bins {
+    const void* pos = nullptr;
+    size_t n = 0;
+    int pdt = 0;
+  };
+  Bins bins;
 };
 
 // Tables of the separable Fourier-space smoothing: per axis, the twiddles

@@ -1,7 +1,8 @@
 """The FFT reconstructions' shifts without wrap run their RSD algebra on the device
 (prh_shift_algebra_*, pyrecon/recon.py:255-325): the same bits as the reference's numpy algebra on the
 host (the class's own host form), for every field, shifted and not, the local and a global line of
-sight, FP32 and FP64 positions and meshes; device positions give device results."""
+sight, FP32 and FP64 positions and meshes; device positions give device results.  Repeated reads of
+the same positions share one binning of the points, which a paint in between invalidates."""
 import numpy as np
 import pytest
 
@@ -46,3 +47,20 @@ def test_device_algebra_same_bits(algorithm, los, dtype, pdtype):
     out = rec.read_shifted_positions(DeviceArray.from_numpy(pos))
     assert isinstance(out, DeviceArray)
     assert np.array_equal(bits(out.numpy()), bits(pos - rec._host_shifts(pos, 'disp+rsd')))
+
+
+def test_read_bins_reused_and_invalidated():
+    # the three component reads of one set of positions share one binning; a paint in between
+    # overwrites the memory that holds it, and the next read bins again: the same values throughout
+    from pyrecon_amd import IterativeFFTReconstruction, DeviceArray
+    rec, randoms = _recon(IterativeFFTReconstruction, 'f8', None)
+    pos = DeviceArray.from_numpy(randoms[:7000])
+    first = [psi.read_cic(pos).numpy() for psi in rec.mesh_psi]
+    again = [psi.read_cic(pos).numpy() for psi in rec.mesh_psi]
+    from pyrecon_amd.mesh import RealMesh
+    scratch = RealMesh(info=rec.info, workspace=rec.workspace)
+    scratch.assign_cic(randoms[3000:9000])  # a paint through the same workspace
+    after = [psi.read_cic(pos).numpy() for psi in rec.mesh_psi]
+    fresh = [psi.read_cic(randoms[:7000]) for psi in rec.mesh_psi]
+    for a, b, c, d in zip(first, again, after, fresh):
+        assert np.array_equal(bits(a), bits(b)) and np.array_equal(bits(a), bits(c)) and np.array_equal(bits(a), bits(d))
